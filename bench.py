@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Throughput of the per-block RHS + TVD-RK3 path on one MI355X.
+
+Metric: cell-updates/s per RK3 step in FP64 (one cell-update = one cell
+advanced one full TVD-RK3 step: three RHS evaluations and three stage
+updates).  Workload: decaying compressible isotropic turbulence on a 256^3
+periodic block (Navier-Stokes, WENO5-JS with local Lax-Friedrichs splitting,
+6th-order central viscous terms, CFL 0.5), synthetic initial field of the
+prescribed spectrum; state resident in HBM before the timed region.
+
+Prints one JSON line with the throughput, the live roofline figure of the
+fused stage kernel (HIP events on its launch stream) and a CPU baseline: the
+reference compiled from its own sources (oracle/_ref) timed on the host cores
+on a bounded sample of the same workload, or the oracle where that build is
+absent.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s per RK3 step FP64: plain WENO5 vs learned hybrid at equal L2 error"
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per cell of the fused stage kernel: stage 0 reads Q
+# (40) and metrics (80) and writes Q1 (40); stages 1 and 2 also read Q^n (40)
+ALG_BYTES_PER_STAGE = (160.0, 200.0, 200.0)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, nsteps=3, ranks=None, timeout=240):
+    """Reference (oracle/_ref, -O2) on a bounded sample of the workload:
+    a 256 x 128 x 32 periodic slab of the same turbulence field, split into
+    `ranks` MPI blocks along x, 3 RK3 steps, mean of steps 2..3."""
+    import numpy as np
+    from nuc3d_amd import cases
+    from oracle import oracle as O
+
+    ranks = ranks or min(16, os.cpu_count() or 1)
+    blocks, phys = cases.isotropic_turbulence(n=n, seed=0)
+    spec = blocks[0]
+    nx, ny, nz = 256, 128, 32
+    sub = cases.BlockSpec((nx, ny, nz), cases.periodic_faces(), spec.jac[:nz, :ny, :nx].copy(),
+                          spec.met[:, :nz, :ny, :nx].copy(), spec.dxyz[:, :nz, :ny, :nx].copy(),
+                          spec.q[:, :nz, :ny, :nx].copy())
+    while nx // ranks < 8:
+        ranks //= 2
+    parts = cases.split_block_x(sub, ranks)
+    p = O.Params(model=O.NS, mach=phys.mach, reynolds=phys.reynolds)
+    cells = nx * ny * nz
+    if O.ref_available(optimised=True):
+        d = tempfile.mkdtemp(prefix="nuc3d_cpu_")
+        try:
+            O.write_ref_case(d, p, parts)
+            out = O.run_ref(d, ranks, "steps", nsteps, 0.5, optimised=True, timeout=timeout)
+            walls = out[0]["walls"]
+            t = float(np.mean(walls[1:]))
+            return {"value": cells / t, "unit": "cell-updates/s", "cores": ranks, "kind": "reference",
+                    "sample": "reference built from its sources at -O2 (oracle/_ref), %d MPI ranks, "
+                              "%dx%dx%d periodic slab of the 256^3 turbulence field in %d x-blocks, "
+                              "mean wall time of RK3 steps 2..%d" % (ranks, nx, ny, nz, ranks, nsteps)}
+        except Exception as e:  # fall back to the oracle below
+            log("cpu baseline: reference run failed (%s); timing the oracle" % e)
+    sub = cases.BlockSpec((64, 64, 32), cases.periodic_faces(), spec.jac[:32, :64, :64].copy(),
+                          spec.met[:, :32, :64, :64].copy(), spec.dxyz[:, :32, :64, :64].copy(),
+                          spec.q[:, :32, :64, :64].copy())
+    oc = O.OracleCase(p, [sub])
+    t0 = time.perf_counter()
+    oc.steps(1, 0.5)
+    t = time.perf_counter() - t0
+    return {"value": 64 * 64 * 32 / t, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "C restatement (oracle/), 1 core, 64x64x32 periodic slab of the 256^3 field, 1 RK3 step"}
+
+
+def read_traffic():
+    """HBM bytes per launch of the stage kernel from the committed rocprofv3
+    counter summary, when present."""
+    path = os.path.join(ROOT, "profiles", "k_stage_iso256_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            try:
+                return float(line.split()[1])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--exact", action="store_true", help="reference operation order (no FMA contraction)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (every block of a case lives on one MI355X); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from nuc3d_amd import cases, solver
+
+    n = args.n
+    log("bench: generating %d^3 isotropic turbulence" % n)
+    blocks, phys = cases.isotropic_turbulence(n=n, seed=0)
+    params = solver.Params(model=solver.NAVIERSTOKES3D, mach=phys.mach, reynolds=phys.reynolds,
+                           exact=1 if args.exact else 0)
+    case = solver.Case(params, blocks)
+    cells = n ** 3
+    log("bench: warmup %d steps" % args.warmup)
+    case.steps(args.warmup, 0.5)
+    case.sync()
+    case.timing(True)
+    case.timing_read()
+    log("bench: timing %d steps" % args.steps)
+    case.sync()
+    t0 = time.perf_counter()
+    case.steps(args.steps, 0.5)
+    case.sync()
+    t1 = time.perf_counter()
+    stage_ms, stage_n, visc_ms, visc_n = case.timing_read()
+    case.timing(False)
+    elapsed = t1 - t0
+    value = cells * args.steps / elapsed
+    bytes_total = cells * args.steps * sum(ALG_BYTES_PER_STAGE)
+    achieved = bytes_total / (stage_ms * 1e-3) / 1e9 if stage_ms > 0 else 0.0
+    traffic = read_traffic()
+    log("bench: %.3f ms/step, stage kernel %.3f ms avg over %d launches, visc %.3f ms avg over %d"
+        % (1e3 * elapsed / args.steps, stage_ms / max(stage_n, 1), stage_n, visc_ms / max(visc_n, 1), visc_n))
+    case.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline")
+        try:
+            cpu = cpu_baseline(n)
+        except Exception as e:
+            log("cpu baseline failed: %s" % e)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "decaying isotropic turbulence %d^3 periodic, Navier-Stokes, WENO5-JS + local LF "
+                        "splitting, cd6 viscous terms, TVD-RK3 CFL 0.5 (plain path)" % n,
+            "cells": cells,
+            "variant": "exact" if args.exact else "fast",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_stage (fused RHS + RK stage)",
+            "avg_launch_ms": stage_ms / max(stage_n, 1),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

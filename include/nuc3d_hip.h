@@ -1,0 +1,126 @@
+/*
+ * nuc3d_hip.h -- C ABI of the MI355X (gfx950) implementation of NUC3d's
+ * per-block right-hand side and TVD-RK3 update.
+ *
+ * The ABI replaces, for all blocks of a case held on one GPU, the calls that
+ * the reference's block::solve (block.cpp:569-594) makes per RK stage:
+ *   myFluxes->solve(myPDE, myOP, mybuffer, myPhys, myMPI, myBC)
+ *       -> EulerData3D::solve (euler3d.cpp:248-262) or
+ *          NaiverStokesData3d::solve (NaiverStokes3d.cpp:36-49)
+ *   myPDE.solve(myOP, cfl, stage)          -> PDEData3d::solve (PDEData3d.cpp:70-88)
+ * together with the MPI halo exchange they contain
+ * (MPICommunicator.cpp:36-115), which becomes on-device ghost copies.
+ *
+ * Plain C types only: host pointers and sizes, an opaque case handle and an
+ * optional hipStream_t passed as void*.  Every call returns a status (0 = OK);
+ * nuc3d_last_error() describes the last failure of the calling thread.  No
+ * call exits the process (the reference's exit(-1)/MPI_Abort paths become
+ * status codes).
+ *
+ * Field arrays are float64, i-fastest (index k*nx*ny + j*nx + i) as in
+ * field3d<double> (field.h:11-81, field.cpp:95-108); a group of fields is
+ * stored one after the other (q5 = five fields of nx*ny*nz).
+ */
+#ifndef NUC3D_HIP_H
+#define NUC3D_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    NUC3D_OK = 0,
+    NUC3D_ERR_BAD_ARG = -1,     /* invalid argument (reference: exit(-1)/exit(0)) */
+    NUC3D_ERR_OOM = -2,         /* device allocation failed */
+    NUC3D_ERR_HIP = -3,         /* a HIP runtime call failed */
+    NUC3D_ERR_UNSUPPORTED = -4, /* configuration outside what this build implements */
+    NUC3D_ERR_NO_DEVICE = -5,   /* no GPU visible */
+    NUC3D_ERR_ORDER = -6        /* RK stages called out of order */
+};
+
+/* Schemes and models, named as in inp/Method.in and inp/PhysModel.in
+ * (fieldOperator.cpp:127-369, physicsModel.cpp:13-112). */
+enum { NUC3D_WENO5JS = 0, NUC3D_WENO5Z = 1, NUC3D_CRWENO5 = 2 };
+enum { NUC3D_CD6 = 0, NUC3D_CD8 = 1 };
+enum { NUC3D_LF = 0, NUC3D_AUSM = 1, NUC3D_AUSMP = 2 };
+enum { NUC3D_EULER3D = 0, NUC3D_NAVIERSTOKES3D = 1 };
+enum { NUC3D_BC_INLET = 0, NUC3D_BC_OUTLET = 1, NUC3D_BC_WALL = 2, NUC3D_BC_SYMMETRIC = 3 };
+
+typedef struct {
+    /* physicsModel parameters (physicsModel.cpp:41-51) */
+    double gamma, mach, reynolds, prandtl, t_ref, t_inf, t_wall;
+    int model;     /* NUC3D_EULER3D / NUC3D_NAVIERSTOKES3D */
+    int riemann;   /* flux splitting: NUC3D_LF / NUC3D_AUSM / NUC3D_AUSMP */
+    int scheme[3]; /* reconstruction per direction (scheme_x_ivs ..) */
+    int diff[3];   /* viscous difference per direction (scheme_x_vis ..) */
+    int width[3];  /* ghost width per direction (bufferSize_xi ..) */
+    int zquirk;    /* 1: reproduce the +inner beta2 of weno5z.cpp:80 */
+    int exact;     /* 1: reference operation order without FMA contraction */
+} nuc3d_params;
+
+typedef struct nuc3d_case nuc3d_case;
+
+/* Allocate a case of nblocks blocks (dims = nblocks x {nx,ny,nz}) on the
+ * current device.  Replaces block::initialData (block.cpp:479-520). */
+int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d_case **out);
+int nuc3d_case_destroy(nuc3d_case *c);
+
+/* Face topology of block b: the six lines of bc_topo/BC_Topo_<b>.in
+ * (boundaryConditions.cpp:25-93).  type6[f] = -1 for a physical face with
+ * id6[f] the BC id (val30[5f..5f+4] = inlet primitives rho,u,v,w,p), else the
+ * neighbour block with id6[f] its face.  Connected faces must join the same
+ * direction and have equal tangential sizes (the raw slab copy of
+ * MPICommunicator.cpp:55-81). */
+int nuc3d_set_faces(nuc3d_case *c, int b, const int *type6, const int *id6, const double *val30);
+
+/* Metrics of block b (block::getJacobians output, block.cpp:237-325):
+ * jac (N), met (9N: xi_x,xi_y,xi_z,eta_x,..,zeta_z) and dxyz (9N:
+ * x_xi,x_eta,x_zeta,y_xi,..,z_zeta; may be NULL unless a face is symmetric). */
+int nuc3d_upload_metrics(nuc3d_case *c, int b, const double *jac, const double *met, const double *dxyz);
+
+/* State of block b: Q_work = conserved / J (5N), PDEData3d::getQ. */
+int nuc3d_upload_q(nuc3d_case *c, int b, const double *q5);
+int nuc3d_download_q(nuc3d_case *c, int b, double *q5);
+/* RHS of the last nuc3d_rhs call (5N), PDEData3d::getRHS. */
+int nuc3d_download_rhs(nuc3d_case *c, int b, double *rhs5);
+
+/* One right-hand-side evaluation of every block on the current state
+ * (EulerData3D::solve / NaiverStokesData3d::solve, including the halo
+ * exchange and boundary conditions). */
+int nuc3d_rhs(nuc3d_case *c, void *stream);
+
+/* RHS evaluation plus PDEData3d::solve for TVD-RK3 stage 0, 1 or 2; stages
+ * must follow each other in that order.  Stage 0 takes dt = cfl * min over
+ * blocks and directions of 1/max eigenvalue of the stage-0 state. */
+int nuc3d_stage(nuc3d_case *c, int stage, double cfl, void *stream);
+
+/* nsteps full TVD-RK3 steps (block::solve, block.cpp:569-594). */
+int nuc3d_steps(nuc3d_case *c, int nsteps, double cfl, void *stream);
+
+/* Wait for the case's work on its streams. */
+int nuc3d_sync(nuc3d_case *c);
+
+/* dt of the current step (PDEData3d::getDt) and per-block max eigenvalues
+ * of the last stage-0 state, or of the state of the last nuc3d_rhs call
+ * (EulerFlux::maxEigen, physicsModel.cpp:756-758). */
+int nuc3d_get_dt(nuc3d_case *c, double *dt_global);
+int nuc3d_get_max_eigen(nuc3d_case *c, int b, double *eig3);
+/* Number of e<0 energy fix-ups applied so far (physicsModel.cpp:324-332). */
+int nuc3d_get_fixups(nuc3d_case *c, long long *count);
+
+/* Kernel timing (no counterpart in the reference, which times whole steps
+ * with MPI_Wtime in block::solve, block.cpp:576-584).  While enabled, every
+ * launch of the fused stage kernel and of the viscous-flux kernels is
+ * bracketed by HIP events on the launch stream; nuc3d_timing_read returns
+ * the summed milliseconds and launch counts since the last read and resets
+ * them. */
+int nuc3d_timing(nuc3d_case *c, int enable);
+int nuc3d_timing_read(nuc3d_case *c, double *stage_ms, int *stage_launches, double *visc_ms, int *visc_launches);
+
+/* Description of the calling thread's last error. */
+const char *nuc3d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

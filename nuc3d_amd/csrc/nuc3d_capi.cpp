@@ -1,0 +1,531 @@
+// nuc3d_capi.cpp -- host side of the C ABI declared in include/nuc3d_hip.h.
+//
+// Owns the device memory of a case (all blocks on the current GPU), the block
+// descriptors and tile tables, the three state buffers of the TVD-RK3 cycle
+// and the small control arrays (dt, eigenvalue maxima, counters), and
+// enqueues the kernels of nuc3d_kernels.hip on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nuc3d_hip.h"
+#include "nuc3d_types.h"
+
+#define NUC3D_LAUNCHERS(SFX)                                                                                         \
+    extern "C" hipError_t nuc3d_launch_stage_##SFX(int, int, int, const nuc3d_block_dev *, const nuc3d_tile *, int,  \
+                                                   nuc3d_kparams, int, int, int, int, double *,                      \
+                                                   const unsigned long long *, unsigned long long *,                 \
+                                                   unsigned long long *, hipStream_t);                               \
+    extern "C" hipError_t nuc3d_launch_visc_##SFX(const nuc3d_block_dev *, int, const nuc3d_tile *, int,             \
+                                                  nuc3d_kparams, int, hipStream_t);                                  \
+    extern "C" hipError_t nuc3d_launch_eigen_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, nuc3d_kparams,  \
+                                                   int, unsigned long long *, hipStream_t);                          \
+    extern "C" hipError_t nuc3d_launch_fixup_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, nuc3d_kparams,  \
+                                                   int, unsigned long long *, hipStream_t);                          \
+    extern "C" hipError_t nuc3d_launch_ghost_fill_##SFX(const nuc3d_block_dev *, int, int, int, hipStream_t);
+NUC3D_LAUNCHERS(fast)
+NUC3D_LAUNCHERS(exact)
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hipfail(hipError_t e, const char *what)
+{
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return NUC3D_ERR_HIP;
+}
+
+#define HIPCHK(x)                                      \
+    do {                                               \
+        hipError_t e_ = (x);                           \
+        if (e_ != hipSuccess) return hipfail(e_, #x);  \
+    } while (0)
+
+struct Launchers {
+    decltype(&nuc3d_launch_stage_fast) stage;
+    decltype(&nuc3d_launch_visc_fast) visc;
+    decltype(&nuc3d_launch_eigen_fast) eigen;
+    decltype(&nuc3d_launch_fixup_fast) fixup;
+    decltype(&nuc3d_launch_ghost_fill_fast) ghost;
+};
+
+const Launchers kFast = {nuc3d_launch_stage_fast, nuc3d_launch_visc_fast, nuc3d_launch_eigen_fast,
+                         nuc3d_launch_fixup_fast, nuc3d_launch_ghost_fill_fast};
+const Launchers kExact = {nuc3d_launch_stage_exact, nuc3d_launch_visc_exact, nuc3d_launch_eigen_exact,
+                          nuc3d_launch_fixup_exact, nuc3d_launch_ghost_fill_exact};
+
+constexpr int KSEG = 32;  // planes per k segment of a tile
+
+}  // namespace
+
+struct nuc3d_case {
+    nuc3d_params p;
+    nuc3d_kparams K;
+    const Launchers *L;
+    int nb = 0;
+    int G = 3;
+    int device = 0;
+    std::vector<nuc3d_block_dev> hb;   // host copy of the descriptors
+    std::vector<void *> chunks;        // one device allocation per block
+    std::vector<long long> padded;     // padded cells per block
+    nuc3d_block_dev *db = nullptr;     // device descriptors
+    nuc3d_tile *dtiles = nullptr;
+    int ntiles = 0;
+    double *ddt = nullptr;             // dt of the current step
+    unsigned long long *dlam = nullptr;   // [2][nb][3] eigenvalue maxima (bit patterns)
+    unsigned long long *dcount = nullptr; // [4] fix-up counter
+    int cur = 0;                       // state buffer holding Q_work
+    int next_stage = 0;
+    int lampar = 0;
+    bool dirty = true;                 // descriptors / ghosts / eigenvalues need refreshing
+    bool faces_set = false;
+    // kernel timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_stage, ev_visc;
+};
+
+namespace {
+
+int check_block(nuc3d_case *c, int b)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    if (b < 0 || b >= c->nb) return fail(NUC3D_ERR_BAD_ARG, "block index out of range");
+    return NUC3D_OK;
+}
+
+// copy an interior field (nz,ny,nx) into / out of the padded device layout
+// through a host staging copy of the whole padded box (ghosts are rebuilt on
+// the device by refresh()).
+int put_field(nuc3d_case *c, int b, double *dst, const double *src)
+{
+    const nuc3d_block_dev &B = c->hb[b];
+    const int G = B.G, nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    std::vector<double> st((size_t)c->padded[b], 0.0);
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            std::memcpy(&st[(size_t)(G + B.sy * (j + G) + B.sz * (k + G))], src + ((size_t)k * ny + j) * nx,
+                        sizeof(double) * nx);
+    HIPCHK(hipMemcpy(dst - (G + B.sy * G + B.sz * G), st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice));
+    return NUC3D_OK;
+}
+
+int get_field(nuc3d_case *c, int b, double *dst, const double *src)
+{
+    const nuc3d_block_dev &B = c->hb[b];
+    const int G = B.G, nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    std::vector<double> st((size_t)c->padded[b]);
+    HIPCHK(hipMemcpy(st.data(), src - (G + B.sy * G + B.sz * G), sizeof(double) * st.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            std::memcpy(dst + ((size_t)k * ny + j) * nx, &st[(size_t)(G + B.sy * (j + G) + B.sz * (k + G))],
+                        sizeof(double) * nx);
+    return NUC3D_OK;
+}
+
+hipEvent_t take_event(nuc3d_case *c)
+{
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// brackets one launch with events when timing is enabled
+struct Timed {
+    nuc3d_case *c;
+    hipStream_t s;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> *dst;
+    hipEvent_t a = nullptr;
+    Timed(nuc3d_case *c_, hipStream_t s_, std::vector<std::pair<hipEvent_t, hipEvent_t>> *d) : c(c_), s(s_), dst(d)
+    {
+        if (c->timing && (a = take_event(c))) (void)hipEventRecord(a, s);
+    }
+    ~Timed()
+    {
+        if (!a) return;
+        hipEvent_t b = take_event(c);
+        if (!b) return;
+        (void)hipEventRecord(b, s);
+        dst->push_back({a, b});
+    }
+};
+
+int refresh(nuc3d_case *c, hipStream_t s)
+{
+    if (!c->dirty) return NUC3D_OK;
+    HIPCHK(hipMemcpy(c->db, c->hb.data(), sizeof(nuc3d_block_dev) * c->nb, hipMemcpyHostToDevice));
+    // ghosts of J and metrics, fix-up and ghosts of the state, eigenvalues
+    HIPCHK(c->L->ghost(c->db, c->nb, 1, c->cur, s));
+    HIPCHK(c->L->fixup(c->db, c->dtiles, c->ntiles, c->K, c->cur, c->dcount, s));
+    HIPCHK(c->L->ghost(c->db, c->nb, 0, c->cur, s));
+    HIPCHK(hipMemsetAsync(c->dlam + 3 * c->nb * c->lampar, 0, sizeof(unsigned long long) * 3 * c->nb, s));
+    HIPCHK(c->L->eigen(c->db, c->dtiles, c->ntiles, c->K, c->cur, c->dlam + 3 * c->nb * c->lampar, s));
+    c->dirty = false;
+    return NUC3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *nuc3d_last_error(void) { return g_err.c_str(); }
+
+int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d_case **out)
+{
+    if (!p || !dims || !out) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    *out = nullptr;
+    if (nblocks < 1 || nblocks > NUC3D_MAX_BLOCKS) return fail(NUC3D_ERR_BAD_ARG, "nblocks out of range");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(NUC3D_ERR_NO_DEVICE, "no HIP device");
+    for (int d = 0; d < 3; d++) {
+        if (p->scheme[d] != p->scheme[0]) return fail(NUC3D_ERR_UNSUPPORTED, "mixed schemes per direction");
+        if (p->scheme[d] != NUC3D_WENO5JS && p->scheme[d] != NUC3D_WENO5Z)
+            return fail(NUC3D_ERR_UNSUPPORTED, "scheme not available in this build");
+        if (p->width[d] != 3) return fail(NUC3D_ERR_UNSUPPORTED, "ghost width must be 3 for WENO5 schemes");
+        if (p->diff[d] != NUC3D_CD6)
+            return fail(NUC3D_ERR_UNSUPPORTED, "cd8 needs ghost width 4 (centraldifference8th.cpp:34,49)");
+    }
+    if (p->model != NUC3D_EULER3D && p->model != NUC3D_NAVIERSTOKES3D) return fail(NUC3D_ERR_BAD_ARG, "model");
+    if (p->riemann < 0 || p->riemann > 2) return fail(NUC3D_ERR_BAD_ARG, "riemann");
+    nuc3d_case *c = new nuc3d_case();
+    c->p = *p;
+    c->nb = nblocks;
+    c->G = 3;
+    c->L = p->exact ? &kExact : &kFast;
+    (void)hipGetDevice(&c->device);
+    nuc3d_kparams &K = c->K;
+    K.gamma = p->gamma;
+    K.mach = p->mach;
+    K.reynolds = p->reynolds;
+    K.prandtl = p->prandtl;
+    K.sref = p->t_ref / p->t_inf;
+    K.twall = p->t_wall;
+    K.cfl = 0.5;
+    K.riemann = p->riemann;
+    for (int d = 0; d < 3; d++) K.width[d] = p->width[d];
+    K.zquirk = p->zquirk;
+    K.nblocks = nblocks;
+    const bool ns = p->model == NUC3D_NAVIERSTOKES3D;
+    const int nfields = 15 + 5 + 1 + 9 + 9 + (ns ? 16 : 0);
+    c->hb.resize(nblocks);
+    std::vector<nuc3d_tile> tiles;
+    for (int b = 0; b < nblocks; b++) {
+        nuc3d_block_dev &B = c->hb[b];
+        std::memset(&B, 0, sizeof B);
+        for (int d = 0; d < 3; d++) {
+            B.n[d] = dims[3 * b + d];
+            if (B.n[d] < 2 * c->G + 2) {
+                nuc3d_case_destroy(c);
+                return fail(NUC3D_ERR_BAD_ARG, "block dimension below 2*ghost+2");
+            }
+        }
+        B.G = c->G;
+        const long long px = B.n[0] + 2 * B.G, py = B.n[1] + 2 * B.G, pz = B.n[2] + 2 * B.G;
+        B.sy = px;
+        B.sz = px * py;
+        const long long P = px * py * pz;
+        c->padded.push_back(P);
+        void *mem = nullptr;
+        if (hipMalloc(&mem, sizeof(double) * P * nfields) != hipSuccess) {
+            nuc3d_case_destroy(c);
+            return fail(NUC3D_ERR_OOM, "device allocation of block fields failed");
+        }
+        c->chunks.push_back(mem);
+        if (hipMemset(mem, 0, sizeof(double) * P * nfields) != hipSuccess) {
+            nuc3d_case_destroy(c);
+            return fail(NUC3D_ERR_HIP, "hipMemset failed");
+        }
+        const long long org = B.G + B.sy * B.G + B.sz * B.G;  // offset of cell (0,0,0)
+        double *base = static_cast<double *>(mem);
+        int f = 0;
+        auto next = [&]() { return base + P * (f++) + org; };
+        for (int s = 0; s < 3; s++)
+            for (int q = 0; q < 5; q++) B.q[s][q] = next();
+        for (int q = 0; q < 5; q++) B.rhs[q] = next();
+        B.jac = next();
+        for (int q = 0; q < 9; q++) B.met[q] = next();
+        for (int q = 0; q < 9; q++) B.dxyz[q] = next();
+        if (ns) {
+            for (int q = 0; q < 4; q++) B.prim[q] = next();
+            for (int q = 0; q < 12; q++) B.fv[q] = next();
+        }
+        for (int fc = 0; fc < 6; fc++) {
+            B.ftype[fc] = -1;
+            B.fid[fc] = NUC3D_BC_OUTLET;
+        }
+        for (int k0 = 0; k0 < B.n[2]; k0 += KSEG)
+            for (int j0 = 0; j0 < B.n[1]; j0 += NUC3D_TILE)
+                for (int i0 = 0; i0 < B.n[0]; i0 += NUC3D_TILE) {
+                    nuc3d_tile t = {b, i0, j0, k0, std::min(KSEG, B.n[2] - k0), 0, 0, 0};
+                    tiles.push_back(t);
+                }
+    }
+    c->ntiles = (int)tiles.size();
+    if (hipMalloc(&c->db, sizeof(nuc3d_block_dev) * nblocks) != hipSuccess ||
+        hipMalloc(&c->dtiles, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess ||
+        hipMalloc(&c->ddt, sizeof(double) * 4) != hipSuccess ||
+        hipMalloc(&c->dlam, sizeof(unsigned long long) * 6 * nblocks) != hipSuccess ||
+        hipMalloc(&c->dcount, sizeof(unsigned long long) * 4) != hipSuccess) {
+        nuc3d_case_destroy(c);
+        return fail(NUC3D_ERR_OOM, "device allocation of control arrays failed");
+    }
+    if (hipMemcpy(c->dtiles, tiles.data(), sizeof(nuc3d_tile) * tiles.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->ddt, 0, sizeof(double) * 4) != hipSuccess ||
+        hipMemset(c->dlam, 0, sizeof(unsigned long long) * 6 * nblocks) != hipSuccess ||
+        hipMemset(c->dcount, 0, sizeof(unsigned long long) * 4) != hipSuccess) {
+        nuc3d_case_destroy(c);
+        return fail(NUC3D_ERR_HIP, "initialising control arrays failed");
+    }
+    *out = c;
+    return NUC3D_OK;
+}
+
+int nuc3d_case_destroy(nuc3d_case *c)
+{
+    if (!c) return NUC3D_OK;
+    (void)hipDeviceSynchronize();
+    for (auto &pr : c->ev_stage) c->ev_pool.push_back(pr.first), c->ev_pool.push_back(pr.second);
+    for (auto &pr : c->ev_visc) c->ev_pool.push_back(pr.first), c->ev_pool.push_back(pr.second);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    for (void *m : c->chunks) (void)hipFree(m);
+    (void)hipFree(c->db);
+    (void)hipFree(c->dtiles);
+    (void)hipFree(c->ddt);
+    (void)hipFree(c->dlam);
+    (void)hipFree(c->dcount);
+    delete c;
+    return NUC3D_OK;
+}
+
+int nuc3d_set_faces(nuc3d_case *c, int b, const int *type6, const int *id6, const double *val30)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!type6 || !id6) return fail(NUC3D_ERR_BAD_ARG, "null face arrays");
+    nuc3d_block_dev &B = c->hb[b];
+    for (int f = 0; f < 6; f++) {
+        const int t = type6[f], id = id6[f];
+        if (t == -1) {
+            if (id < 0 || id > 3) return fail(NUC3D_ERR_BAD_ARG, "BC id must be 0..3 (boundaryConditions.hpp:56-61)");
+        } else {
+            if (t < 0 || t >= c->nb) return fail(NUC3D_ERR_BAD_ARG, "neighbour block out of range");
+            if (id < 0 || id > 5 || id / 2 != f / 2)
+                return fail(NUC3D_ERR_UNSUPPORTED, "connected faces must join the same direction");
+            const nuc3d_block_dev &N = c->hb[t];
+            const int d = f / 2;
+            if (N.n[(d + 1) % 3] != B.n[(d + 1) % 3] || N.n[(d + 2) % 3] != B.n[(d + 2) % 3])
+                return fail(NUC3D_ERR_BAD_ARG, "connected faces differ in tangential size");
+        }
+        B.ftype[f] = t;
+        B.fid[f] = id;
+        for (int q = 0; q < 5; q++) B.fval[f][q] = val30 ? val30[5 * f + q] : 0.0;
+    }
+    c->dirty = true;
+    c->faces_set = true;
+    return NUC3D_OK;
+}
+
+int nuc3d_upload_metrics(nuc3d_case *c, int b, const double *jac, const double *met, const double *dxyz)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!jac || !met) return fail(NUC3D_ERR_BAD_ARG, "null metrics");
+    nuc3d_block_dev &B = c->hb[b];
+    const long long N = (long long)B.n[0] * B.n[1] * B.n[2];
+    if ((rc = put_field(c, b, B.jac, jac))) return rc;
+    for (int q = 0; q < 9; q++)
+        if ((rc = put_field(c, b, B.met[q], met + q * N))) return rc;
+    if (dxyz)
+        for (int q = 0; q < 9; q++)
+            if ((rc = put_field(c, b, B.dxyz[q], dxyz + q * N))) return rc;
+    c->dirty = true;
+    return NUC3D_OK;
+}
+
+int nuc3d_upload_q(nuc3d_case *c, int b, const double *q5)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!q5) return fail(NUC3D_ERR_BAD_ARG, "null state");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "upload in the middle of an RK step");
+    nuc3d_block_dev &B = c->hb[b];
+    const long long N = (long long)B.n[0] * B.n[1] * B.n[2];
+    for (int q = 0; q < 5; q++)
+        if ((rc = put_field(c, b, B.q[c->cur][q], q5 + q * N))) return rc;
+    c->dirty = true;
+    return NUC3D_OK;
+}
+
+int nuc3d_download_q(nuc3d_case *c, int b, double *q5)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!q5) return fail(NUC3D_ERR_BAD_ARG, "null output");
+    HIPCHK(hipDeviceSynchronize());
+    nuc3d_block_dev &B = c->hb[b];
+    const long long N = (long long)B.n[0] * B.n[1] * B.n[2];
+    for (int q = 0; q < 5; q++)
+        if ((rc = get_field(c, b, q5 + q * N, B.q[c->cur][q]))) return rc;
+    return NUC3D_OK;
+}
+
+int nuc3d_download_rhs(nuc3d_case *c, int b, double *rhs5)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!rhs5) return fail(NUC3D_ERR_BAD_ARG, "null output");
+    HIPCHK(hipDeviceSynchronize());
+    nuc3d_block_dev &B = c->hb[b];
+    const long long N = (long long)B.n[0] * B.n[1] * B.n[2];
+    for (int q = 0; q < 5; q++)
+        if ((rc = get_field(c, b, rhs5 + q * N, B.rhs[q]))) return rc;
+    return NUC3D_OK;
+}
+
+int nuc3d_rhs(nuc3d_case *c, void *stream)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = refresh(c, s);
+    if (rc) return rc;
+    // eigenvalues of the evaluated state (the stage-0 maxima stay untouched)
+    unsigned long long *lam = c->dlam + 3 * c->nb * (1 - c->lampar);
+    if (c->next_stage != 0) {
+        HIPCHK(hipMemsetAsync(lam, 0, sizeof(unsigned long long) * 3 * c->nb, s));
+        HIPCHK(c->L->eigen(c->db, c->dtiles, c->ntiles, c->K, c->cur, lam, s));
+    }
+    const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
+    if (ns) HIPCHK(c->L->visc(c->db, c->nb, c->dtiles, c->ntiles, c->K, c->cur, s));
+    HIPCHK(c->L->stage(c->p.scheme[0], ns, 0, c->db, c->dtiles, c->ntiles, c->K, 0, c->cur, 0, 0, c->ddt,
+                       c->dlam, lam, c->dcount, s));
+    return NUC3D_OK;
+}
+
+int nuc3d_stage(nuc3d_case *c, int stage, double cfl, void *stream)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    if (stage < 0 || stage > 2) return fail(NUC3D_ERR_BAD_ARG, "stage must be 0, 1 or 2");
+    if (stage != c->next_stage) return fail(NUC3D_ERR_ORDER, "RK stages must run 0, 1, 2 in order");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = refresh(c, s);
+    if (rc) return rc;
+    c->K.cfl = cfl;
+    const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
+    // buffers: stage 0 A->B, stage 1 (B, u_n=A)->C, stage 2 (C, u_n=A)->A
+    const int bin = stage, bun = 0, bout = stage == 2 ? 0 : stage + 1;
+    if (ns) {
+        Timed t(c, s, &c->ev_visc);
+        HIPCHK(c->L->visc(c->db, c->nb, c->dtiles, c->ntiles, c->K, bin, s));
+    }
+    Timed t(c, s, &c->ev_stage);
+    unsigned long long *lcur = c->dlam + 3 * c->nb * c->lampar;
+    unsigned long long *lnext = c->dlam + 3 * c->nb * (1 - c->lampar);
+    HIPCHK(c->L->stage(c->p.scheme[0], ns, 1, c->db, c->dtiles, c->ntiles, c->K, stage, bin, bun, bout, c->ddt, lcur,
+                       lnext, c->dcount, s));
+    c->cur = bout;
+    c->next_stage = (stage + 1) % 3;
+    if (stage == 2) c->lampar ^= 1;
+    return NUC3D_OK;
+}
+
+int nuc3d_steps(nuc3d_case *c, int nsteps, double cfl, void *stream)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    if (nsteps < 0) return fail(NUC3D_ERR_BAD_ARG, "negative step count");
+    for (int n = 0; n < nsteps; n++)
+        for (int st = 0; st < 3; st++) {
+            int rc = nuc3d_stage(c, st, cfl, stream);
+            if (rc) return rc;
+        }
+    return NUC3D_OK;
+}
+
+int nuc3d_sync(nuc3d_case *c)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    HIPCHK(hipDeviceSynchronize());
+    return NUC3D_OK;
+}
+
+int nuc3d_get_dt(nuc3d_case *c, double *dt)
+{
+    if (!c || !dt) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(dt, c->ddt, sizeof(double), hipMemcpyDeviceToHost));
+    return NUC3D_OK;
+}
+
+int nuc3d_get_max_eigen(nuc3d_case *c, int b, double *eig3)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!eig3) return fail(NUC3D_ERR_BAD_ARG, "null output");
+    HIPCHK(hipDeviceSynchronize());
+    // after a step the maxima of the next stage-0 state sit in lampar;
+    // after a lone RHS evaluation in the middle of a step, in the other half.
+    const int half = c->next_stage == 0 ? c->lampar : 1 - c->lampar;
+    unsigned long long v[3];
+    HIPCHK(hipMemcpy(v, c->dlam + 3 * c->nb * half + 3 * b, sizeof v, hipMemcpyDeviceToHost));
+    for (int d = 0; d < 3; d++) std::memcpy(&eig3[d], &v[d], sizeof(double));
+    return NUC3D_OK;
+}
+
+int nuc3d_get_fixups(nuc3d_case *c, long long *count)
+{
+    if (!c || !count) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    HIPCHK(hipDeviceSynchronize());
+    unsigned long long v = 0;
+    HIPCHK(hipMemcpy(&v, c->dcount, sizeof v, hipMemcpyDeviceToHost));
+    *count = (long long)v;
+    return NUC3D_OK;
+}
+
+int nuc3d_timing(nuc3d_case *c, int enable)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    c->timing = enable != 0;
+    return NUC3D_OK;
+}
+
+int nuc3d_timing_read(nuc3d_case *c, double *stage_ms, int *stage_launches, double *visc_ms, int *visc_launches)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    HIPCHK(hipDeviceSynchronize());
+    double acc[2] = {0.0, 0.0};
+    int cnt[2] = {0, 0};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> *lists[2] = {&c->ev_stage, &c->ev_visc};
+    for (int l = 0; l < 2; l++) {
+        for (auto &pr : *lists[l]) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+            acc[l] += ms;
+            cnt[l]++;
+            c->ev_pool.push_back(pr.first);
+            c->ev_pool.push_back(pr.second);
+        }
+        lists[l]->clear();
+    }
+    if (stage_ms) *stage_ms = acc[0];
+    if (stage_launches) *stage_launches = cnt[0];
+    if (visc_ms) *visc_ms = acc[1];
+    if (visc_launches) *visc_launches = cnt[1];
+    return NUC3D_OK;
+}
+
+}  // extern "C"

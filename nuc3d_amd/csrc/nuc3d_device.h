@@ -1,0 +1,332 @@
+// nuc3d_device.h -- device-side arithmetic of the per-block RHS path (gfx950).
+//
+// Two variants are compiled from this header (see nuc3d_kernels.hip):
+//   NUC3D_EXACT=1  reference operation order, no FMA contraction
+//                  (the TU is built with -ffp-contract=off); pow(x,2) -> x*x.
+//   NUC3D_EXACT=0  FMA contraction and a one-division form of the WENO
+//                  weights; agrees with the reference to ~1e-13 relative.
+//
+// Reference formulas: schemes.hpp:17-44 (coefficients), weno5js.cpp:75-108,
+// weno5z.cpp:74-109, crweno5.cpp:91-218, reconstructionBoundaryScheme.cpp:
+// 52-474, centraldifference6th.cpp:43-45, centraldifference8th.cpp:49-52,
+// boundaryScheme.cpp:100-126, physicsModel.cpp:258-357 (con2prim),
+// :940-979 (local Lax-Friedrichs), :762-938 (AUSM, AUSM+), :1145-1398
+// (point splitters), :199-234 (Sutherland), TVD-RK3.cpp:37-137.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#ifndef NUC3D_EXACT
+#define NUC3D_EXACT 0
+#endif
+
+namespace nuc3d_dev {
+
+// ---- scheme constants (schemes.hpp) ----
+constexpr double A00 = 1.0 / 3.0, A01 = -7.0 / 6.0, A02 = 11.0 / 6.0;
+constexpr double A10 = -1.0 / 6.0, A11 = 5.0 / 6.0, A12 = 1.0 / 3.0;
+constexpr double A20 = 1.0 / 3.0, A21 = 5.0 / 6.0, A22 = -1.0 / 6.0;
+constexpr double C0 = 0.1, C1 = 0.6, C2 = 0.3;          // WENO5 linear weights
+constexpr double R0 = 0.2, R1 = 0.5, R2 = 0.3;          // CRWENO5 linear weights
+constexpr double G0 = 13.0 / 12.0, G1 = 1.0 / 4.0;
+constexpr double EPS = 1.0e-6;
+constexpr double CD6_0 = 3.0 / 4.0, CD6_1 = -3.0 / 20.0, CD6_2 = 1.0 / 60.0;
+constexpr double CD8_0 = 4.0 / 5.0, CD8_1 = -1.0 / 5.0, CD8_2 = 4.0 / 105.0, CD8_3 = -1.0 / 280.0;
+constexpr double CD4_0 = 2.0 / 3.0, CD4_1 = -1.0 / 12.0;
+
+enum { SCH_WENO5JS = 0, SCH_WENO5Z = 1, SCH_CRWENO5 = 2 };
+enum { RIE_LF = 0, RIE_AUSM = 1, RIE_AUSMP = 2 };
+enum { BC_INLET = 0, BC_OUTLET = 1, BC_WALL = 2, BC_SYMM = 3 };
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+
+// ---- smoothness indicators on an oriented stencil f0..f4 ----
+// zq: the +inner beta2 of weno5z.cpp:80 ((f2-2f3+f3)^2 in place of (f2-2f3+f4)^2)
+__device__ __forceinline__ void smooth5(double f0, double f1, double f2, double f3, double f4, bool zq,
+                                        double &b0, double &b1, double &b2)
+{
+    b0 = G0 * sq(f0 - 2.0 * f1 + f2) + G1 * sq(f0 - 4.0 * f1 + 3.0 * f2);
+    b1 = G0 * sq(f1 - 2.0 * f2 + f3) + G1 * sq(f1 - f3);
+    b2 = G0 * sq(zq ? (f2 - 2.0 * f3 + f3) : (f2 - 2.0 * f3 + f4)) + G1 * sq(3.0 * f2 - 4.0 * f3 + f4);
+}
+
+__device__ __forceinline__ void cand5(double f0, double f1, double f2, double f3, double f4, double &q0, double &q1,
+                                      double &q2)
+{
+    q0 = A00 * f0 + A01 * f1 + A02 * f2;
+    q1 = A10 * f1 + A11 * f2 + A12 * f3;
+    q2 = A20 * f2 + A21 * f3 + A22 * f4;
+}
+
+// WENO5-JS value at the face of an oriented 5-point stencil.
+__device__ __forceinline__ double weno5js(double f0, double f1, double f2, double f3, double f4)
+{
+    double b0, b1, b2, q0, q1, q2;
+    smooth5(f0, f1, f2, f3, f4, false, b0, b1, b2);
+    cand5(f0, f1, f2, f3, f4, q0, q1, q2);
+#if NUC3D_EXACT
+    double a0 = C0 / sq(EPS + b0);
+    double a1 = C1 / sq(EPS + b1);
+    double a2 = C2 / sq(EPS + b2);
+    double as = a0 + a1 + a2;
+    double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
+    return o0 * q0 + o1 * q1 + o2 * q2;
+#else
+    // alpha_k = c_k/d_k with d_k=(eps+beta_k)^2; multiplying numerator and
+    // denominator of sum(alpha q)/sum(alpha) by d0 d1 d2 leaves one division.
+    double d0 = sq(EPS + b0), d1 = sq(EPS + b1), d2 = sq(EPS + b2);
+    double w0 = C0 * (d1 * d2), w1 = C1 * (d0 * d2), w2 = C2 * (d0 * d1);
+    return (w0 * q0 + w1 * q1 + w2 * q2) / (w0 + w1 + w2);
+#endif
+}
+
+// WENO5-Z value (weno5z.cpp:96-109).
+__device__ __forceinline__ double weno5z(double f0, double f1, double f2, double f3, double f4, bool zq)
+{
+    double b0, b1, b2, q0, q1, q2;
+    smooth5(f0, f1, f2, f3, f4, zq, b0, b1, b2);
+    cand5(f0, f1, f2, f3, f4, q0, q1, q2);
+    double tau = fabs(b2 - b0);
+#if NUC3D_EXACT
+    double a0 = C0 * (1.0 + sq(tau / (b0 + EPS)));
+    double a1 = C1 * (1.0 + sq(tau / (b1 + EPS)));
+    double a2 = C2 * (1.0 + sq(tau / (b2 + EPS)));
+    double as = a0 + a1 + a2;
+    double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
+    return o0 * q0 + o1 * q1 + o2 * q2;
+#else
+    double d0 = sq(b0 + EPS), d1 = sq(b1 + EPS), d2 = sq(b2 + EPS), t2 = tau * tau;
+    double w0 = C0 * (d0 + t2) * (d1 * d2), w1 = C1 * (d1 + t2) * (d0 * d2), w2 = C2 * (d2 + t2) * (d0 * d1);
+    return (w0 * q0 + w1 * q1 + w2 * q2) / (w0 + w1 + w2);
+#endif
+}
+
+// third-order closure (reconstructionBoundaryScheme.cpp:327-343)
+__device__ __forceinline__ double third_order(double f0, double f1, double f2)
+{
+    double is0 = sq(f0 - f1), is1 = sq(f1 - f2);
+    double q20 = -0.5 * f0 + 1.5 * f1;
+    double q21 = 0.5 * f1 + 0.5 * f2;
+    double aa0 = 1.0 / 3.0 / sq(1.0e-6 + is0);
+    double aa1 = 2.0 / 3.0 / sq(1.0e-6 + is1);
+    double w0 = aa0 / (aa0 + aa1);
+    double w1 = aa1 / (aa0 + aa1);
+    return w0 * q20 + w1 * q21;
+}
+
+// Interior face value for the + (upwind>0) or - stencil.  s[0..5] holds the
+// split flux at cells m-3..m+2 of face m (between cells m-1 and m).
+template <int SCHEME>
+__device__ __forceinline__ double face_interior(const double *s, bool plus, bool zq)
+{
+    if (plus) {
+        if (SCHEME == SCH_WENO5Z) return weno5z(s[0], s[1], s[2], s[3], s[4], zq);
+        return weno5js(s[0], s[1], s[2], s[3], s[4]);
+    }
+    if (SCHEME == SCH_WENO5Z) return weno5z(s[5], s[4], s[3], s[2], s[1], false);
+    return weno5js(s[5], s[4], s[3], s[2], s[1]);
+}
+
+// Exterior (physical face) closure at face m, k faces from the physical
+// face (k = m on the low side, k = n - m on the high side).  s[0..5] holds the
+// split flux at cells m-3..m+2, where the slot of ghost cell -1 (+ flux) or n
+// (- flux) carries the BC ghost flux.  Mapped onto s, the left and right
+// closures of reconstructionBoundaryScheme.cpp:52-312 coincide:
+//   +: k=0 s2 (ghost[-1] / c[n-1]); k=1 (s2+s3)/2; k=2 third(s1,s2,s3); k=3 WENO5(s0..s4)
+//   -: k=0 s3 (c[0] / ghost[n]);    k=1 (s3+s2)/2; k=2 third(s4,s3,s2); k=3 WENO5(s5..s1)
+__device__ __forceinline__ double face_exterior(const double *s, bool plus, int k)
+{
+    if (plus) {
+        switch (k) {
+        case 0: return s[2];
+        case 1: return 0.5 * (s[2] + s[3]);
+        case 2: return third_order(s[1], s[2], s[3]);
+        default: return weno5js(s[0], s[1], s[2], s[3], s[4]);
+        }
+    }
+    switch (k) {
+    case 0: return s[3];
+    case 1: return 0.5 * (s[3] + s[2]);
+    case 2: return third_order(s[4], s[3], s[2]);
+    default: return weno5js(s[5], s[4], s[3], s[2], s[1]);
+    }
+}
+
+// ---- physics ----
+struct PhysParams {
+    double gamma, mach, reynolds, prandtl, sref, twall;
+    int riemann;
+};
+
+// con2prim (physicsModel.cpp:313-353).  Returns p; sets rho,u,v,w,c.
+__device__ __forceinline__ void con2prim(const double *Q, double jac, const PhysParams &P, double &rho, double &u,
+                                         double &v, double &w, double &p, double &c)
+{
+    rho = Q[0] * jac;
+    u = Q[1] / rho * jac;
+    v = Q[2] / rho * jac;
+    w = Q[3] / rho * jac;
+    double E = Q[4] * jac;
+    p = (E - 0.5 * rho * (u * u + v * v + w * w)) * (P.gamma - 1.0);
+    c = sqrt(fabs(P.gamma * p / rho));
+}
+
+// e<0 fix-up of physicsModel.cpp:322-332 applied to a freshly written state.
+// Returns true if Q[4] was replaced.
+__device__ __forceinline__ bool fixup_energy(double *Q, double jac, const PhysParams &P)
+{
+    double rho = Q[0] * jac;
+    double u = Q[1] / rho * jac;
+    double v = Q[2] / rho * jac;
+    double w = Q[3] / rho * jac;
+    double E = Q[4] * jac;
+    double e0 = E - 0.5 * rho * (u * u + v * v + w * w);
+    if (e0 < 0.0) {
+        double g = P.gamma, m = P.mach;
+        E = (pow(rho, g) / ((g * m * m) * (g - 1.0)) + 0.5 * rho * (u * u + v * v + w * w));
+        Q[4] = E / jac;
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ double machL(double m) { return fabs(m) < 1.0 ? 0.25 * sq(m + 1.0) + 0.125 * sq(m * m - 1) : 0.50 * (m + fabs(m)); }
+__device__ __forceinline__ double machR(double m) { return fabs(m) < 1.0 ? -0.25 * sq(m - 1.0) - 0.125 * sq(m * m - 1) : 0.50 * (m - fabs(m)); }
+__device__ __forceinline__ double presL(double m, double p, bool plus_variant)
+{
+    if (fabs(m) < 1.0)
+        return plus_variant ? p * (0.25 * sq(m + 1.0) * (2.0 - m) + 0.1875 * m * sq(m * m - 1.0)) : p * (m + 1.0) / 2.0;
+    return 0.50 * p * (m + fabs(m)) / m;
+}
+__device__ __forceinline__ double presR(double m, double p, bool plus_variant)
+{
+    if (fabs(m) < 1.0)
+        return plus_variant ? p * (0.25 * sq(m - 1.0) * (2.0 + m) - 0.1875 * m * sq(m * m - 1.0)) : -p * (m - 1.0) / 2.0;
+    return 0.5 * p * (m - fabs(m)) / m;
+}
+
+// Split flux of one cell in one direction (RiemannSolver loop body,
+// physicsModel.cpp:557-597 with RiemannLF/AUSM/AUSMp).  Q = conserved/J.
+// Returns |U| + c|grad xi| (the local eigenvalue of :597).
+__device__ __forceinline__ double split_cell(const double *Q, double jac, double mx, double my, double mz, double u,
+                                             double v, double w, double p, double c, int riemann, double *fp,
+                                             double *fn)
+{
+    double theta = sqrt(mx * mx + my * my + mz * mz);
+    double U0 = (mx * u + my * v + mz * w);
+    double al = c * theta;
+    double lam = fabs(U0) + al;
+    if (riemann == RIE_LF) {
+        double f0 = U0 * Q[0];
+#if NUC3D_EXACT
+        double f1 = U0 * Q[1] + mx * p / jac;
+        double f2 = U0 * Q[2] + my * p / jac;
+        double f3 = U0 * Q[3] + mz * p / jac;
+        double f4 = U0 * (Q[4] + p / jac);
+#else
+        double pj = p / jac;
+        double f1 = U0 * Q[1] + mx * pj;
+        double f2 = U0 * Q[2] + my * pj;
+        double f3 = U0 * Q[3] + mz * pj;
+        double f4 = U0 * (Q[4] + pj);
+#endif
+        fp[0] = 0.5 * (f0 + lam * Q[0]);
+        fp[1] = 0.5 * (f1 + lam * Q[1]);
+        fp[2] = 0.5 * (f2 + lam * Q[2]);
+        fp[3] = 0.5 * (f3 + lam * Q[3]);
+        fp[4] = 0.5 * (f4 + lam * Q[4]);
+        fn[0] = 0.5 * (f0 - lam * Q[0]);
+        fn[1] = 0.5 * (f1 - lam * Q[1]);
+        fn[2] = 0.5 * (f2 - lam * Q[2]);
+        fn[3] = 0.5 * (f3 - lam * Q[3]);
+        fn[4] = 0.5 * (f4 - lam * Q[4]);
+        return lam;
+    }
+    bool pv = riemann == RIE_AUSMP;
+    double mach = U0 / al;
+    double mp = machL(mach), mn = machR(mach);
+    double pp = presL(mach, p, pv), pn = presR(mach, p, pv);
+    fp[0] = mp * al * Q[0];
+    fp[1] = mp * al * Q[1] + mx * pp / jac;
+    fp[2] = mp * al * Q[2] + my * pp / jac;
+    fp[3] = mp * al * Q[3] + mz * pp / jac;
+    fp[4] = mp * al * Q[4] + mp * al * p / jac;
+    fn[0] = mn * al * Q[0];
+    fn[1] = mn * al * Q[1] + mx * pn / jac;
+    fn[2] = mn * al * Q[2] + my * pn / jac;
+    fn[3] = mn * al * Q[3] + mz * pn / jac;
+    fn[4] = mn * al * Q[4] + mn * al * p / jac;
+    return lam;
+}
+
+// Point splitter of a ghost primitive state (physicsModel.cpp:1145-1398).
+__device__ __forceinline__ void split_point(const double *prim, double jac, double mx, double my, double mz,
+                                            const PhysParams &P, double *fl, double *fr)
+{
+    double rho = prim[0], u = prim[1], v = prim[2], w = prim[3], p = prim[4];
+    double g = P.gamma;
+    double _rho = rho, _rhou = rho * u, _rhov = rho * v, _rhow = rho * w;
+    double _E = p / (g - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
+    double _p = (_E - 0.5 * rho * (u * u + v * v + w * w)) * (g - 1.0);
+    double _alpha = sqrt(fabs(g * _p / rho));
+    double theta = sqrt(mx * mx + my * my + mz * mz);
+    double alpha = _alpha * theta;
+    double U0 = (mx * u + my * v + mz * w);
+    if (P.riemann == RIE_LF) {
+        double lam = fabs(U0) + alpha;
+        double f0 = U0 * _rho / jac;
+        double f1 = U0 * _rhou / jac + mx * p / jac;
+        double f2 = U0 * _rhov / jac + my * p / jac;
+        double f3 = U0 * _rhow / jac + mz * p / jac;
+        double f4 = U0 * (_E / jac + p / jac);
+        fl[0] = 0.5 * (f0 + lam * _rho / jac);
+        fl[1] = 0.5 * (f1 + lam * _rhou / jac);
+        fl[2] = 0.5 * (f2 + lam * _rhov / jac);
+        fl[3] = 0.5 * (f3 + lam * _rhow / jac);
+        fl[4] = 0.5 * (f4 + lam * _E / jac);
+        fr[0] = 0.5 * (f0 - lam * _rho / jac);
+        fr[1] = 0.5 * (f1 - lam * _rhou / jac);
+        fr[2] = 0.5 * (f2 - lam * _rhov / jac);
+        fr[3] = 0.5 * (f3 - lam * _rhow / jac);
+        fr[4] = 0.5 * (f4 - lam * _E / jac);
+        return;
+    }
+    bool pv = P.riemann == RIE_AUSMP;
+    double mach = U0 / alpha;
+    double mp = machL(mach), mn = machR(mach);
+    double pp = presL(mach, p, pv), pn = presR(mach, p, pv);
+    fl[0] = mp * alpha * _rho / jac;
+    fl[1] = mp * alpha * _rhou / jac + mx * pp / jac;
+    fl[2] = mp * alpha * _rhov / jac + my * pp / jac;
+    fl[3] = mp * alpha * _rhow / jac + mz * pp / jac;
+    fl[4] = mp * alpha * _E / jac + mp * alpha * p / jac;
+    fr[0] = mn * alpha * _rho / jac;
+    fr[1] = mn * alpha * _rhou / jac + mx * pn / jac;
+    fr[2] = mn * alpha * _rhov / jac + my * pn / jac;
+    fr[3] = mn * alpha * _rhow / jac + mz * pn / jac;
+    fr[4] = mn * alpha * _E / jac + mn * alpha * p / jac;
+}
+
+// Central differences (centraldifference6th.cpp:43-45, centraldifference8th.cpp:49-52);
+// s points at the centre cell, st = element stride along the line.
+__device__ __forceinline__ double cd6(const double *s, long st)
+{
+    return CD6_2 * (s[3 * st] - s[-3 * st]) + CD6_1 * (s[2 * st] - s[-2 * st]) + CD6_0 * (s[st] - s[-st]);
+}
+__device__ __forceinline__ double cd8(const double *s, long st)
+{
+    return CD8_3 * (s[4 * st] - s[-4 * st]) + CD8_2 * (s[3 * st] - s[-3 * st]) + CD8_1 * (s[2 * st] - s[-2 * st]) +
+           CD8_0 * (s[st] - s[-st]);
+}
+// Exterior derivative closure (boundaryScheme.cpp:100-126); k = distance from
+// the physical face, left = the face is on the low side.
+__device__ __forceinline__ double diff_exterior(const double *s, long st, int k, bool left)
+{
+    switch (k) {
+    case 0: return left ? s[st] - s[0] : s[0] - s[-st];
+    case 1: return 0.5 * (s[st] - s[-st]);
+    case 2: return CD4_1 * (s[2 * st] - s[-2 * st]) + CD4_0 * (s[st] - s[-st]);
+    default: return CD6_2 * (s[3 * st] - s[-3 * st]) + CD6_1 * (s[2 * st] - s[-2 * st]) + CD4_0 * (s[st] - s[-st]);
+    }
+}
+
+}  // namespace nuc3d_dev

@@ -1,0 +1,42 @@
+// nuc3d_types.h -- data layout shared by the HIP kernels and the host side.
+//
+// Every block lives on the device as SoA FP64 fields in a padded box of
+// (nx+2G)(ny+2G)(nz+2G) cells, i-fastest.  Field pointers point at cell
+// (0,0,0), so cell (i,j,k) with i in [-G, nx+G) is ptr[i + sy*j + sz*k].
+// Only the six face slabs of the ghost frame are used (the stencils are
+// one-dimensional); corner and edge ghosts are never read.
+#pragma once
+#include <stdint.h>
+
+#define NUC3D_MAX_BLOCKS 256
+#define NUC3D_TILE 16   // tile edge in i and j (one thread per column)
+#define NUC3D_HALO 3    // stencil half-width of WENO5 / cd6
+
+struct nuc3d_block_dev {
+    int n[3];            // interior cells nx, ny, nz
+    int G;               // ghost layers in storage (the reference's bufferWidth)
+    long long sy, sz;    // strides of j and k (stride of i is 1)
+    double *q[3][5];     // three state buffers (conserved / J)
+    double *rhs[5];      // RHS output (single RHS evaluations)
+    double *jac;         // J = 1/det(dx/dxi)
+    double *met[9];      // xi_x, xi_y, xi_z, eta_x .. zeta_z
+    double *dxyz[9];     // x_xi, x_eta, x_zeta, y_xi .. z_zeta (symmetric BC only)
+    double *prim[4];     // u, v, w, T (Navier-Stokes)
+    double *fv[12];      // viscous flux: fv[4*d + c - 1], d = xi/eta/zeta, c = 1..4
+    int ftype[6];        // -1: physical face, else neighbour block index
+    int fid[6];          // BC id (0 inlet, 1 outlet, 2 wall, 3 symmetric) or neighbour face
+    double fval[6][5];   // inlet primitive state per face
+};
+
+struct nuc3d_tile {
+    int b, i0, j0, k0, kn, pad0, pad1, pad2;
+};
+
+struct nuc3d_kparams {
+    double gamma, mach, reynolds, prandtl, sref, twall;
+    double cfl;
+    int riemann;
+    int width[3];
+    int zquirk;
+    int nblocks;
+};

@@ -1,0 +1,214 @@
+"""Python view of the C ABI (include/nuc3d_hip.h).
+
+`Case` holds every block of a case on the current GPU and mirrors the calls
+the reference's driver makes per time step (block::solve, block.cpp:569-594):
+`rhs()` is EulerData3D::solve / NaiverStokesData3d::solve, `stage(s, cfl)`
+adds PDEData3d::solve for RK stage s, `steps(n, cfl)` runs whole TVD-RK3
+steps.  Errors of the library raise `Nuc3dError` with its message.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import LIB_PATH
+
+WENO5JS, WENO5Z, CRWENO5 = 0, 1, 2
+CD6, CD8 = 0, 1
+LF, AUSM, AUSMP = 0, 1, 2
+EULER3D, NAVIERSTOKES3D = 0, 1
+
+
+class Nuc3dError(RuntimeError):
+    pass
+
+
+class _CParams(ctypes.Structure):
+    _fields_ = [
+        ("gamma", ctypes.c_double), ("mach", ctypes.c_double), ("reynolds", ctypes.c_double),
+        ("prandtl", ctypes.c_double), ("t_ref", ctypes.c_double), ("t_inf", ctypes.c_double),
+        ("t_wall", ctypes.c_double), ("model", ctypes.c_int), ("riemann", ctypes.c_int),
+        ("scheme", ctypes.c_int * 3), ("diff", ctypes.c_int * 3), ("width", ctypes.c_int * 3),
+        ("zquirk", ctypes.c_int), ("exact", ctypes.c_int),
+    ]
+
+
+@dataclass
+class Params:
+    """inp/PhysModel.in + inp/Method.in of the reference (physicsModel.cpp:41-69,
+    fieldOperator.cpp:20-31), plus `exact` (reference operation order)."""
+    gamma: float = 1.4
+    mach: float = 0.1
+    reynolds: float = 1600.0
+    prandtl: float = 0.72
+    t_ref: float = 110.4
+    t_inf: float = 288.0
+    t_wall: float = 1.0
+    model: int = NAVIERSTOKES3D
+    riemann: int = LF
+    scheme: Sequence[int] = (WENO5JS, WENO5JS, WENO5JS)
+    diff: Sequence[int] = (CD6, CD6, CD6)
+    width: Sequence[int] = (3, 3, 3)
+    zquirk: int = 1
+    exact: int = 0
+
+    def to_c(self) -> _CParams:
+        p = _CParams()
+        for k in ("gamma", "mach", "reynolds", "prandtl", "t_ref", "t_inf", "t_wall", "model", "riemann", "zquirk",
+                  "exact"):
+            setattr(p, k, getattr(self, k))
+        for i in range(3):
+            p.scheme[i] = int(self.scheme[i])
+            p.diff[i] = int(self.diff[i])
+            p.width[i] = int(self.width[i])
+        return p
+
+
+_LIB = None
+
+
+def lib():
+    """Load libnuc3d_hip.so; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise Nuc3dError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "nuc3d_case_create": [ctypes.POINTER(_CParams), ctypes.c_int, ip, ctypes.POINTER(vp)],
+        "nuc3d_case_destroy": [vp],
+        "nuc3d_set_faces": [vp, ctypes.c_int, ip, ip, dp],
+        "nuc3d_upload_metrics": [vp, ctypes.c_int, dp, dp, dp],
+        "nuc3d_upload_q": [vp, ctypes.c_int, dp],
+        "nuc3d_download_q": [vp, ctypes.c_int, dp],
+        "nuc3d_download_rhs": [vp, ctypes.c_int, dp],
+        "nuc3d_rhs": [vp, vp],
+        "nuc3d_stage": [vp, ctypes.c_int, ctypes.c_double, vp],
+        "nuc3d_steps": [vp, ctypes.c_int, ctypes.c_double, vp],
+        "nuc3d_sync": [vp],
+        "nuc3d_get_dt": [vp, dp],
+        "nuc3d_get_max_eigen": [vp, ctypes.c_int, dp],
+        "nuc3d_get_fixups": [vp, ctypes.POINTER(ctypes.c_longlong)],
+        "nuc3d_timing": [vp, ctypes.c_int],
+        "nuc3d_timing_read": [vp, dp, ctypes.POINTER(ctypes.c_int), dp, ctypes.POINTER(ctypes.c_int)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.nuc3d_last_error.argtypes = []
+    L.nuc3d_last_error.restype = ctypes.c_char_p
+    _LIB = L
+    return L
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().nuc3d_last_error().decode(errors="replace")
+        raise Nuc3dError(f"{what} failed ({rc}): {msg}")
+
+
+class Case:
+    """All blocks of a case resident on the current GPU."""
+
+    def __init__(self, params: Params, blocks, stream: Optional[int] = None):
+        L = lib()
+        self.params = params
+        self.blocks = list(blocks)
+        self.stream = ctypes.c_void_p(stream) if stream else None
+        self._cp = params.to_c()
+        dims = (ctypes.c_int * (3 * len(self.blocks)))(*[int(d) for b in self.blocks for d in b.dims])
+        h = ctypes.c_void_p()
+        _check(L.nuc3d_case_create(ctypes.byref(self._cp), len(self.blocks), dims, ctypes.byref(h)), "case_create")
+        self.h = h
+        for bi, b in enumerate(self.blocks):
+            t = (ctypes.c_int * 6)(*[int(f[0]) for f in b.faces])
+            i = (ctypes.c_int * 6)(*[int(f[1]) for f in b.faces])
+            vals = np.zeros(30)
+            for fi, f in enumerate(b.faces):
+                if len(f) > 2 and f[2] is not None:
+                    vals[5 * fi:5 * fi + len(f[2])] = f[2]
+            _check(L.nuc3d_set_faces(self.h, bi, t, i, _dp(vals)), "set_faces")
+            jac = np.ascontiguousarray(b.jac, dtype=np.float64)
+            met = np.ascontiguousarray(b.met, dtype=np.float64)
+            dxyz = np.ascontiguousarray(b.dxyz, dtype=np.float64)
+            _check(L.nuc3d_upload_metrics(self.h, bi, _dp(jac), _dp(met), _dp(dxyz)), "upload_metrics")
+            self.set_q(bi, b.q)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nuc3d_case_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shape(self, bi, nc):
+        nx, ny, nz = self.blocks[bi].dims
+        return (nc, nz, ny, nx)
+
+    def set_q(self, bi, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        _check(lib().nuc3d_upload_q(self.h, bi, _dp(q)), "upload_q")
+
+    def get_q(self, bi=0):
+        out = np.empty(self._shape(bi, 5))
+        _check(lib().nuc3d_download_q(self.h, bi, _dp(out)), "download_q")
+        return out
+
+    def get_rhs(self, bi=0):
+        out = np.empty(self._shape(bi, 5))
+        _check(lib().nuc3d_download_rhs(self.h, bi, _dp(out)), "download_rhs")
+        return out
+
+    def rhs(self):
+        _check(lib().nuc3d_rhs(self.h, self.stream), "rhs")
+
+    def stage(self, s, cfl):
+        _check(lib().nuc3d_stage(self.h, s, cfl, self.stream), "stage")
+
+    def steps(self, n, cfl):
+        _check(lib().nuc3d_steps(self.h, n, cfl, self.stream), "steps")
+
+    def sync(self):
+        _check(lib().nuc3d_sync(self.h), "sync")
+
+    def dt(self):
+        v = ctypes.c_double()
+        _check(lib().nuc3d_get_dt(self.h, ctypes.byref(v)), "get_dt")
+        return v.value
+
+    def max_eigen(self, bi=0):
+        out = np.empty(3)
+        _check(lib().nuc3d_get_max_eigen(self.h, bi, _dp(out)), "get_max_eigen")
+        return out
+
+    def fixups(self):
+        v = ctypes.c_longlong()
+        _check(lib().nuc3d_get_fixups(self.h, ctypes.byref(v)), "get_fixups")
+        return v.value
+
+    def timing(self, enable=True):
+        _check(lib().nuc3d_timing(self.h, 1 if enable else 0), "timing")
+
+    def timing_read(self):
+        """(stage_ms_total, stage_launches, visc_ms_total, visc_launches) since the last read."""
+        sm, vm = ctypes.c_double(), ctypes.c_double()
+        sl, vl = ctypes.c_int(), ctypes.c_int()
+        _check(lib().nuc3d_timing_read(self.h, ctypes.byref(sm), ctypes.byref(sl), ctypes.byref(vm),
+                                       ctypes.byref(vl)), "timing_read")
+        return sm.value, sl.value, vm.value, vl.value

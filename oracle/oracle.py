@@ -328,6 +328,7 @@ def read_ref_output(path: str, rank: int):
     w = sh(raw[o:o + 5 * N], 5); o += 5 * N
     w0 = sh(raw[o:o + 3 * N], 3); o += 3 * N
     dt_local, dt_global, e0, e1, e2, ns = raw[o:o + 6]; o += 6
-    dts = raw[o:o + int(ns)]
+    dts = raw[o:o + int(ns)]; o += int(ns)
+    walls = raw[o:o + int(ns)]
     return dict(rhs=rhs, q=q, prim=np.concatenate([w, w0]), dt_local=dt_local, dt_global=dt_global,
-                max_eigen=np.array([e0, e1, e2]), dts=dts)
+                max_eigen=np.array([e0, e1, e2]), dts=dts, walls=walls)

@@ -19,7 +19,8 @@
 //                 (met: xi_x..zeta_z; dxyz: x_xi,x_eta,x_zeta,y_xi..z_zeta;
 //                  Q = conserved/J, i-fastest)
 // out_<rank>.bin: float64 RHS[5N], Q[5N], W[5N] (rho,u,v,w,p), W0[3N] (T,e,c),
-//                 dt_local, dt_global, maxEigen[3], nsteps, dt[nsteps]
+//                 dt_local, dt_global, maxEigen[3], nsteps, dt[nsteps],
+//                 wall[nsteps] (MPI_Wtime around each step, max over ranks)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -106,16 +107,21 @@ int main(int argc, char **argv)
         }
         for (int c = 0; c < 5; c++) load(pde.getQ()[c], &buf[(19 + c) * N]);
 
-        std::vector<double> dts;
+        std::vector<double> dts, walls;
         if (mode == "rhs") {
             flux->solve(pde, op, bufs, phys, mpi, bc);
         } else {
             for (int s = 0; s < nsteps; s++) {
+                MPI_Barrier(MPI_COMM_WORLD);
+                double t0 = MPI_Wtime();
                 for (int stage = 0; stage < op.getSteps(); stage++) {
                     flux->solve(pde, op, bufs, phys, mpi, bc);
                     pde.solve(op, cfl, stage);
                 }
+                double w = MPI_Wtime() - t0, wmax = 0.0;
+                MPI_Allreduce(&w, &wmax, 1, MPI_DOUBLE, MPI_MAX, MPI_COMM_WORLD);
                 dts.push_back(pde.getDt());
+                walls.push_back(wmax);
             }
         }
 
@@ -129,6 +135,7 @@ int main(int argc, char **argv)
                           flux->Flux_zeta.maxEigen, (double)dts.size()};
         std::fwrite(tail, sizeof(double), 6, fp);
         if (!dts.empty()) std::fwrite(dts.data(), sizeof(double), dts.size(), fp);
+        if (!walls.empty()) std::fwrite(walls.data(), sizeof(double), walls.size(), fp);
         std::fclose(fp);
     }
     return 0;

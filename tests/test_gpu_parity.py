@@ -1,0 +1,135 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same inputs.  The oracle agrees bit-for-bit with the reference compiled from
+its own sources (tests/test_oracle_ref.py, tests/golden/).
+
+Bar (north star): <= 1e-12 relative max-norm per conserved variable,
+   err_c = max|gpu_c - ref_c| / max|ref_c|.
+"""
+import numpy as np
+import pytest
+
+from nuc3d_amd import cases, solver
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def relerr(a, b):
+    return max(np.abs(a[c] - b[c]).max() / max(np.abs(b[c]).max(), 1e-300) for c in range(a.shape[0]))
+
+
+def both_params(**kw):
+    return solver.Params(**kw), O.Params(**{k: v for k, v in kw.items() if k != "exact"})
+
+
+def run_rhs(sp, op, blocks):
+    g = solver.Case(sp, blocks)
+    g.rhs()
+    o = O.OracleCase(op, blocks)
+    o.rhs()
+    return g, o
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+@pytest.mark.parametrize("model", [solver.EULER3D, solver.NAVIERSTOKES3D])
+@pytest.mark.parametrize("scheme", [solver.WENO5JS, solver.WENO5Z])
+def test_rhs_tgv(exact, model, scheme):
+    print(f"case: rhs tgv 20x18x22 model={model} scheme={scheme} exact={exact}", flush=True)
+    blocks = cases.taylor_green(dims=(20, 18, 22), noise=1e-3, seed=3)
+    sp, op = both_params(model=model, scheme=(scheme,) * 3, exact=exact)
+    g, o = run_rhs(sp, op, blocks)
+    err = relerr(g.get_rhs(0), o.get_rhs(0))
+    print("  rel err %.3e" % err, flush=True)
+    assert err <= TOL
+    np.testing.assert_allclose(g.max_eigen(0), o.max_eigen(0), rtol=1e-14)
+    g.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+def test_steps_tgv_ns(exact):
+    print(f"case: 3 steps NS weno5js tgv 16^3 exact={exact}", flush=True)
+    blocks = cases.taylor_green(16, noise=1e-3, seed=4)
+    sp, op = both_params(exact=exact)
+    g = solver.Case(sp, blocks)
+    o = O.OracleCase(op, blocks)
+    g.steps(3, 0.5)
+    dts = o.steps(3, 0.5)
+    err = relerr(g.get_q(0), o.get_q(0))
+    print("  rel err %.3e dt %.17g %.17g" % (err, g.dt(), dts[-1]), flush=True)
+    assert err <= TOL
+    assert abs(g.dt() - dts[-1]) <= 1e-14 * dts[-1]
+    g.close()
+
+
+def test_multiblock_weno5z():
+    print("case: 3 blocks split in x, NS weno5z, 2 steps", flush=True)
+    spec = cases.taylor_green(dims=(36, 12, 14), noise=1e-3, seed=5)[0]
+    blocks = cases.split_block_x(spec, 3)
+    sp, op = both_params(scheme=(solver.WENO5Z,) * 3)
+    g = solver.Case(sp, blocks)
+    o = O.OracleCase(op, blocks)
+    g.steps(2, 0.5)
+    o.steps(2, 0.5)
+    for b in range(3):
+        err = relerr(g.get_q(b), o.get_q(b))
+        print("  block %d rel err %.3e" % (b, err), flush=True)
+        assert err <= TOL
+    g.close()
+
+
+@pytest.mark.parametrize("scheme", [solver.WENO5JS, solver.WENO5Z])
+@pytest.mark.parametrize("riemann", [solver.LF, solver.AUSM, solver.AUSMP])
+def test_shu_osher_inflow_outflow(scheme, riemann):
+    print(f"case: shu-osher 48x10x8 euler scheme={scheme} riemann={riemann}", flush=True)
+    blocks = cases.shu_osher(48, 10, 8, seed=2)
+    sp, op = both_params(model=solver.EULER3D, mach=1.0, scheme=(scheme,) * 3, riemann=riemann)
+    g = solver.Case(sp, blocks)
+    o = O.OracleCase(op, blocks)
+    g.steps(3, 0.4)
+    o.steps(3, 0.4)
+    err = relerr(g.get_q(0), o.get_q(0))
+    print("  rel err %.3e" % err, flush=True)
+    assert err <= TOL
+    g.close()
+
+
+@pytest.mark.parametrize("faces", [
+    [(-1, 2, None), (-1, 1, None), (-1, 2, None), (-1, 3, None), (0, 5, None), (0, 4, None)],
+    [(-1, 3, None), (-1, 3, None), (-1, 0, [1.0, 0.1, 0.0, 0.0, 71.0]), (-1, 1, None), (-1, 2, None), (-1, 3, None)],
+])
+def test_physical_faces_ns(faces):
+    print(f"case: NS walls/symmetry/inlet faces={faces}", flush=True)
+    b = cases.taylor_green(dims=(16, 12, 10), noise=1e-3, seed=6)[0]
+    b.faces = faces
+    sp, op = both_params(t_wall=1.2)
+    g, o = run_rhs(sp, op, [b])
+    err = relerr(g.get_rhs(0), o.get_rhs(0))
+    print("  rhs rel err %.3e" % err, flush=True)
+    assert err <= TOL
+    g.close()
+    g = solver.Case(sp, [b])
+    o = O.OracleCase(op, [b])
+    g.steps(2, 0.5)
+    o.steps(2, 0.5)
+    err = relerr(g.get_q(0), o.get_q(0))
+    print("  steps rel err %.3e" % err, flush=True)
+    assert err <= TOL
+    g.close()
+
+
+def test_conservation_periodic_64():
+    """Periodic block: the cell sum of every RHS component telescopes to zero,
+    so mass, momentum and energy are conserved to round-off."""
+    print("case: conservation NS tgv 64^3", flush=True)
+    blocks = cases.taylor_green(64, noise=1e-3, seed=7)
+    g = solver.Case(solver.Params(), blocks)
+    q0 = g.get_q(0)
+    g.steps(2, 0.5)
+    q1 = g.get_q(0)
+    for c in range(5):
+        s0, s1 = q0[c].sum(), q1[c].sum()
+        scale = np.abs(q0[c]).sum()
+        print("  comp %d drift %.3e" % (c, abs(s1 - s0) / scale), flush=True)
+        assert abs(s1 - s0) <= 1e-13 * scale
+    g.close()
