@@ -27,9 +27,29 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s per RK3 step FP64: plain WENO5 vs learned hybrid at equal L2 error"
 HBM_PEAK_GBS = 8000.0
-# algorithmic HBM bytes per cell of the fused stage kernel: stage 0 reads Q
-# (40) and metrics (80) and writes Q1 (40); stages 1 and 2 also read Q^n (40)
-ALG_BYTES_PER_STAGE = (160.0, 200.0, 200.0)
+# Algorithmic HBM bytes per cell-update of one fused TVD-RK3 step: stage 0
+# reads Q (40) and J + nine metric terms (80) and writes Q1 (40); stages 1
+# and 2 also read Q^n (40): 160 + 200 + 200 = 560.
+ALG_BYTES_PER_STEP = 560.0
+
+
+def kernel_alg_bytes(cls, ns):
+    """Compulsory bytes per cell of one launch of a kernel class, averaged over
+    the three stages: every input field read once and every output written once.
+      x sweep:  Q 40 + J 8 + xi metrics 24 + RHS write 40                 = 112
+      y sweep:  Q 40 + J 8 + eta metrics 24 + RHS read/write 80            = 152
+      z sweep:  Q 40 + J 8 + zeta metrics 24 + RHS read 40 + Q_out 40
+                + Q^n 40 (stages 1, 2) + xi,eta metrics 48 (stage 2 dt)
+                + viscous fluxes 96 (Navier-Stokes)
+      viscous:  Q 40 + J 8 + metrics 72 + viscous fluxes 96 (write)        = 216
+    """
+    if cls == "x_sweep":
+        return 112.0
+    if cls == "y_sweep":
+        return 152.0
+    if cls == "z_sweep_update":
+        return 152.0 + (0 + 40 + 40) / 3.0 + 48.0 / 3.0 + (96.0 if ns else 0.0)
+    return 216.0
 
 
 def log(msg):
@@ -129,15 +149,21 @@ def main():
     case.steps(args.steps, 0.5)
     case.sync()
     t1 = time.perf_counter()
-    stage_ms, stage_n, visc_ms, visc_n = case.timing_read()
+    tm = case.timing_read()
     case.timing(False)
     elapsed = t1 - t0
     value = cells * args.steps / elapsed
-    bytes_total = cells * args.steps * sum(ALG_BYTES_PER_STAGE)
-    achieved = bytes_total / (stage_ms * 1e-3) / 1e9 if stage_ms > 0 else 0.0
+    ns = params.model == solver.NAVIERSTOKES3D
+    for k, (ms, cnt) in tm.items():
+        if cnt:
+            log("bench: %-15s %8.3f ms avg over %d launches" % (k, ms / cnt, cnt))
+    dom = max(tm, key=lambda k: tm[k][0])
+    dom_ms, dom_n = tm[dom]
+    avg_ms = dom_ms / max(dom_n, 1)
+    achieved = kernel_alg_bytes(dom, ns) * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    step_gbs = ALG_BYTES_PER_STEP * value / 1e9
     traffic = read_traffic()
-    log("bench: %.3f ms/step, stage kernel %.3f ms avg over %d launches, visc %.3f ms avg over %d"
-        % (1e3 * elapsed / args.steps, stage_ms / max(stage_n, 1), stage_n, visc_ms / max(visc_n, 1), visc_n))
+    log("bench: %.3f ms/step; step-level algorithmic HBM rate %.1f GB/s" % (1e3 * elapsed / args.steps, step_gbs))
     case.close()
 
     cpu = None
@@ -174,8 +200,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_stage (fused RHS + RK stage)",
-            "avg_launch_ms": stage_ms / max(stage_n, 1),
+            "kernel": "k_sweep " + dom,
+            "avg_launch_ms": avg_ms,
+            "alg_bytes_per_cell": kernel_alg_bytes(dom, ns),
+            "step_level_gbs": step_gbs,
+            "kernel_ms_per_step": {k: v[0] / args.steps for k, v in tm.items()},
         },
         "cpu_baseline": cpu,
     }

@@ -110,12 +110,12 @@ int nuc3d_get_fixups(nuc3d_case *c, long long *count);
 
 /* Kernel timing (no counterpart in the reference, which times whole steps
  * with MPI_Wtime in block::solve, block.cpp:576-584).  While enabled, every
- * launch of the fused stage kernel and of the viscous-flux kernels is
- * bracketed by HIP events on the launch stream; nuc3d_timing_read returns
- * the summed milliseconds and launch counts since the last read and resets
- * them. */
+ * kernel launch is bracketed by HIP events on its stream, in four classes:
+ * 0 x sweep, 1 y sweep, 2 z sweep with the RK update, 3 viscous fluxes.
+ * nuc3d_timing_read returns the summed milliseconds and launch counts per
+ * class since the last read (arrays of 4) and resets them. */
 int nuc3d_timing(nuc3d_case *c, int enable);
-int nuc3d_timing_read(nuc3d_case *c, double *stage_ms, int *stage_launches, double *visc_ms, int *visc_launches);
+int nuc3d_timing_read(nuc3d_case *c, double *ms4, int *launches4);
 
 /* Description of the calling thread's last error. */
 const char *nuc3d_last_error(void);

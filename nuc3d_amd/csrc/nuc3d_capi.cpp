@@ -16,7 +16,7 @@
 #include "nuc3d_types.h"
 
 #define NUC3D_LAUNCHERS(SFX)                                                                                         \
-    extern "C" hipError_t nuc3d_launch_stage_##SFX(int, int, int, const nuc3d_block_dev *, const nuc3d_tile *, int,  \
+    extern "C" hipError_t nuc3d_launch_sweep_##SFX(int, int, int, int, const nuc3d_block_dev *, const int4 *, int,   \
                                                    nuc3d_kparams, int, int, int, int, double *,                      \
                                                    const unsigned long long *, unsigned long long *,                 \
                                                    unsigned long long *, hipStream_t);                               \
@@ -53,16 +53,16 @@ int hipfail(hipError_t e, const char *what)
     } while (0)
 
 struct Launchers {
-    decltype(&nuc3d_launch_stage_fast) stage;
+    decltype(&nuc3d_launch_sweep_fast) sweep;
     decltype(&nuc3d_launch_visc_fast) visc;
     decltype(&nuc3d_launch_eigen_fast) eigen;
     decltype(&nuc3d_launch_fixup_fast) fixup;
     decltype(&nuc3d_launch_ghost_fill_fast) ghost;
 };
 
-const Launchers kFast = {nuc3d_launch_stage_fast, nuc3d_launch_visc_fast, nuc3d_launch_eigen_fast,
+const Launchers kFast = {nuc3d_launch_sweep_fast, nuc3d_launch_visc_fast, nuc3d_launch_eigen_fast,
                          nuc3d_launch_fixup_fast, nuc3d_launch_ghost_fill_fast};
-const Launchers kExact = {nuc3d_launch_stage_exact, nuc3d_launch_visc_exact, nuc3d_launch_eigen_exact,
+const Launchers kExact = {nuc3d_launch_sweep_exact, nuc3d_launch_visc_exact, nuc3d_launch_eigen_exact,
                           nuc3d_launch_fixup_exact, nuc3d_launch_ghost_fill_exact};
 
 constexpr int KSEG = 32;  // planes per k segment of a tile
@@ -80,8 +80,10 @@ struct nuc3d_case {
     std::vector<void *> chunks;        // one device allocation per block
     std::vector<long long> padded;     // padded cells per block
     nuc3d_block_dev *db = nullptr;     // device descriptors
-    nuc3d_tile *dtiles = nullptr;
+    nuc3d_tile *dtiles = nullptr;      // 16x16 column tiles with k segments (pointwise kernels)
     int ntiles = 0;
+    int4 *dsweep[3] = {nullptr, nullptr, nullptr};  // direction-sweep tiles
+    int nsweep[3] = {0, 0, 0};
     double *ddt = nullptr;             // dt of the current step
     unsigned long long *dlam = nullptr;   // [2][nb][3] eigenvalue maxima (bit patterns)
     unsigned long long *dcount = nullptr; // [4] fix-up counter
@@ -93,7 +95,7 @@ struct nuc3d_case {
     // kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_stage, ev_visc;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];  // x sweep, y sweep, z sweep + update, viscous
 };
 
 namespace {
@@ -177,6 +179,24 @@ int refresh(nuc3d_case *c, hipStream_t s)
     HIPCHK(hipMemsetAsync(c->dlam + 3 * c->nb * c->lampar, 0, sizeof(unsigned long long) * 3 * c->nb, s));
     HIPCHK(c->L->eigen(c->db, c->dtiles, c->ntiles, c->K, c->cur, c->dlam + 3 * c->nb * c->lampar, s));
     c->dirty = false;
+    return NUC3D_OK;
+}
+
+// viscous fluxes (Navier-Stokes), then the three sweeps; the z sweep
+// completes the RHS (mode 0) or applies RK stage `stage` (mode 1)
+int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, const unsigned long long *lcur,
+               unsigned long long *lnext, hipStream_t s)
+{
+    const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
+    if (ns) {
+        Timed t(c, s, &c->ev[3]);
+        HIPCHK(c->L->visc(c->db, c->nb, c->dtiles, c->ntiles, c->K, bin, s));
+    }
+    for (int d = 0; d < 3; d++) {
+        Timed t(c, s, &c->ev[d]);
+        HIPCHK(c->L->sweep(d, c->p.scheme[0], ns, mode, c->db, c->dsweep[d], c->nsweep[d], c->K, stage, bin, bun,
+                           bout, c->ddt, lcur, lnext, c->dcount, s));
+    }
     return NUC3D_OK;
 }
 
@@ -277,6 +297,28 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
                 }
     }
     c->ntiles = (int)tiles.size();
+    // direction sweeps: 15 cells along the sweep axis x 16 lines
+    std::vector<int4> sw[3];
+    for (int b = 0; b < nblocks; b++) {
+        const nuc3d_block_dev &B = c->hb[b];
+        for (int k = 0; k < B.n[2]; k++)
+            for (int l0 = 0; l0 < B.n[1]; l0 += 16)
+                for (int a0 = 0; a0 < B.n[0]; a0 += 15) sw[0].push_back(make_int4(b, a0, l0, k));
+        for (int k = 0; k < B.n[2]; k++)
+            for (int l0 = 0; l0 < B.n[0]; l0 += 16)
+                for (int a0 = 0; a0 < B.n[1]; a0 += 15) sw[1].push_back(make_int4(b, a0, l0, k));
+        for (int j = 0; j < B.n[1]; j++)
+            for (int l0 = 0; l0 < B.n[0]; l0 += 16)
+                for (int a0 = 0; a0 < B.n[2]; a0 += 15) sw[2].push_back(make_int4(b, a0, l0, j));
+    }
+    for (int d = 0; d < 3; d++) {
+        c->nsweep[d] = (int)sw[d].size();
+        if (hipMalloc(&c->dsweep[d], sizeof(int4) * sw[d].size()) != hipSuccess ||
+            hipMemcpy(c->dsweep[d], sw[d].data(), sizeof(int4) * sw[d].size(), hipMemcpyHostToDevice) != hipSuccess) {
+            nuc3d_case_destroy(c);
+            return fail(NUC3D_ERR_OOM, "device allocation of sweep tiles failed");
+        }
+    }
     if (hipMalloc(&c->db, sizeof(nuc3d_block_dev) * nblocks) != hipSuccess ||
         hipMalloc(&c->dtiles, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess ||
         hipMalloc(&c->ddt, sizeof(double) * 4) != hipSuccess ||
@@ -300,12 +342,13 @@ int nuc3d_case_destroy(nuc3d_case *c)
 {
     if (!c) return NUC3D_OK;
     (void)hipDeviceSynchronize();
-    for (auto &pr : c->ev_stage) c->ev_pool.push_back(pr.first), c->ev_pool.push_back(pr.second);
-    for (auto &pr : c->ev_visc) c->ev_pool.push_back(pr.first), c->ev_pool.push_back(pr.second);
+    for (auto &l : c->ev)
+        for (auto &pr : l) c->ev_pool.push_back(pr.first), c->ev_pool.push_back(pr.second);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (void *m : c->chunks) (void)hipFree(m);
     (void)hipFree(c->db);
     (void)hipFree(c->dtiles);
+    for (int d = 0; d < 3; d++) (void)hipFree(c->dsweep[d]);
     (void)hipFree(c->ddt);
     (void)hipFree(c->dlam);
     (void)hipFree(c->dcount);
@@ -410,11 +453,7 @@ int nuc3d_rhs(nuc3d_case *c, void *stream)
         HIPCHK(hipMemsetAsync(lam, 0, sizeof(unsigned long long) * 3 * c->nb, s));
         HIPCHK(c->L->eigen(c->db, c->dtiles, c->ntiles, c->K, c->cur, lam, s));
     }
-    const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
-    if (ns) HIPCHK(c->L->visc(c->db, c->nb, c->dtiles, c->ntiles, c->K, c->cur, s));
-    HIPCHK(c->L->stage(c->p.scheme[0], ns, 0, c->db, c->dtiles, c->ntiles, c->K, 0, c->cur, 0, 0, c->ddt,
-                       c->dlam, lam, c->dcount, s));
-    return NUC3D_OK;
+    return launch_rhs(c, 0, 0, c->cur, 0, 0, c->dlam, lam, s);
 }
 
 int nuc3d_stage(nuc3d_case *c, int stage, double cfl, void *stream)
@@ -429,15 +468,10 @@ int nuc3d_stage(nuc3d_case *c, int stage, double cfl, void *stream)
     const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
     // buffers: stage 0 A->B, stage 1 (B, u_n=A)->C, stage 2 (C, u_n=A)->A
     const int bin = stage, bun = 0, bout = stage == 2 ? 0 : stage + 1;
-    if (ns) {
-        Timed t(c, s, &c->ev_visc);
-        HIPCHK(c->L->visc(c->db, c->nb, c->dtiles, c->ntiles, c->K, bin, s));
-    }
-    Timed t(c, s, &c->ev_stage);
+    (void)ns;
     unsigned long long *lcur = c->dlam + 3 * c->nb * c->lampar;
     unsigned long long *lnext = c->dlam + 3 * c->nb * (1 - c->lampar);
-    HIPCHK(c->L->stage(c->p.scheme[0], ns, 1, c->db, c->dtiles, c->ntiles, c->K, stage, bin, bun, bout, c->ddt, lcur,
-                       lnext, c->dcount, s));
+    if ((rc = launch_rhs(c, 1, stage, bin, bun, bout, lcur, lnext, s))) return rc;
     c->cur = bout;
     c->next_stage = (stage + 1) % 3;
     if (stage == 2) c->lampar ^= 1;
@@ -503,28 +537,25 @@ int nuc3d_timing(nuc3d_case *c, int enable)
     return NUC3D_OK;
 }
 
-int nuc3d_timing_read(nuc3d_case *c, double *stage_ms, int *stage_launches, double *visc_ms, int *visc_launches)
+int nuc3d_timing_read(nuc3d_case *c, double *ms4, int *launches4)
 {
     if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
     HIPCHK(hipDeviceSynchronize());
-    double acc[2] = {0.0, 0.0};
-    int cnt[2] = {0, 0};
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> *lists[2] = {&c->ev_stage, &c->ev_visc};
-    for (int l = 0; l < 2; l++) {
-        for (auto &pr : *lists[l]) {
+    for (int l = 0; l < 4; l++) {
+        double acc = 0.0;
+        int cnt = 0;
+        for (auto &pr : c->ev[l]) {
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
-            acc[l] += ms;
-            cnt[l]++;
+            acc += ms;
+            cnt++;
             c->ev_pool.push_back(pr.first);
             c->ev_pool.push_back(pr.second);
         }
-        lists[l]->clear();
+        c->ev[l].clear();
+        if (ms4) ms4[l] = acc;
+        if (launches4) launches4[l] = cnt;
     }
-    if (stage_ms) *stage_ms = acc[0];
-    if (stage_launches) *stage_launches = cnt[0];
-    if (visc_ms) *visc_ms = acc[1];
-    if (visc_launches) *visc_launches = cnt[1];
     return NUC3D_OK;
 }
 

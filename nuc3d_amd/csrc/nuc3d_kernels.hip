@@ -4,21 +4,17 @@
 // _exact) and with -DNUC3D_EXACT=0 (names end in _fast).  The host side picks
 // one set per case.
 //
-// The hot kernel is k_stage: one workgroup owns a 16x16 column tile of one
-// block and marches along k over a segment of planes.  Per plane it
-//   * computes the split fluxes of its cells in all three directions
-//     (physicsModel.cpp:528-754), x and y ones with a 3-cell halo into LDS,
-//     z ones into a six-plane register ring,
-//   * reconstructs every x and y face of the tile from LDS and every z face
-//     from the ring (WENO5-JS / WENO5-Z, or the physical-face closures),
-//   * differences the face fluxes (euler3d.cpp:143-241), adds the viscous
-//     divergence for Navier-Stokes (NaiverStokes3d.cpp:401-513), and
-//   * applies the TVD-RK3 stage (TVD-RK3.cpp:37-137) to the cell three planes
-//     behind, writing a second state buffer and pushing the new values into
-//     the ghost layers of neighbouring blocks (the on-device form of
-//     MPICommunicator.cpp:55-115).
-// Stage 2 also accumulates the maximum eigenvalues of the new state, from
-// which the next step's dt is taken (euler3d.cpp:429-435, PDEData3d.cpp:74-80).
+// One RK stage is three direction sweeps (k_sweep<0>, <1>, <2>): each
+// workgroup stages the split fluxes of 16 lines x 21 cells in LDS
+// (physicsModel.cpp:528-754), reconstructs 16 faces per line with one face
+// per thread (WENO5-JS / WENO5-Z or the physical closures) and differences
+// them into the RHS (euler3d.cpp:143-241).  The z sweep also subtracts the
+// viscous divergence for Navier-Stokes (NaiverStokes3d.cpp:401-513) and
+// applies the TVD-RK3 stage (TVD-RK3.cpp:37-137) into a second state buffer,
+// pushing the new values into the ghost layers of neighbouring blocks (the
+// on-device form of MPICommunicator.cpp:55-115).  Stage 2 also accumulates
+// the maximum eigenvalues of the new state, from which the next step's dt is
+// taken (euler3d.cpp:429-435, PDEData3d.cpp:74-80).
 #include "nuc3d_device.h"
 #include "nuc3d_types.h"
 
@@ -252,36 +248,89 @@ __device__ __forceinline__ double global_dt(const unsigned long long *lam, int n
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Fused RHS + stage kernel.  MODE 0 writes the RHS (one evaluation of
-// EulerData3D::solve / NaiverStokesData3d::solve); MODE 1 applies TVD-RK3
-// stage `stage` from buffer bin (u_i) and bun (u_n) into bout.
-template <int SCHEME, bool NS, int MODE>
-__global__ __launch_bounds__(256) void KNAME(k_stage)(const nuc3d_block_dev *__restrict__ blocks,
-                                                      const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K,
-                                                      int stage, int bin, int bun, int bout, double *dtbuf,
+// Direction sweep.  A workgroup owns 16 lines of one block along direction
+// DIR and 15 consecutive cells of each line (16 faces).  It computes the split
+// fluxes of the 21 cells of every line that its faces reach (3-cell halo,
+// physicsModel.cpp:528-754) into LDS, reconstructs the 16 faces per line
+// (one face per thread: WENO5-JS / WENO5-Z or the physical closures), and
+// differences them into the RHS (euler3d.cpp:143-241, :368-427):
+//   DIR 0: rhs  = dF/dxi
+//   DIR 1: rhs += dG/deta
+//   DIR 2: rhs += dH/dzeta, then for Navier-Stokes
+//          rhs -= dFv/dxi, dGv/deta, dHv/dzeta (NaiverStokes3d.cpp:401-513),
+//          then (MODE 1) the TVD-RK3 stage (TVD-RK3.cpp:37-137) into buffer
+//          bout with the e<0 fix-up and the ghost push to neighbours.
+// Lines run along j at fixed k (DIR 0) and along i otherwise, so that global
+// loads and stores of a wavefront touch contiguous i.
+// tile: x = block, y = first cell a0 along DIR, z = first line l0, w = third index.
+template <int DIR, int SCHEME, bool NS, int MODE>
+__global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__restrict__ blocks,
+                                                      const int4 *__restrict__ tiles, nuc3d_kparams K, int stage,
+                                                      int bin, int bun, int bout, double *dtbuf,
                                                       const unsigned long long *lam_cur,
                                                       unsigned long long *lam_next, unsigned long long *counters)
 {
-    constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
-    constexpr int NX_ITEMS = (T + 1) * T;  // faces per direction
-    // F+/F- of x (rows) and y (columns) with halo; after the face phase the
-    // same storage holds the reconstructed fluxes of the tile's faces.
-    __shared__ double FX[10][T][L];
-    __shared__ double FY[10][T][L];
-    double *HX = &FX[0][0][0];               // [5][T][T+1]
-    double *HY = &FX[0][0][0] + 5 * T * (T + 1);
+    constexpr int NL = 16, NF = 16, NC = NF - 1, H = NUC3D_HALO, NSL = NC + 2 * H;  // 21 slots per line
+    __shared__ double S[10][NSL * NL];
+    __shared__ double HF[5][NF * NL];
 
-    const nuc3d_tile tl = tiles[blockIdx.x];
-    const nuc3d_block_dev &B = blocks[tl.b];
+    const int4 t = tiles[blockIdx.x];
+    const nuc3d_block_dev &B = blocks[t.x];
     const PhysParams P = phys_of(K);
-    const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
-    const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
-    const bool incell = ti < tw && tj < th;
-    const int i = tl.i0 + ti, j = tl.j0 + tj;
+    const int a0 = t.y, l0 = t.z, cc = t.w;
+    const int nc = min(NC, B.n[DIR] - a0);
+    const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
     double *const *qin = B.q[bin];
 
+    // (line, slot) -> LDS offset: slot-fastest for DIR 0, line-fastest otherwise
+    auto sidx = [](int line, int slot) { return DIR == 0 ? line * NSL + slot : slot * NL + line; };
+    auto hidx = [](int line, int f) { return DIR == 0 ? line * NF + f : f * NL + line; };
+    auto cell = [&](int line, int a, int &i, int &j, int &k) {
+        if (DIR == 0) { i = a; j = l0 + line; k = cc; }
+        else if (DIR == 1) { i = l0 + line; j = a; k = cc; }
+        else { i = l0 + line; j = cc; k = a; }
+    };
+
+    const int nslots = nc + 2 * H;
+    for (int e = threadIdx.x; e < NSL * NL; e += 256) {
+        const int line = DIR == 0 ? e / NSL : e % NL;
+        const int sl = DIR == 0 ? e % NSL : e / NL;
+        if (line < nlines && sl < nslots) {
+            int i, j, k;
+            cell(line, a0 - H + sl, i, j, k);
+            double fp[5], fn[5];
+            split_any(B, qin, i, j, k, DIR, P, fp, fn);
+            const int o = sidx(line, sl);
+#pragma unroll
+            for (int c = 0; c < 5; c++) {
+                S[c][o] = fp[c];
+                S[5 + c][o] = fn[c];
+            }
+        }
+    }
+    __syncthreads();
+
+    const int line = DIR == 0 ? threadIdx.x / NF : threadIdx.x % NL;
+    const int f = DIR == 0 ? threadIdx.x % NF : threadIdx.x / NL;
+    if (line < nlines && f <= nc) {
+        const int m = a0 + f;
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            double sp[6], sn[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                sp[q] = S[c][sidx(line, f + q)];
+                sn[q] = S[5 + c][sidx(line, f + q)];
+            }
+            HF[c][hidx(line, f)] = face_value<SCHEME>(B, K, DIR, m, sp, sn);
+        }
+    }
+    __syncthreads();
+
+    double lmax[3] = {0.0, 0.0, 0.0};
+    unsigned long long nfix = 0;
     double dt = 0.0;
-    if (MODE == 1) {
+    if (DIR == 2 && MODE == 1) {
         if (stage == 0) {
             dt = global_dt(lam_cur, K.nblocks, K.cfl);
             if (blockIdx.x == 0 && threadIdx.x == 0) dtbuf[0] = dt;
@@ -291,211 +340,68 @@ __global__ __launch_bounds__(256) void KNAME(k_stage)(const nuc3d_block_dev *__r
             dt = dtbuf[0];
         }
     }
-
-    double zr[6][10];   // z split fluxes of planes p-5..p (slot 5 newest)
-    double rr[3][5];    // x+y RHS of planes p-3..p-1
-    double hz_prev[5];  // z face flux of the previous face
-    double lmax[3] = {0.0, 0.0, 0.0};
-    unsigned long long nfix = 0;
-#pragma unroll
-    for (int a = 0; a < 6; a++)
-#pragma unroll
-        for (int c = 0; c < 10; c++) zr[a][c] = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int c = 0; c < 5; c++) rr[a][c] = 0.0;
-#pragma unroll
-    for (int c = 0; c < 5; c++) hz_prev[c] = 0.0;
-
-    const int kb = tl.k0, ke = tl.k0 + tl.kn;
-    for (int p = kb - H; p < ke + H; p++) {
-        const bool axy = p >= kb && p < ke;
-        __syncthreads();
-        // ---- split fluxes of this plane
-#pragma unroll
-        for (int a = 0; a < 5; a++)
-#pragma unroll
-            for (int c = 0; c < 10; c++) zr[a][c] = zr[a + 1][c];
-        if (incell) {
-            split_any(B, qin, i, j, p, 2, P, &zr[5][0], &zr[5][5]);
-        } else {
-#pragma unroll
-            for (int c = 0; c < 10; c++) zr[5][c] = 0.0;
-        }
-        if (axy) {
-            if (incell) {
-                double fp[5], fn[5];
-                const long long x = cidx(B, i, j, p);
-                cell_split(B, qin, x, 0, P, fp, fn);
-#pragma unroll
-                for (int c = 0; c < 5; c++) {
-                    FX[c][tj][ti + H] = fp[c];
-                    FX[5 + c][tj][ti + H] = fn[c];
-                }
-                cell_split(B, qin, x, 1, P, fp, fn);
-#pragma unroll
-                for (int c = 0; c < 5; c++) {
-                    FY[c][ti][tj + H] = fp[c];
-                    FY[5 + c][ti][tj + H] = fn[c];
-                }
-            }
-            // halo: 6 cells per row (x) and per column (y)
-            const int h = threadIdx.x;
-            if (h < 2 * T * 2 * H) {
-                const int dir = h / (T * 2 * H);
-                const int r = (h % (T * 2 * H)) / (2 * H), s = h % (2 * H);
-                const int lim = dir == 0 ? th : tw, ext = dir == 0 ? tw : th;
-                if (r < lim) {
-                    const int off = s < H ? s - H : ext + (s - H);
-                    const int slot = s < H ? s : ext + s;
-                    double fp[5], fn[5];
-                    if (dir == 0) {
-                        split_any(B, qin, tl.i0 + off, tl.j0 + r, p, 0, P, fp, fn);
-#pragma unroll
-                        for (int c = 0; c < 5; c++) {
-                            FX[c][r][slot] = fp[c];
-                            FX[5 + c][r][slot] = fn[c];
-                        }
-                    } else {
-                        split_any(B, qin, tl.i0 + r, tl.j0 + off, p, 1, P, fp, fn);
-#pragma unroll
-                        for (int c = 0; c < 5; c++) {
-                            FY[c][r][slot] = fp[c];
-                            FY[5 + c][r][slot] = fn[c];
-                        }
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        // ---- x and y faces of the tile
-        double hres[3][5];
-        if (axy) {
-#pragma unroll
-            for (int rnd = 0; rnd < 3; rnd++) {
-                const int it = threadIdx.x + 256 * rnd;
-#pragma unroll
-                for (int c = 0; c < 5; c++) hres[rnd][c] = 0.0;
-                if (it < 2 * NX_ITEMS) {
-                    const int dir = it / NX_ITEMS, rem = it % NX_ITEMS;
-                    const int line = rem / (T + 1), fi = rem % (T + 1);
-                    const bool ok = dir == 0 ? (line < th && fi <= tw) : (line < tw && fi <= th);
-                    if (ok) {
-                        const int m = (dir == 0 ? tl.i0 : tl.j0) + fi;
-#pragma unroll
-                        for (int c = 0; c < 5; c++) {
-                            const double *sp = dir == 0 ? &FX[c][line][fi] : &FY[c][line][fi];
-                            const double *sn = dir == 0 ? &FX[5 + c][line][fi] : &FY[5 + c][line][fi];
-                            hres[rnd][c] = face_value<SCHEME>(B, K, dir, m, sp, sn);
-                        }
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        if (axy) {
-#pragma unroll
-            for (int rnd = 0; rnd < 3; rnd++) {
-                const int it = threadIdx.x + 256 * rnd;
-                if (it < 2 * NX_ITEMS) {
-                    const int dir = it / NX_ITEMS, rem = it % NX_ITEMS;
-                    const int line = rem / (T + 1), fi = rem % (T + 1);
-                    double *dst = dir == 0 ? HX : HY;
-#pragma unroll
-                    for (int c = 0; c < 5; c++) dst[(c * T + line) * (T + 1) + fi] = hres[rnd][c];
-                }
-            }
-        }
-        __syncthreads();
-        double rnew[5];
-#pragma unroll
-        for (int c = 0; c < 5; c++) rnew[c] = 0.0;
-        if (axy && incell) {
-#pragma unroll
-            for (int c = 0; c < 5; c++) {
-                const double dfx = HX[(c * T + tj) * (T + 1) + ti + 1] - HX[(c * T + tj) * (T + 1) + ti];
-                const double dgy = HY[(c * T + ti) * (T + 1) + tj + 1] - HY[(c * T + ti) * (T + 1) + tj];
-                rnew[c] = dfx + dgy;  // rhs = df; rhs += dg (euler3d.cpp:383-408)
-            }
-        }
-        // ---- z face p-2 and the update of cell p-3
-        const int zf = p - 2;
-        if (incell && zf >= kb && zf <= ke) {
-            double hz[5];
-#pragma unroll
-            for (int c = 0; c < 5; c++) {
-                double sp[6], sn[6];
-#pragma unroll
-                for (int a = 0; a < 6; a++) {
-                    sp[a] = zr[a][c];
-                    sn[a] = zr[a][5 + c];
-                }
-                hz[c] = face_value<SCHEME>(B, K, 2, zf, sp, sn);
-            }
-            const int kc = zf - 1;
-            if (kc >= kb) {
-                const long long x = cidx(B, i, j, kc);
-                double r[5];
-#pragma unroll
-                for (int c = 0; c < 5; c++) r[c] = rr[0][c] + (hz[c] - hz_prev[c]);  // rhs += dh
-                if (NS) {
-#pragma unroll
-                    for (int c = 1; c < 5; c++) {
-                        const double gx = line_diff(B, K, B.fv[c - 1] + x, 0, i);
-                        const double gy = line_diff(B, K, B.fv[4 + c - 1] + x, 1, j);
-                        const double gz = line_diff(B, K, B.fv[8 + c - 1] + x, 2, kc);
-                        r[c] = ((r[c] - gx) - gy) - gz;
-                    }
-                }
-                if (MODE == 0) {
-#pragma unroll
-                    for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
-                } else {
-                    double qn[5];
-#pragma unroll
-                    for (int c = 0; c < 5; c++) {
-                        const double rhs = -r[c];
-                        const double u = qin[c][x];
-                        if (stage == 0)
-                            qn[c] = 1.0 * u + 1.0 * rhs * dt;
-                        else if (stage == 1)
-                            qn[c] = 3.0 / 4.0 * B.q[bun][c][x] + 1.0 / 4.0 * (rhs * dt + u);
-                        else
-                            qn[c] = 1.0 / 3.0 * B.q[bun][c][x] + 2.0 / 3.0 * (rhs * dt + u);
-                    }
-                    const double jac = B.jac[x];
-                    if (fixup_energy(qn, jac, P)) nfix++;
-#pragma unroll
-                    for (int c = 0; c < 5; c++) B.q[bout][c][x] = qn[c];
-                    push_q(blocks, B, bout, i, j, kc, qn);
-                    if (stage == 2) {
-                        double rho, u, v, w, pr, cs;
-                        con2prim(qn, jac, P, rho, u, v, w, pr, cs);
-#pragma unroll
-                        for (int d = 0; d < 3; d++) {
-                            const double mx = B.met[3 * d][x], my = B.met[3 * d + 1][x], mz = B.met[3 * d + 2][x];
-                            const double theta = sqrt(mx * mx + my * my + mz * mz);
-                            const double U0 = (mx * u + my * v + mz * w);
-                            const double lam = fabs(U0) + cs * theta;
-                            lmax[d] = (lmax[d] < lam) ? lam : lmax[d];
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 5; c++) hz_prev[c] = hz[c];
-        }
+    if (line < nlines && f >= 1 && f <= nc) {
+        int i, j, k;
+        cell(line, a0 + f - 1, i, j, k);
+        const long long x = cidx(B, i, j, k);
+        double r[5];
 #pragma unroll
         for (int c = 0; c < 5; c++) {
-            rr[0][c] = rr[1][c];
-            rr[1][c] = rr[2][c];
-            rr[2][c] = rnew[c];
+            const double d = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
+            r[c] = DIR == 0 ? d : B.rhs[c][x] + d;
+        }
+        if (DIR < 2) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
+        } else {
+            if (NS) {
+#pragma unroll
+                for (int c = 1; c < 5; c++) {
+                    const double gx = line_diff(B, K, B.fv[c - 1] + x, 0, i);
+                    const double gy = line_diff(B, K, B.fv[4 + c - 1] + x, 1, j);
+                    const double gz = line_diff(B, K, B.fv[8 + c - 1] + x, 2, k);
+                    r[c] = ((r[c] - gx) - gy) - gz;
+                }
+            }
+            if (MODE == 0) {
+#pragma unroll
+                for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
+            } else {
+                double qn[5];
+#pragma unroll
+                for (int c = 0; c < 5; c++) {
+                    const double rhs = -r[c];
+                    const double u = qin[c][x];
+                    if (stage == 0)
+                        qn[c] = 1.0 * u + 1.0 * rhs * dt;
+                    else if (stage == 1)
+                        qn[c] = 3.0 / 4.0 * B.q[bun][c][x] + 1.0 / 4.0 * (rhs * dt + u);
+                    else
+                        qn[c] = 1.0 / 3.0 * B.q[bun][c][x] + 2.0 / 3.0 * (rhs * dt + u);
+                }
+                const double jac = B.jac[x];
+                if (fixup_energy(qn, jac, P)) nfix++;
+#pragma unroll
+                for (int c = 0; c < 5; c++) B.q[bout][c][x] = qn[c];
+                push_q(blocks, B, bout, i, j, k, qn);
+                if (stage == 2) {
+                    double rho, u, v, w, pr, cs;
+                    con2prim(qn, jac, P, rho, u, v, w, pr, cs);
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const double mx = B.met[3 * d][x], my = B.met[3 * d + 1][x], mz = B.met[3 * d + 2][x];
+                        const double theta = sqrt(mx * mx + my * my + mz * mz);
+                        const double U0 = (mx * u + my * v + mz * w);
+                        const double lam = fabs(U0) + cs * theta;
+                        lmax[d] = (lmax[d] < lam) ? lam : lmax[d];
+                    }
+                }
+            }
         }
     }
-    if (MODE == 1) {
+    if (DIR == 2 && MODE == 1) {
         if (nfix) atomicAdd(counters, nfix);
-        if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * tl.b);
+        if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * t.x);
     }
 }
 
@@ -690,32 +596,51 @@ __global__ __launch_bounds__(256) void KNAME(k_ghost_fill)(const nuc3d_block_dev
 
 // ---------------------------------------------------------------------------
 // host-side launchers (the only symbols the C-ABI layer uses)
-template <int SCHEME, bool NS, int MODE>
-static void launch_stage(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles, int ntiles, nuc3d_kparams K,
-                         int stage, int bin, int bun, int bout, double *dtbuf, const unsigned long long *lam_cur,
+template <int DIR, int SCHEME, bool NS, int MODE>
+static void launch_sweep(const nuc3d_block_dev *blocks, const int4 *tiles, int ntiles, nuc3d_kparams K, int stage,
+                         int bin, int bun, int bout, double *dtbuf, const unsigned long long *lam_cur,
                          unsigned long long *lam_next, unsigned long long *counters, hipStream_t st)
 {
-    hipLaunchKernelGGL((KNAME(k_stage)<SCHEME, NS, MODE>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, stage,
-                       bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
+    hipLaunchKernelGGL((KNAME(k_sweep)<DIR, SCHEME, NS, MODE>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K,
+                       stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
 }
 
-extern "C" hipError_t KNAME(nuc3d_launch_stage)(int scheme, int ns, int mode, const nuc3d_block_dev *blocks,
-                                                const nuc3d_tile *tiles, int ntiles, nuc3d_kparams K, int stage,
-                                                int bin, int bun, int bout, double *dtbuf,
+template <int SCHEME, bool NS>
+static void sweep_dispatch(int dir, int mode, const nuc3d_block_dev *blocks, const int4 *tiles, int ntiles,
+                           nuc3d_kparams K, int stage, int bin, int bun, int bout, double *dtbuf,
+                           const unsigned long long *lam_cur, unsigned long long *lam_next,
+                           unsigned long long *counters, hipStream_t st)
+{
+#define NUC3D_A blocks, tiles, ntiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, st
+    if (dir == 0)
+        launch_sweep<0, SCHEME, NS, 0>(NUC3D_A);
+    else if (dir == 1)
+        launch_sweep<1, SCHEME, NS, 0>(NUC3D_A);
+    else if (mode == 0)
+        launch_sweep<2, SCHEME, NS, 0>(NUC3D_A);
+    else
+        launch_sweep<2, SCHEME, NS, 1>(NUC3D_A);
+#undef NUC3D_A
+}
+
+// dir 0/1 accumulate the RHS; dir 2 completes it (mode 0) or applies the RK stage (mode 1)
+extern "C" hipError_t KNAME(nuc3d_launch_sweep)(int dir, int scheme, int ns, int mode, const nuc3d_block_dev *blocks,
+                                                const int4 *tiles, int ntiles, nuc3d_kparams K, int stage, int bin,
+                                                int bun, int bout, double *dtbuf,
                                                 const unsigned long long *lam_cur, unsigned long long *lam_next,
                                                 unsigned long long *counters, hipStream_t st)
 {
-#define NUC3D_L(S, N, M) launch_stage<S, N, M>(blocks, tiles, ntiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, st)
+#define NUC3D_A dir, mode, blocks, tiles, ntiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, st
     if (scheme == SCH_WENO5JS) {
-        if (ns) { if (mode) NUC3D_L(SCH_WENO5JS, true, 1); else NUC3D_L(SCH_WENO5JS, true, 0); }
-        else { if (mode) NUC3D_L(SCH_WENO5JS, false, 1); else NUC3D_L(SCH_WENO5JS, false, 0); }
+        if (ns) sweep_dispatch<SCH_WENO5JS, true>(NUC3D_A);
+        else sweep_dispatch<SCH_WENO5JS, false>(NUC3D_A);
     } else if (scheme == SCH_WENO5Z) {
-        if (ns) { if (mode) NUC3D_L(SCH_WENO5Z, true, 1); else NUC3D_L(SCH_WENO5Z, true, 0); }
-        else { if (mode) NUC3D_L(SCH_WENO5Z, false, 1); else NUC3D_L(SCH_WENO5Z, false, 0); }
+        if (ns) sweep_dispatch<SCH_WENO5Z, true>(NUC3D_A);
+        else sweep_dispatch<SCH_WENO5Z, false>(NUC3D_A);
     } else {
         return hipErrorInvalidValue;
     }
-#undef NUC3D_L
+#undef NUC3D_A
     return hipGetLastError();
 }
 

@@ -96,7 +96,7 @@ def lib():
         "nuc3d_get_max_eigen": [vp, ctypes.c_int, dp],
         "nuc3d_get_fixups": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "nuc3d_timing": [vp, ctypes.c_int],
-        "nuc3d_timing_read": [vp, dp, ctypes.POINTER(ctypes.c_int), dp, ctypes.POINTER(ctypes.c_int)],
+        "nuc3d_timing_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -205,10 +205,11 @@ class Case:
     def timing(self, enable=True):
         _check(lib().nuc3d_timing(self.h, 1 if enable else 0), "timing")
 
+    TIMING_CLASSES = ("x_sweep", "y_sweep", "z_sweep_update", "viscous")
+
     def timing_read(self):
-        """(stage_ms_total, stage_launches, visc_ms_total, visc_launches) since the last read."""
-        sm, vm = ctypes.c_double(), ctypes.c_double()
-        sl, vl = ctypes.c_int(), ctypes.c_int()
-        _check(lib().nuc3d_timing_read(self.h, ctypes.byref(sm), ctypes.byref(sl), ctypes.byref(vm),
-                                       ctypes.byref(vl)), "timing_read")
-        return sm.value, sl.value, vm.value, vl.value
+        """{class: (total_ms, launches)} since the last read."""
+        ms = (ctypes.c_double * 4)()
+        n = (ctypes.c_int * 4)()
+        _check(lib().nuc3d_timing_read(self.h, ms, n), "timing_read")
+        return {k: (ms[i], n[i]) for i, k in enumerate(self.TIMING_CLASSES)}
