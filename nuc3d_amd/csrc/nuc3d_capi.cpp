@@ -20,7 +20,7 @@
                                                    nuc3d_kparams, int, int, int, int, double *,                      \
                                                    const unsigned long long *, unsigned long long *,                 \
                                                    unsigned long long *, hipStream_t);                               \
-    extern "C" hipError_t nuc3d_launch_visc_##SFX(const nuc3d_block_dev *, int, const nuc3d_tile *, int,             \
+    extern "C" hipError_t nuc3d_launch_visc_##SFX(const nuc3d_block_dev *, int, const int4 *, int,                   \
                                                   nuc3d_kparams, int, hipStream_t);                                  \
     extern "C" hipError_t nuc3d_launch_eigen_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, nuc3d_kparams,  \
                                                    int, unsigned long long *, hipStream_t);                          \
@@ -190,7 +190,7 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
     const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
     if (ns) {
         Timed t(c, s, &c->ev[3]);
-        HIPCHK(c->L->visc(c->db, c->nb, c->dtiles, c->ntiles, c->K, bin, s));
+        HIPCHK(c->L->visc(c->db, c->nb, c->dsweep[1], c->nsweep[1], c->K, bin, s));
     }
     for (int d = 0; d < 3; d++) {
         Timed t(c, s, &c->ev[d]);

@@ -165,6 +165,16 @@ __device__ __forceinline__ double line_diff(const nuc3d_block_dev &B, const nuc3
     return cd6(f, st);
 }
 
+// Same derivative on a line held in LDS (element stride st along the line).
+__device__ __forceinline__ double line_diff_lds(const nuc3d_block_dev &B, const nuc3d_kparams &K, const double *f,
+                                                int st, int d, int t)
+{
+    const int n = B.n[d], w = K.width[d];
+    if (B.ftype[2 * d] == -1 && t < w) return diff_exterior(f, st, t, true);
+    if (B.ftype[2 * d + 1] == -1 && t >= n - w) return diff_exterior(f, st, n - 1 - t, false);
+    return cd6(f, st);
+}
+
 // Ghost cell of the neighbour that receives cell (i,j,k) through face f: the
 // raw slab copy of bufferData::setBufferSend + MPI_Irecv (bufferData.cpp:56-192,
 // MPICommunicator.cpp:55-81).  Face f packs cells [0,G) (even f) or [n-G,n)
@@ -273,6 +283,10 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     constexpr int NL = 16, NF = 16, NC = NF - 1, H = NUC3D_HALO, NSL = NC + 2 * H;  // 21 slots per line
     __shared__ double S[10][NSL * NL];
     __shared__ double HF[5][NF * NL];
+    // viscous flux components 1..4 of this direction along the lines (fast
+    // variant: each sweep subtracts its own viscous divergence)
+    constexpr bool VLDS = NS && !NUC3D_EXACT;
+    __shared__ double V[VLDS ? 4 : 1][VLDS ? NSL * NL : 1];
 
     const int4 t = tiles[blockIdx.x];
     const nuc3d_block_dev &B = blocks[t.x];
@@ -305,6 +319,11 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
             for (int c = 0; c < 5; c++) {
                 S[c][o] = fp[c];
                 S[5 + c][o] = fn[c];
+            }
+            if (VLDS) {
+                const long long x = cidx(B, i, j, k);
+#pragma unroll
+                for (int c = 0; c < 4; c++) V[c][o] = B.fv[4 * DIR + c][x];
             }
         }
     }
@@ -347,14 +366,18 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
         double r[5];
 #pragma unroll
         for (int c = 0; c < 5; c++) {
-            const double d = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
+            double d = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
+            if (VLDS && c > 0) {
+                const int st = DIR == 0 ? 1 : NL;
+                d -= line_diff_lds(B, K, &V[c - 1][sidx(line, f + 2)], st, DIR, a0 + f - 1);
+            }
             r[c] = DIR == 0 ? d : B.rhs[c][x] + d;
         }
         if (DIR < 2) {
 #pragma unroll
             for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
         } else {
-            if (NS) {
+            if (NS && !VLDS) {  // reference order: rhs = df + dg + dh - dfv - dgv - dhv
 #pragma unroll
                 for (int c = 1; c < 5; c++) {
                     const double gx = line_diff(B, K, B.fv[c - 1] + x, 0, i);
@@ -440,63 +463,60 @@ __global__ __launch_bounds__(256) void KNAME(k_prim)(const nuc3d_block_dev *__re
 }
 
 __global__ __launch_bounds__(256) void KNAME(k_visc_flux)(const nuc3d_block_dev *__restrict__ blocks,
-                                                          const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K)
+                                                          const int4 *__restrict__ tiles, nuc3d_kparams K)
 {
-    constexpr int T = NUC3D_TILE;
-    const nuc3d_tile tl = tiles[blockIdx.x];
-    const nuc3d_block_dev &B = blocks[tl.b];
-    const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
-    const int i = tl.i0 + ti, j = tl.j0 + tj;
-    if (i >= B.n[0] || j >= B.n[1]) return;
+    // one thread per cell on the tiles of the y sweep: 16 i (lines) x 15 j
+    const int4 t = tiles[blockIdx.x];
+    const nuc3d_block_dev &B = blocks[t.x];
+    const int i = t.z + (threadIdx.x & 15), j = t.y + (threadIdx.x >> 4), k = t.w;
+    if (i >= B.n[0] || j >= B.n[1] || (threadIdx.x >> 4) >= 15) return;
     const double S = K.sref;
-    for (int k = tl.k0; k < tl.k0 + tl.kn; k++) {
-        const long long x = cidx(B, i, j, k);
-        double g[4][3];
+    const long long x = cidx(B, i, j, k);
+    double g[4][3];
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const double a = line_diff(B, K, B.prim[v] + x, 0, i);
-            const double bb = line_diff(B, K, B.prim[v] + x, 1, j);
-            const double cc = line_diff(B, K, B.prim[v] + x, 2, k);
-            g[v][0] = a * B.met[0][x] + bb * B.met[3][x] + cc * B.met[6][x];
-            g[v][1] = a * B.met[1][x] + bb * B.met[4][x] + cc * B.met[7][x];
-            g[v][2] = a * B.met[2][x] + bb * B.met[5][x] + cc * B.met[8][x];
-        }
-        const double T0 = B.prim[3][x];
-        const double miu0 = (1.0 + S) / (T0 + S) * pow(T0, 1.5) / K.reynolds;  // physicsModel.cpp:226
-        const double coeff0 = miu0 / ((K.gamma - 1.0) * K.mach * K.mach * K.prandtl);
-        const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
-        const double vx = g[1][0], vy = g[1][1], vz = g[1][2];
-        const double wx = g[2][0], wy = g[2][1], wz = g[2][2];
-        const double grad = ux + vy + wz;
-        const double txx = miu0 * (2.0 * ux - 2.0 / 3.0 * grad);
-        const double txy = miu0 * (uy + vx - 2.0 / 3.0 * grad);
-        const double txz = miu0 * (uz + wx - 2.0 / 3.0 * grad);
-        const double tyy = miu0 * (2.0 * vy - 2.0 / 3.0 * grad);
-        const double tyz = miu0 * (vz + wy - 2.0 / 3.0 * grad);
-        const double tzz = miu0 * (2.0 * wz - 2.0 / 3.0 * grad);
-        const double ttx = coeff0 * g[3][0], tty = coeff0 * g[3][1], ttz = coeff0 * g[3][2];
-        const double u = B.prim[0][x], vv = B.prim[1][x], w = B.prim[2][x];
-        const double fv[4] = {txx, txy, txz, u * txx + vv * txy + w * txz + ttx};
-        const double gv[4] = {txy, tyy, tyz, u * txy + vv * tyy + w * tyz + tty};
-        const double hv[4] = {txz, tyz, tzz, u * txz + vv * tyz + w * tzz + ttz};
-        const double jac = B.jac[x];
-        double out[12];
+    for (int v = 0; v < 4; v++) {
+        const double a = line_diff(B, K, B.prim[v] + x, 0, i);
+        const double bb = line_diff(B, K, B.prim[v] + x, 1, j);
+        const double cc = line_diff(B, K, B.prim[v] + x, 2, k);
+        g[v][0] = a * B.met[0][x] + bb * B.met[3][x] + cc * B.met[6][x];
+        g[v][1] = a * B.met[1][x] + bb * B.met[4][x] + cc * B.met[7][x];
+        g[v][2] = a * B.met[2][x] + bb * B.met[5][x] + cc * B.met[8][x];
+    }
+    const double T0 = B.prim[3][x];
+    const double miu0 = (1.0 + S) / (T0 + S) * pow(T0, 1.5) / K.reynolds;  // physicsModel.cpp:226
+    const double coeff0 = miu0 / ((K.gamma - 1.0) * K.mach * K.mach * K.prandtl);
+    const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
+    const double vx = g[1][0], vy = g[1][1], vz = g[1][2];
+    const double wx = g[2][0], wy = g[2][1], wz = g[2][2];
+    const double grad = ux + vy + wz;
+    const double txx = miu0 * (2.0 * ux - 2.0 / 3.0 * grad);
+    const double txy = miu0 * (uy + vx - 2.0 / 3.0 * grad);
+    const double txz = miu0 * (uz + wx - 2.0 / 3.0 * grad);
+    const double tyy = miu0 * (2.0 * vy - 2.0 / 3.0 * grad);
+    const double tyz = miu0 * (vz + wy - 2.0 / 3.0 * grad);
+    const double tzz = miu0 * (2.0 * wz - 2.0 / 3.0 * grad);
+    const double ttx = coeff0 * g[3][0], tty = coeff0 * g[3][1], ttz = coeff0 * g[3][2];
+    const double u = B.prim[0][x], vv = B.prim[1][x], w = B.prim[2][x];
+    const double fv[4] = {txx, txy, txz, u * txx + vv * txy + w * txz + ttx};
+    const double gv[4] = {txy, tyy, tyz, u * txy + vv * tyy + w * tyz + tty};
+    const double hv[4] = {txz, tyz, tzz, u * txz + vv * tyz + w * tzz + ttz};
+    const double jac = B.jac[x];
+    double out[12];
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const double mx = B.met[3 * d][x], my = B.met[3 * d + 1][x], mz = B.met[3 * d + 2][x];
+    for (int d = 0; d < 3; d++) {
+        const double mx = B.met[3 * d][x], my = B.met[3 * d + 1][x], mz = B.met[3 * d + 2][x];
 #pragma unroll
-            for (int c = 0; c < 4; c++) out[4 * d + c] = (mx * fv[c] + my * gv[c] + mz * hv[c]) / jac;
-        }
+        for (int c = 0; c < 4; c++) out[4 * d + c] = (mx * fv[c] + my * gv[c] + mz * hv[c]) / jac;
+    }
 #pragma unroll
-        for (int e = 0; e < 12; e++) B.fv[e][x] = out[e];
+    for (int e = 0; e < 12; e++) B.fv[e][x] = out[e];
 #pragma unroll
-        for (int f = 0; f < 6; f++) {
-            int nb;
-            long long y;
-            if (ghost_target(blocks, B, f, i, j, k, nb, y)) {
+    for (int f = 0; f < 6; f++) {
+        int nb;
+        long long y;
+        if (ghost_target(blocks, B, f, i, j, k, nb, y)) {
 #pragma unroll
-                for (int e = 0; e < 12; e++) blocks[nb].fv[e][y] = out[e];
-            }
+            for (int e = 0; e < 12; e++) blocks[nb].fv[e][y] = out[e];
         }
     }
 }
@@ -644,11 +664,11 @@ extern "C" hipError_t KNAME(nuc3d_launch_sweep)(int dir, int scheme, int ns, int
     return hipGetLastError();
 }
 
-extern "C" hipError_t KNAME(nuc3d_launch_visc)(const nuc3d_block_dev *blocks, int nblocks, const nuc3d_tile *tiles,
-                                               int ntiles, nuc3d_kparams K, int bin, hipStream_t st)
+extern "C" hipError_t KNAME(nuc3d_launch_visc)(const nuc3d_block_dev *blocks, int nblocks, const int4 *ytiles,
+                                               int nytiles, nuc3d_kparams K, int bin, hipStream_t st)
 {
-    hipLaunchKernelGGL(KNAME(k_prim), dim3(512, nblocks), dim3(256), 0, st, blocks, nblocks, K, bin);
-    hipLaunchKernelGGL(KNAME(k_visc_flux), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K);
+    hipLaunchKernelGGL(KNAME(k_prim), dim3(1024, nblocks), dim3(256), 0, st, blocks, nblocks, K, bin);
+    hipLaunchKernelGGL(KNAME(k_visc_flux), dim3(nytiles), dim3(256), 0, st, blocks, ytiles, K);
     return hipGetLastError();
 }
 

@@ -108,6 +108,11 @@ def lib():
     return L
 
 
+def crweno5_available():
+    """Whether the library implements the CRWENO5 line solves."""
+    return hasattr(lib(), "nuc3d_has_crweno5") and lib().nuc3d_has_crweno5() == 1
+
+
 def _dp(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))

@@ -1,0 +1,107 @@
+"""Writes the golden fixtures of tests/golden/ from the reference.
+
+The reference is compiled from its own sources by oracle/Makefile (target
+`ref`) into oracle/_ref/ref_harness, which drives the reference's classes
+(EulerData3D / NaiverStokesData3d::solve, PDEData3d::solve, boundaryCondition,
+the MPI halo exchange with one block per rank).  Each fixture holds the inputs
+(face topology, metrics, Q) and the reference's outputs (RHS of one
+evaluation, or Q and dt after n RK3 steps).  Run in this container only:
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from nuc3d_amd import cases  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def tgv(dims, seed):
+    return cases.taylor_green(dims=dims, noise=1e-3, seed=seed, pressure="reference")
+
+
+def walls(dims, faces, seed):
+    b = tgv(dims, seed)[0]
+    b.faces = faces
+    return [b]
+
+
+CASES = [
+    # name, params, blocks, mode, nsteps, cfl
+    ("tgv_ns_weno5js_rhs", dict(model=O.NS), lambda: tgv((16, 12, 10), 11), "rhs", 0, 0.5),
+    ("tgv_ns_weno5js_steps3", dict(model=O.NS), lambda: tgv((16, 12, 10), 12), "steps", 3, 0.5),
+    ("tgv_ns_weno5z_steps3", dict(model=O.NS, scheme=(O.WENO5Z,) * 3), lambda: tgv((14, 16, 12), 13), "steps", 3,
+     0.5),
+    ("tgv_euler_crweno5_steps2", dict(model=O.EULER, scheme=(O.CRWENO5,) * 3), lambda: tgv((16, 14, 12), 14),
+     "steps", 2, 0.5),
+    ("multiblock3_ns_weno5z_steps2", dict(model=O.NS, scheme=(O.WENO5Z,) * 3),
+     lambda: cases.split_block_x(tgv((36, 12, 10), 15)[0], 3), "steps", 2, 0.5),
+    ("shuosher_euler_weno5z_steps3", dict(model=O.EULER, mach=1.0, scheme=(O.WENO5Z,) * 3),
+     lambda: cases.shu_osher(40, 10, 8, seed=1), "steps", 3, 0.5),
+    ("shuosher_euler_crweno5_steps3", dict(model=O.EULER, mach=1.0, scheme=(O.CRWENO5,) * 3),
+     lambda: cases.shu_osher(40, 10, 8, seed=1), "steps", 3, 0.5),
+    ("shuosher_euler_ausm_steps2", dict(model=O.EULER, mach=1.0, riemann=O.AUSM),
+     lambda: cases.shu_osher(32, 10, 8, seed=2), "steps", 2, 0.3),
+    ("shuosher_euler_ausmp_steps2", dict(model=O.EULER, mach=1.0, riemann=O.AUSMP),
+     lambda: cases.shu_osher(32, 10, 8, seed=2), "steps", 2, 0.3),
+    ("walls_ns_rhs", dict(model=O.NS, t_wall=1.2),
+     lambda: walls((16, 12, 10), [(-1, 2, None), (-1, 1, None), (-1, 2, None), (-1, 3, None), (0, 5, None),
+                                  (0, 4, None)], 16), "rhs", 0, 0.5),
+    ("walls_ns_steps2", dict(model=O.NS, t_wall=1.2),
+     lambda: walls((16, 12, 10), [(-1, 3, None), (-1, 3, None), (-1, 0, [1.0, 0.1, 0.0, 0.0, 71.0]),
+                                  (-1, 1, None), (-1, 2, None), (-1, 3, None)], 17), "steps", 2, 0.5),
+]
+
+
+def faces_to_array(faces):
+    t = np.array([f[0] for f in faces], dtype=np.int32)
+    i = np.array([f[1] for f in faces], dtype=np.int32)
+    v = np.zeros((6, 5))
+    for k, f in enumerate(faces):
+        if len(f) > 2 and f[2] is not None:
+            v[k, :len(f[2])] = f[2]
+    return t, i, v
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("reference harness missing: make -C oracle ref")
+    for name, pk, mk, mode, nsteps, cfl in CASES:
+        params = O.Params(**pk)
+        blocks = mk()
+        d = tempfile.mkdtemp(prefix="golden_")
+        O.write_ref_case(d, params, blocks)
+        out = O.run_ref(d, len(blocks), mode, nsteps, cfl)
+        arrays = {}
+        for b, (blk, o) in enumerate(zip(blocks, out)):
+            t, i, v = faces_to_array(blk.faces)
+            arrays.update({
+                "b%d_dims" % b: np.asarray(blk.dims, dtype=np.int32),
+                "b%d_ftype" % b: t, "b%d_fid" % b: i, "b%d_fval" % b: v,
+                "b%d_jac" % b: blk.jac, "b%d_met" % b: blk.met, "b%d_dxyz" % b: blk.dxyz,
+                "b%d_q_in" % b: blk.q,
+                "b%d_out_q" % b: o["q"], "b%d_out_prim" % b: o["prim"],
+                "b%d_out_max_eigen" % b: o["max_eigen"], "b%d_out_dt_local" % b: np.float64(o["dt_local"]),
+            })
+            if mode == "rhs":
+                arrays["b%d_out_rhs" % b] = o["rhs"]
+        arrays["out_dts"] = out[0]["dts"]
+        meta = dict(name=name, params={k: (list(v) if isinstance(v, tuple) else v) for k, v in
+                                       vars(params).items()},
+                    mode=mode, nsteps=nsteps, cfl=cfl, nblocks=len(blocks),
+                    source="reference src/ compiled -O0 with oracle/ref_harness.cpp (oracle/_ref/ref_harness)")
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **arrays)
+        print("%-34s %7.1f KB" % (name, os.path.getsize(path) / 1024))
+
+
+if __name__ == "__main__":
+    main()

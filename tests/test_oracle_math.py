@@ -1,0 +1,198 @@
+"""Pins of the oracle's building blocks by what the mathematics fixes:
+closed forms, exactness on polynomials, convergence orders, symmetries and
+conservation.  Each check is chosen so that a plausible slip (a dropped term,
+a wrong sign or index, swapped operands) makes it fail."""
+import numpy as np
+import pytest
+
+from nuc3d_amd import cases
+from oracle import oracle as O
+
+
+def ext_line(f, w):
+    """cell values f(i) for i in [-w, n+w) -> ext array"""
+    return np.asarray(f, dtype=np.float64)
+
+
+def line_of(fun, n, w, h=1.0):
+    x = (np.arange(-w, n + w) + 0.5) * h
+    return fun(x), x
+
+
+@pytest.mark.parametrize("scheme", [O.WENO5JS, O.WENO5Z, O.CRWENO5])
+@pytest.mark.parametrize("upwind", [1, -1])
+def test_constant_and_linear_data_exact(scheme, upwind):
+    n, w = 24, 3
+    c, _ = line_of(lambda x: 0 * x + 3.25, n, w)
+    np.testing.assert_allclose(O.recon_line(scheme, upwind, c, w), 3.25, rtol=0, atol=1e-14)
+    # linear data: all smoothness indicators equal, every candidate exact -> face value exact
+    f, x = line_of(lambda x: 2.0 * x - 1.0, n, w)
+    faces = np.arange(n + 1) * 1.0
+    np.testing.assert_allclose(O.recon_line(scheme, upwind, f, w), 2.0 * faces - 1.0, rtol=0, atol=1e-12)
+
+
+def test_weno5_linear_weights_closed_form():
+    # equal beta -> omega = (0.1,0.6,0.3) -> (2 f0 - 13 f1 + 47 f2 + 27 f3 - 3 f4)/60 at face 3
+    n, w = 12, 3
+    f = 1.0 + 0.5 * np.arange(n + 2 * w)
+    out = O.recon_line(O.WENO5JS, 1, f, w)
+    m = 6
+    s = f[w + m - 3: w + m + 2]
+    np.testing.assert_allclose(out[m], (2 * s[0] - 13 * s[1] + 47 * s[2] + 27 * s[3] - 3 * s[4]) / 60, rtol=1e-15)
+
+
+@pytest.mark.parametrize("scheme,zquirk,order", [(O.WENO5JS, 1, 4.6), (O.WENO5Z, 0, 4.6), (O.CRWENO5, 1, 4.6)])
+def test_convergence_order_smooth(scheme, zquirk, order):
+    # cell averages of sin -> face point values, 5th order (weights near optimal)
+    errs = []
+    for n in (32, 64, 128):
+        h = 2 * np.pi / n
+        w = 3
+        xc = (np.arange(-w, n + w) + 0.5) * h
+        avg = (np.cos(xc - h / 2) - np.cos(xc + h / 2)) / h  # exact cell averages of sin
+        out = O.recon_line(scheme, 1, avg, w, zquirk=zquirk)
+        xf = np.arange(n + 1) * h
+        errs.append(np.abs(out - np.sin(xf)).max())
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert rates[-1] >= order, (errs, rates)
+
+
+def test_weno5z_quirk_degrades_order():
+    # with the +inner beta2 of weno5z.cpp:80, beta2 no longer measures the
+    # curvature of its stencil and WENO5-Z drops to about third order
+    errs = []
+    for n in (64, 128, 256):
+        h = 2 * np.pi / n
+        xc = (np.arange(-3, n + 3) + 0.5) * h
+        avg = (np.cos(xc - h / 2) - np.cos(xc + h / 2)) / h
+        out = O.recon_line(O.WENO5Z, 1, avg, 3, zquirk=1)
+        errs.append(np.abs(out - np.sin(np.arange(n + 1) * h)).max())
+    rate = np.log2(errs[-2] / errs[-1])
+    assert 2.5 < rate < 3.6, (errs, rate)
+
+
+def test_crweno5_more_accurate_than_weno5():
+    n, w = 64, 3
+    h = 2 * np.pi / n
+    xc = (np.arange(-w, n + w) + 0.5) * h
+    avg = (np.cos(xc - h / 2) - np.cos(xc + h / 2)) / h
+    xf = np.arange(n + 1) * h
+    e_js = np.abs(O.recon_line(O.WENO5JS, 1, avg, w) - np.sin(xf))[w + 1:n - w].max()
+    e_cr = np.abs(O.recon_line(O.CRWENO5, 1, avg, w) - np.sin(xf))[w + 1:n - w].max()
+    assert e_cr < 0.5 * e_js
+
+
+@pytest.mark.parametrize("scheme", [O.WENO5JS, O.CRWENO5])
+def test_minus_is_mirror_of_plus(scheme):
+    rng = np.random.default_rng(0)
+    n, w = 20, 3
+    f = rng.standard_normal(n + 2 * w)
+    plus_rev = O.recon_line(scheme, 1, f[::-1].copy(), w)[::-1]
+    minus = O.recon_line(scheme, -1, f, w)
+    np.testing.assert_allclose(minus, plus_rev, rtol=1e-14, atol=1e-14)
+
+
+def test_weno5z_quirk_only_on_plus_inner_faces():
+    rng = np.random.default_rng(1)
+    n, w = 20, 3
+    f = rng.standard_normal(n + 2 * w)
+    q = O.recon_line(O.WENO5Z, 1, f, w, zquirk=1)
+    t = O.recon_line(O.WENO5Z, 1, f, w, zquirk=0)
+    np.testing.assert_array_equal(q[:w], t[:w])
+    np.testing.assert_array_equal(q[n - w + 1:], t[n - w + 1:])
+    assert np.all(q[w:n - w + 1] != t[w:n - w + 1])
+    np.testing.assert_array_equal(O.recon_line(O.WENO5Z, -1, f, w, zquirk=1), O.recon_line(O.WENO5Z, -1, f, w, zquirk=0))
+
+
+@pytest.mark.parametrize("scheme,deg,w", [(O.CD6, 6, 3), (O.CD8, 8, 4)])
+def test_central_difference_polynomial_exact(scheme, deg, w):
+    n = 20
+    x = np.arange(-w, n + w) * 1.0
+    coef = np.arange(1, deg + 2) / 7.0
+    f = np.polyval(coef, x / n)
+    df = np.polyval(np.polyder(coef), x / n) / n
+    out = O.diff_line(scheme, f, w)
+    np.testing.assert_allclose(out, df[w:n + w], rtol=0, atol=1e-12)
+
+
+def test_exterior_derivative_closure_orders():
+    # one-sided 1st (linear exact), central 2nd (quadratic exact), central 4th (quartic exact)
+    n, w = 16, 3
+    x = np.arange(-w, n + w) * 1.0
+    for deg, k in ((1, 0), (2, 1), (4, 2)):
+        coef = np.ones(deg + 1)
+        f = np.polyval(coef, x / 10.0)
+        out = O.diff_line(O.CD6, f, w, typeL=-1, typeR=-1)
+        df = np.polyval(np.polyder(coef), x / 10.0) / 10.0
+        np.testing.assert_allclose(out[k], df[w + k], rtol=1e-12)
+        np.testing.assert_allclose(out[n - 1 - k], df[w + n - 1 - k], rtol=1e-12)
+
+
+@pytest.mark.parametrize("riemann", [O.LF, O.AUSM, O.AUSMP])
+def test_flux_splitting_sums_to_flux(riemann):
+    p = O.Params(model=O.EULER, mach=1.0, riemann=riemann)
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        rho, u, v, w_, pr = 0.5 + rng.random(), *rng.standard_normal(3), 0.5 + rng.random()
+        jac = 0.3 + rng.random()
+        m = rng.standard_normal(3)
+        fl, fr = O.split_point(p, [rho, u, v, w_, pr], jac, m)
+        U = m @ np.array([u, v, w_])
+        E = pr / 0.4 + 0.5 * rho * (u * u + v * v + w_ * w_)
+        F = np.array([U * rho, U * rho * u + m[0] * pr, U * rho * v + m[1] * pr, U * rho * w_ + m[2] * pr,
+                      U * (E + pr)]) / jac
+        np.testing.assert_allclose(fl + fr, F, rtol=1e-12, atol=1e-12)
+        if riemann == O.LF:
+            lam = abs(U) + np.sqrt(1.4 * pr / rho) * np.linalg.norm(m)
+            np.testing.assert_allclose(fl - fr, lam * np.array([rho, rho * u, rho * v, rho * w_, E]) / jac,
+                                       rtol=1e-12)
+
+
+def test_rk3_amplification_factor():
+    # u' = -R with R = a u: one TVD-RK3 step multiplies u by 1 + z + z^2/2 + z^3/6, z = -a dt
+    a, dt = 1.7, 0.3
+    u0 = np.array([2.0])
+    un = u0.copy()
+    ui = u0.copy()
+    for s in range(3):
+        ui = O.rk3(s, a * ui, un, ui, dt)
+    z = -a * dt
+    np.testing.assert_allclose(ui, u0 * (1 + z + z * z / 2 + z ** 3 / 6), rtol=1e-15)
+
+
+@pytest.mark.parametrize("model", [O.EULER, O.NS])
+@pytest.mark.parametrize("scheme", [O.WENO5JS, O.WENO5Z, O.CRWENO5])
+def test_periodic_rhs_telescopes(model, scheme):
+    blocks = cases.taylor_green(dims=(16, 12, 14), noise=1e-2, seed=8)
+    oc = O.OracleCase(O.Params(model=model, scheme=(scheme,) * 3), blocks)
+    oc.rhs()
+    r = oc.get_rhs(0)
+    for c in range(5):
+        assert abs(r[c].sum()) <= 1e-13 * np.abs(r[c]).sum() + 1e-300
+
+
+@pytest.mark.parametrize("model", [O.EULER, O.NS])
+def test_free_stream_preserved(model):
+    n = 12
+    jac, met, dxyz = cases.cartesian_metrics((n, n, n), (0.3, 0.2, 0.25))
+    one = np.ones((n, n, n))
+    q = cases.conserved_over_j(one, 0.3 * one, -0.2 * one, 0.1 * one, 71.0 * one, 1.4, jac)
+    b = cases.BlockSpec((n, n, n), cases.periodic_faces(), jac, met, dxyz, q)
+    oc = O.OracleCase(O.Params(model=model), [b])
+    oc.rhs()
+    r = oc.get_rhs(0)
+    scale = np.abs(q).max() / 0.2
+    assert np.abs(r).max() <= 1e-13 * scale
+
+
+def test_multiblock_equals_single_block():
+    # cutting a periodic block into slabs must not change anything (halo = periodic wrap)
+    spec = cases.taylor_green(dims=(24, 10, 12), noise=1e-3, seed=9)[0]
+    p = O.Params(model=O.NS)
+    one = O.OracleCase(p, [spec])
+    one.steps(1, 0.5)
+    parts = cases.split_block_x(spec, 3)
+    many = O.OracleCase(p, parts)
+    many.steps(1, 0.5)
+    got = np.concatenate([many.get_q(b) for b in range(3)], axis=3)
+    np.testing.assert_array_equal(got, one.get_q(0))
