@@ -51,7 +51,7 @@ typedef struct {
     double gamma, mach, reynolds, prandtl, t_ref, t_inf, t_wall;
     int model;     /* NUC3D_EULER3D / NUC3D_NAVIERSTOKES3D */
     int riemann;   /* flux splitting: NUC3D_LF / NUC3D_AUSM / NUC3D_AUSMP */
-    int scheme[3]; /* reconstruction per direction (scheme_x_ivs ..) */
+    int scheme[3]; /* reconstruction per direction (scheme_x_ivs ..); may differ */
     int diff[3];   /* viscous difference per direction (scheme_x_vis ..) */
     int width[3];  /* ghost width per direction (bufferSize_xi ..) */
     int zquirk;    /* 1: reproduce the +inner beta2 of weno5z.cpp:80 */
@@ -116,6 +116,10 @@ int nuc3d_get_fixups(nuc3d_case *c, long long *count);
  * class since the last read (arrays of 4) and resets them. */
 int nuc3d_timing(nuc3d_case *c, int enable);
 int nuc3d_timing_read(nuc3d_case *c, double *ms4, int *launches4);
+
+/* 1 when this build implements the CRWENO5 line solves (no counterpart in the
+ * reference: a capability query for callers). */
+int nuc3d_has_crweno5(void);
 
 /* Description of the calling thread's last error. */
 const char *nuc3d_last_error(void);

@@ -26,7 +26,11 @@
                                                    int, unsigned long long *, hipStream_t);                          \
     extern "C" hipError_t nuc3d_launch_fixup_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, nuc3d_kparams,  \
                                                    int, unsigned long long *, hipStream_t);                          \
-    extern "C" hipError_t nuc3d_launch_ghost_fill_##SFX(const nuc3d_block_dev *, int, int, int, hipStream_t);
+    extern "C" hipError_t nuc3d_launch_ghost_fill_##SFX(const nuc3d_block_dev *, int, int, int, hipStream_t);     \
+    extern "C" hipError_t nuc3d_launch_crweno_##SFX(int, int, int, const nuc3d_block_dev *, const int4 *, int, int, \
+                                                    nuc3d_kparams, int, int, int, int, double *,                     \
+                                                    const unsigned long long *, unsigned long long *,                \
+                                                    unsigned long long *, double *, hipStream_t);
 NUC3D_LAUNCHERS(fast)
 NUC3D_LAUNCHERS(exact)
 
@@ -58,12 +62,13 @@ struct Launchers {
     decltype(&nuc3d_launch_eigen_fast) eigen;
     decltype(&nuc3d_launch_fixup_fast) fixup;
     decltype(&nuc3d_launch_ghost_fill_fast) ghost;
+    decltype(&nuc3d_launch_crweno_fast) crweno;
 };
 
 const Launchers kFast = {nuc3d_launch_sweep_fast, nuc3d_launch_visc_fast, nuc3d_launch_eigen_fast,
-                         nuc3d_launch_fixup_fast, nuc3d_launch_ghost_fill_fast};
+                         nuc3d_launch_fixup_fast, nuc3d_launch_ghost_fill_fast, nuc3d_launch_crweno_fast};
 const Launchers kExact = {nuc3d_launch_sweep_exact, nuc3d_launch_visc_exact, nuc3d_launch_eigen_exact,
-                          nuc3d_launch_fixup_exact, nuc3d_launch_ghost_fill_exact};
+                          nuc3d_launch_fixup_exact, nuc3d_launch_ghost_fill_exact, nuc3d_launch_crweno_exact};
 
 constexpr int KSEG = 32;  // planes per k segment of a tile
 
@@ -84,6 +89,10 @@ struct nuc3d_case {
     int ntiles = 0;
     int4 *dsweep[3] = {nullptr, nullptr, nullptr};  // direction-sweep tiles
     int nsweep[3] = {0, 0, 0};
+    int4 *dlines[3] = {nullptr, nullptr, nullptr};  // whole lines (CRWENO5 directions)
+    int nlines[3] = {0, 0, 0};
+    int nmax[3] = {0, 0, 0};
+    double *dscratch = nullptr;                     // Thomas c', d' of the CRWENO5 lines
     double *ddt = nullptr;             // dt of the current step
     unsigned long long *dlam = nullptr;   // [2][nb][3] eigenvalue maxima (bit patterns)
     unsigned long long *dcount = nullptr; // [4] fix-up counter
@@ -194,8 +203,12 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
     }
     for (int d = 0; d < 3; d++) {
         Timed t(c, s, &c->ev[d]);
-        HIPCHK(c->L->sweep(d, c->p.scheme[0], ns, mode, c->db, c->dsweep[d], c->nsweep[d], c->K, stage, bin, bun,
-                           bout, c->ddt, lcur, lnext, c->dcount, s));
+        if (c->p.scheme[d] == NUC3D_CRWENO5)
+            HIPCHK(c->L->crweno(d, ns, mode, c->db, c->dlines[d], c->nlines[d], c->nmax[d], c->K, stage, bin, bun,
+                                bout, c->ddt, lcur, lnext, c->dcount, c->dscratch, s));
+        else
+            HIPCHK(c->L->sweep(d, c->p.scheme[d], ns, mode, c->db, c->dsweep[d], c->nsweep[d], c->K, stage, bin,
+                               bun, bout, c->ddt, lcur, lnext, c->dcount, s));
     }
     return NUC3D_OK;
 }
@@ -206,6 +219,8 @@ extern "C" {
 
 const char *nuc3d_last_error(void) { return g_err.c_str(); }
 
+int nuc3d_has_crweno5(void) { return 1; }
+
 int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d_case **out)
 {
     if (!p || !dims || !out) return fail(NUC3D_ERR_BAD_ARG, "null argument");
@@ -214,8 +229,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(NUC3D_ERR_NO_DEVICE, "no HIP device");
     for (int d = 0; d < 3; d++) {
-        if (p->scheme[d] != p->scheme[0]) return fail(NUC3D_ERR_UNSUPPORTED, "mixed schemes per direction");
-        if (p->scheme[d] != NUC3D_WENO5JS && p->scheme[d] != NUC3D_WENO5Z)
+        if (p->scheme[d] != NUC3D_WENO5JS && p->scheme[d] != NUC3D_WENO5Z && p->scheme[d] != NUC3D_CRWENO5)
             return fail(NUC3D_ERR_UNSUPPORTED, "scheme not available in this build");
         if (p->width[d] != 3) return fail(NUC3D_ERR_UNSUPPORTED, "ghost width must be 3 for WENO5 schemes");
         if (p->diff[d] != NUC3D_CD6)
@@ -311,6 +325,34 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
             for (int l0 = 0; l0 < B.n[0]; l0 += 16)
                 for (int a0 = 0; a0 < B.n[2]; a0 += 15) sw[2].push_back(make_int4(b, a0, l0, j));
     }
+    // whole lines of the CRWENO5 directions and their Thomas scratch
+    size_t scratch = 0;
+    for (int d = 0; d < 3; d++) {
+        if (p->scheme[d] != NUC3D_CRWENO5) continue;
+        std::vector<int4> ln;
+        for (int b = 0; b < nblocks; b++) {
+            const nuc3d_block_dev &B = c->hb[b];
+            const int d2 = d == 0 ? 1 : 0, d3 = d == 2 ? 1 : 2;  // transverse axes: (j,k) / (i,k) / (i,j)
+            for (int a3 = 0; a3 < B.n[d3]; a3++)
+                for (int a2 = 0; a2 < B.n[d2]; a2++) ln.push_back(make_int4(b, a2, a3, 0));
+            c->nmax[d] = std::max(c->nmax[d], B.n[d]);
+        }
+        if (c->nmax[d] > 1024) {
+            nuc3d_case_destroy(c);
+            return fail(NUC3D_ERR_UNSUPPORTED, "CRWENO5 lines longer than 1024 cells");
+        }
+        c->nlines[d] = (int)ln.size();
+        scratch = std::max(scratch, ln.size() * 20 * (size_t)c->nmax[d]);
+        if (hipMalloc(&c->dlines[d], sizeof(int4) * ln.size()) != hipSuccess ||
+            hipMemcpy(c->dlines[d], ln.data(), sizeof(int4) * ln.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            nuc3d_case_destroy(c);
+            return fail(NUC3D_ERR_OOM, "device allocation of CRWENO5 lines failed");
+        }
+    }
+    if (scratch && hipMalloc(&c->dscratch, sizeof(double) * scratch) != hipSuccess) {
+        nuc3d_case_destroy(c);
+        return fail(NUC3D_ERR_OOM, "device allocation of CRWENO5 scratch failed");
+    }
     for (int d = 0; d < 3; d++) {
         c->nsweep[d] = (int)sw[d].size();
         if (hipMalloc(&c->dsweep[d], sizeof(int4) * sw[d].size()) != hipSuccess ||
@@ -349,6 +391,8 @@ int nuc3d_case_destroy(nuc3d_case *c)
     (void)hipFree(c->db);
     (void)hipFree(c->dtiles);
     for (int d = 0; d < 3; d++) (void)hipFree(c->dsweep[d]);
+    for (int d = 0; d < 3; d++) (void)hipFree(c->dlines[d]);
+    (void)hipFree(c->dscratch);
     (void)hipFree(c->ddt);
     (void)hipFree(c->dlam);
     (void)hipFree(c->dcount);

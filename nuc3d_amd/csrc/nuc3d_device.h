@@ -1,10 +1,14 @@
 // nuc3d_device.h -- device-side arithmetic of the per-block RHS path (gfx950).
 //
-// Two variants are compiled from this header (see nuc3d_kernels.hip):
-//   NUC3D_EXACT=1  reference operation order, no FMA contraction
-//                  (the TU is built with -ffp-contract=off); pow(x,2) -> x*x.
-//   NUC3D_EXACT=0  FMA contraction and a one-division form of the WENO
-//                  weights; agrees with the reference to ~1e-13 relative.
+// Two variants are compiled from this header (see nuc3d_kernels.hip); both
+// translation units are built with -ffp-contract=off.
+//   NUC3D_EXACT=1  reference operation order everywhere; pow(x,2) -> x*x.
+//   NUC3D_EXACT=0  inside the WENO reconstructions only: FMA contraction and
+//                  a one-division form of the nonlinear weights.  Flux
+//                  splitting, differences and the RK update keep the
+//                  reference order, because reassociating the large p/J
+//                  terms there moves near-zero momentum components by
+//                  ~1e-12 of their size.
 //
 // Reference formulas: schemes.hpp:17-44 (coefficients), weno5js.cpp:75-108,
 // weno5z.cpp:74-109, crweno5.cpp:91-218, reconstructionBoundaryScheme.cpp:
@@ -41,9 +45,16 @@ __device__ __forceinline__ double sq(double x) { return x * x; }
 
 // ---- smoothness indicators on an oriented stencil f0..f4 ----
 // zq: the +inner beta2 of weno5z.cpp:80 ((f2-2f3+f3)^2 in place of (f2-2f3+f4)^2)
+#if NUC3D_EXACT
+#define NUC3D_WENO_FP
+#else
+#define NUC3D_WENO_FP _Pragma("clang fp contract(fast)")
+#endif
+
 __device__ __forceinline__ void smooth5(double f0, double f1, double f2, double f3, double f4, bool zq,
                                         double &b0, double &b1, double &b2)
 {
+    NUC3D_WENO_FP
     b0 = G0 * sq(f0 - 2.0 * f1 + f2) + G1 * sq(f0 - 4.0 * f1 + 3.0 * f2);
     b1 = G0 * sq(f1 - 2.0 * f2 + f3) + G1 * sq(f1 - f3);
     b2 = G0 * sq(zq ? (f2 - 2.0 * f3 + f3) : (f2 - 2.0 * f3 + f4)) + G1 * sq(3.0 * f2 - 4.0 * f3 + f4);
@@ -52,6 +63,7 @@ __device__ __forceinline__ void smooth5(double f0, double f1, double f2, double 
 __device__ __forceinline__ void cand5(double f0, double f1, double f2, double f3, double f4, double &q0, double &q1,
                                       double &q2)
 {
+    NUC3D_WENO_FP
     q0 = A00 * f0 + A01 * f1 + A02 * f2;
     q1 = A10 * f1 + A11 * f2 + A12 * f3;
     q2 = A20 * f2 + A21 * f3 + A22 * f4;
@@ -60,6 +72,7 @@ __device__ __forceinline__ void cand5(double f0, double f1, double f2, double f3
 // WENO5-JS value at the face of an oriented 5-point stencil.
 __device__ __forceinline__ double weno5js(double f0, double f1, double f2, double f3, double f4)
 {
+    NUC3D_WENO_FP
     double b0, b1, b2, q0, q1, q2;
     smooth5(f0, f1, f2, f3, f4, false, b0, b1, b2);
     cand5(f0, f1, f2, f3, f4, q0, q1, q2);
@@ -82,6 +95,7 @@ __device__ __forceinline__ double weno5js(double f0, double f1, double f2, doubl
 // WENO5-Z value (weno5z.cpp:96-109).
 __device__ __forceinline__ double weno5z(double f0, double f1, double f2, double f3, double f4, bool zq)
 {
+    NUC3D_WENO_FP
     double b0, b1, b2, q0, q1, q2;
     smooth5(f0, f1, f2, f3, f4, zq, b0, b1, b2);
     cand5(f0, f1, f2, f3, f4, q0, q1, q2);
@@ -217,18 +231,10 @@ __device__ __forceinline__ double split_cell(const double *Q, double jac, double
     double lam = fabs(U0) + al;
     if (riemann == RIE_LF) {
         double f0 = U0 * Q[0];
-#if NUC3D_EXACT
         double f1 = U0 * Q[1] + mx * p / jac;
         double f2 = U0 * Q[2] + my * p / jac;
         double f3 = U0 * Q[3] + mz * p / jac;
         double f4 = U0 * (Q[4] + p / jac);
-#else
-        double pj = p / jac;
-        double f1 = U0 * Q[1] + mx * pj;
-        double f2 = U0 * Q[2] + my * pj;
-        double f3 = U0 * Q[3] + mz * pj;
-        double f4 = U0 * (Q[4] + pj);
-#endif
         fp[0] = 0.5 * (f0 + lam * Q[0]);
         fp[1] = 0.5 * (f1 + lam * Q[1]);
         fp[2] = 0.5 * (f2 + lam * Q[2]);

@@ -255,6 +255,91 @@ __device__ __forceinline__ double global_dt(const unsigned long long *lam, int n
     return m * cfl;
 }
 
+// dt of the stage: stage 0 takes cfl * min 1/lambda from the eigenvalue
+// maxima of the stage-0 state (and workgroup 0 publishes it and clears the
+// maxima the coming stage 2 accumulates); stages 1 and 2 reuse it.
+__device__ __forceinline__ double stage_dt(const nuc3d_kparams &K, int stage, double *dtbuf,
+                                           const unsigned long long *lam_cur, unsigned long long *lam_next)
+{
+    if (stage != 0) return dtbuf[0];
+    const double dt = global_dt(lam_cur, K.nblocks, K.cfl);
+    if (blockIdx.x == 0 && threadIdx.x == 0) dtbuf[0] = dt;
+    if (blockIdx.x == 0)
+        for (int e = threadIdx.x; e < 3 * K.nblocks; e += blockDim.x) lam_next[e] = 0ULL;
+    return dt;
+}
+
+// Cell (i,j,k) receives the flux difference dF of direction DIR:
+//   DIR 0: rhs = dF;  DIR 1: rhs += dF;  DIR 2: rhs += dF, then for
+//   Navier-Stokes rhs -= dFv/dxi, dGv/deta, dHv/dzeta in the reference's order
+//   (NaiverStokes3d.cpp:401-513), then the RHS (MODE 0) or the TVD-RK3 stage
+//   into buffer bout with the e<0 fix-up, the ghost push to neighbours and,
+//   at stage 2, the eigenvalue maxima of the new state.
+template <int DIR, bool NS, int MODE>
+__device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
+                                            const nuc3d_kparams &K, const PhysParams &P, int i, int j, int k,
+                                            const double *dF, int stage, int bin, int bun, int bout, double dt,
+                                            double *lmax, unsigned long long &nfix)
+{
+    const long long x = cidx(B, i, j, k);
+    double r[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) r[c] = DIR == 0 ? dF[c] : B.rhs[c][x] + dF[c];
+    if (DIR < 2 || MODE == 0) {
+        if (DIR == 2 && NS) {
+#pragma unroll
+            for (int c = 1; c < 5; c++) {
+                const double gx = line_diff(B, K, B.fv[c - 1] + x, 0, i);
+                const double gy = line_diff(B, K, B.fv[4 + c - 1] + x, 1, j);
+                const double gz = line_diff(B, K, B.fv[8 + c - 1] + x, 2, k);
+                r[c] = ((r[c] - gx) - gy) - gz;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
+        return;
+    }
+    if (NS) {  // reference order: rhs = df + dg + dh - dfv - dgv - dhv
+#pragma unroll
+        for (int c = 1; c < 5; c++) {
+            const double gx = line_diff(B, K, B.fv[c - 1] + x, 0, i);
+            const double gy = line_diff(B, K, B.fv[4 + c - 1] + x, 1, j);
+            const double gz = line_diff(B, K, B.fv[8 + c - 1] + x, 2, k);
+            r[c] = ((r[c] - gx) - gy) - gz;
+        }
+    }
+    double *const *qin = B.q[bin];
+    double qn[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        const double rhs = -r[c];
+        const double u = qin[c][x];
+        if (stage == 0)
+            qn[c] = 1.0 * u + 1.0 * rhs * dt;
+        else if (stage == 1)
+            qn[c] = 3.0 / 4.0 * B.q[bun][c][x] + 1.0 / 4.0 * (rhs * dt + u);
+        else
+            qn[c] = 1.0 / 3.0 * B.q[bun][c][x] + 2.0 / 3.0 * (rhs * dt + u);
+    }
+    const double jac = B.jac[x];
+    if (fixup_energy(qn, jac, P)) nfix++;
+#pragma unroll
+    for (int c = 0; c < 5; c++) B.q[bout][c][x] = qn[c];
+    push_q(blocks, B, bout, i, j, k, qn);
+    if (stage == 2) {
+        double rho, u, v, w, pr, cs;
+        con2prim(qn, jac, P, rho, u, v, w, pr, cs);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double mx = B.met[3 * d][x], my = B.met[3 * d + 1][x], mz = B.met[3 * d + 2][x];
+            const double theta = sqrt(mx * mx + my * my + mz * mz);
+            const double U0 = (mx * u + my * v + mz * w);
+            const double lam = fabs(U0) + cs * theta;
+            lmax[d] = (lmax[d] < lam) ? lam : lmax[d];
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -283,10 +368,6 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     constexpr int NL = 16, NF = 16, NC = NF - 1, H = NUC3D_HALO, NSL = NC + 2 * H;  // 21 slots per line
     __shared__ double S[10][NSL * NL];
     __shared__ double HF[5][NF * NL];
-    // viscous flux components 1..4 of this direction along the lines (fast
-    // variant: each sweep subtracts its own viscous divergence)
-    constexpr bool VLDS = NS && !NUC3D_EXACT;
-    __shared__ double V[VLDS ? 4 : 1][VLDS ? NSL * NL : 1];
 
     const int4 t = tiles[blockIdx.x];
     const nuc3d_block_dev &B = blocks[t.x];
@@ -320,11 +401,6 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
                 S[c][o] = fp[c];
                 S[5 + c][o] = fn[c];
             }
-            if (VLDS) {
-                const long long x = cidx(B, i, j, k);
-#pragma unroll
-                for (int c = 0; c < 4; c++) V[c][o] = B.fv[4 * DIR + c][x];
-            }
         }
     }
     __syncthreads();
@@ -348,83 +424,166 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
 
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
-    double dt = 0.0;
-    if (DIR == 2 && MODE == 1) {
-        if (stage == 0) {
-            dt = global_dt(lam_cur, K.nblocks, K.cfl);
-            if (blockIdx.x == 0 && threadIdx.x == 0) dtbuf[0] = dt;
-            if (blockIdx.x == 0)
-                for (int e = threadIdx.x; e < 3 * K.nblocks; e += blockDim.x) lam_next[e] = 0ULL;
-        } else {
-            dt = dtbuf[0];
-        }
-    }
+    const double dt = (DIR == 2 && MODE == 1) ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
     if (line < nlines && f >= 1 && f <= nc) {
         int i, j, k;
         cell(line, a0 + f - 1, i, j, k);
-        const long long x = cidx(B, i, j, k);
-        double r[5];
+        double dF[5];
 #pragma unroll
-        for (int c = 0; c < 5; c++) {
-            double d = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
-            if (VLDS && c > 0) {
-                const int st = DIR == 0 ? 1 : NL;
-                d -= line_diff_lds(B, K, &V[c - 1][sidx(line, f + 2)], st, DIR, a0 + f - 1);
-            }
-            r[c] = DIR == 0 ? d : B.rhs[c][x] + d;
-        }
-        if (DIR < 2) {
-#pragma unroll
-            for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
-        } else {
-            if (NS && !VLDS) {  // reference order: rhs = df + dg + dh - dfv - dgv - dhv
-#pragma unroll
-                for (int c = 1; c < 5; c++) {
-                    const double gx = line_diff(B, K, B.fv[c - 1] + x, 0, i);
-                    const double gy = line_diff(B, K, B.fv[4 + c - 1] + x, 1, j);
-                    const double gz = line_diff(B, K, B.fv[8 + c - 1] + x, 2, k);
-                    r[c] = ((r[c] - gx) - gy) - gz;
-                }
-            }
-            if (MODE == 0) {
-#pragma unroll
-                for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
-            } else {
-                double qn[5];
-#pragma unroll
-                for (int c = 0; c < 5; c++) {
-                    const double rhs = -r[c];
-                    const double u = qin[c][x];
-                    if (stage == 0)
-                        qn[c] = 1.0 * u + 1.0 * rhs * dt;
-                    else if (stage == 1)
-                        qn[c] = 3.0 / 4.0 * B.q[bun][c][x] + 1.0 / 4.0 * (rhs * dt + u);
-                    else
-                        qn[c] = 1.0 / 3.0 * B.q[bun][c][x] + 2.0 / 3.0 * (rhs * dt + u);
-                }
-                const double jac = B.jac[x];
-                if (fixup_energy(qn, jac, P)) nfix++;
-#pragma unroll
-                for (int c = 0; c < 5; c++) B.q[bout][c][x] = qn[c];
-                push_q(blocks, B, bout, i, j, k, qn);
-                if (stage == 2) {
-                    double rho, u, v, w, pr, cs;
-                    con2prim(qn, jac, P, rho, u, v, w, pr, cs);
-#pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        const double mx = B.met[3 * d][x], my = B.met[3 * d + 1][x], mz = B.met[3 * d + 2][x];
-                        const double theta = sqrt(mx * mx + my * my + mz * mz);
-                        const double U0 = (mx * u + my * v + mz * w);
-                        const double lam = fabs(U0) + cs * theta;
-                        lmax[d] = (lmax[d] < lam) ? lam : lmax[d];
-                    }
-                }
-            }
-        }
+        for (int c = 0; c < 5; c++) dF[c] = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
+        finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix);
     }
     if (DIR == 2 && MODE == 1) {
         if (nfix) atomicAdd(counters, nfix);
         if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * t.x);
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// CRWENO5 along direction DIR (crweno5.cpp:44-401): one wavefront per line.
+// The split fluxes of the whole line and its halo are staged in LDS; faces
+// w..n-w form a tridiagonal system per component and upwind side whose end
+// rows are explicit WENO5-JS and whose inner rows carry the compact weights
+// (crweno5.cpp:130-160, :310-340).  The ten systems (5 components x +/-) are
+// solved side by side by ten lanes with the Thomas recurrence of
+// crweno5.cpp:207-218, coefficients computed on the fly; the solutions for
+// c' and d' live in global scratch.  Faces outside the system use the
+// WENO5-JS face variants or the physical closures.  There is no line-length
+// cap (the reference's stack arrays hold 256 rows, crweno5.cpp:69-72).
+// line: x = block, y, z = transverse coordinates (j,k / i,k / i,j).
+__device__ __forceinline__ void crweno_row(const double *f, int st, bool plus, double &a, double &b, double &c,
+                                           double &d)
+{
+    NUC3D_WENO_FP
+    // f points at cell i of face i+1; plus: cells i-2..i+2, minus: i+3..i-1
+    double f0, f1, f2, f3, f4;
+    if (plus) { f0 = f[-2 * st]; f1 = f[-st]; f2 = f[0]; f3 = f[st]; f4 = f[2 * st]; }
+    else { f0 = f[3 * st]; f1 = f[2 * st]; f2 = f[st]; f3 = f[0]; f4 = f[-st]; }
+    double b0, b1, b2;
+    smooth5(f0, f1, f2, f3, f4, false, b0, b1, b2);
+    const double a0 = R0 / sq(EPS + b0), a1 = R1 / sq(EPS + b1), a2 = R2 / sq(EPS + b2);
+    const double as = a0 + a1 + a2;
+    const double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
+    const double lo = 2.0 / 3.0 * o0 + 1.0 / 3.0 * o1;   // coefficient of the upwind-side face
+    const double hi = 1.0 / 3.0 * o2;                    // coefficient of the downwind-side face
+    b = 1.0 / 3.0 * o0 + 2.0 / 3.0 * (o1 + o2);
+    if (plus) { a = lo; c = hi; }
+    else { c = lo; a = hi; }
+    d = 1.0 / 6.0 * o0 * f1 + (5.0 * (o0 + o1) + o2) * f2 / 6.0 + (o1 + 5.0 * o2) * f3 / 6.0;
+}
+
+template <int DIR, bool NS, int MODE>
+__global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__restrict__ blocks,
+                                                      const int4 *__restrict__ lines, nuc3d_kparams K, int stage,
+                                                      int bin, int bun, int bout, double *dtbuf,
+                                                      const unsigned long long *lam_cur,
+                                                      unsigned long long *lam_next, unsigned long long *counters,
+                                                      double *scratch, int nmax)
+{
+    constexpr int H = NUC3D_HALO;
+    extern __shared__ double lds[];
+    const int4 L = lines[blockIdx.x];
+    const nuc3d_block_dev &B = blocks[L.x];
+    const PhysParams P = phys_of(K);
+    const int n = B.n[DIR], w = K.width[DIR], M = n - 2 * w + 1;
+    const int ns = n + 2 * H;              // slots: cell a at slot a + H
+    double *F = lds;                       // [10][ns]
+    double *HF = lds + 10 * ns;            // [5][n+1]
+    double *cp = scratch + (long long)blockIdx.x * 20 * nmax;  // [10][M] c', then [10][M] d'
+    double *dp = cp + 10 * nmax;
+    double *const *qin = B.q[bin];
+    auto cell = [&](int a, int &i, int &j, int &k) {
+        if (DIR == 0) { i = a; j = L.y; k = L.z; }
+        else if (DIR == 1) { i = L.y; j = a; k = L.z; }
+        else { i = L.y; j = L.z; k = a; }
+    };
+    for (int sl = threadIdx.x; sl < ns; sl += 64) {
+        int i, j, k;
+        cell(sl - H, i, j, k);
+        double fp[5], fn[5];
+        split_any(B, qin, i, j, k, DIR, P, fp, fn);
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            F[c * ns + sl] = fp[c];
+            F[(5 + c) * ns + sl] = fn[c];
+        }
+    }
+    __syncthreads();
+    // explicit faces: 0..w-1 and n-w+1..n (halo face variants or physical closures)
+    for (int e = threadIdx.x; e < 2 * w * 5; e += 64) {
+        const int c = e / (2 * w), q = e % (2 * w);
+        const int m = q < w ? q : n - w + 1 + (q - w);
+        double sp[6], sn[6];
+#pragma unroll
+        for (int o = 0; o < 6; o++) {
+            sp[o] = F[c * ns + m + o];
+            sn[o] = F[(5 + c) * ns + m + o];
+        }
+        HF[c * (n + 1) + m] = face_value<SCH_WENO5JS>(B, K, DIR, m, sp, sn);
+    }
+    // the ten tridiagonal systems of faces w..n-w
+    if (threadIdx.x < 10) {
+        const int sys = threadIdx.x, c = sys % 5;
+        const bool plus = sys < 5;
+        const double *f = F + (plus ? c : 5 + c) * ns;
+        double *C = cp + sys * nmax, *D = dp + sys * nmax;
+        double cprev = 0.0, dprev = 0.0;
+        for (int r = 0; r < M; r++) {
+            const int i = w - 1 + r;       // reference loop index, face i+1
+            double a, b, cc, d;
+            if (r == 0 || r == M - 1) {
+                const double *s6 = f + i + 1;   // slots of cells i-2..i+3 (face i+1 = m, slot m-3+H = m)
+                a = 0.0; b = 1.0; cc = 0.0;
+                d = plus ? weno5js(s6[0], s6[1], s6[2], s6[3], s6[4]) : weno5js(s6[5], s6[4], s6[3], s6[2], s6[1]);
+            } else {
+                crweno_row(f + i + H, 1, plus, a, b, cc, d);
+            }
+            if (r > 0) {
+                cc = cc / (b - cprev * a);
+                d = (d - dprev * a) / (b - cprev * a);
+            }
+            C[r] = cc;
+            D[r] = d;
+            cprev = cc;
+            dprev = d;
+        }
+        // back substitution: out[n-w] = d'[M-1]; out[w+r] = d'[r] - c'[r] out[w+r+1]
+        double x = D[M - 1];
+        D[M - 1] = x;
+        for (int r = M - 2; r >= 0; r--) {
+            x = D[r] - C[r] * x;
+            D[r] = x;
+        }
+    }
+    __syncthreads();
+    // combine F^+ and F^- solutions into the inner face fluxes
+    for (int e = threadIdx.x; e < 5 * M; e += 64) {
+        const int c = e / M, r = e % M;
+        HF[c * (n + 1) + w + r] = dp[c * nmax + r] + dp[(5 + c) * nmax + r];
+    }
+    __syncthreads();
+    double lmax[3] = {0.0, 0.0, 0.0};
+    unsigned long long nfix = 0;
+    const double dt = (DIR == 2 && MODE == 1) ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
+    for (int a = threadIdx.x; a < n; a += 64) {
+        int i, j, k;
+        cell(a, i, j, k);
+        double dF[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) dF[c] = HF[c * (n + 1) + a + 1] - HF[c * (n + 1) + a];
+        finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix);
+    }
+    if (DIR == 2 && MODE == 1) {
+        if (nfix) atomicAdd(counters, nfix);
+        if (stage == 2) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double v = warp_max(lmax[d]);
+                if (threadIdx.x == 0 && v > 0.0)
+                    atomicMax(lam_next + 3 * L.x + d, (unsigned long long)__double_as_longlong(v));
+            }
+        }
     }
 }
 
@@ -465,24 +624,40 @@ __global__ __launch_bounds__(256) void KNAME(k_prim)(const nuc3d_block_dev *__re
 __global__ __launch_bounds__(256) void KNAME(k_visc_flux)(const nuc3d_block_dev *__restrict__ blocks,
                                                           const int4 *__restrict__ tiles, nuc3d_kparams K)
 {
-    // one thread per cell on the tiles of the y sweep: 16 i (lines) x 15 j
+    // one thread per cell on the tiles of the y sweep (16 i x 15 j at one k);
+    // u, v, w, T of the tile and its 3-cell halo in i and j are staged in LDS
+    constexpr int TI = 16, TJ = 15, H = NUC3D_HALO, LI = TI + 2 * H, LJ = TJ + 2 * H;
+    __shared__ double Wp[4][LJ][LI];
     const int4 t = tiles[blockIdx.x];
     const nuc3d_block_dev &B = blocks[t.x];
-    const int i = t.z + (threadIdx.x & 15), j = t.y + (threadIdx.x >> 4), k = t.w;
-    if (i >= B.n[0] || j >= B.n[1] || (threadIdx.x >> 4) >= 15) return;
+    const int i0 = t.z, j0 = t.y, k = t.w;
+    const int ni = min(TI, B.n[0] - i0), nj = min(TJ, B.n[1] - j0);
+    for (int e = threadIdx.x; e < LI * LJ; e += 256) {
+        const int li = e % LI, lj = e / LI;
+        const bool inx = li >= H && li < H + ni, iny = lj >= H && lj < H + nj;
+        if ((inx && lj < nj + 2 * H) || (iny && li < ni + 2 * H)) {
+            const long long y = cidx(B, i0 + li - H, j0 + lj - H, k);
+#pragma unroll
+            for (int v = 0; v < 4; v++) Wp[v][lj][li] = B.prim[v][y];
+        }
+    }
+    __syncthreads();
+    const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
+    if (ti >= ni || tj >= nj) return;
+    const int i = i0 + ti, j = j0 + tj;
     const double S = K.sref;
     const long long x = cidx(B, i, j, k);
     double g[4][3];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const double a = line_diff(B, K, B.prim[v] + x, 0, i);
-        const double bb = line_diff(B, K, B.prim[v] + x, 1, j);
+        const double a = line_diff_lds(B, K, &Wp[v][tj + H][ti + H], 1, 0, i);
+        const double bb = line_diff_lds(B, K, &Wp[v][tj + H][ti + H], LI, 1, j);
         const double cc = line_diff(B, K, B.prim[v] + x, 2, k);
         g[v][0] = a * B.met[0][x] + bb * B.met[3][x] + cc * B.met[6][x];
         g[v][1] = a * B.met[1][x] + bb * B.met[4][x] + cc * B.met[7][x];
         g[v][2] = a * B.met[2][x] + bb * B.met[5][x] + cc * B.met[8][x];
     }
-    const double T0 = B.prim[3][x];
+    const double T0 = Wp[3][tj + H][ti + H];
     const double miu0 = (1.0 + S) / (T0 + S) * pow(T0, 1.5) / K.reynolds;  // physicsModel.cpp:226
     const double coeff0 = miu0 / ((K.gamma - 1.0) * K.mach * K.mach * K.prandtl);
     const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
@@ -496,7 +671,7 @@ __global__ __launch_bounds__(256) void KNAME(k_visc_flux)(const nuc3d_block_dev 
     const double tyz = miu0 * (vz + wy - 2.0 / 3.0 * grad);
     const double tzz = miu0 * (2.0 * wz - 2.0 / 3.0 * grad);
     const double ttx = coeff0 * g[3][0], tty = coeff0 * g[3][1], ttz = coeff0 * g[3][2];
-    const double u = B.prim[0][x], vv = B.prim[1][x], w = B.prim[2][x];
+    const double u = Wp[0][tj + H][ti + H], vv = Wp[1][tj + H][ti + H], w = Wp[2][tj + H][ti + H];
     const double fv[4] = {txx, txy, txz, u * txx + vv * txy + w * txz + ttx};
     const double gv[4] = {txy, tyy, tyz, u * txy + vv * tyy + w * tyz + tty};
     const double hv[4] = {txz, tyz, tzz, u * txz + vv * tyz + w * tzz + ttz};
@@ -662,6 +837,40 @@ extern "C" hipError_t KNAME(nuc3d_launch_sweep)(int dir, int scheme, int ns, int
     }
 #undef NUC3D_A
     return hipGetLastError();
+}
+
+template <int DIR, bool NS, int MODE>
+static hipError_t launch_crweno(const nuc3d_block_dev *blocks, const int4 *lines, int nlines, int nmax,
+                                nuc3d_kparams K, int stage, int bin, int bun, int bout, double *dtbuf,
+                                const unsigned long long *lam_cur, unsigned long long *lam_next,
+                                unsigned long long *counters, double *scratch, hipStream_t st)
+{
+    const size_t lds = sizeof(double) * (10 * (nmax + 2 * NUC3D_HALO) + 5 * (nmax + 1));
+    static bool attr_set = false;
+    if (!attr_set && lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&KNAME(k_crweno)<DIR, NS, MODE>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((KNAME(k_crweno)<DIR, NS, MODE>), dim3(nlines), dim3(64), lds, st, blocks, lines, K, stage,
+                       bin, bun, bout, dtbuf, lam_cur, lam_next, counters, scratch, nmax);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t KNAME(nuc3d_launch_crweno)(int dir, int ns, int mode, const nuc3d_block_dev *blocks,
+                                                 const int4 *lines, int nlines, int nmax, nuc3d_kparams K,
+                                                 int stage, int bin, int bun, int bout, double *dtbuf,
+                                                 const unsigned long long *lam_cur, unsigned long long *lam_next,
+                                                 unsigned long long *counters, double *scratch, hipStream_t st)
+{
+    if (nmax > 1024) return hipErrorInvalidValue;
+#define NUC3D_A blocks, lines, nlines, nmax, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, scratch, st
+    if (dir == 0) return ns ? launch_crweno<0, true, 0>(NUC3D_A) : launch_crweno<0, false, 0>(NUC3D_A);
+    if (dir == 1) return ns ? launch_crweno<1, true, 0>(NUC3D_A) : launch_crweno<1, false, 0>(NUC3D_A);
+    if (mode == 0) return ns ? launch_crweno<2, true, 0>(NUC3D_A) : launch_crweno<2, false, 0>(NUC3D_A);
+    return ns ? launch_crweno<2, true, 1>(NUC3D_A) : launch_crweno<2, false, 1>(NUC3D_A);
+#undef NUC3D_A
 }
 
 extern "C" hipError_t KNAME(nuc3d_launch_visc)(const nuc3d_block_dev *blocks, int nblocks, const int4 *ytiles,

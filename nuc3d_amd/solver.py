@@ -102,6 +102,8 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
+    L.nuc3d_has_crweno5.argtypes = []
+    L.nuc3d_has_crweno5.restype = ctypes.c_int
     L.nuc3d_last_error.argtypes = []
     L.nuc3d_last_error.restype = ctypes.c_char_p
     _LIB = L

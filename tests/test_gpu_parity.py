@@ -134,3 +134,46 @@ def test_conservation_periodic_64():
         print("  comp %d drift %.3e" % (c, abs(s1 - s0) / scale), flush=True)
         assert abs(s1 - s0) <= 1e-13 * scale
     g.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+@pytest.mark.parametrize("model", [solver.EULER3D, solver.NAVIERSTOKES3D])
+def test_crweno5_rhs_tgv(exact, model):
+    print(f"case: crweno5 rhs tgv 20x18x22 model={model} exact={exact}", flush=True)
+    blocks = cases.taylor_green(dims=(20, 18, 22), noise=1e-3, seed=31)
+    sp, op = both_params(model=model, scheme=(solver.CRWENO5,) * 3, exact=exact)
+    g, o = run_rhs(sp, op, blocks)
+    err = relerr(g.get_rhs(0), o.get_rhs(0))
+    print("  rel err %.3e" % err, flush=True)
+    assert err <= TOL
+    g.close()
+
+
+def test_crweno5_long_lines_shu_osher():
+    """x-lines of 300 cells: past the 261-cell limit of the reference's stack
+    arrays (crweno5.cpp:69-72); pinned to the oracle, which has no cap."""
+    print("case: crweno5 shu-osher 300x8x8 euler 2 steps", flush=True)
+    blocks = cases.shu_osher(300, 8, 8, seed=4)
+    sp, op = both_params(model=solver.EULER3D, mach=1.0, scheme=(solver.CRWENO5,) * 3)
+    g = solver.Case(sp, blocks)
+    o = O.OracleCase(op, blocks)
+    g.steps(2, 0.5)
+    o.steps(2, 0.5)
+    err = relerr(g.get_q(0), o.get_q(0))
+    print("  rel err %.3e" % err, flush=True)
+    assert err <= TOL
+    g.close()
+
+
+def test_mixed_schemes_per_direction():
+    print("case: x crweno5, y weno5z, z weno5js, NS, 2 steps", flush=True)
+    blocks = cases.taylor_green(dims=(18, 16, 14), noise=1e-3, seed=32)
+    sp, op = both_params(scheme=(solver.CRWENO5, solver.WENO5Z, solver.WENO5JS))
+    g = solver.Case(sp, blocks)
+    o = O.OracleCase(op, blocks)
+    g.steps(2, 0.5)
+    o.steps(2, 0.5)
+    err = relerr(g.get_q(0), o.get_q(0))
+    print("  rel err %.3e" % err, flush=True)
+    assert err <= TOL
+    g.close()
