@@ -33,23 +33,28 @@ HBM_PEAK_GBS = 8000.0
 ALG_BYTES_PER_STEP = 560.0
 
 
-def kernel_alg_bytes(cls, ns):
-    """Compulsory bytes per cell of one launch of a kernel class, averaged over
-    the three stages: every input field read once and every output written once.
-      x sweep:  Q 40 + J 8 + xi metrics 24 + RHS write 40                 = 112
-      y sweep:  Q 40 + J 8 + eta metrics 24 + RHS read/write 80            = 152
-      z sweep:  Q 40 + J 8 + zeta metrics 24 + RHS read 40 + Q_out 40
-                + Q^n 40 (stages 1, 2) + xi,eta metrics 48 (stage 2 dt)
-                + viscous fluxes 96 (Navier-Stokes)
-      viscous:  Q 40 + J 8 + metrics 72 + viscous fluxes 96 (write)        = 216
+def kernel_alg_bytes(cls, ns, stage_avg=True):
+    """Compulsory HBM bytes per cell of one launch of a kernel class, averaged
+    over the three RK stages (every input field read once, every output
+    written once; ghost layers and halo re-reads not counted):
+      x sweep:     Q 40 + J 8 + xi metrics 24 + RHS write 40                  = 112
+      y sweep:     Q 40 + J 8 + eta metrics 24 + RHS read and write 80         = 152
+      z sweep:     Q 40 + J 8 + zeta metrics 24 + RHS read and write 80        = 152
+                   (Euler: + Q^n 40 at stages 1, 2, Q_out write instead of RHS,
+                    xi/eta metrics 48 at stage 2)
+      viscous flux: Q 40 + J 8 + nine metrics 72 + fv write 96                 = 216
+      viscous div:  fv read 96 + RHS 40 + Q 40 + J 8 + Q_out 40
+                    + Q^n 40 (stages 1, 2) + nine metrics 72 (stage 2)
     """
     if cls == "x_sweep":
         return 112.0
     if cls == "y_sweep":
         return 152.0
-    if cls == "z_sweep_update":
-        return 152.0 + (0 + 40 + 40) / 3.0 + 48.0 / 3.0 + (96.0 if ns else 0.0)
-    return 216.0
+    if cls == "z_sweep":
+        return 152.0 if ns else 112.0 + (0 + 40 + 40) / 3.0 + 40.0 + 48.0 / 3.0
+    if cls == "viscous_flux":
+        return 216.0
+    return 96.0 + 40.0 + 40.0 + 8.0 + 40.0 + (0 + 40 + 40) / 3.0 + 72.0 / 3.0
 
 
 def log(msg):

@@ -20,8 +20,12 @@
                                                    nuc3d_kparams, int, int, int, int, double *,                      \
                                                    const unsigned long long *, unsigned long long *,                 \
                                                    unsigned long long *, hipStream_t);                               \
-    extern "C" hipError_t nuc3d_launch_visc_##SFX(const nuc3d_block_dev *, int, const int4 *, int,                   \
-                                                  nuc3d_kparams, int, hipStream_t);                                  \
+    extern "C" hipError_t nuc3d_launch_visc_flux_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int,           \
+                                                       nuc3d_kparams, int, hipStream_t);                             \
+    extern "C" hipError_t nuc3d_launch_visc_div_##SFX(int, const nuc3d_block_dev *, const nuc3d_tile *, int,        \
+                                                      nuc3d_kparams, int, int, int, int, double *,                   \
+                                                      const unsigned long long *, unsigned long long *,              \
+                                                      unsigned long long *, hipStream_t);                            \
     extern "C" hipError_t nuc3d_launch_eigen_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, nuc3d_kparams,  \
                                                    int, unsigned long long *, hipStream_t);                          \
     extern "C" hipError_t nuc3d_launch_fixup_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, nuc3d_kparams,  \
@@ -58,16 +62,17 @@ int hipfail(hipError_t e, const char *what)
 
 struct Launchers {
     decltype(&nuc3d_launch_sweep_fast) sweep;
-    decltype(&nuc3d_launch_visc_fast) visc;
+    decltype(&nuc3d_launch_visc_flux_fast) vflux;
+    decltype(&nuc3d_launch_visc_div_fast) vdiv;
     decltype(&nuc3d_launch_eigen_fast) eigen;
     decltype(&nuc3d_launch_fixup_fast) fixup;
     decltype(&nuc3d_launch_ghost_fill_fast) ghost;
     decltype(&nuc3d_launch_crweno_fast) crweno;
 };
 
-const Launchers kFast = {nuc3d_launch_sweep_fast, nuc3d_launch_visc_fast, nuc3d_launch_eigen_fast,
+const Launchers kFast = {nuc3d_launch_sweep_fast, nuc3d_launch_visc_flux_fast, nuc3d_launch_visc_div_fast, nuc3d_launch_eigen_fast,
                          nuc3d_launch_fixup_fast, nuc3d_launch_ghost_fill_fast, nuc3d_launch_crweno_fast};
-const Launchers kExact = {nuc3d_launch_sweep_exact, nuc3d_launch_visc_exact, nuc3d_launch_eigen_exact,
+const Launchers kExact = {nuc3d_launch_sweep_exact, nuc3d_launch_visc_flux_exact, nuc3d_launch_visc_div_exact, nuc3d_launch_eigen_exact,
                           nuc3d_launch_fixup_exact, nuc3d_launch_ghost_fill_exact, nuc3d_launch_crweno_exact};
 
 constexpr int KSEG = 32;  // planes per k segment of a tile
@@ -104,7 +109,8 @@ struct nuc3d_case {
     // kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];  // x sweep, y sweep, z sweep + update, viscous
+    // x sweep, y sweep, z sweep, viscous fluxes, viscous divergence (+ update)
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[5];
 };
 
 namespace {
@@ -199,7 +205,7 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
     const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
     if (ns) {
         Timed t(c, s, &c->ev[3]);
-        HIPCHK(c->L->visc(c->db, c->nb, c->dsweep[1], c->nsweep[1], c->K, bin, s));
+        HIPCHK(c->L->vflux(c->db, c->dtiles, c->ntiles, c->K, bin, s));
     }
     for (int d = 0; d < 3; d++) {
         Timed t(c, s, &c->ev[d]);
@@ -209,6 +215,11 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
         else
             HIPCHK(c->L->sweep(d, c->p.scheme[d], ns, mode, c->db, c->dsweep[d], c->nsweep[d], c->K, stage, bin,
                                bun, bout, c->ddt, lcur, lnext, c->dcount, s));
+    }
+    if (ns) {
+        Timed t(c, s, &c->ev[4]);
+        HIPCHK(c->L->vdiv(mode, c->db, c->dtiles, c->ntiles, c->K, stage, bin, bun, bout, c->ddt, lcur, lnext,
+                          c->dcount, s));
     }
     return NUC3D_OK;
 }
@@ -581,11 +592,11 @@ int nuc3d_timing(nuc3d_case *c, int enable)
     return NUC3D_OK;
 }
 
-int nuc3d_timing_read(nuc3d_case *c, double *ms4, int *launches4)
+int nuc3d_timing_read(nuc3d_case *c, double *ms5, int *launches5)
 {
     if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
     HIPCHK(hipDeviceSynchronize());
-    for (int l = 0; l < 4; l++) {
+    for (int l = 0; l < 5; l++) {
         double acc = 0.0;
         int cnt = 0;
         for (auto &pr : c->ev[l]) {
@@ -597,8 +608,8 @@ int nuc3d_timing_read(nuc3d_case *c, double *ms4, int *launches4)
             c->ev_pool.push_back(pr.second);
         }
         c->ev[l].clear();
-        if (ms4) ms4[l] = acc;
-        if (launches4) launches4[l] = cnt;
+        if (ms5) ms5[l] = acc;
+        if (launches5) launches5[l] = cnt;
     }
     return NUC3D_OK;
 }

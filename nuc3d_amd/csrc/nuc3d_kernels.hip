@@ -269,45 +269,14 @@ __device__ __forceinline__ double stage_dt(const nuc3d_kparams &K, int stage, do
     return dt;
 }
 
-// Cell (i,j,k) receives the flux difference dF of direction DIR:
-//   DIR 0: rhs = dF;  DIR 1: rhs += dF;  DIR 2: rhs += dF, then for
-//   Navier-Stokes rhs -= dFv/dxi, dGv/deta, dHv/dzeta in the reference's order
-//   (NaiverStokes3d.cpp:401-513), then the RHS (MODE 0) or the TVD-RK3 stage
-//   into buffer bout with the e<0 fix-up, the ghost push to neighbours and,
-//   at stage 2, the eigenvalue maxima of the new state.
-template <int DIR, bool NS, int MODE>
-__device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
-                                            const nuc3d_kparams &K, const PhysParams &P, int i, int j, int k,
-                                            const double *dF, int stage, int bin, int bun, int bout, double dt,
-                                            double *lmax, unsigned long long &nfix)
+// TVD-RK3 stage of cell x from RHS r (TVD-RK3.cpp:37-137), e<0 fix-up
+// (physicsModel.cpp:322-332), ghost push, and the eigenvalue maxima of the new
+// state at stage 2 (physicsModel.cpp:557-597).
+__device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
+                                          const PhysParams &P, int i, int j, int k, long long x, const double *r,
+                                          int stage, int bin, int bun, int bout, double dt, double *lmax,
+                                          unsigned long long &nfix)
 {
-    const long long x = cidx(B, i, j, k);
-    double r[5];
-#pragma unroll
-    for (int c = 0; c < 5; c++) r[c] = DIR == 0 ? dF[c] : B.rhs[c][x] + dF[c];
-    if (DIR < 2 || MODE == 0) {
-        if (DIR == 2 && NS) {
-#pragma unroll
-            for (int c = 1; c < 5; c++) {
-                const double gx = line_diff(B, K, B.fv[c - 1] + x, 0, i);
-                const double gy = line_diff(B, K, B.fv[4 + c - 1] + x, 1, j);
-                const double gz = line_diff(B, K, B.fv[8 + c - 1] + x, 2, k);
-                r[c] = ((r[c] - gx) - gy) - gz;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
-        return;
-    }
-    if (NS) {  // reference order: rhs = df + dg + dh - dfv - dgv - dhv
-#pragma unroll
-        for (int c = 1; c < 5; c++) {
-            const double gx = line_diff(B, K, B.fv[c - 1] + x, 0, i);
-            const double gy = line_diff(B, K, B.fv[4 + c - 1] + x, 1, j);
-            const double gz = line_diff(B, K, B.fv[8 + c - 1] + x, 2, k);
-            r[c] = ((r[c] - gx) - gy) - gz;
-        }
-    }
     double *const *qin = B.q[bin];
     double qn[5];
 #pragma unroll
@@ -338,6 +307,28 @@ __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const
             lmax[d] = (lmax[d] < lam) ? lam : lmax[d];
         }
     }
+}
+
+// Cell (i,j,k) receives the flux difference dF of direction DIR:
+//   DIR 0: rhs = dF;  DIR 1, 2: rhs += dF (euler3d.cpp:383-422).  For Euler
+//   the z sweep then writes the RHS (MODE 0) or applies the RK stage; for
+//   Navier-Stokes k_visc_div finishes the cell.
+template <int DIR, bool NS, int MODE>
+__device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
+                                            const nuc3d_kparams &K, const PhysParams &P, int i, int j, int k,
+                                            const double *dF, int stage, int bin, int bun, int bout, double dt,
+                                            double *lmax, unsigned long long &nfix)
+{
+    const long long x = cidx(B, i, j, k);
+    double r[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) r[c] = DIR == 0 ? dF[c] : B.rhs[c][x] + dF[c];
+    if (DIR < 2 || NS || MODE == 0) {  // Navier-Stokes: k_visc_div completes the cell
+#pragma unroll
+        for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
+        return;
+    }
+    rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix);
 }
 
 }  // namespace
@@ -588,111 +579,292 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 }
 
 // ---------------------------------------------------------------------------
-// Navier-Stokes viscous fluxes (NaiverStokes3d.cpp:63-326): gradients of u, v,
-// w, T by cd6 along each direction, Sutherland viscosity, stresses and heat
-// flux, contravariant viscous fluxes divided by J.  Writes fv of the interior
-// and pushes them into the neighbours' ghost layers.
-__global__ __launch_bounds__(256) void KNAME(k_prim)(const nuc3d_block_dev *__restrict__ blocks, int nblocks,
-                                                     nuc3d_kparams K, int bin)
+// Navier-Stokes viscous terms in two k-marching passes over 16x16 column
+// tiles (NaiverStokes3d.cpp:51-513).
+//
+// k_visc_flux: u, v, w, T (con2prim, physicsModel.cpp:313-353) of each plane
+// with a 3-cell halo in i and j go to LDS for the xi and eta derivatives; a
+// seven-plane register ring of the thread's own column gives the zeta
+// derivative (cd6 or the exterior closure, NaiverStokes3d.cpp:86-168).  Three
+// planes later the cell's viscous fluxes (Sutherland viscosity, stresses,
+// heat flux, contravariant fluxes over J: NaiverStokes3d.cpp:170-326) are
+// written and pushed to the neighbours' ghost layers.
+//
+// k_visc_div: after the three inviscid sweeps, rhs -= dFv/dxi, dGv/deta,
+// dHv/dzeta in the reference's order (NaiverStokes3d.cpp:401-513), fv_xi and
+// fv_eta from LDS plane tiles and fv_zeta from a register ring; then the RHS
+// (MODE 0) or the TVD-RK3 stage (MODE 1).
+
+__device__ __forceinline__ void prim4(const nuc3d_block_dev &B, double *const *qb, long long x, const PhysParams &P,
+                                      double *w4)
 {
-    const int b = blockIdx.y;
-    if (b >= nblocks) return;
-    const nuc3d_block_dev &B = blocks[b];
-    const PhysParams P = phys_of(K);
-    const int G = B.G;
-    const long long px = B.n[0] + 2 * G, py = B.n[1] + 2 * G, pz = B.n[2] + 2 * G;
-    const long long tot = px * py * pz;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(e % px) - G, j = (int)((e / px) % py) - G, k = (int)(e / (px * py)) - G;
-        const int outside = (i < 0 || i >= B.n[0]) + (j < 0 || j >= B.n[1]) + (k < 0 || k >= B.n[2]);
-        if (outside > 1) continue;  // corner/edge ghosts are never read
-        const long long x = cidx(B, i, j, k);
-        double Q[5];
-#pragma unroll
-        for (int c = 0; c < 5; c++) Q[c] = B.q[bin][c][x];
-        const double jac = B.jac[x];
-        if (jac == 0.0) continue;
-        double rho, u, v, w, p, cs;
-        con2prim(Q, jac, P, rho, u, v, w, p, cs);
-        B.prim[0][x] = u;
-        B.prim[1][x] = v;
-        B.prim[2][x] = w;
-        B.prim[3][x] = P.gamma * P.mach * P.mach * p / rho;  // physicsModel.cpp:1029
+    const double jac = B.jac[x];
+    if (jac == 0.0) {  // ghost beyond a physical face: never read
+        w4[0] = w4[1] = w4[2] = w4[3] = 0.0;
+        return;
     }
+    double Q[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) Q[c] = qb[c][x];
+    double rho, u, v, w, p, cs;
+    con2prim(Q, jac, P, rho, u, v, w, p, cs);
+    w4[0] = u;
+    w4[1] = v;
+    w4[2] = w;
+    w4[3] = P.gamma * P.mach * P.mach * p / rho;  // physicsModel.cpp:1029
 }
 
 __global__ __launch_bounds__(256) void KNAME(k_visc_flux)(const nuc3d_block_dev *__restrict__ blocks,
-                                                          const int4 *__restrict__ tiles, nuc3d_kparams K)
+                                                          const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K,
+                                                          int bin)
 {
-    // one thread per cell on the tiles of the y sweep (16 i x 15 j at one k);
-    // u, v, w, T of the tile and its 3-cell halo in i and j are staged in LDS
-    constexpr int TI = 16, TJ = 15, H = NUC3D_HALO, LI = TI + 2 * H, LJ = TJ + 2 * H;
-    __shared__ double Wp[4][LJ][LI];
-    const int4 t = tiles[blockIdx.x];
-    const nuc3d_block_dev &B = blocks[t.x];
-    const int i0 = t.z, j0 = t.y, k = t.w;
-    const int ni = min(TI, B.n[0] - i0), nj = min(TJ, B.n[1] - j0);
-    for (int e = threadIdx.x; e < LI * LJ; e += 256) {
-        const int li = e % LI, lj = e / LI;
-        const bool inx = li >= H && li < H + ni, iny = lj >= H && lj < H + nj;
-        if ((inx && lj < nj + 2 * H) || (iny && li < ni + 2 * H)) {
-            const long long y = cidx(B, i0 + li - H, j0 + lj - H, k);
-#pragma unroll
-            for (int v = 0; v < 4; v++) Wp[v][lj][li] = B.prim[v][y];
-        }
-    }
-    __syncthreads();
-    const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
-    if (ti >= ni || tj >= nj) return;
-    const int i = i0 + ti, j = j0 + tj;
+    constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
+    __shared__ double Wp[4][L][L];
+    const nuc3d_tile tl = tiles[blockIdx.x];
+    const nuc3d_block_dev &B = blocks[tl.b];
+    const PhysParams P = phys_of(K);
+    const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
+    const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
+    const bool incell = ti < tw && tj < th;
+    const int i = tl.i0 + ti, j = tl.j0 + tj;
+    double *const *qin = B.q[bin];
     const double S = K.sref;
-    const long long x = cidx(B, i, j, k);
-    double g[4][3];
+
+    double zr[7][4];   // u,v,w,T of the own column, planes p-6..p
+    double dxy[3][8];  // xi and eta derivatives of u,v,w,T, planes p-3..p-1
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const double a = line_diff_lds(B, K, &Wp[v][tj + H][ti + H], 1, 0, i);
-        const double bb = line_diff_lds(B, K, &Wp[v][tj + H][ti + H], LI, 1, j);
-        const double cc = line_diff(B, K, B.prim[v] + x, 2, k);
-        g[v][0] = a * B.met[0][x] + bb * B.met[3][x] + cc * B.met[6][x];
-        g[v][1] = a * B.met[1][x] + bb * B.met[4][x] + cc * B.met[7][x];
-        g[v][2] = a * B.met[2][x] + bb * B.met[5][x] + cc * B.met[8][x];
-    }
-    const double T0 = Wp[3][tj + H][ti + H];
-    const double miu0 = (1.0 + S) / (T0 + S) * pow(T0, 1.5) / K.reynolds;  // physicsModel.cpp:226
-    const double coeff0 = miu0 / ((K.gamma - 1.0) * K.mach * K.mach * K.prandtl);
-    const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
-    const double vx = g[1][0], vy = g[1][1], vz = g[1][2];
-    const double wx = g[2][0], wy = g[2][1], wz = g[2][2];
-    const double grad = ux + vy + wz;
-    const double txx = miu0 * (2.0 * ux - 2.0 / 3.0 * grad);
-    const double txy = miu0 * (uy + vx - 2.0 / 3.0 * grad);
-    const double txz = miu0 * (uz + wx - 2.0 / 3.0 * grad);
-    const double tyy = miu0 * (2.0 * vy - 2.0 / 3.0 * grad);
-    const double tyz = miu0 * (vz + wy - 2.0 / 3.0 * grad);
-    const double tzz = miu0 * (2.0 * wz - 2.0 / 3.0 * grad);
-    const double ttx = coeff0 * g[3][0], tty = coeff0 * g[3][1], ttz = coeff0 * g[3][2];
-    const double u = Wp[0][tj + H][ti + H], vv = Wp[1][tj + H][ti + H], w = Wp[2][tj + H][ti + H];
-    const double fv[4] = {txx, txy, txz, u * txx + vv * txy + w * txz + ttx};
-    const double gv[4] = {txy, tyy, tyz, u * txy + vv * tyy + w * tyz + tty};
-    const double hv[4] = {txz, tyz, tzz, u * txz + vv * tyz + w * tzz + ttz};
-    const double jac = B.jac[x];
-    double out[12];
+    for (int a = 0; a < 7; a++)
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double mx = B.met[3 * d][x], my = B.met[3 * d + 1][x], mz = B.met[3 * d + 2][x];
+        for (int v = 0; v < 4; v++) zr[a][v] = 0.0;
 #pragma unroll
-        for (int c = 0; c < 4; c++) out[4 * d + c] = (mx * fv[c] + my * gv[c] + mz * hv[c]) / jac;
-    }
+    for (int a = 0; a < 3; a++)
 #pragma unroll
-    for (int e = 0; e < 12; e++) B.fv[e][x] = out[e];
+        for (int v = 0; v < 8; v++) dxy[a][v] = 0.0;
+
+    const int kb = tl.k0, ke = tl.k0 + tl.kn;
+    for (int p = kb - H; p < ke + H; p++) {
+        __syncthreads();
 #pragma unroll
-    for (int f = 0; f < 6; f++) {
-        int nb;
-        long long y;
-        if (ghost_target(blocks, B, f, i, j, k, nb, y)) {
+        for (int a = 0; a < 6; a++)
 #pragma unroll
-            for (int e = 0; e < 12; e++) blocks[nb].fv[e][y] = out[e];
+            for (int v = 0; v < 4; v++) zr[a][v] = zr[a + 1][v];
+        if (incell) prim4(B, qin, cidx(B, i, j, p), P, zr[6]);
+        const bool axy = p >= kb && p < ke;
+        if (axy) {
+            if (incell)
+#pragma unroll
+                for (int v = 0; v < 4; v++) Wp[v][tj + H][ti + H] = zr[6][v];
+            const int h = threadIdx.x;
+            if (h < 2 * T * 2 * H) {
+                const int dir = h / (T * 2 * H);
+                const int r = (h % (T * 2 * H)) / (2 * H), sl = h % (2 * H);
+                const int lim = dir == 0 ? th : tw, ext = dir == 0 ? tw : th;
+                if (r < lim) {
+                    const int off = sl < H ? sl - H : ext + (sl - H);
+                    double w4[4];
+                    if (dir == 0) {
+                        prim4(B, qin, cidx(B, tl.i0 + off, tl.j0 + r, p), P, w4);
+#pragma unroll
+                        for (int v = 0; v < 4; v++) Wp[v][r + H][off + H] = w4[v];
+                    } else {
+                        prim4(B, qin, cidx(B, tl.i0 + r, tl.j0 + off, p), P, w4);
+#pragma unroll
+                        for (int v = 0; v < 4; v++) Wp[v][off + H][r + H] = w4[v];
+                    }
+                }
+            }
         }
+        __syncthreads();
+        double dnew[8];
+#pragma unroll
+        for (int v = 0; v < 8; v++) dnew[v] = 0.0;
+        if (axy && incell) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                dnew[v] = line_diff_lds(B, K, &Wp[v][tj + H][ti + H], 1, 0, i);
+                dnew[4 + v] = line_diff_lds(B, K, &Wp[v][tj + H][ti + H], L, 1, j);
+            }
+        }
+        const int kc = p - H;
+        if (incell && kc >= kb && kc < ke) {
+            const long long x = cidx(B, i, j, kc);
+            double g[4][3];
+            const double xi_x = B.met[0][x], xi_y = B.met[1][x], xi_z = B.met[2][x];
+            const double eta_x = B.met[3][x], eta_y = B.met[4][x], eta_z = B.met[5][x];
+            const double zeta_x = B.met[6][x], zeta_y = B.met[7][x], zeta_z = B.met[8][x];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                double col[7];
+#pragma unroll
+                for (int a = 0; a < 7; a++) col[a] = zr[a][v];
+                const double a = dxy[0][v], bb = dxy[0][4 + v];
+                const double cc = line_diff_lds(B, K, &col[3], 1, 2, kc);
+                g[v][0] = a * xi_x + bb * eta_x + cc * zeta_x;
+                g[v][1] = a * xi_y + bb * eta_y + cc * zeta_y;
+                g[v][2] = a * xi_z + bb * eta_z + cc * zeta_z;
+            }
+            const double T0 = zr[3][3];
+            const double miu0 = (1.0 + S) / (T0 + S) * pow(T0, 1.5) / K.reynolds;  // physicsModel.cpp:226
+            const double coeff0 = miu0 / ((K.gamma - 1.0) * K.mach * K.mach * K.prandtl);
+            const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
+            const double vx = g[1][0], vy = g[1][1], vz = g[1][2];
+            const double wx = g[2][0], wy = g[2][1], wz = g[2][2];
+            const double grad = ux + vy + wz;
+            const double txx = miu0 * (2.0 * ux - 2.0 / 3.0 * grad);
+            const double txy = miu0 * (uy + vx - 2.0 / 3.0 * grad);
+            const double txz = miu0 * (uz + wx - 2.0 / 3.0 * grad);
+            const double tyy = miu0 * (2.0 * vy - 2.0 / 3.0 * grad);
+            const double tyz = miu0 * (vz + wy - 2.0 / 3.0 * grad);
+            const double tzz = miu0 * (2.0 * wz - 2.0 / 3.0 * grad);
+            const double ttx = coeff0 * g[3][0], tty = coeff0 * g[3][1], ttz = coeff0 * g[3][2];
+            const double u = zr[3][0], vv = zr[3][1], w = zr[3][2];
+            const double fv[4] = {txx, txy, txz, u * txx + vv * txy + w * txz + ttx};
+            const double gv[4] = {txy, tyy, tyz, u * txy + vv * tyy + w * tyz + tty};
+            const double hv[4] = {txz, tyz, tzz, u * txz + vv * tyz + w * tzz + ttz};
+            const double jac = B.jac[x];
+            double out[12];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                out[c] = (xi_x * fv[c] + xi_y * gv[c] + xi_z * hv[c]) / jac;
+                out[4 + c] = (eta_x * fv[c] + eta_y * gv[c] + eta_z * hv[c]) / jac;
+                out[8 + c] = (zeta_x * fv[c] + zeta_y * gv[c] + zeta_z * hv[c]) / jac;
+            }
+#pragma unroll
+            for (int e = 0; e < 12; e++) B.fv[e][x] = out[e];
+#pragma unroll
+            for (int f = 0; f < 6; f++) {
+                int nb;
+                long long y;
+                if (ghost_target(blocks, B, f, i, j, kc, nb, y)) {
+#pragma unroll
+                    for (int e = 0; e < 12; e++) blocks[nb].fv[e][y] = out[e];
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            dxy[0][v] = dxy[1][v];
+            dxy[1][v] = dxy[2][v];
+            dxy[2][v] = dnew[v];
+        }
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void KNAME(k_visc_div)(const nuc3d_block_dev *__restrict__ blocks,
+                                                         const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K,
+                                                         int stage, int bin, int bun, int bout, double *dtbuf,
+                                                         const unsigned long long *lam_cur,
+                                                         unsigned long long *lam_next, unsigned long long *counters)
+{
+    constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
+    __shared__ double Vx[4][T][L];  // fv_xi (components 1..4) of rows j, x halo
+    __shared__ double Vy[4][L][T];  // fv_eta of columns i, y halo
+    const nuc3d_tile tl = tiles[blockIdx.x];
+    const nuc3d_block_dev &B = blocks[tl.b];
+    const PhysParams P = phys_of(K);
+    const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
+    const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
+    const bool incell = ti < tw && tj < th;
+    const int i = tl.i0 + ti, j = tl.j0 + tj;
+    const double dt = MODE == 1 ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
+
+    double zr[7][4];   // fv_zeta of the own column, planes p-6..p
+    double dd[3][8];   // d fv_xi / dxi and d fv_eta / deta, planes p-3..p-1
+#pragma unroll
+    for (int a = 0; a < 7; a++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) zr[a][v] = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int v = 0; v < 8; v++) dd[a][v] = 0.0;
+    double lmax[3] = {0.0, 0.0, 0.0};
+    unsigned long long nfix = 0;
+
+    const int kb = tl.k0, ke = tl.k0 + tl.kn;
+    for (int p = kb - H; p < ke + H; p++) {
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 6; a++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) zr[a][v] = zr[a + 1][v];
+        if (incell) {
+            const long long x = cidx(B, i, j, p);
+#pragma unroll
+            for (int v = 0; v < 4; v++) zr[6][v] = B.fv[8 + v][x];
+        }
+        const bool axy = p >= kb && p < ke;
+        if (axy) {
+            if (incell) {
+                const long long x = cidx(B, i, j, p);
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    Vx[v][tj][ti + H] = B.fv[v][x];
+                    Vy[v][tj + H][ti] = B.fv[4 + v][x];
+                }
+            }
+            const int h = threadIdx.x;
+            if (h < 2 * T * 2 * H) {
+                const int dir = h / (T * 2 * H);
+                const int r = (h % (T * 2 * H)) / (2 * H), sl = h % (2 * H);
+                const int lim = dir == 0 ? th : tw, ext = dir == 0 ? tw : th;
+                if (r < lim) {
+                    const int off = sl < H ? sl - H : ext + (sl - H);
+                    if (dir == 0) {
+                        const long long y = cidx(B, tl.i0 + off, tl.j0 + r, p);
+#pragma unroll
+                        for (int v = 0; v < 4; v++) Vx[v][r][off + H] = B.fv[v][y];
+                    } else {
+                        const long long y = cidx(B, tl.i0 + r, tl.j0 + off, p);
+#pragma unroll
+                        for (int v = 0; v < 4; v++) Vy[v][off + H][r] = B.fv[4 + v][y];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        double dnew[8];
+#pragma unroll
+        for (int v = 0; v < 8; v++) dnew[v] = 0.0;
+        if (axy && incell) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                dnew[v] = line_diff_lds(B, K, &Vx[v][tj][ti + H], 1, 0, i);
+                dnew[4 + v] = line_diff_lds(B, K, &Vy[v][tj + H][ti], T, 1, j);
+            }
+        }
+        const int kc = p - H;
+        if (incell && kc >= kb && kc < ke) {
+            const long long x = cidx(B, i, j, kc);
+            double r[5];
+#pragma unroll
+            for (int c = 0; c < 5; c++) r[c] = B.rhs[c][x];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                double col[7];
+#pragma unroll
+                for (int a = 0; a < 7; a++) col[a] = zr[a][v];
+                const double gz = line_diff_lds(B, K, &col[3], 1, 2, kc);
+                r[1 + v] = ((r[1 + v] - dd[0][v]) - dd[0][4 + v]) - gz;
+            }
+            if (MODE == 0) {
+#pragma unroll
+                for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
+            } else {
+                rk_update(blocks, B, P, i, j, kc, x, r, stage, bin, bun, bout, dt, lmax, nfix);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            dd[0][v] = dd[1][v];
+            dd[1][v] = dd[2][v];
+            dd[2][v] = dnew[v];
+        }
+    }
+    if (MODE == 1) {
+        if (nfix) atomicAdd(counters, nfix);
+        if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * tl.b);
     }
 }
 
@@ -873,11 +1045,25 @@ extern "C" hipError_t KNAME(nuc3d_launch_crweno)(int dir, int ns, int mode, cons
 #undef NUC3D_A
 }
 
-extern "C" hipError_t KNAME(nuc3d_launch_visc)(const nuc3d_block_dev *blocks, int nblocks, const int4 *ytiles,
-                                               int nytiles, nuc3d_kparams K, int bin, hipStream_t st)
+extern "C" hipError_t KNAME(nuc3d_launch_visc_flux)(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles,
+                                                    int ntiles, nuc3d_kparams K, int bin, hipStream_t st)
 {
-    hipLaunchKernelGGL(KNAME(k_prim), dim3(1024, nblocks), dim3(256), 0, st, blocks, nblocks, K, bin);
-    hipLaunchKernelGGL(KNAME(k_visc_flux), dim3(nytiles), dim3(256), 0, st, blocks, ytiles, K);
+    hipLaunchKernelGGL(KNAME(k_visc_flux), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t KNAME(nuc3d_launch_visc_div)(int mode, const nuc3d_block_dev *blocks, const nuc3d_tile *tiles,
+                                                   int ntiles, nuc3d_kparams K, int stage, int bin, int bun,
+                                                   int bout, double *dtbuf, const unsigned long long *lam_cur,
+                                                   unsigned long long *lam_next, unsigned long long *counters,
+                                                   hipStream_t st)
+{
+    if (mode)
+        hipLaunchKernelGGL(KNAME(k_visc_div)<1>, dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, stage, bin, bun,
+                           bout, dtbuf, lam_cur, lam_next, counters);
+    else
+        hipLaunchKernelGGL(KNAME(k_visc_div)<0>, dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, stage, bin, bun,
+                           bout, dtbuf, lam_cur, lam_next, counters);
     return hipGetLastError();
 }
 

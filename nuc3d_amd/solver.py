@@ -212,11 +212,11 @@ class Case:
     def timing(self, enable=True):
         _check(lib().nuc3d_timing(self.h, 1 if enable else 0), "timing")
 
-    TIMING_CLASSES = ("x_sweep", "y_sweep", "z_sweep_update", "viscous")
+    TIMING_CLASSES = ("x_sweep", "y_sweep", "z_sweep", "viscous_flux", "viscous_div_update")
 
     def timing_read(self):
         """{class: (total_ms, launches)} since the last read."""
-        ms = (ctypes.c_double * 4)()
-        n = (ctypes.c_int * 4)()
+        ms = (ctypes.c_double * 5)()
+        n = (ctypes.c_int * 5)()
         _check(lib().nuc3d_timing_read(self.h, ms, n), "timing_read")
         return {k: (ms[i], n[i]) for i, k in enumerate(self.TIMING_CLASSES)}
