@@ -9,7 +9,8 @@ periodic block (Navier-Stokes, WENO5-JS with local Lax-Friedrichs splitting,
 prescribed spectrum; state resident in HBM before the timed region.
 
 Prints one JSON line with the throughput, the live roofline figure of the
-fused stage kernel (HIP events on its launch stream) and a CPU baseline: the
+kernel class that takes the most time per step (HIP events recorded on the
+stream the kernels are launched on) and a CPU baseline: the
 reference compiled from its own sources (oracle/_ref) timed on the host cores
 on a bounded sample of the same workload, or the oracle where that build is
 absent.
@@ -105,10 +106,16 @@ def cpu_baseline(n, nsteps=3, ranks=None, timeout=240):
             "sample": "C restatement (oracle/), 1 core, 64x64x32 periodic slab of the 256^3 field, 1 RK3 step"}
 
 
-def read_traffic():
-    """HBM bytes per launch of the stage kernel from the committed rocprofv3
-    counter summary, when present."""
-    path = os.path.join(ROOT, "profiles", "k_stage_iso256_traffic.txt")
+# timing class -> kernel (rocprofv3 names: k_sweep_<variant><DIR,...>, ...)
+KERNEL_OF_CLASS = {"x_sweep": "k_sweep_x", "y_sweep": "k_sweep_y", "z_sweep": "k_sweep_z",
+                   "viscous_flux": "k_visc_flux", "viscous_div_update": "k_visc_div"}
+
+
+def read_traffic(cls, n, variant):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy
+    kernel calibration), when present for this workload and variant."""
+    path = os.path.join(ROOT, "profiles", "%s_iso%d_%s_traffic.txt" % (KERNEL_OF_CLASS[cls], n, variant))
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -120,19 +127,39 @@ def read_traffic():
     return None
 
 
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of known size, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters: reads and writes `nbytes` each."""
+    import torch
+    n = nbytes // 8
+    x = torch.arange(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    assert torch.equal(x, y)
+    print(json.dumps({"copy_bytes_each_way": n * 8, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--exact", action="store_true", help="reference operation order (no FMA contraction)")
+    ap.add_argument("--fast", action="store_true",
+                    help="contracted WENO arithmetic (within 1e-12 of the reference on well-conditioned fields only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every block of a case lives on one MI355X); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from nuc3d_amd import cases, solver
 
@@ -140,7 +167,7 @@ def main():
     log("bench: generating %d^3 isotropic turbulence" % n)
     blocks, phys = cases.isotropic_turbulence(n=n, seed=0)
     params = solver.Params(model=solver.NAVIERSTOKES3D, mach=phys.mach, reynolds=phys.reynolds,
-                           exact=1 if args.exact else 0)
+                           exact=0 if args.fast else 1)
     case = solver.Case(params, blocks)
     cells = n ** 3
     log("bench: warmup %d steps" % args.warmup)
@@ -167,7 +194,8 @@ def main():
     avg_ms = dom_ms / max(dom_n, 1)
     achieved = kernel_alg_bytes(dom, ns) * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     step_gbs = ALG_BYTES_PER_STEP * value / 1e9
-    traffic = read_traffic()
+    variant = "fast" if args.fast else "exact"
+    traffic = read_traffic(dom, n, variant)
     log("bench: %.3f ms/step; step-level algorithmic HBM rate %.1f GB/s" % (1e3 * elapsed / args.steps, step_gbs))
     case.close()
 
@@ -196,7 +224,7 @@ def main():
             "workload": "decaying isotropic turbulence %d^3 periodic, Navier-Stokes, WENO5-JS + local LF "
                         "splitting, cd6 viscous terms, TVD-RK3 CFL 0.5 (plain path)" % n,
             "cells": cells,
-            "variant": "exact" if args.exact else "fast",
+            "variant": variant,
         },
         "roofline": {
             "bound": "hbm",
@@ -205,7 +233,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_sweep " + dom,
+            "kernel": KERNEL_OF_CLASS[dom],
             "avg_launch_ms": avg_ms,
             "alg_bytes_per_cell": kernel_alg_bytes(dom, ns),
             "step_level_gbs": step_gbs,

@@ -122,6 +122,15 @@ int check_block(nuc3d_case *c, int b)
     return NUC3D_OK;
 }
 
+// Padded field layout: a row holds XLEAD slots before cell i=0 (the last G of
+// them are the ghost cells), the n0 interior cells, G ghosts and padding to a
+// multiple of ROWALIGN doubles.  With 256-byte aligned allocations every
+// interior row therefore starts on a 128-byte L2 line, so a 16-cell row
+// segment of a tile is one line, not two.
+constexpr int XLEAD = 16, ROWALIGN = 16;
+
+long long origin(const nuc3d_block_dev &B) { return XLEAD + B.sy * B.G + B.sz * B.G; }
+
 // copy an interior field (nz,ny,nx) into / out of the padded device layout
 // through a host staging copy of the whole padded box (ghosts are rebuilt on
 // the device by refresh()).
@@ -132,9 +141,9 @@ int put_field(nuc3d_case *c, int b, double *dst, const double *src)
     std::vector<double> st((size_t)c->padded[b], 0.0);
     for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
-            std::memcpy(&st[(size_t)(G + B.sy * (j + G) + B.sz * (k + G))], src + ((size_t)k * ny + j) * nx,
+            std::memcpy(&st[(size_t)(XLEAD + B.sy * (j + G) + B.sz * (k + G))], src + ((size_t)k * ny + j) * nx,
                         sizeof(double) * nx);
-    HIPCHK(hipMemcpy(dst - (G + B.sy * G + B.sz * G), st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dst - origin(B), st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice));
     return NUC3D_OK;
 }
 
@@ -143,10 +152,10 @@ int get_field(nuc3d_case *c, int b, double *dst, const double *src)
     const nuc3d_block_dev &B = c->hb[b];
     const int G = B.G, nx = B.n[0], ny = B.n[1], nz = B.n[2];
     std::vector<double> st((size_t)c->padded[b]);
-    HIPCHK(hipMemcpy(st.data(), src - (G + B.sy * G + B.sz * G), sizeof(double) * st.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(st.data(), src - origin(B), sizeof(double) * st.size(), hipMemcpyDeviceToHost));
     for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
-            std::memcpy(dst + ((size_t)k * ny + j) * nx, &st[(size_t)(G + B.sy * (j + G) + B.sz * (k + G))],
+            std::memcpy(dst + ((size_t)k * ny + j) * nx, &st[(size_t)(XLEAD + B.sy * (j + G) + B.sz * (k + G))],
                         sizeof(double) * nx);
     return NUC3D_OK;
 }
@@ -281,7 +290,8 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
             }
         }
         B.G = c->G;
-        const long long px = B.n[0] + 2 * B.G, py = B.n[1] + 2 * B.G, pz = B.n[2] + 2 * B.G;
+        const long long px = (XLEAD + B.n[0] + B.G + ROWALIGN - 1) / ROWALIGN * ROWALIGN;
+        const long long py = B.n[1] + 2 * B.G, pz = B.n[2] + 2 * B.G;
         B.sy = px;
         B.sz = px * py;
         const long long P = px * py * pz;
@@ -296,7 +306,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
             nuc3d_case_destroy(c);
             return fail(NUC3D_ERR_HIP, "hipMemset failed");
         }
-        const long long org = B.G + B.sy * B.G + B.sz * B.G;  // offset of cell (0,0,0)
+        const long long org = origin(B);  // offset of cell (0,0,0)
         double *base = static_cast<double *>(mem);
         int f = 0;
         auto next = [&]() { return base + P * (f++) + org; };

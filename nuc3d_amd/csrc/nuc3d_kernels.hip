@@ -41,6 +41,17 @@ __device__ __forceinline__ PhysParams phys_of(const nuc3d_kparams &K)
     return P;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  Tile lists are ordered so that neighbouring tiles (whose
+// halos overlap) are adjacent; this gives XCD x the contiguous range of tiles
+// [x*per, ...) instead of every eighth tile, so halo lines are shared in L2.
+__device__ __forceinline__ int xcd_tile(int w, int nw)
+{
+    constexpr int NXCD = 8;
+    const int per = nw / NXCD, rem = nw % NXCD, x = w % NXCD, l = w / NXCD;
+    return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
+}
+
 __device__ __forceinline__ long long cidx(const nuc3d_block_dev &B, int i, int j, int k)
 {
     return (long long)i + B.sy * j + B.sz * k;
@@ -360,7 +371,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     __shared__ double S[10][NSL * NL];
     __shared__ double HF[5][NF * NL];
 
-    const int4 t = tiles[blockIdx.x];
+    const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[t.x];
     const PhysParams P = phys_of(K);
     const int a0 = t.y, l0 = t.z, cc = t.w;
@@ -474,7 +485,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 {
     constexpr int H = NUC3D_HALO;
     extern __shared__ double lds[];
-    const int4 L = lines[blockIdx.x];
+    const int4 L = lines[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[L.x];
     const PhysParams P = phys_of(K);
     const int n = B.n[DIR], w = K.width[DIR], M = n - 2 * w + 1;
@@ -620,7 +631,7 @@ __global__ __launch_bounds__(256) void KNAME(k_visc_flux)(const nuc3d_block_dev 
 {
     constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
     __shared__ double Wp[4][L][L];
-    const nuc3d_tile tl = tiles[blockIdx.x];
+    const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K);
     const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
@@ -760,7 +771,7 @@ __global__ __launch_bounds__(256) void KNAME(k_visc_div)(const nuc3d_block_dev *
     constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
     __shared__ double Vx[4][T][L];  // fv_xi (components 1..4) of rows j, x halo
     __shared__ double Vy[4][L][T];  // fv_eta of columns i, y halo
-    const nuc3d_tile tl = tiles[blockIdx.x];
+    const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K);
     const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
@@ -877,7 +888,7 @@ __global__ __launch_bounds__(256) void KNAME(k_eigen)(const nuc3d_block_dev *__r
                                                       unsigned long long *lam)
 {
     constexpr int T = NUC3D_TILE;
-    const nuc3d_tile tl = tiles[blockIdx.x];
+    const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K);
     const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T;
@@ -909,7 +920,7 @@ __global__ __launch_bounds__(256) void KNAME(k_fixup)(const nuc3d_block_dev *__r
                                                       unsigned long long *counters)
 {
     constexpr int T = NUC3D_TILE;
-    const nuc3d_tile tl = tiles[blockIdx.x];
+    const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K);
     const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T;

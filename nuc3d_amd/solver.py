@@ -40,7 +40,10 @@ class _CParams(ctypes.Structure):
 @dataclass
 class Params:
     """inp/PhysModel.in + inp/Method.in of the reference (physicsModel.cpp:41-69,
-    fieldOperator.cpp:20-31), plus `exact` (reference operation order)."""
+    fieldOperator.cpp:20-31), plus `exact`: 1 (default) keeps the reference's
+    operation order everywhere; 0 contracts the WENO arithmetic and uses a
+    one-division form of the nonlinear weights (about a third faster per
+    sweep, within 1e-12 on well-conditioned fields only)."""
     gamma: float = 1.4
     mach: float = 0.1
     reynolds: float = 1600.0
@@ -54,7 +57,7 @@ class Params:
     diff: Sequence[int] = (CD6, CD6, CD6)
     width: Sequence[int] = (3, 3, 3)
     zquirk: int = 1
-    exact: int = 0
+    exact: int = 1
 
     def to_c(self) -> _CParams:
         p = _CParams()
