@@ -590,287 +590,211 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 }
 
 // ---------------------------------------------------------------------------
-// Navier-Stokes viscous terms in two k-marching passes over 16x16 column
-// tiles (NaiverStokes3d.cpp:51-513).
+// Navier-Stokes viscous terms (NaiverStokes3d.cpp:51-513) in three passes
+// over 16x16 column tiles marching in k, one thread per column, with no LDS
+// and no barriers: waves never wait for each other, and neighbouring columns
+// of a tile share L1/L2 lines.
 //
-// k_visc_flux: u, v, w, T (con2prim, physicsModel.cpp:313-353) of each plane
-// with a 3-cell halo in i and j go to LDS for the xi and eta derivatives; a
-// seven-plane register ring of the thread's own column gives the zeta
-// derivative (cd6 or the exterior closure, NaiverStokes3d.cpp:86-168).  Three
-// planes later the cell's viscous fluxes (Sutherland viscosity, stresses,
-// heat flux, contravariant fluxes over J: NaiverStokes3d.cpp:170-326) are
-// written and pushed to the neighbours' ghost layers.
-//
-// k_visc_div: after the three inviscid sweeps, rhs -= dFv/dxi, dGv/deta,
-// dHv/dzeta in the reference's order (NaiverStokes3d.cpp:401-513), fv_xi and
-// fv_eta from LDS plane tiles and fv_zeta from a register ring; then the RHS
-// (MODE 0) or the TVD-RK3 stage (MODE 1).
+// k_prim:      u, v, w, T of every cell (con2prim, physicsModel.cpp:313-353,
+//              T = gamma Ma^2 p / rho, physicsModel.cpp:1029), pushed to the
+//              neighbours' ghost layers like Q.
+// k_visc_flux: cd6 (or exterior closure) derivatives of u, v, w, T along
+//              xi, eta (gathered from L1/L2) and zeta (a seven-plane register
+//              ring of the thread's own column, so the L2 working set is one
+//              plane deep) (NaiverStokes3d.cpp:86-168), chain rule with the
+//              metrics, Sutherland viscosity, stresses, heat flux and the
+//              contravariant fluxes over J (NaiverStokes3d.cpp:170-326).
+// k_visc_div:  after the three inviscid sweeps, rhs -= dFv/dxi, dGv/deta,
+//              dHv/dzeta in the reference's order (NaiverStokes3d.cpp:401-513),
+//              fv_zeta from a register ring as above, then the RHS (MODE 0) or
+//              the TVD-RK3 stage (MODE 1).
+
+// Waves per SIMD the viscous kernels are register-limited to (measured at
+// 256^3: the flux pass is fastest with all of its loads in flight at 2 waves,
+// the divergence pass with 4 waves and a few spills).
+#ifndef NUC3D_VISC_FLUX_WAVES
+#define NUC3D_VISC_FLUX_WAVES 2
+#endif
+#ifndef NUC3D_VISC_DIV_WAVES
+#define NUC3D_VISC_DIV_WAVES 4
+#endif
 
 __device__ __forceinline__ void prim4(const nuc3d_block_dev &B, double *const *qb, long long x, const PhysParams &P,
                                       double *w4)
 {
-    const double jac = B.jac[x];
-    if (jac == 0.0) {  // ghost beyond a physical face: never read
-        w4[0] = w4[1] = w4[2] = w4[3] = 0.0;
-        return;
-    }
     double Q[5];
 #pragma unroll
     for (int c = 0; c < 5; c++) Q[c] = qb[c][x];
     double rho, u, v, w, p, cs;
-    con2prim(Q, jac, P, rho, u, v, w, p, cs);
+    con2prim(Q, B.jac[x], P, rho, u, v, w, p, cs);
     w4[0] = u;
     w4[1] = v;
     w4[2] = w;
     w4[3] = P.gamma * P.mach * P.mach * p / rho;  // physicsModel.cpp:1029
 }
 
-__global__ __launch_bounds__(256) void KNAME(k_visc_flux)(const nuc3d_block_dev *__restrict__ blocks,
-                                                          const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K,
-                                                          int bin)
+__global__ __launch_bounds__(256) void KNAME(k_prim)(const nuc3d_block_dev *__restrict__ blocks,
+                                                     const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int bin)
 {
-    constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
-    __shared__ double Wp[4][L][L];
+    constexpr int T = NUC3D_TILE;
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K);
-    const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
-    const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
-    const bool incell = ti < tw && tj < th;
-    const int i = tl.i0 + ti, j = tl.j0 + tj;
+    const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T;
+    if (i >= B.n[0] || j >= B.n[1]) return;
     double *const *qin = B.q[bin];
-    const double S = K.sref;
-
-    double zr[7][4];   // u,v,w,T of the own column, planes p-6..p
-    double dxy[3][8];  // xi and eta derivatives of u,v,w,T, planes p-3..p-1
+    for (int k = tl.k0; k < tl.k0 + tl.kn; k++) {
+        const long long x = cidx(B, i, j, k);
+        double w4[4];
+        prim4(B, qin, x, P, w4);
 #pragma unroll
-    for (int a = 0; a < 7; a++)
+        for (int v = 0; v < 4; v++) B.prim[v][x] = w4[v];
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            int nb;
+            long long y;
+            if (ghost_target(blocks, B, f, i, j, k, nb, y)) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) blocks[nb].prim[v][y] = w4[v];
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_FLUX_WAVES))) void KNAME(k_visc_flux)(const nuc3d_block_dev *__restrict__ blocks,
+                                                          const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K)
+{
+    constexpr int T = NUC3D_TILE, H = NUC3D_HALO;
+    const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const nuc3d_block_dev &B = blocks[tl.b];
+    const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T;
+    if (i >= B.n[0] || j >= B.n[1]) return;
+    const double S = K.sref;
+    double zr[2 * H + 1][4];  // u, v, w, T of the own column, planes p-6..p
+#pragma unroll
+    for (int a = 0; a < 2 * H + 1; a++)
 #pragma unroll
         for (int v = 0; v < 4; v++) zr[a][v] = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int v = 0; v < 8; v++) dxy[a][v] = 0.0;
-
     const int kb = tl.k0, ke = tl.k0 + tl.kn;
     for (int p = kb - H; p < ke + H; p++) {
-        __syncthreads();
 #pragma unroll
-        for (int a = 0; a < 6; a++)
+        for (int a = 0; a < 2 * H; a++)
 #pragma unroll
             for (int v = 0; v < 4; v++) zr[a][v] = zr[a + 1][v];
-        if (incell) prim4(B, qin, cidx(B, i, j, p), P, zr[6]);
-        const bool axy = p >= kb && p < ke;
-        if (axy) {
-            if (incell)
+        const long long xp = cidx(B, i, j, p);
 #pragma unroll
-                for (int v = 0; v < 4; v++) Wp[v][tj + H][ti + H] = zr[6][v];
-            const int h = threadIdx.x;
-            if (h < 2 * T * 2 * H) {
-                const int dir = h / (T * 2 * H);
-                const int r = (h % (T * 2 * H)) / (2 * H), sl = h % (2 * H);
-                const int lim = dir == 0 ? th : tw, ext = dir == 0 ? tw : th;
-                if (r < lim) {
-                    const int off = sl < H ? sl - H : ext + (sl - H);
-                    double w4[4];
-                    if (dir == 0) {
-                        prim4(B, qin, cidx(B, tl.i0 + off, tl.j0 + r, p), P, w4);
+        for (int v = 0; v < 4; v++) zr[2 * H][v] = B.prim[v][xp];
+        const int k = p - H;
+        if (k < kb) continue;
+        const long long x = cidx(B, i, j, k);
+        const double xi_x = B.met[0][x], xi_y = B.met[1][x], xi_z = B.met[2][x];
+        const double eta_x = B.met[3][x], eta_y = B.met[4][x], eta_z = B.met[5][x];
+        const double zeta_x = B.met[6][x], zeta_y = B.met[7][x], zeta_z = B.met[8][x];
+        double g[4][3];
 #pragma unroll
-                        for (int v = 0; v < 4; v++) Wp[v][r + H][off + H] = w4[v];
-                    } else {
-                        prim4(B, qin, cidx(B, tl.i0 + r, tl.j0 + off, p), P, w4);
+        for (int v = 0; v < 4; v++) {
+            const double *f = B.prim[v] + x;
+            double col[2 * H + 1];
 #pragma unroll
-                        for (int v = 0; v < 4; v++) Wp[v][off + H][r + H] = w4[v];
-                    }
-                }
-            }
+            for (int a = 0; a < 2 * H + 1; a++) col[a] = zr[a][v];
+            const double a = line_diff(B, K, f, 0, i);
+            const double bb = line_diff(B, K, f, 1, j);
+            const double cc = line_diff_lds(B, K, &col[H], 1, 2, k);
+            g[v][0] = a * xi_x + bb * eta_x + cc * zeta_x;
+            g[v][1] = a * xi_y + bb * eta_y + cc * zeta_y;
+            g[v][2] = a * xi_z + bb * eta_z + cc * zeta_z;
         }
-        __syncthreads();
-        double dnew[8];
+        const double u = zr[H][0], vv = zr[H][1], w = zr[H][2], T0 = zr[H][3];
+        const double miu0 = (1.0 + S) / (T0 + S) * pow(T0, 1.5) / K.reynolds;  // physicsModel.cpp:226
+        const double coeff0 = miu0 / ((K.gamma - 1.0) * K.mach * K.mach * K.prandtl);
+        const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
+        const double vx = g[1][0], vy = g[1][1], vz = g[1][2];
+        const double wx = g[2][0], wy = g[2][1], wz = g[2][2];
+        const double grad = ux + vy + wz;
+        const double txx = miu0 * (2.0 * ux - 2.0 / 3.0 * grad);
+        const double txy = miu0 * (uy + vx - 2.0 / 3.0 * grad);
+        const double txz = miu0 * (uz + wx - 2.0 / 3.0 * grad);
+        const double tyy = miu0 * (2.0 * vy - 2.0 / 3.0 * grad);
+        const double tyz = miu0 * (vz + wy - 2.0 / 3.0 * grad);
+        const double tzz = miu0 * (2.0 * wz - 2.0 / 3.0 * grad);
+        const double ttx = coeff0 * g[3][0], tty = coeff0 * g[3][1], ttz = coeff0 * g[3][2];
+        const double fv[4] = {txx, txy, txz, u * txx + vv * txy + w * txz + ttx};
+        const double gv[4] = {txy, tyy, tyz, u * txy + vv * tyy + w * tyz + tty};
+        const double hv[4] = {txz, tyz, tzz, u * txz + vv * tyz + w * tzz + ttz};
+        const double jac = B.jac[x];
+        double out[12];
 #pragma unroll
-        for (int v = 0; v < 8; v++) dnew[v] = 0.0;
-        if (axy && incell) {
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                dnew[v] = line_diff_lds(B, K, &Wp[v][tj + H][ti + H], 1, 0, i);
-                dnew[4 + v] = line_diff_lds(B, K, &Wp[v][tj + H][ti + H], L, 1, j);
-            }
-        }
-        const int kc = p - H;
-        if (incell && kc >= kb && kc < ke) {
-            const long long x = cidx(B, i, j, kc);
-            double g[4][3];
-            const double xi_x = B.met[0][x], xi_y = B.met[1][x], xi_z = B.met[2][x];
-            const double eta_x = B.met[3][x], eta_y = B.met[4][x], eta_z = B.met[5][x];
-            const double zeta_x = B.met[6][x], zeta_y = B.met[7][x], zeta_z = B.met[8][x];
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                double col[7];
-#pragma unroll
-                for (int a = 0; a < 7; a++) col[a] = zr[a][v];
-                const double a = dxy[0][v], bb = dxy[0][4 + v];
-                const double cc = line_diff_lds(B, K, &col[3], 1, 2, kc);
-                g[v][0] = a * xi_x + bb * eta_x + cc * zeta_x;
-                g[v][1] = a * xi_y + bb * eta_y + cc * zeta_y;
-                g[v][2] = a * xi_z + bb * eta_z + cc * zeta_z;
-            }
-            const double T0 = zr[3][3];
-            const double miu0 = (1.0 + S) / (T0 + S) * pow(T0, 1.5) / K.reynolds;  // physicsModel.cpp:226
-            const double coeff0 = miu0 / ((K.gamma - 1.0) * K.mach * K.mach * K.prandtl);
-            const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
-            const double vx = g[1][0], vy = g[1][1], vz = g[1][2];
-            const double wx = g[2][0], wy = g[2][1], wz = g[2][2];
-            const double grad = ux + vy + wz;
-            const double txx = miu0 * (2.0 * ux - 2.0 / 3.0 * grad);
-            const double txy = miu0 * (uy + vx - 2.0 / 3.0 * grad);
-            const double txz = miu0 * (uz + wx - 2.0 / 3.0 * grad);
-            const double tyy = miu0 * (2.0 * vy - 2.0 / 3.0 * grad);
-            const double tyz = miu0 * (vz + wy - 2.0 / 3.0 * grad);
-            const double tzz = miu0 * (2.0 * wz - 2.0 / 3.0 * grad);
-            const double ttx = coeff0 * g[3][0], tty = coeff0 * g[3][1], ttz = coeff0 * g[3][2];
-            const double u = zr[3][0], vv = zr[3][1], w = zr[3][2];
-            const double fv[4] = {txx, txy, txz, u * txx + vv * txy + w * txz + ttx};
-            const double gv[4] = {txy, tyy, tyz, u * txy + vv * tyy + w * tyz + tty};
-            const double hv[4] = {txz, tyz, tzz, u * txz + vv * tyz + w * tzz + ttz};
-            const double jac = B.jac[x];
-            double out[12];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                out[c] = (xi_x * fv[c] + xi_y * gv[c] + xi_z * hv[c]) / jac;
-                out[4 + c] = (eta_x * fv[c] + eta_y * gv[c] + eta_z * hv[c]) / jac;
-                out[8 + c] = (zeta_x * fv[c] + zeta_y * gv[c] + zeta_z * hv[c]) / jac;
-            }
-#pragma unroll
-            for (int e = 0; e < 12; e++) B.fv[e][x] = out[e];
-#pragma unroll
-            for (int f = 0; f < 6; f++) {
-                int nb;
-                long long y;
-                if (ghost_target(blocks, B, f, i, j, kc, nb, y)) {
-#pragma unroll
-                    for (int e = 0; e < 12; e++) blocks[nb].fv[e][y] = out[e];
-                }
-            }
+        for (int c = 0; c < 4; c++) {
+            out[c] = (xi_x * fv[c] + xi_y * gv[c] + xi_z * hv[c]) / jac;
+            out[4 + c] = (eta_x * fv[c] + eta_y * gv[c] + eta_z * hv[c]) / jac;
+            out[8 + c] = (zeta_x * fv[c] + zeta_y * gv[c] + zeta_z * hv[c]) / jac;
         }
 #pragma unroll
-        for (int v = 0; v < 8; v++) {
-            dxy[0][v] = dxy[1][v];
-            dxy[1][v] = dxy[2][v];
-            dxy[2][v] = dnew[v];
+        for (int e = 0; e < 12; e++) B.fv[e][x] = out[e];
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            int nb;
+            long long y;
+            if (ghost_target(blocks, B, f, i, j, k, nb, y)) {
+#pragma unroll
+                for (int e = 0; e < 12; e++) blocks[nb].fv[e][y] = out[e];
+            }
         }
     }
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void KNAME(k_visc_div)(const nuc3d_block_dev *__restrict__ blocks,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_DIV_WAVES))) void KNAME(k_visc_div)(const nuc3d_block_dev *__restrict__ blocks,
                                                          const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K,
                                                          int stage, int bin, int bun, int bout, double *dtbuf,
                                                          const unsigned long long *lam_cur,
                                                          unsigned long long *lam_next, unsigned long long *counters)
 {
-    constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
-    __shared__ double Vx[4][T][L];  // fv_xi (components 1..4) of rows j, x halo
-    __shared__ double Vy[4][L][T];  // fv_eta of columns i, y halo
+    constexpr int T = NUC3D_TILE, H = NUC3D_HALO;
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K);
-    const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
-    const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
-    const bool incell = ti < tw && tj < th;
-    const int i = tl.i0 + ti, j = tl.j0 + tj;
+    const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T;
     const double dt = MODE == 1 ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
-
-    double zr[7][4];   // fv_zeta of the own column, planes p-6..p
-    double dd[3][8];   // d fv_xi / dxi and d fv_eta / deta, planes p-3..p-1
-#pragma unroll
-    for (int a = 0; a < 7; a++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) zr[a][v] = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int v = 0; v < 8; v++) dd[a][v] = 0.0;
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
-
-    const int kb = tl.k0, ke = tl.k0 + tl.kn;
-    for (int p = kb - H; p < ke + H; p++) {
-        __syncthreads();
+    if (i < B.n[0] && j < B.n[1]) {
+        double zr[2 * H + 1][4];  // fv_zeta (components 1..4) of the own column, planes p-6..p
 #pragma unroll
-        for (int a = 0; a < 6; a++)
+        for (int a = 0; a < 2 * H + 1; a++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) zr[a][v] = zr[a + 1][v];
-        if (incell) {
-            const long long x = cidx(B, i, j, p);
+            for (int v = 0; v < 4; v++) zr[a][v] = 0.0;
+        const int kb = tl.k0, ke = tl.k0 + tl.kn;
+        for (int p = kb - H; p < ke + H; p++) {
 #pragma unroll
-            for (int v = 0; v < 4; v++) zr[6][v] = B.fv[8 + v][x];
-        }
-        const bool axy = p >= kb && p < ke;
-        if (axy) {
-            if (incell) {
-                const long long x = cidx(B, i, j, p);
+            for (int a = 0; a < 2 * H; a++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    Vx[v][tj][ti + H] = B.fv[v][x];
-                    Vy[v][tj + H][ti] = B.fv[4 + v][x];
-                }
-            }
-            const int h = threadIdx.x;
-            if (h < 2 * T * 2 * H) {
-                const int dir = h / (T * 2 * H);
-                const int r = (h % (T * 2 * H)) / (2 * H), sl = h % (2 * H);
-                const int lim = dir == 0 ? th : tw, ext = dir == 0 ? tw : th;
-                if (r < lim) {
-                    const int off = sl < H ? sl - H : ext + (sl - H);
-                    if (dir == 0) {
-                        const long long y = cidx(B, tl.i0 + off, tl.j0 + r, p);
+                for (int v = 0; v < 4; v++) zr[a][v] = zr[a + 1][v];
+            const long long xp = cidx(B, i, j, p);
 #pragma unroll
-                        for (int v = 0; v < 4; v++) Vx[v][r][off + H] = B.fv[v][y];
-                    } else {
-                        const long long y = cidx(B, tl.i0 + r, tl.j0 + off, p);
-#pragma unroll
-                        for (int v = 0; v < 4; v++) Vy[v][off + H][r] = B.fv[4 + v][y];
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        double dnew[8];
-#pragma unroll
-        for (int v = 0; v < 8; v++) dnew[v] = 0.0;
-        if (axy && incell) {
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                dnew[v] = line_diff_lds(B, K, &Vx[v][tj][ti + H], 1, 0, i);
-                dnew[4 + v] = line_diff_lds(B, K, &Vy[v][tj + H][ti], T, 1, j);
-            }
-        }
-        const int kc = p - H;
-        if (incell && kc >= kb && kc < ke) {
-            const long long x = cidx(B, i, j, kc);
+            for (int v = 0; v < 4; v++) zr[2 * H][v] = B.fv[8 + v][xp];
+            const int k = p - H;
+            if (k < kb) continue;
+            const long long x = cidx(B, i, j, k);
             double r[5];
 #pragma unroll
             for (int c = 0; c < 5; c++) r[c] = B.rhs[c][x];
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                double col[7];
+                double col[2 * H + 1];
 #pragma unroll
-                for (int a = 0; a < 7; a++) col[a] = zr[a][v];
-                const double gz = line_diff_lds(B, K, &col[3], 1, 2, kc);
-                r[1 + v] = ((r[1 + v] - dd[0][v]) - dd[0][4 + v]) - gz;
+                for (int a = 0; a < 2 * H + 1; a++) col[a] = zr[a][v];
+                const double gx = line_diff(B, K, B.fv[v] + x, 0, i);
+                const double gy = line_diff(B, K, B.fv[4 + v] + x, 1, j);
+                const double gz = line_diff_lds(B, K, &col[H], 1, 2, k);
+                r[1 + v] = ((r[1 + v] - gx) - gy) - gz;
             }
             if (MODE == 0) {
 #pragma unroll
                 for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
             } else {
-                rk_update(blocks, B, P, i, j, kc, x, r, stage, bin, bun, bout, dt, lmax, nfix);
+                rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix);
             }
-        }
-#pragma unroll
-        for (int v = 0; v < 8; v++) {
-            dd[0][v] = dd[1][v];
-            dd[1][v] = dd[2][v];
-            dd[2][v] = dnew[v];
         }
     }
     if (MODE == 1) {
@@ -1059,7 +983,8 @@ extern "C" hipError_t KNAME(nuc3d_launch_crweno)(int dir, int ns, int mode, cons
 extern "C" hipError_t KNAME(nuc3d_launch_visc_flux)(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles,
                                                     int ntiles, nuc3d_kparams K, int bin, hipStream_t st)
 {
-    hipLaunchKernelGGL(KNAME(k_visc_flux), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+    hipLaunchKernelGGL(KNAME(k_prim), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+    hipLaunchKernelGGL(KNAME(k_visc_flux), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K);
     return hipGetLastError();
 }
 
