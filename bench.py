@@ -34,27 +34,30 @@ HBM_PEAK_GBS = 8000.0
 ALG_BYTES_PER_STEP = 560.0
 
 
-def kernel_alg_bytes(cls, ns, stage_avg=True):
+def kernel_alg_bytes(cls, ns):
     """Compulsory HBM bytes per cell of one launch of a kernel class, averaged
     over the three RK stages (every input field read once, every output
     written once; ghost layers and halo re-reads not counted):
-      x sweep:     Q 40 + J 8 + xi metrics 24 + RHS write 40                  = 112
-      y sweep:     Q 40 + J 8 + eta metrics 24 + RHS read and write 80         = 152
-      z sweep:     Q 40 + J 8 + zeta metrics 24 + RHS read and write 80        = 152
-                   (Euler: + Q^n 40 at stages 1, 2, Q_out write instead of RHS,
-                    xi/eta metrics 48 at stage 2)
-      viscous flux: Q 40 + J 8 + nine metrics 72 + fv write 96                 = 216
-      viscous div:  fv read 96 + RHS 40 + Q 40 + J 8 + Q_out 40
-                    + Q^n 40 (stages 1, 2) + nine metrics 72 (stage 2)
+      x sweep:      Q 40 + J 8 + xi metrics 24 + RHS write 40                 = 112
+      y sweep:      Q 40 + J 8 + eta metrics 24 + RHS read and write 80        = 152
+      z sweep:      Q 40 + J 8 + zeta metrics 24 + RHS read and write 80       = 152
+                    (Euler: RHS read 40, Q^n 40 at stages 1 and 2, Q_out write
+                     40, xi/eta metrics 48 at stage 2 for the eigenvalues)
+      primitives:   Q 40 + J 8 read, u v w T write 32                         = 80
+      viscous flux: u v w T 32 + J 8 + nine metrics 72 read, fv write 96      = 208
+      viscous div:  fv 96 + RHS 40 + Q 40 + J 8 read, Q_out 40 write,
+                    Q^n 40 at stages 1 and 2, nine metrics 72 at stage 2      = 274.7
     """
     if cls == "x_sweep":
         return 112.0
-    if cls == "y_sweep":
+    if cls in ("y_sweep", "z_sweep") and (ns or cls == "y_sweep"):
         return 152.0
     if cls == "z_sweep":
-        return 152.0 if ns else 112.0 + (0 + 40 + 40) / 3.0 + 40.0 + 48.0 / 3.0
+        return 112.0 + (0 + 40 + 40) / 3.0 + 40.0 + 48.0 / 3.0
+    if cls == "primitives":
+        return 80.0
     if cls == "viscous_flux":
-        return 216.0
+        return 208.0
     return 96.0 + 40.0 + 40.0 + 8.0 + 40.0 + (0 + 40 + 40) / 3.0 + 72.0 / 3.0
 
 
@@ -108,7 +111,7 @@ def cpu_baseline(n, nsteps=3, ranks=None, timeout=240):
 
 # timing class -> kernel (rocprofv3 names: k_sweep_<variant><DIR,...>, ...)
 KERNEL_OF_CLASS = {"x_sweep": "k_sweep_x", "y_sweep": "k_sweep_y", "z_sweep": "k_sweep_z",
-                   "viscous_flux": "k_visc_flux", "viscous_div_update": "k_visc_div"}
+                   "primitives": "k_prim", "viscous_flux": "k_visc_flux", "viscous_div_update": "k_visc_div"}
 
 
 def read_traffic(cls, n, variant):

@@ -110,13 +110,15 @@ int nuc3d_get_fixups(nuc3d_case *c, long long *count);
 
 /* Kernel timing (no counterpart in the reference, which times whole steps
  * with MPI_Wtime in block::solve, block.cpp:576-584).  While enabled, every
- * kernel launch is bracketed by HIP events on its stream, in five classes:
- * 0 x sweep, 1 y sweep, 2 z sweep (with the RK update for Euler),
- * 3 viscous fluxes, 4 viscous divergence with the RK update (Navier-Stokes).
- * nuc3d_timing_read returns the summed milliseconds and launch counts per
- * class since the last read (arrays of 5) and resets them. */
+ * kernel launch is bracketed by HIP events on its stream, in NUC3D_NTIMING
+ * classes: 0 x sweep, 1 y sweep, 2 z sweep (with the RK update for Euler),
+ * and for Navier-Stokes 3 primitives u,v,w,T, 4 viscous fluxes, 5 viscous
+ * divergence with the RK update.  nuc3d_timing_read returns the summed
+ * milliseconds and launch counts per class since the last read (arrays of
+ * NUC3D_NTIMING) and resets them. */
+#define NUC3D_NTIMING 6
 int nuc3d_timing(nuc3d_case *c, int enable);
-int nuc3d_timing_read(nuc3d_case *c, double *ms5, int *launches5);
+int nuc3d_timing_read(nuc3d_case *c, double *ms, int *launches);
 
 /* 1 when this build implements the CRWENO5 line solves (no counterpart in the
  * reference: a capability query for callers). */

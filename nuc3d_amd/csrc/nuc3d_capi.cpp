@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/nuc3d_hip.h"
+#include "../../include/nuc3d_ops.h"
 #include "nuc3d_types.h"
 
 #define NUC3D_LAUNCHERS(SFX)                                                                                         \
@@ -20,6 +21,8 @@
                                                    nuc3d_kparams, int, int, int, int, double *,                      \
                                                    const unsigned long long *, unsigned long long *,                 \
                                                    unsigned long long *, hipStream_t);                               \
+    extern "C" hipError_t nuc3d_launch_prim_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, nuc3d_kparams,  \
+                                                  int, hipStream_t);                                                 \
     extern "C" hipError_t nuc3d_launch_visc_flux_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int,           \
                                                        nuc3d_kparams, int, hipStream_t);                             \
     extern "C" hipError_t nuc3d_launch_visc_div_##SFX(int, const nuc3d_block_dev *, const nuc3d_tile *, int,        \
@@ -62,6 +65,7 @@ int hipfail(hipError_t e, const char *what)
 
 struct Launchers {
     decltype(&nuc3d_launch_sweep_fast) sweep;
+    decltype(&nuc3d_launch_prim_fast) prim;
     decltype(&nuc3d_launch_visc_flux_fast) vflux;
     decltype(&nuc3d_launch_visc_div_fast) vdiv;
     decltype(&nuc3d_launch_eigen_fast) eigen;
@@ -70,9 +74,9 @@ struct Launchers {
     decltype(&nuc3d_launch_crweno_fast) crweno;
 };
 
-const Launchers kFast = {nuc3d_launch_sweep_fast, nuc3d_launch_visc_flux_fast, nuc3d_launch_visc_div_fast, nuc3d_launch_eigen_fast,
+const Launchers kFast = {nuc3d_launch_sweep_fast, nuc3d_launch_prim_fast, nuc3d_launch_visc_flux_fast, nuc3d_launch_visc_div_fast, nuc3d_launch_eigen_fast,
                          nuc3d_launch_fixup_fast, nuc3d_launch_ghost_fill_fast, nuc3d_launch_crweno_fast};
-const Launchers kExact = {nuc3d_launch_sweep_exact, nuc3d_launch_visc_flux_exact, nuc3d_launch_visc_div_exact, nuc3d_launch_eigen_exact,
+const Launchers kExact = {nuc3d_launch_sweep_exact, nuc3d_launch_prim_exact, nuc3d_launch_visc_flux_exact, nuc3d_launch_visc_div_exact, nuc3d_launch_eigen_exact,
                           nuc3d_launch_fixup_exact, nuc3d_launch_ghost_fill_exact, nuc3d_launch_crweno_exact};
 
 constexpr int KSEG = 32;  // planes per k segment of a tile
@@ -110,7 +114,7 @@ struct nuc3d_case {
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     // x sweep, y sweep, z sweep, viscous fluxes, viscous divergence (+ update)
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[5];
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[NUC3D_NTIMING];
 };
 
 namespace {
@@ -213,7 +217,11 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
 {
     const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
     if (ns) {
-        Timed t(c, s, &c->ev[3]);
+        {
+            Timed t(c, s, &c->ev[3]);
+            HIPCHK(c->L->prim(c->db, c->dtiles, c->ntiles, c->K, bin, s));
+        }
+        Timed t(c, s, &c->ev[4]);
         HIPCHK(c->L->vflux(c->db, c->dtiles, c->ntiles, c->K, bin, s));
     }
     for (int d = 0; d < 3; d++) {
@@ -226,7 +234,7 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
                                bun, bout, c->ddt, lcur, lnext, c->dcount, s));
     }
     if (ns) {
-        Timed t(c, s, &c->ev[4]);
+        Timed t(c, s, &c->ev[5]);
         HIPCHK(c->L->vdiv(mode, c->db, c->dtiles, c->ntiles, c->K, stage, bin, bun, bout, c->ddt, lcur, lnext,
                           c->dcount, s));
     }
@@ -602,11 +610,11 @@ int nuc3d_timing(nuc3d_case *c, int enable)
     return NUC3D_OK;
 }
 
-int nuc3d_timing_read(nuc3d_case *c, double *ms5, int *launches5)
+int nuc3d_timing_read(nuc3d_case *c, double *ms, int *launches)
 {
     if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
     HIPCHK(hipDeviceSynchronize());
-    for (int l = 0; l < 5; l++) {
+    for (int l = 0; l < NUC3D_NTIMING; l++) {
         double acc = 0.0;
         int cnt = 0;
         for (auto &pr : c->ev[l]) {
@@ -618,8 +626,172 @@ int nuc3d_timing_read(nuc3d_case *c, double *ms5, int *launches5)
             c->ev_pool.push_back(pr.second);
         }
         c->ev[l].clear();
-        if (ms5) ms5[l] = acc;
-        if (launches5) launches5[l] = cnt;
+        if (ms) ms[l] = acc;
+        if (launches) launches[l] = cnt;
+    }
+    return NUC3D_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Operator-level ABI (include/nuc3d_ops.h): one Field operation per call, on
+// device copies of the caller's host Fields.
+
+extern "C" hipError_t nuc3d_launch_op_interp(int, int, int, int, int, int, int, int, int, const double *,
+                                             const double *, const double *, double *, hipStream_t);
+extern "C" hipError_t nuc3d_launch_op_crweno(int, int, int, int, const double *, double *, double *, hipStream_t);
+extern "C" hipError_t nuc3d_launch_op_diff(int, int, int, int, int, int, int, const double *, const double *,
+                                           const double *, double *, hipStream_t);
+extern "C" hipError_t nuc3d_launch_op_rk3(int, double, long long, const double *, const double *, double *,
+                                          hipStream_t);
+
+namespace {
+
+// device buffers of one operator call, freed on return
+struct OpBuffers {
+    std::vector<void *> p;
+    ~OpBuffers()
+    {
+        for (void *q : p) (void)hipFree(q);
+    }
+    double *get(size_t n)
+    {
+        void *q = nullptr;
+        if (hipMalloc(&q, sizeof(double) * std::max<size_t>(n, 1)) != hipSuccess) return nullptr;
+        p.push_back(q);
+        return static_cast<double *>(q);
+    }
+};
+
+int op_device_ready()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return fail(NUC3D_ERR_NO_DEVICE, "no GPU visible");
+    return NUC3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nuc3d_op_interpolate(int scheme, int part, int upwind, int w, int zquirk, int nx, int ny, int nz,
+                         const double *in, const double *ghost, double *out, void *stream)
+{
+    if (!in || !out) return fail(NUC3D_ERR_BAD_ARG, "null Field");
+    if (scheme < NUC3D_WENO5JS || scheme > NUC3D_RECON_BOUNDARY) return fail(NUC3D_ERR_BAD_ARG, "unknown scheme");
+    if (part < NUC3D_PART_INNER || part > NUC3D_PART_RIGHT) return fail(NUC3D_ERR_BAD_ARG, "unknown part");
+    if (upwind != 1 && upwind != -1) return fail(NUC3D_ERR_BAD_ARG, "upwind must be +1 or -1");
+    if (w < 2 || w > 4) return fail(NUC3D_ERR_UNSUPPORTED, "ghost width must be 2..4");
+    if (nx < 2 * w + 1 || ny < 1 || nz < 1) return fail(NUC3D_ERR_BAD_ARG, "line shorter than 2w+1 or empty Field");
+    if (scheme == NUC3D_RECON_BOUNDARY && part == NUC3D_PART_INNER)
+        return fail(NUC3D_ERR_BAD_ARG, "the physical-face closure has no inner part");
+    // the five-point stencils of faces w.. read cells w-3..: the reference runs
+    // them with a ghost width of 3 (fieldOperator.cpp:150-160)
+    if (scheme != NUC3D_RECON_BOUNDARY && w < 3)
+        return fail(NUC3D_ERR_UNSUPPORTED, "five-point reconstructions need a ghost width of 3 or more");
+    if (part != NUC3D_PART_INNER && !ghost) return fail(NUC3D_ERR_BAD_ARG, "null ghost Field");
+    if (int r = op_device_ready()) return r;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long nl = (long long)ny * nz;
+    OpBuffers B;
+    double *din = B.get(nl * nx), *dout = B.get(nl * (nx + 1));
+    double *dg = part == NUC3D_PART_INNER ? nullptr : B.get(nl * w);
+    if (!din || !dout || (part != NUC3D_PART_INNER && !dg)) return fail(NUC3D_ERR_OOM, "device allocation failed");
+    HIPCHK(hipMemcpyAsync(din, in, sizeof(double) * nl * nx, hipMemcpyHostToDevice, s));
+    if (dg) HIPCHK(hipMemcpyAsync(dg, ghost, sizeof(double) * nl * w, hipMemcpyHostToDevice, s));
+    const double *gl = part == NUC3D_PART_LEFT ? dg : nullptr, *gr = part == NUC3D_PART_RIGHT ? dg : nullptr;
+    const int plus = upwind > 0;
+    int m0, nf;
+    if (part == NUC3D_PART_INNER) {
+        m0 = w;
+        nf = nx - 2 * w + 1;
+        if (scheme == NUC3D_CRWENO5) {
+            double *scr = B.get(nl * 2 * nf);
+            if (!scr) return fail(NUC3D_ERR_OOM, "device allocation failed");
+            HIPCHK(nuc3d_launch_op_crweno(plus, nx, w, (int)nl, din, dout, scr, s));
+        } else {
+            HIPCHK(nuc3d_launch_op_interp(scheme, 0, plus, zquirk != 0 && plus && scheme == NUC3D_WENO5Z, nx, w,
+                                          (int)nl, m0, nf, din, gl, gr, dout, s));
+        }
+    } else {
+        m0 = part == NUC3D_PART_LEFT ? 0 : nx - w + 1;
+        nf = w;
+        // halo faces of every scheme use the WENO5 face formula of that scheme
+        // (crweno5.cpp:424-836 is WENO5-JS; weno5z.cpp:195-640 the textbook
+        // beta2); kind 1 / 2 is the physical-face closure
+        const int kind = scheme == NUC3D_RECON_BOUNDARY ? part : 0;
+        const int sch = scheme == NUC3D_WENO5Z ? NUC3D_WENO5Z : NUC3D_WENO5JS;
+        HIPCHK(nuc3d_launch_op_interp(sch, kind, plus, 0, nx, w, (int)nl, m0, nf, din, gl, gr, dout, s));
+    }
+    HIPCHK(hipMemcpy2DAsync(out + m0, sizeof(double) * (nx + 1), dout + m0, sizeof(double) * (nx + 1),
+                            sizeof(double) * nf, nl, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return NUC3D_OK;
+}
+
+int nuc3d_op_differentiate(int scheme, int part, int w, int nx, int ny, int nz, const double *in,
+                           const double *ghost, double *out, void *stream)
+{
+    if (!in || !out) return fail(NUC3D_ERR_BAD_ARG, "null Field");
+    if (scheme < NUC3D_CD6 || scheme > NUC3D_DIFF_BOUNDARY) return fail(NUC3D_ERR_BAD_ARG, "unknown scheme");
+    if (part < NUC3D_PART_INNER || part > NUC3D_PART_RIGHT) return fail(NUC3D_ERR_BAD_ARG, "unknown part");
+    if (w < 2 || w > 4) return fail(NUC3D_ERR_UNSUPPORTED, "ghost width must be 2..4");
+    if (scheme == NUC3D_CD6 && w < 3) return fail(NUC3D_ERR_UNSUPPORTED, "cd6 needs a ghost width of 3 or more");
+    // the reference's cd8 reads a fourth ghost layer (centraldifference8th.cpp:49-52)
+    if (scheme == NUC3D_CD8 && w < 4) return fail(NUC3D_ERR_UNSUPPORTED, "cd8 needs a ghost width of 4");
+    if (nx < 2 * w + 1 || ny < 1 || nz < 1) return fail(NUC3D_ERR_BAD_ARG, "line shorter than 2w+1 or empty Field");
+    if (scheme == NUC3D_DIFF_BOUNDARY && part == NUC3D_PART_INNER)
+        return fail(NUC3D_ERR_BAD_ARG, "the physical-face closure has no inner part");
+    const bool needs_ghost = part != NUC3D_PART_INNER && scheme != NUC3D_DIFF_BOUNDARY;
+    if (needs_ghost && !ghost) return fail(NUC3D_ERR_BAD_ARG, "null ghost Field");
+    if (int r = op_device_ready()) return r;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long nl = (long long)ny * nz;
+    OpBuffers B;
+    double *din = B.get(nl * nx), *dout = B.get(nl * nx);
+    double *dg = needs_ghost ? B.get(nl * w) : nullptr;
+    if (!din || !dout || (needs_ghost && !dg)) return fail(NUC3D_ERR_OOM, "device allocation failed");
+    HIPCHK(hipMemcpyAsync(din, in, sizeof(double) * nl * nx, hipMemcpyHostToDevice, s));
+    if (dg) HIPCHK(hipMemcpyAsync(dg, ghost, sizeof(double) * nl * w, hipMemcpyHostToDevice, s));
+    const double *gl = part == NUC3D_PART_LEFT ? dg : nullptr, *gr = part == NUC3D_PART_RIGHT ? dg : nullptr;
+    int t0, nt;
+    if (part == NUC3D_PART_INNER) {
+        t0 = w;
+        nt = nx - 2 * w;
+    } else {
+        t0 = part == NUC3D_PART_LEFT ? 0 : nx - w;
+        nt = w;
+    }
+    const int kind = scheme == NUC3D_DIFF_BOUNDARY ? part : 0;
+    if (nt > 0) {
+        HIPCHK(nuc3d_launch_op_diff(scheme == NUC3D_CD8, kind, nx, w, (int)nl, t0, nt, din, gl, gr, dout, s));
+        HIPCHK(hipMemcpy2DAsync(out + t0, sizeof(double) * nx, dout + t0, sizeof(double) * nx, sizeof(double) * nt,
+                                nl, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return NUC3D_OK;
+}
+
+int nuc3d_op_rk3(int stage, double dt, int nfields, long long n, const double *const *rhs,
+                 const double *const *u_n, double *const *u_i, void *stream)
+{
+    if (stage < 0 || stage > 2) return fail(NUC3D_ERR_BAD_ARG, "stage must be 0, 1 or 2");
+    if (nfields < 1 || n < 1 || !rhs || !u_i || (stage > 0 && !u_n)) return fail(NUC3D_ERR_BAD_ARG, "null field");
+    for (int f = 0; f < nfields; f++)
+        if (!rhs[f] || !u_i[f] || (stage > 0 && !u_n[f])) return fail(NUC3D_ERR_BAD_ARG, "null field");
+    if (int r = op_device_ready()) return r;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    OpBuffers B;
+    double *dr = B.get(n), *du = B.get(n), *dn = stage > 0 ? B.get(n) : nullptr;
+    if (!dr || !du || (stage > 0 && !dn)) return fail(NUC3D_ERR_OOM, "device allocation failed");
+    for (int f = 0; f < nfields; f++) {
+        HIPCHK(hipMemcpyAsync(dr, rhs[f], sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(du, u_i[f], sizeof(double) * n, hipMemcpyHostToDevice, s));
+        if (dn) HIPCHK(hipMemcpyAsync(dn, u_n[f], sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIPCHK(nuc3d_launch_op_rk3(stage, dt, n, dr, dn, du, s));
+        HIPCHK(hipMemcpyAsync(u_i[f], du, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
     }
     return NUC3D_OK;
 }

@@ -140,6 +140,31 @@ __device__ __forceinline__ double face_interior(const double *s, bool plus, bool
     return weno5js(s[5], s[4], s[3], s[2], s[1]);
 }
 
+// One inner row of the CRWENO5 system (crweno5.cpp:130-160 for +,
+// :310-340 for -): f points at cell i of face i+1 (element stride st); plus
+// uses cells i-2..i+2, minus cells i+3..i-1.  a, b, c are the coefficients of
+// faces i, i+1, i+2 (in line order), d the right-hand side.
+__device__ __forceinline__ void crweno_row(const double *f, int st, bool plus, double &a, double &b, double &c,
+                                           double &d)
+{
+    NUC3D_WENO_FP
+    // f points at cell i of face i+1; plus: cells i-2..i+2, minus: i+3..i-1
+    double f0, f1, f2, f3, f4;
+    if (plus) { f0 = f[-2 * st]; f1 = f[-st]; f2 = f[0]; f3 = f[st]; f4 = f[2 * st]; }
+    else { f0 = f[3 * st]; f1 = f[2 * st]; f2 = f[st]; f3 = f[0]; f4 = f[-st]; }
+    double b0, b1, b2;
+    smooth5(f0, f1, f2, f3, f4, false, b0, b1, b2);
+    const double a0 = R0 / sq(EPS + b0), a1 = R1 / sq(EPS + b1), a2 = R2 / sq(EPS + b2);
+    const double as = a0 + a1 + a2;
+    const double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
+    const double lo = 2.0 / 3.0 * o0 + 1.0 / 3.0 * o1;   // coefficient of the upwind-side face
+    const double hi = 1.0 / 3.0 * o2;                    // coefficient of the downwind-side face
+    b = 1.0 / 3.0 * o0 + 2.0 / 3.0 * (o1 + o2);
+    if (plus) { a = lo; c = hi; }
+    else { c = lo; a = hi; }
+    d = 1.0 / 6.0 * o0 * f1 + (5.0 * (o0 + o1) + o2) * f2 / 6.0 + (o1 + 5.0 * o2) * f3 / 6.0;
+}
+
 // Exterior (physical face) closure at face m, k faces from the physical
 // face (k = m on the low side, k = n - m on the high side).  s[0..5] holds the
 // split flux at cells m-3..m+2, where the slot of ghost cell -1 (+ flux) or n

@@ -454,27 +454,6 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
 // WENO5-JS face variants or the physical closures.  There is no line-length
 // cap (the reference's stack arrays hold 256 rows, crweno5.cpp:69-72).
 // line: x = block, y, z = transverse coordinates (j,k / i,k / i,j).
-__device__ __forceinline__ void crweno_row(const double *f, int st, bool plus, double &a, double &b, double &c,
-                                           double &d)
-{
-    NUC3D_WENO_FP
-    // f points at cell i of face i+1; plus: cells i-2..i+2, minus: i+3..i-1
-    double f0, f1, f2, f3, f4;
-    if (plus) { f0 = f[-2 * st]; f1 = f[-st]; f2 = f[0]; f3 = f[st]; f4 = f[2 * st]; }
-    else { f0 = f[3 * st]; f1 = f[2 * st]; f2 = f[st]; f3 = f[0]; f4 = f[-st]; }
-    double b0, b1, b2;
-    smooth5(f0, f1, f2, f3, f4, false, b0, b1, b2);
-    const double a0 = R0 / sq(EPS + b0), a1 = R1 / sq(EPS + b1), a2 = R2 / sq(EPS + b2);
-    const double as = a0 + a1 + a2;
-    const double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
-    const double lo = 2.0 / 3.0 * o0 + 1.0 / 3.0 * o1;   // coefficient of the upwind-side face
-    const double hi = 1.0 / 3.0 * o2;                    // coefficient of the downwind-side face
-    b = 1.0 / 3.0 * o0 + 2.0 / 3.0 * (o1 + o2);
-    if (plus) { a = lo; c = hi; }
-    else { c = lo; a = hi; }
-    d = 1.0 / 6.0 * o0 * f1 + (5.0 * (o0 + o1) + o2) * f2 / 6.0 + (o1 + 5.0 * o2) * f3 / 6.0;
-}
-
 template <int DIR, bool NS, int MODE>
 __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__restrict__ blocks,
                                                       const int4 *__restrict__ lines, nuc3d_kparams K, int stage,
@@ -980,10 +959,17 @@ extern "C" hipError_t KNAME(nuc3d_launch_crweno)(int dir, int ns, int mode, cons
 #undef NUC3D_A
 }
 
+extern "C" hipError_t KNAME(nuc3d_launch_prim)(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles, int ntiles,
+                                               nuc3d_kparams K, int bin, hipStream_t st)
+{
+    hipLaunchKernelGGL(KNAME(k_prim), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t KNAME(nuc3d_launch_visc_flux)(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles,
                                                     int ntiles, nuc3d_kparams K, int bin, hipStream_t st)
 {
-    hipLaunchKernelGGL(KNAME(k_prim), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+    (void)bin;
     hipLaunchKernelGGL(KNAME(k_visc_flux), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K);
     return hipGetLastError();
 }

@@ -1,0 +1,187 @@
+// Operator-level kernels: one reconstruction, derivative or TVD-RK3 update
+// of whole Fields, with the semantics of the reference's plugin interface
+// (interoplation / differential / integration, fieldOperator.h:15-80).
+// A Field is nx*ny*nz float64, i-fastest; the operator acts along x on the
+// ny*nz lines.  A ghost Field ("boundary") holds w cells per line: the left
+// one cells -w..-1 at slots 0..w-1, the right one cells n..n+w-1
+// (weno5js.cpp:255-266, :470-485).  These kernels keep the reference's
+// operation order (the same device functions as the stage kernels' exact
+// variant); they are the parity path of the drop-in operators, not the
+// performance path.
+#define NUC3D_EXACT 1
+#include "nuc3d_device.h"
+
+using namespace nuc3d_dev;
+
+namespace {
+
+// cell a of line l, -w <= a < n + w: interior, left ghost or right ghost.  A
+// part that has no ghost Field on a side passes NULL there; its stencils never
+// use those cells, and the value returned for them is 0.
+__device__ __forceinline__ double cell_at(const double *in, const double *gl, const double *gr, int n, int w, int l,
+                                          int a)
+{
+    if (a < 0) return gl ? gl[(long long)l * w + (a + w)] : 0.0;
+    if (a >= n) return gr ? gr[(long long)l * w + (a - n)] : 0.0;
+    return in[(long long)l * n + a];
+}
+
+// Faces [m0, m0+nf) of every line.  kind 0: scheme face (interior formula,
+// with the WENO5-Z quirk on + inner faces when zq); kind 1: exterior closure
+// on the low side; kind 2: exterior closure on the high side
+// (reconstructionBoundaryScheme.cpp:52-312).
+template <int SCHEME>
+__global__ __launch_bounds__(256) void k_op_interp(int kind, int plus, int zq, int n, int w, int nlines, int m0,
+                                                   int nf, const double *__restrict__ in,
+                                                   const double *__restrict__ gl, const double *__restrict__ gr,
+                                                   double *__restrict__ out)
+{
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= (long long)nf * nlines) return;
+    const int l = (int)(e / nf), m = m0 + (int)(e % nf);
+    double s[6];
+#pragma unroll
+    for (int o = 0; o < 6; o++) {
+        const int a = m - 3 + o;
+        // slots a stencil never reads may lie beyond the ghost frame
+        s[o] = (a >= -w && a < n + w) ? cell_at(in, gl, gr, n, w, l, a) : 0.0;
+    }
+    double v;
+    if (kind == 1)
+        v = face_exterior(s, plus != 0, m);
+    else if (kind == 2)
+        v = face_exterior(s, plus != 0, n - m);
+    else
+        v = face_interior<SCHEME>(s, plus != 0, zq != 0);
+    out[(long long)l * (n + 1) + m] = v;
+}
+
+// CRWENO5 inner faces w..n-w of every line (crweno5.cpp:44-222 for +,
+// :224-401 for -): rows 0 and M-1 are explicit WENO5-JS, the inner rows the
+// compact system; Thomas forward sweep and back substitution
+// (crweno5.cpp:207-218, :387-398).  One thread per line; c' and d' in scratch.
+__global__ __launch_bounds__(64) void k_op_crweno(int plus, int n, int w, int nlines, const double *__restrict__ in,
+                                                  double *__restrict__ out, double *__restrict__ scratch)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlines) return;
+    const int M = n - 2 * w + 1;
+    const double *p = in + (long long)l * n;
+    double *C = scratch + (long long)l * 2 * M, *D = C + M;
+    double cprev = 0.0, dprev = 0.0;
+    for (int r = 0; r < M; r++) {
+        const int i = w - 1 + r;  // reference loop index, face i+1
+        double a, b, c, d;
+        if (r == 0 || r == M - 1) {
+            const double *s6 = p + i - 2;  // cells i-2..i+3
+            a = 0.0;
+            b = 1.0;
+            c = 0.0;
+            d = plus ? weno5js(s6[0], s6[1], s6[2], s6[3], s6[4]) : weno5js(s6[5], s6[4], s6[3], s6[2], s6[1]);
+        } else {
+            crweno_row(p + i, 1, plus != 0, a, b, c, d);
+        }
+        if (r > 0) {
+            c = c / (b - cprev * a);
+            d = (d - dprev * a) / (b - cprev * a);
+        }
+        C[r] = c;
+        D[r] = d;
+        cprev = c;
+        dprev = d;
+    }
+    double *o = out + (long long)l * (n + 1) + w;
+    double x = D[M - 1];
+    o[M - 1] = x;
+    for (int r = M - 2; r >= 0; r--) {
+        x = D[r] - C[r] * x;
+        o[r] = x;
+    }
+}
+
+// Derivative along x of cells [t0, t0+nt) (centraldifference6th.cpp:16-183,
+// centraldifference8th.cpp:22-192); kind 1 / 2: exterior closure on the low /
+// high side (boundaryScheme.cpp:31-126).
+__global__ __launch_bounds__(256) void k_op_diff(int scheme, int kind, int n, int w, int nlines, int t0, int nt,
+                                                 const double *__restrict__ in, const double *__restrict__ gl,
+                                                 const double *__restrict__ gr, double *__restrict__ out)
+{
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= (long long)nt * nlines) return;
+    const int l = (int)(e / nt), t = t0 + (int)(e % nt);
+    // stencil radius: 4 for cd8, 3 for cd6 and the exterior closure
+    const int R = (kind == 0 && scheme == 1) ? 4 : 3;
+    double s[9];
+#pragma unroll
+    for (int o = 0; o < 9; o++) {
+        const int a = t - 4 + o;
+        s[o] = (a >= t - R && a <= t + R && a >= -w && a < n + w) ? cell_at(in, gl, gr, n, w, l, a) : 0.0;
+    }
+    double v;
+    if (kind == 1)
+        v = diff_exterior(&s[4], 1, t, true);
+    else if (kind == 2)
+        v = diff_exterior(&s[4], 1, n - 1 - t, false);
+    else
+        v = scheme == 1 ? cd8(&s[4], 1) : cd6(&s[4], 1);
+    out[(long long)l * n + t] = v;
+}
+
+// TVD-RK3 stage over one field (TVD-RK3.cpp:37-137)
+__global__ __launch_bounds__(256) void k_op_rk3(int stage, double dt, long long n, const double *__restrict__ rhs,
+                                                const double *__restrict__ un, double *__restrict__ ui)
+{
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double r = -rhs[e];
+    const double u = ui[e];
+    double v;
+    if (stage == 0)
+        v = 1.0 * u + 1.0 * r * dt;
+    else if (stage == 1)
+        v = 3.0 / 4.0 * un[e] + 1.0 / 4.0 * (r * dt + u);
+    else
+        v = 1.0 / 3.0 * un[e] + 2.0 / 3.0 * (r * dt + u);
+    ui[e] = v;
+}
+
+unsigned grid_of(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+extern "C" hipError_t nuc3d_launch_op_interp(int scheme, int kind, int plus, int zq, int n, int w, int nlines, int m0,
+                                             int nf, const double *in, const double *gl, const double *gr, double *out,
+                                             hipStream_t st)
+{
+    const unsigned g = grid_of((long long)nf * nlines, 256);
+    if (scheme == SCH_WENO5Z)
+        hipLaunchKernelGGL(k_op_interp<SCH_WENO5Z>, dim3(g), dim3(256), 0, st, kind, plus, zq, n, w, nlines, m0, nf,
+                           in, gl, gr, out);
+    else
+        hipLaunchKernelGGL(k_op_interp<SCH_WENO5JS>, dim3(g), dim3(256), 0, st, kind, plus, zq, n, w, nlines, m0, nf,
+                           in, gl, gr, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t nuc3d_launch_op_crweno(int plus, int n, int w, int nlines, const double *in, double *out,
+                                             double *scratch, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_op_crweno, dim3(grid_of(nlines, 64)), dim3(64), 0, st, plus, n, w, nlines, in, out, scratch);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t nuc3d_launch_op_diff(int scheme, int kind, int n, int w, int nlines, int t0, int nt,
+                                           const double *in, const double *gl, const double *gr, double *out,
+                                           hipStream_t st)
+{
+    hipLaunchKernelGGL(k_op_diff, dim3(grid_of((long long)nt * nlines, 256)), dim3(256), 0, st, scheme, kind, n, w,
+                       nlines, t0, nt, in, gl, gr, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t nuc3d_launch_op_rk3(int stage, double dt, long long n, const double *rhs, const double *un,
+                                          double *ui, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_op_rk3, dim3(grid_of(n, 256)), dim3(256), 0, st, stage, dt, n, rhs, un, ui);
+    return hipGetLastError();
+}

@@ -19,6 +19,9 @@ from . import LIB_PATH
 
 WENO5JS, WENO5Z, CRWENO5 = 0, 1, 2
 CD6, CD8 = 0, 1
+RECON_BOUNDARY = 3   # physical-face reconstruction closure (operator level)
+DIFF_BOUNDARY = 2    # physical-face derivative closure (operator level)
+PART_INNER, PART_LEFT, PART_RIGHT = 0, 1, 2
 LF, AUSM, AUSMP = 0, 1, 2
 EULER3D, NAVIERSTOKES3D = 0, 1
 
@@ -100,6 +103,10 @@ def lib():
         "nuc3d_get_fixups": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "nuc3d_timing": [vp, ctypes.c_int],
         "nuc3d_timing_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
+        "nuc3d_op_interpolate": [ctypes.c_int] * 8 + [dp, dp, dp, vp],
+        "nuc3d_op_differentiate": [ctypes.c_int] * 6 + [dp, dp, dp, vp],
+        "nuc3d_op_rk3": [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_longlong,
+                         ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp), vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -215,11 +222,57 @@ class Case:
     def timing(self, enable=True):
         _check(lib().nuc3d_timing(self.h, 1 if enable else 0), "timing")
 
-    TIMING_CLASSES = ("x_sweep", "y_sweep", "z_sweep", "viscous_flux", "viscous_div_update")
+    TIMING_CLASSES = ("x_sweep", "y_sweep", "z_sweep", "primitives", "viscous_flux", "viscous_div_update")
 
     def timing_read(self):
         """{class: (total_ms, launches)} since the last read."""
-        ms = (ctypes.c_double * 5)()
-        n = (ctypes.c_int * 5)()
+        ms = (ctypes.c_double * len(self.TIMING_CLASSES))()
+        n = (ctypes.c_int * len(self.TIMING_CLASSES))()
         _check(lib().nuc3d_timing_read(self.h, ms, n), "timing_read")
         return {k: (ms[i], n[i]) for i, k in enumerate(self.TIMING_CLASSES)}
+
+
+# ---------------------------------------------------------------------------
+# operator level (include/nuc3d_ops.h): one Field operation per call on host
+# arrays of shape (nz, ny, nx), the operator acting along x
+
+
+def _field(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def op_interpolate(scheme, part, upwind, w, field, ghost, out, zquirk=1):
+    """interpolationInner / BoundaryL / BoundaryR of the reference's scheme
+    classes (fieldOperator.h:23-38) into `out` (nz, ny, nx+1), in place."""
+    f = _field(field)
+    nz, ny, nx = f.shape
+    g = None if ghost is None else _field(ghost)
+    assert out.dtype == np.float64 and out.flags["C_CONTIGUOUS"] and out.shape == (nz, ny, nx + 1)
+    _check(lib().nuc3d_op_interpolate(scheme, part, upwind, w, zquirk, nx, ny, nz, _dp(f),
+                                      None if g is None else _dp(g), _dp(out), None), "op_interpolate")
+    return out
+
+
+def op_differentiate(scheme, part, w, field, ghost, out):
+    """differentialInner / BoundaryL / BoundaryR (fieldOperator.h:68-79) into
+    `out` (nz, ny, nx), in place."""
+    f = _field(field)
+    nz, ny, nx = f.shape
+    g = None if ghost is None else _field(ghost)
+    assert out.dtype == np.float64 and out.flags["C_CONTIGUOUS"] and out.shape == f.shape
+    _check(lib().nuc3d_op_differentiate(scheme, part, w, nx, ny, nz, _dp(f), None if g is None else _dp(g),
+                                        _dp(out), None), "op_differentiate")
+    return out
+
+
+def op_rk3(stage, dt, rhs, u_n, u_i):
+    """tvdrk3rd::integrationAll (TVD-RK3.cpp:12-137) over lists of fields;
+    updates the arrays of u_i in place."""
+    rs = [_field(a) for a in rhs]
+    ns = [_field(a) for a in u_n]
+    for a in u_i:
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    P = ctypes.POINTER(ctypes.c_double)
+    arr = lambda xs: (P * len(xs))(*[_dp(x) for x in xs])
+    _check(lib().nuc3d_op_rk3(stage, dt, len(u_i), u_i[0].size, arr(rs), arr(ns), arr(u_i), None), "op_rk3")
+    return u_i

@@ -19,6 +19,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libnuc3d_oracle.so")
 REF_BIN = os.path.join(HERE, "_ref", "ref_harness")
 REF_BIN_O2 = os.path.join(HERE, "_ref", "ref_harness_O2")
+# the reference's driver code with its operator units replaced by the drop-in
+# GPU definitions (nuc3d_amd/shim/): the caller of the operator-level ABI under
+# test, checked against REF_BIN
+REF_BIN_GPUOPS = os.path.join(HERE, "_ref", "ref_harness_gpuops")
 
 WENO5JS, WENO5Z, CRWENO5 = 0, 1, 2
 CD6, CD8 = 0, 1
@@ -301,9 +305,10 @@ def write_ref_case(path: str, params: Params, blocks: Sequence[Block]):
 
 
 def run_ref(path: str, nranks: int, mode: str = "rhs", nsteps: int = 0, cfl: float = 0.5,
-            optimised: bool = False, timeout: float = 600.0):
-    """Run the reference harness in `path`; return per-rank output dicts."""
-    exe = REF_BIN_O2 if optimised else REF_BIN
+            optimised: bool = False, timeout: float = 600.0, exe: str = None):
+    """Run the reference harness in `path` (or `exe`, another build of it);
+    return per-rank output dicts."""
+    exe = exe or (REF_BIN_O2 if optimised else REF_BIN)
     args = [exe, mode] + ([str(nsteps), repr(cfl)] if mode == "steps" else [])
     env = dict(os.environ)
     if nranks == 1:

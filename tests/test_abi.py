@@ -43,3 +43,40 @@ def test_bad_arguments_return_status():
     assert b"null" in L.nuc3d_last_error()
     assert L.nuc3d_rhs(None, None) == -1
     assert L.nuc3d_steps(None, 1, 0.5, None) == -1
+
+
+def test_operator_level_argument_errors():
+    """Argument checks of include/nuc3d_ops.h come before any device work."""
+    import numpy as np
+    from nuc3d_amd import solver
+    f = np.zeros((2, 3, 10))
+    g = np.zeros((2, 3, 3))
+    o = np.zeros((2, 3, 11))
+    d = np.zeros((2, 3, 10))
+    with pytest.raises(solver.Nuc3dError, match=r"\(-1\).*upwind"):
+        solver.op_interpolate(solver.WENO5JS, solver.PART_INNER, 0, 3, f, None, o)
+    with pytest.raises(solver.Nuc3dError, match=r"\(-1\).*inner part"):
+        solver.op_interpolate(solver.RECON_BOUNDARY, solver.PART_INNER, 1, 3, f, None, o)
+    with pytest.raises(solver.Nuc3dError, match=r"\(-1\).*ghost"):
+        solver.op_interpolate(solver.WENO5JS, solver.PART_LEFT, 1, 3, f, None, o)
+    with pytest.raises(solver.Nuc3dError, match=r"\(-4\).*cd8"):
+        solver.op_differentiate(solver.CD8, solver.PART_INNER, 3, f, None, d)
+    with pytest.raises(solver.Nuc3dError, match=r"\(-1\).*shorter"):
+        solver.op_differentiate(solver.CD6, solver.PART_INNER, 3, np.zeros((1, 1, 6)), None, np.zeros((1, 1, 6)))
+    with pytest.raises(solver.Nuc3dError, match=r"\(-1\).*stage"):
+        solver.op_rk3(3, 0.1, [f.ravel()], [f.ravel()], [f.ravel().copy()])
+
+
+def test_shim_sources_compile_against_the_reference_headers():
+    """The drop-in operator definitions (nuc3d_amd/shim/) compile against the
+    reference's own class declarations, where the reference tree is present."""
+    import shutil
+    import subprocess
+    src = "/root/reference/src"
+    if not os.path.isdir(src) or not shutil.which("g++"):
+        pytest.skip("reference headers absent")
+    shim = os.path.join(PKG_DIR, "shim")
+    for f in sorted(glob.glob(os.path.join(shim, "*.cpp"))):
+        r = subprocess.run(["g++", "-std=c++11", "-w", "-fsyntax-only", "-I" + src, "-I" + os.path.join(ROOT, "include"),
+                            f], capture_output=True, text=True)
+        assert r.returncode == 0, (f, r.stderr[-2000:])

@@ -1,0 +1,165 @@
+"""The operator-level ABI (include/nuc3d_ops.h) and the drop-in operator
+definitions (nuc3d_amd/shim/) on the GPU.
+
+1. Each Field operation -- interpolationInner/BoundaryL/BoundaryR of weno5js,
+   weno5z, crweno5 and the physical-face closure, differentialInner/... of
+   cd6, cd8 and the derivative closure, and tvdrk3rd::integrationAll -- is
+   compared line by line with the oracle (oracle/, bit-exact against the
+   reference build) on seeded random Fields and ghost Fields.
+2. The reference's own driver code (fieldOperator3d, EulerData3D,
+   NaiverStokesData3d, PDEData3d, physicsModel, boundary conditions, MPI halo
+   exchange), relinked with its eight operator units replaced by the drop-in
+   GPU definitions (oracle/_ref/ref_harness_gpuops), is run on the same cases
+   as the unmodified reference (oracle/_ref/ref_harness) and must agree to
+   the parity bar of tests/parity.py.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from nuc3d_amd import cases, solver
+from oracle import oracle as O
+from tests.parity import TOL, relerr
+
+pytestmark = pytest.mark.gpu
+
+SCHEMES = [(solver.WENO5JS, O.WENO5JS), (solver.WENO5Z, O.WENO5Z), (solver.CRWENO5, O.CRWENO5)]
+# (left type, right type): 0 = neighbour (ghost cells), -1 = physical face (closure)
+FACE_TYPES = [(0, 0), (-1, -1), (-1, 0)]
+
+
+def lines(seed, nz, ny, nx, w):
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal((nz, ny, nx)) + 2.0
+    gl = rng.standard_normal((nz, ny, w)) + 2.0
+    gr = rng.standard_normal((nz, ny, w)) + 2.0
+    return f, gl, gr
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("n", [7, 20, 300])
+@pytest.mark.parametrize("types", FACE_TYPES)
+@pytest.mark.parametrize("upwind", [1, -1])
+@pytest.mark.parametrize("scheme", SCHEMES, ids=["weno5js", "weno5z", "crweno5"])
+def test_op_interpolate_matches_oracle(scheme, upwind, types, n):
+    gs, os_ = scheme
+    w = 3
+    print("case: op_interpolate scheme=%d upwind=%d types=%s n=%d" % (gs, upwind, types, n), flush=True)
+    f, gl, gr = lines(100 + n, 2, 3, n, w)
+    out = np.full((2, 3, n + 1), np.nan)
+    solver.op_interpolate(gs, solver.PART_INNER, upwind, w, f, None, out)
+    solver.op_interpolate(solver.RECON_BOUNDARY if types[0] == -1 else gs, solver.PART_LEFT, upwind, w, f, gl, out)
+    solver.op_interpolate(solver.RECON_BOUNDARY if types[1] == -1 else gs, solver.PART_RIGHT, upwind, w, f, gr, out)
+    want = np.empty_like(out)
+    for k in range(2):
+        for j in range(3):
+            ext = np.concatenate([gl[k, j], f[k, j], gr[k, j]])
+            want[k, j] = O.recon_line(os_, upwind, ext, w, types[0], types[1], zquirk=1)
+    assert not np.isnan(out).any()
+    err = rel(out, want)
+    print("  rel err %.3e" % err, flush=True)
+    assert err <= 1e-13
+
+
+def test_op_interpolate_leaves_other_faces():
+    """Each part writes only its own faces, as the reference's methods do."""
+    f, gl, gr = lines(7, 1, 2, 16, 3)
+    out = np.full((1, 2, 17), -7.0)
+    solver.op_interpolate(solver.WENO5JS, solver.PART_LEFT, 1, 3, f, gl, out)
+    assert np.all(out[..., 3:] == -7.0) and np.all(out[..., :3] != -7.0)
+    solver.op_interpolate(solver.WENO5JS, solver.PART_INNER, 1, 3, f, None, out)
+    assert np.all(out[..., 14:] == -7.0) and np.all(out[..., :14] != -7.0)
+
+
+@pytest.mark.parametrize("n", [9, 40])
+@pytest.mark.parametrize("types", FACE_TYPES)
+@pytest.mark.parametrize("scheme,w", [((solver.CD6, O.CD6), 3), ((solver.CD8, O.CD8), 4)], ids=["cd6", "cd8"])
+def test_op_differentiate_matches_oracle(scheme, w, types, n):
+    gs, os_ = scheme
+    print("case: op_differentiate scheme=%d w=%d types=%s n=%d" % (gs, w, types, n), flush=True)
+    f, gl, gr = lines(200 + n, 2, 2, n, w)
+    out = np.full((2, 2, n), np.nan)
+    solver.op_differentiate(gs, solver.PART_INNER, w, f, None, out)
+    solver.op_differentiate(solver.DIFF_BOUNDARY if types[0] == -1 else gs, solver.PART_LEFT, w, f, gl, out)
+    solver.op_differentiate(solver.DIFF_BOUNDARY if types[1] == -1 else gs, solver.PART_RIGHT, w, f, gr, out)
+    want = np.empty_like(out)
+    for k in range(2):
+        for j in range(2):
+            ext = np.concatenate([gl[k, j], f[k, j], gr[k, j]])
+            want[k, j] = O.diff_line(os_, ext, w, types[0], types[1])
+    assert not np.isnan(out).any()
+    np.testing.assert_array_equal(out, want)
+
+
+@pytest.mark.parametrize("stage", [0, 1, 2])
+def test_op_rk3_bit_exact(stage):
+    rng = np.random.default_rng(30 + stage)
+    n = 4 * 5 * 6
+    rhs = [rng.standard_normal(n) for _ in range(5)]
+    un = [rng.standard_normal(n) for _ in range(5)]
+    ui = [rng.standard_normal(n) for _ in range(5)]
+    want = [O.rk3(stage, rhs[c], un[c], ui[c].copy(), 0.37) for c in range(5)]
+    got = solver.op_rk3(stage, 0.37, rhs, un, [u.copy() for u in ui])
+    for c in range(5):
+        np.testing.assert_array_equal(got[c], want[c])
+
+
+# ---------------------------------------------------------------------------
+# the reference's driver on the drop-in GPU operators
+
+needs_gpuops = pytest.mark.skipif(not (O.ref_available() and os.path.exists(O.REF_BIN_GPUOPS)),
+                                  reason="reference harness builds (oracle/_ref) absent")
+
+
+def run_both(params, blocks, mode, nsteps=0, cfl=0.5):
+    outs = []
+    for exe in (O.REF_BIN, O.REF_BIN_GPUOPS):
+        d = tempfile.mkdtemp(prefix="nuc3d_dropin_")
+        O.write_ref_case(d, params, blocks)
+        outs.append(O.run_ref(d, len(blocks), mode, nsteps, cfl, exe=exe, timeout=300))
+    return outs
+
+
+def walls_case():
+    """NS block with isothermal walls, symmetry, inflow and outflow faces: the
+    reconstruction and derivative closures and every BC ghost setter."""
+    b = cases.taylor_green(dims=(16, 12, 10), noise=1e-3, seed=45)[0]
+    b.faces = [(-1, 3, None), (-1, 2, None), (-1, 0, [1.0, 0.1, 0.0, 0.0, 71.0]), (-1, 1, None), (-1, 2, None),
+               (-1, 3, None)]
+    return [b]
+
+
+DROPIN_CASES = {
+    "tgv_ns_weno5js_rhs": (lambda: (O.Params(model=O.NS), cases.taylor_green(dims=(16, 12, 10), noise=1e-2, seed=41),
+                                    "rhs", 0)),
+    "tgv_euler_crweno5_steps2": (lambda: (O.Params(model=O.EULER, scheme=(O.CRWENO5,) * 3),
+                                          cases.taylor_green(dims=(14, 12, 10), noise=1e-2, seed=42), "steps", 2)),
+    "shuosher_ausm_weno5z_steps2": (lambda: (O.Params(model=O.EULER, mach=1.0, riemann=O.AUSM,
+                                                      scheme=(O.WENO5Z,) * 3),
+                                             cases.shu_osher(24, 8, 8, seed=43), "steps", 2)),
+    "walls_ns_steps2": (lambda: (O.Params(model=O.NS, t_wall=1.2), walls_case(), "steps", 2)),
+    "multiblock2_ns_weno5z_steps1": (lambda: (O.Params(model=O.NS, scheme=(O.WENO5Z,) * 3),
+                                              cases.split_block_x(cases.taylor_green(dims=(20, 10, 9), noise=1e-3,
+                                                                                     seed=44)[0], 2),
+                                              "steps", 1)),
+}
+
+
+@needs_gpuops
+@pytest.mark.parametrize("name", sorted(DROPIN_CASES))
+def test_reference_driver_on_gpu_operators(name):
+    params, blocks, mode, nsteps = DROPIN_CASES[name]()
+    print("case: drop-in %s (%d blocks)" % (name, len(blocks)), flush=True)
+    ref, gpu = run_both(params, blocks, mode, nsteps)
+    key = "rhs" if mode == "rhs" else "q"
+    for b in range(len(blocks)):
+        err = relerr(gpu[b][key], ref[b][key])
+        print("  block %d %s rel err %.3e" % (b, key, err), flush=True)
+        assert err <= TOL
+    if mode == "steps":
+        np.testing.assert_allclose(gpu[0]["dts"], ref[0]["dts"], rtol=1e-13)
