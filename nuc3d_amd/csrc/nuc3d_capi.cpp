@@ -79,7 +79,7 @@ const Launchers kFast = {nuc3d_launch_sweep_fast, nuc3d_launch_prim_fast, nuc3d_
 const Launchers kExact = {nuc3d_launch_sweep_exact, nuc3d_launch_prim_exact, nuc3d_launch_visc_flux_exact, nuc3d_launch_visc_div_exact, nuc3d_launch_eigen_exact,
                           nuc3d_launch_fixup_exact, nuc3d_launch_ghost_fill_exact, nuc3d_launch_crweno_exact};
 
-constexpr int KSEG = 32;  // planes per k segment of a tile
+constexpr int KSEG = NUC3D_KSEG;  // planes per k segment of a tile
 
 }  // namespace
 

@@ -280,28 +280,27 @@ __device__ __forceinline__ double stage_dt(const nuc3d_kparams &K, int stage, do
     return dt;
 }
 
-// TVD-RK3 stage of cell x from RHS r (TVD-RK3.cpp:37-137), e<0 fix-up
-// (physicsModel.cpp:322-332), ghost push, and the eigenvalue maxima of the new
-// state at stage 2 (physicsModel.cpp:557-597).
-__device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
-                                          const PhysParams &P, int i, int j, int k, long long x, const double *r,
-                                          int stage, int bin, int bun, int bout, double dt, double *lmax,
-                                          unsigned long long &nfix)
+// TVD-RK3 stage of cell x from RHS r (TVD-RK3.cpp:37-137) with the cell's
+// current state u5, u_n (un5, stages 1 and 2) and J already loaded; e<0
+// fix-up (physicsModel.cpp:322-332), ghost push, and the eigenvalue maxima of
+// the new state at stage 2 (physicsModel.cpp:557-597).
+__device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
+                                               const PhysParams &P, int i, int j, int k, long long x, const double *r,
+                                               const double *u5, const double *un5, double jac, int stage, int bout,
+                                               double dt, double *lmax, unsigned long long &nfix)
 {
-    double *const *qin = B.q[bin];
     double qn[5];
 #pragma unroll
     for (int c = 0; c < 5; c++) {
         const double rhs = -r[c];
-        const double u = qin[c][x];
+        const double u = u5[c];
         if (stage == 0)
             qn[c] = 1.0 * u + 1.0 * rhs * dt;
         else if (stage == 1)
-            qn[c] = 3.0 / 4.0 * B.q[bun][c][x] + 1.0 / 4.0 * (rhs * dt + u);
+            qn[c] = 3.0 / 4.0 * un5[c] + 1.0 / 4.0 * (rhs * dt + u);
         else
-            qn[c] = 1.0 / 3.0 * B.q[bun][c][x] + 2.0 / 3.0 * (rhs * dt + u);
+            qn[c] = 1.0 / 3.0 * un5[c] + 2.0 / 3.0 * (rhs * dt + u);
     }
-    const double jac = B.jac[x];
     if (fixup_energy(qn, jac, P)) nfix++;
 #pragma unroll
     for (int c = 0; c < 5; c++) B.q[bout][c][x] = qn[c];
@@ -318,6 +317,21 @@ __device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const n
             lmax[d] = (lmax[d] < lam) ? lam : lmax[d];
         }
     }
+}
+
+// The same, loading the cell's state from buffers bin / bun.
+__device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
+                                          const PhysParams &P, int i, int j, int k, long long x, const double *r,
+                                          int stage, int bin, int bun, int bout, double dt, double *lmax,
+                                          unsigned long long &nfix)
+{
+    double u5[5], un5[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        u5[c] = B.q[bin][c][x];
+        un5[c] = stage == 0 ? 0.0 : B.q[bun][c][x];
+    }
+    rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, B.jac[x], stage, bout, dt, lmax, nfix);
 }
 
 // Cell (i,j,k) receives the flux difference dF of direction DIR:
@@ -570,32 +584,35 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 
 // ---------------------------------------------------------------------------
 // Navier-Stokes viscous terms (NaiverStokes3d.cpp:51-513) in three passes
-// over 16x16 column tiles marching in k, one thread per column, with no LDS
-// and no barriers: waves never wait for each other, and neighbouring columns
-// of a tile share L1/L2 lines.
+// over 16x16 column tiles marching in k.
 //
 // k_prim:      u, v, w, T of every cell (con2prim, physicsModel.cpp:313-353,
 //              T = gamma Ma^2 p / rho, physicsModel.cpp:1029), pushed to the
 //              neighbours' ghost layers like Q.
-// k_visc_flux: cd6 (or exterior closure) derivatives of u, v, w, T along
-//              xi, eta (gathered from L1/L2) and zeta (a seven-plane register
-//              ring of the thread's own column, so the L2 working set is one
-//              plane deep) (NaiverStokes3d.cpp:86-168), chain rule with the
+// k_visc_flux: cd6 (or exterior closure) derivatives of u, v, w, T along xi
+//              and eta from a double-buffered LDS plane with its xy halo, and
+//              along zeta from a seven-plane register ring of the thread's own
+//              column (NaiverStokes3d.cpp:86-168); chain rule with the
 //              metrics, Sutherland viscosity, stresses, heat flux and the
 //              contravariant fluxes over J (NaiverStokes3d.cpp:170-326).
 // k_visc_div:  after the three inviscid sweeps, rhs -= dFv/dxi, dGv/deta,
 //              dHv/dzeta in the reference's order (NaiverStokes3d.cpp:401-513),
-//              fv_zeta from a register ring as above, then the RHS (MODE 0) or
-//              the TVD-RK3 stage (MODE 1).
+//              fv_xi / fv_eta from LDS planes and fv_zeta from a register ring,
+//              then the RHS (MODE 0) or the TVD-RK3 stage (MODE 1).
+//
+// The loads of plane k+1 (ring, halo, centre values, and J + metrics or the
+// RHS) are issued before plane k is computed, so memory latency overlaps the
+// arithmetic; one barrier per plane separates the LDS fill from its use.
 
-// Waves per SIMD the viscous kernels are register-limited to (measured at
-// 256^3: the flux pass is fastest with all of its loads in flight at 2 waves,
-// the divergence pass with 4 waves and a few spills).
+// Waves per SIMD the viscous kernels are register-limited to.  Measured at
+// 256^3: both run fastest unconstrained (about 224 / 231 VGPRs, 2 waves per
+// SIMD, every load of the next plane in flight); capping them at 3 or 4 waves
+// spills and is slower.
 #ifndef NUC3D_VISC_FLUX_WAVES
 #define NUC3D_VISC_FLUX_WAVES 2
 #endif
 #ifndef NUC3D_VISC_DIV_WAVES
-#define NUC3D_VISC_DIV_WAVES 4
+#define NUC3D_VISC_DIV_WAVES 2
 #endif
 
 __device__ __forceinline__ void prim4(const nuc3d_block_dev &B, double *const *qb, long long x, const PhysParams &P,
@@ -612,102 +629,174 @@ __device__ __forceinline__ void prim4(const nuc3d_block_dev &B, double *const *q
     w4[3] = P.gamma * P.mach * P.mach * p / rho;  // physicsModel.cpp:1029
 }
 
+// one workgroup per (tile, plane): blockIdx.y = plane within the tile's k segment
 __global__ __launch_bounds__(256) void KNAME(k_prim)(const nuc3d_block_dev *__restrict__ blocks,
                                                      const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int bin)
 {
     constexpr int T = NUC3D_TILE;
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    if ((int)blockIdx.y >= tl.kn) return;
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K);
-    const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T;
+    const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T, k = tl.k0 + blockIdx.y;
     if (i >= B.n[0] || j >= B.n[1]) return;
-    double *const *qin = B.q[bin];
-    for (int k = tl.k0; k < tl.k0 + tl.kn; k++) {
-        const long long x = cidx(B, i, j, k);
-        double w4[4];
-        prim4(B, qin, x, P, w4);
+    const long long x = cidx(B, i, j, k);
+    double w4[4];
+    prim4(B, B.q[bin], x, P, w4);
 #pragma unroll
-        for (int v = 0; v < 4; v++) B.prim[v][x] = w4[v];
+    for (int v = 0; v < 4; v++) B.prim[v][x] = w4[v];
 #pragma unroll
-        for (int f = 0; f < 6; f++) {
-            int nb;
-            long long y;
-            if (ghost_target(blocks, B, f, i, j, k, nb, y)) {
+    for (int f = 0; f < 6; f++) {
+        int nb;
+        long long y;
+        if (ghost_target(blocks, B, f, i, j, k, nb, y)) {
 #pragma unroll
-                for (int v = 0; v < 4; v++) blocks[nb].prim[v][y] = w4[v];
-            }
+            for (int v = 0; v < 4; v++) blocks[nb].prim[v][y] = w4[v];
         }
     }
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_FLUX_WAVES))) void KNAME(k_visc_flux)(const nuc3d_block_dev *__restrict__ blocks,
-                                                          const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K)
+// Halo cell h (0..191) of a 16x16 column tile: h < 96 are the strips of 3
+// cells left and right of each row (x halo), h >= 96 the strips above and
+// below each column (y halo); no corners (the derivatives are not mixed).
+// Tile-local (li, lj); false for rows/columns beyond a partial tile.
+__device__ __forceinline__ bool halo_cell(int h, int tw, int th, int &li, int &lj)
 {
-    constexpr int T = NUC3D_TILE, H = NUC3D_HALO;
+    constexpr int H = NUC3D_HALO;
+    if (h < 16 * 2 * H) {
+        const int r = h / (2 * H), s = h % (2 * H);
+        if (r >= th) return false;
+        lj = r;
+        li = s < H ? s - H : tw + (s - H);
+    } else {
+        const int c = (h - 16 * 2 * H) / (2 * H), s = (h - 16 * 2 * H) % (2 * H);
+        if (c >= tw) return false;
+        li = c;
+        lj = s < H ? s - H : th + (s - H);
+    }
+    return true;
+}
+
+// Viscous fluxes of one cell from its u,v,w,T gradients along xi, eta, zeta
+// (dxi/deta/dzeta [4] each) and centre values (NaiverStokes3d.cpp:170-326).
+__device__ __forceinline__ void visc_flux_cell(const nuc3d_kparams &K, const double *dxi, const double *deta,
+                                               const double *dzeta, const double *w4, const double *m, double jac,
+                                               double *out)
+{
+    const double xi_x = m[0], xi_y = m[1], xi_z = m[2];
+    const double eta_x = m[3], eta_y = m[4], eta_z = m[5];
+    const double zeta_x = m[6], zeta_y = m[7], zeta_z = m[8];
+    double g[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const double a = dxi[v], bb = deta[v], cc = dzeta[v];
+        g[v][0] = a * xi_x + bb * eta_x + cc * zeta_x;
+        g[v][1] = a * xi_y + bb * eta_y + cc * zeta_y;
+        g[v][2] = a * xi_z + bb * eta_z + cc * zeta_z;
+    }
+    const double S = K.sref;
+    const double u = w4[0], vv = w4[1], w = w4[2], T0 = w4[3];
+    const double miu0 = (1.0 + S) / (T0 + S) * pow(T0, 1.5) / K.reynolds;  // physicsModel.cpp:226
+    const double coeff0 = miu0 / ((K.gamma - 1.0) * K.mach * K.mach * K.prandtl);
+    const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
+    const double vx = g[1][0], vy = g[1][1], vz = g[1][2];
+    const double wx = g[2][0], wy = g[2][1], wz = g[2][2];
+    const double grad = ux + vy + wz;
+    const double txx = miu0 * (2.0 * ux - 2.0 / 3.0 * grad);
+    const double txy = miu0 * (uy + vx - 2.0 / 3.0 * grad);
+    const double txz = miu0 * (uz + wx - 2.0 / 3.0 * grad);
+    const double tyy = miu0 * (2.0 * vy - 2.0 / 3.0 * grad);
+    const double tyz = miu0 * (vz + wy - 2.0 / 3.0 * grad);
+    const double tzz = miu0 * (2.0 * wz - 2.0 / 3.0 * grad);
+    const double ttx = coeff0 * g[3][0], tty = coeff0 * g[3][1], ttz = coeff0 * g[3][2];
+    const double fv[4] = {txx, txy, txz, u * txx + vv * txy + w * txz + ttx};
+    const double gv[4] = {txy, tyy, tyz, u * txy + vv * tyy + w * tyz + tty};
+    const double hv[4] = {txz, tyz, tzz, u * txz + vv * tyz + w * tzz + ttz};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        out[c] = (xi_x * fv[c] + xi_y * gv[c] + xi_z * hv[c]) / jac;
+        out[4 + c] = (eta_x * fv[c] + eta_y * gv[c] + eta_z * hv[c]) / jac;
+        out[8 + c] = (zeta_x * fv[c] + zeta_y * gv[c] + zeta_z * hv[c]) / jac;
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_FLUX_WAVES))) void KNAME(k_visc_flux)(
+    const nuc3d_block_dev *__restrict__ blocks, const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K)
+{
+    constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
+    __shared__ double Wp[2][4][L][L];  // u,v,w,T of plane k with the xy halo, double-buffered
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
-    const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T;
-    if (i >= B.n[0] || j >= B.n[1]) return;
-    const double S = K.sref;
-    double zr[2 * H + 1][4];  // u, v, w, T of the own column, planes p-6..p
+    const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
+    const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
+    const bool incell = ti < tw && tj < th;
+    const int i = tl.i0 + ti, j = tl.j0 + tj;
+    int hi = 0, hj = 0;
+    const bool hv = threadIdx.x < 2 * (2 * H * T) && halo_cell(threadIdx.x, tw, th, hi, hj);
+    const long long xc = cidx(B, i, j, 0), xh = cidx(B, tl.i0 + hi, tl.j0 + hj, 0), sz = B.sz;
+    const int kb = tl.k0, ke = tl.k0 + tl.kn;
+
+    double zr[2 * H + 1][4];  // u,v,w,T of the own column, planes p-6..p
 #pragma unroll
     for (int a = 0; a < 2 * H + 1; a++)
 #pragma unroll
         for (int v = 0; v < 4; v++) zr[a][v] = 0.0;
-    const int kb = tl.k0, ke = tl.k0 + tl.kn;
+    double pc[4] = {0.0, 0.0, 0.0, 0.0}, ph[4] = {0.0, 0.0, 0.0, 0.0};  // prefetched plane p / halo plane k
+    double pm[10];                                                       // prefetched J + metrics of cell k
+#pragma unroll
+    for (int e = 0; e < 10; e++) pm[e] = 0.0;
+    if (incell)
+#pragma unroll
+        for (int v = 0; v < 4; v++) pc[v] = B.prim[v][xc + (kb - H) * sz];
+
     for (int p = kb - H; p < ke + H; p++) {
+        const int k = p - H, buf = p & 1;
 #pragma unroll
         for (int a = 0; a < 2 * H; a++)
 #pragma unroll
             for (int v = 0; v < 4; v++) zr[a][v] = zr[a + 1][v];
-        const long long xp = cidx(B, i, j, p);
 #pragma unroll
-        for (int v = 0; v < 4; v++) zr[2 * H][v] = B.prim[v][xp];
-        const int k = p - H;
-        if (k < kb) continue;
-        const long long x = cidx(B, i, j, k);
-        const double xi_x = B.met[0][x], xi_y = B.met[1][x], xi_z = B.met[2][x];
-        const double eta_x = B.met[3][x], eta_y = B.met[4][x], eta_z = B.met[5][x];
-        const double zeta_x = B.met[6][x], zeta_y = B.met[7][x], zeta_z = B.met[8][x];
-        double g[4][3];
+        for (int v = 0; v < 4; v++) zr[2 * H][v] = pc[v];
+        double m[10];
+#pragma unroll
+        for (int e = 0; e < 10; e++) m[e] = pm[e];
+        if (k >= kb) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                if (incell) Wp[buf][v][tj + H][ti + H] = zr[H][v];
+                if (hv) Wp[buf][v][hj + H][hi + H] = ph[v];
+            }
+        }
+        // loads for the next plane, in flight during this plane's work
+        if (incell && p + 1 < ke + H)
+#pragma unroll
+            for (int v = 0; v < 4; v++) pc[v] = B.prim[v][xc + (p + 1) * sz];
+        if (k + 1 >= kb && k + 1 < ke) {
+            if (hv)
+#pragma unroll
+                for (int v = 0; v < 4; v++) ph[v] = B.prim[v][xh + (k + 1) * sz];
+            if (incell) {
+                pm[0] = B.jac[xc + (k + 1) * sz];
+#pragma unroll
+                for (int e = 0; e < 9; e++) pm[1 + e] = B.met[e][xc + (k + 1) * sz];
+            }
+        }
+        __syncthreads();
+        if (k < kb || !incell) continue;
+        double dxi[4], deta[4], dzeta[4], w4[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const double *f = B.prim[v] + x;
             double col[2 * H + 1];
 #pragma unroll
             for (int a = 0; a < 2 * H + 1; a++) col[a] = zr[a][v];
-            const double a = line_diff(B, K, f, 0, i);
-            const double bb = line_diff(B, K, f, 1, j);
-            const double cc = line_diff_lds(B, K, &col[H], 1, 2, k);
-            g[v][0] = a * xi_x + bb * eta_x + cc * zeta_x;
-            g[v][1] = a * xi_y + bb * eta_y + cc * zeta_y;
-            g[v][2] = a * xi_z + bb * eta_z + cc * zeta_z;
+            dxi[v] = line_diff_lds(B, K, &Wp[buf][v][tj + H][ti + H], 1, 0, i);
+            deta[v] = line_diff_lds(B, K, &Wp[buf][v][tj + H][ti + H], L, 1, j);
+            dzeta[v] = line_diff_lds(B, K, &col[H], 1, 2, k);
+            w4[v] = zr[H][v];
         }
-        const double u = zr[H][0], vv = zr[H][1], w = zr[H][2], T0 = zr[H][3];
-        const double miu0 = (1.0 + S) / (T0 + S) * pow(T0, 1.5) / K.reynolds;  // physicsModel.cpp:226
-        const double coeff0 = miu0 / ((K.gamma - 1.0) * K.mach * K.mach * K.prandtl);
-        const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
-        const double vx = g[1][0], vy = g[1][1], vz = g[1][2];
-        const double wx = g[2][0], wy = g[2][1], wz = g[2][2];
-        const double grad = ux + vy + wz;
-        const double txx = miu0 * (2.0 * ux - 2.0 / 3.0 * grad);
-        const double txy = miu0 * (uy + vx - 2.0 / 3.0 * grad);
-        const double txz = miu0 * (uz + wx - 2.0 / 3.0 * grad);
-        const double tyy = miu0 * (2.0 * vy - 2.0 / 3.0 * grad);
-        const double tyz = miu0 * (vz + wy - 2.0 / 3.0 * grad);
-        const double tzz = miu0 * (2.0 * wz - 2.0 / 3.0 * grad);
-        const double ttx = coeff0 * g[3][0], tty = coeff0 * g[3][1], ttz = coeff0 * g[3][2];
-        const double fv[4] = {txx, txy, txz, u * txx + vv * txy + w * txz + ttx};
-        const double gv[4] = {txy, tyy, tyz, u * txy + vv * tyy + w * tyz + tty};
-        const double hv[4] = {txz, tyz, tzz, u * txz + vv * tyz + w * tzz + ttz};
-        const double jac = B.jac[x];
         double out[12];
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            out[c] = (xi_x * fv[c] + xi_y * gv[c] + xi_z * hv[c]) / jac;
-            out[4 + c] = (eta_x * fv[c] + eta_y * gv[c] + eta_z * hv[c]) / jac;
-            out[8 + c] = (zeta_x * fv[c] + zeta_y * gv[c] + zeta_z * hv[c]) / jac;
-        }
+        visc_flux_cell(K, dxi, deta, dzeta, w4, m + 1, m[0], out);
+        const long long x = xc + k * sz;
 #pragma unroll
         for (int e = 0; e < 12; e++) B.fv[e][x] = out[e];
 #pragma unroll
@@ -723,57 +812,105 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_DIV_WAVES))) void KNAME(k_visc_div)(const nuc3d_block_dev *__restrict__ blocks,
-                                                         const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K,
-                                                         int stage, int bin, int bun, int bout, double *dtbuf,
-                                                         const unsigned long long *lam_cur,
-                                                         unsigned long long *lam_next, unsigned long long *counters)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_DIV_WAVES))) void KNAME(k_visc_div)(
+    const nuc3d_block_dev *__restrict__ blocks, const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int stage,
+    int bin, int bun, int bout, double *dtbuf, const unsigned long long *lam_cur, unsigned long long *lam_next,
+    unsigned long long *counters)
 {
-    constexpr int T = NUC3D_TILE, H = NUC3D_HALO;
+    constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
+    __shared__ double Vx[2][4][T][L];  // fv_xi (components 1..4) of plane k, x halo, double-buffered
+    __shared__ double Vy[2][4][L][T];  // fv_eta, y halo
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K);
-    const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T;
+    const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
+    const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
+    const bool incell = ti < tw && tj < th;
+    const int i = tl.i0 + ti, j = tl.j0 + tj;
+    int hi = 0, hj = 0;
+    const bool hv = threadIdx.x < 2 * (2 * H * T) && halo_cell(threadIdx.x, tw, th, hi, hj);
+    const bool hx = threadIdx.x < 2 * H * T;  // x halo (fv_xi) or y halo (fv_eta)
+    const long long xc = cidx(B, i, j, 0), xh = cidx(B, tl.i0 + hi, tl.j0 + hj, 0), sz = B.sz;
+    const int kb = tl.k0, ke = tl.k0 + tl.kn;
     const double dt = MODE == 1 ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
-    if (i < B.n[0] && j < B.n[1]) {
-        double zr[2 * H + 1][4];  // fv_zeta (components 1..4) of the own column, planes p-6..p
+
+    double zr[2 * H + 1][4];  // fv_zeta (components 1..4) of the own column, planes p-6..p
 #pragma unroll
-        for (int a = 0; a < 2 * H + 1; a++)
+    for (int a = 0; a < 2 * H + 1; a++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) zr[a][v] = 0.0;
-        const int kb = tl.k0, ke = tl.k0 + tl.kn;
-        for (int p = kb - H; p < ke + H; p++) {
+        for (int v = 0; v < 4; v++) zr[a][v] = 0.0;
+    double pz[4] = {0.0, 0.0, 0.0, 0.0}, ph[4] = {0.0, 0.0, 0.0, 0.0}, pxy[8];
+    double pr[5];  // RHS of cell k (prefetched; the state is loaded when the plane is computed)
 #pragma unroll
-            for (int a = 0; a < 2 * H; a++)
+    for (int e = 0; e < 8; e++) pxy[e] = 0.0;
 #pragma unroll
-                for (int v = 0; v < 4; v++) zr[a][v] = zr[a + 1][v];
-            const long long xp = cidx(B, i, j, p);
+    for (int c = 0; c < 5; c++) pr[c] = 0.0;
+    if (incell)
 #pragma unroll
-            for (int v = 0; v < 4; v++) zr[2 * H][v] = B.fv[8 + v][xp];
-            const int k = p - H;
-            if (k < kb) continue;
-            const long long x = cidx(B, i, j, k);
-            double r[5];
+        for (int v = 0; v < 4; v++) pz[v] = B.fv[8 + v][xc + (kb - H) * sz];
+
+    for (int p = kb - H; p < ke + H; p++) {
+        const int k = p - H, buf = p & 1;
 #pragma unroll
-            for (int c = 0; c < 5; c++) r[c] = B.rhs[c][x];
+        for (int a = 0; a < 2 * H; a++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) zr[a][v] = zr[a + 1][v];
+#pragma unroll
+        for (int v = 0; v < 4; v++) zr[2 * H][v] = pz[v];
+        double r[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) r[c] = pr[c];
+        if (k >= kb) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                double col[2 * H + 1];
-#pragma unroll
-                for (int a = 0; a < 2 * H + 1; a++) col[a] = zr[a][v];
-                const double gx = line_diff(B, K, B.fv[v] + x, 0, i);
-                const double gy = line_diff(B, K, B.fv[4 + v] + x, 1, j);
-                const double gz = line_diff_lds(B, K, &col[H], 1, 2, k);
-                r[1 + v] = ((r[1 + v] - gx) - gy) - gz;
+                if (incell) {
+                    Vx[buf][v][tj][ti + H] = pxy[v];
+                    Vy[buf][v][tj + H][ti] = pxy[4 + v];
+                }
+                if (hv) {
+                    if (hx)
+                        Vx[buf][v][hj][hi + H] = ph[v];
+                    else
+                        Vy[buf][v][hj + H][hi] = ph[v];
+                }
             }
-            if (MODE == 0) {
+        }
+        // loads for the next plane, in flight during this plane's work
+        if (incell && p + 1 < ke + H)
 #pragma unroll
-                for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
-            } else {
-                rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix);
+            for (int v = 0; v < 4; v++) pz[v] = B.fv[8 + v][xc + (p + 1) * sz];
+        if (k + 1 >= kb && k + 1 < ke) {
+            const long long x1 = xc + (k + 1) * sz;
+            if (hv)
+#pragma unroll
+                for (int v = 0; v < 4; v++) ph[v] = B.fv[(hx ? 0 : 4) + v][xh + (k + 1) * sz];
+            if (incell) {
+#pragma unroll
+                for (int e = 0; e < 8; e++) pxy[e] = B.fv[e][x1];
+#pragma unroll
+                for (int c = 0; c < 5; c++) pr[c] = B.rhs[c][x1];
             }
+        }
+        __syncthreads();
+        if (k < kb || !incell) continue;
+        const long long x = xc + k * sz;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double col[2 * H + 1];
+#pragma unroll
+            for (int a = 0; a < 2 * H + 1; a++) col[a] = zr[a][v];
+            const double gx = line_diff_lds(B, K, &Vx[buf][v][tj][ti + H], 1, 0, i);
+            const double gy = line_diff_lds(B, K, &Vy[buf][v][tj + H][ti], T, 1, j);
+            const double gz = line_diff_lds(B, K, &col[H], 1, 2, k);
+            r[1 + v] = ((r[1 + v] - gx) - gy) - gz;
+        }
+        if (MODE == 0) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
+        } else {
+            rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix);
         }
     }
     if (MODE == 1) {
@@ -962,7 +1099,7 @@ extern "C" hipError_t KNAME(nuc3d_launch_crweno)(int dir, int ns, int mode, cons
 extern "C" hipError_t KNAME(nuc3d_launch_prim)(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles, int ntiles,
                                                nuc3d_kparams K, int bin, hipStream_t st)
 {
-    hipLaunchKernelGGL(KNAME(k_prim), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+    hipLaunchKernelGGL(KNAME(k_prim), dim3(ntiles, NUC3D_KSEG), dim3(256), 0, st, blocks, tiles, K, bin);
     return hipGetLastError();
 }
 
