@@ -59,11 +59,12 @@ def test_steps_tgv_ns(exact):
     g.close()
 
 
-def test_multiblock_weno5z():
-    print("case: 3 blocks split in x, NS weno5z, 2 steps", flush=True)
+@pytest.mark.parametrize("exact", [1, 0])
+def test_multiblock_weno5z(exact):
+    print("case: 3 blocks split in x, NS weno5z, 2 steps, exact=%d" % exact, flush=True)
     spec = cases.taylor_green(dims=(36, 12, 14), noise=1e-3, seed=5)[0]
     blocks = cases.split_block_x(spec, 3)
-    sp, op = both_params(scheme=(solver.WENO5Z,) * 3)
+    sp, op = both_params(scheme=(solver.WENO5Z,) * 3, exact=exact)
     g = solver.Case(sp, blocks)
     o = O.OracleCase(op, blocks)
     g.steps(2, 0.5)
@@ -75,12 +76,13 @@ def test_multiblock_weno5z():
     g.close()
 
 
+@pytest.mark.parametrize("exact", [1, 0])
 @pytest.mark.parametrize("scheme", [solver.WENO5JS, solver.WENO5Z])
 @pytest.mark.parametrize("riemann", [solver.LF, solver.AUSM, solver.AUSMP])
-def test_shu_osher_inflow_outflow(scheme, riemann):
-    print(f"case: shu-osher 48x10x8 euler scheme={scheme} riemann={riemann}", flush=True)
+def test_shu_osher_inflow_outflow(scheme, riemann, exact):
+    print(f"case: shu-osher 48x10x8 euler scheme={scheme} riemann={riemann} exact={exact}", flush=True)
     blocks = cases.shu_osher(48, 10, 8, seed=2)
-    sp, op = both_params(model=solver.EULER3D, mach=1.0, scheme=(scheme,) * 3, riemann=riemann)
+    sp, op = both_params(model=solver.EULER3D, mach=1.0, scheme=(scheme,) * 3, riemann=riemann, exact=exact)
     g = solver.Case(sp, blocks)
     o = O.OracleCase(op, blocks)
     g.steps(3, 0.4)
@@ -95,11 +97,12 @@ def test_shu_osher_inflow_outflow(scheme, riemann):
     [(-1, 2, None), (-1, 1, None), (-1, 2, None), (-1, 3, None), (0, 5, None), (0, 4, None)],
     [(-1, 3, None), (-1, 3, None), (-1, 0, [1.0, 0.1, 0.0, 0.0, 71.0]), (-1, 1, None), (-1, 2, None), (-1, 3, None)],
 ])
-def test_physical_faces_ns(faces):
-    print(f"case: NS walls/symmetry/inlet faces={faces}", flush=True)
+@pytest.mark.parametrize("exact", [1, 0])
+def test_physical_faces_ns(faces, exact):
+    print(f"case: NS walls/symmetry/inlet faces={faces} exact={exact}", flush=True)
     b = cases.taylor_green(dims=(16, 12, 10), noise=1e-3, seed=6)[0]
     b.faces = faces
-    sp, op = both_params(t_wall=1.2)
+    sp, op = both_params(t_wall=1.2, exact=exact)
     g, o = run_rhs(sp, op, [b])
     err = relerr(g.get_rhs(0), o.get_rhs(0))
     print("  rhs rel err %.3e" % err, flush=True)
@@ -115,12 +118,13 @@ def test_physical_faces_ns(faces):
     g.close()
 
 
-def test_conservation_periodic_64():
+@pytest.mark.parametrize("exact", [1, 0])
+def test_conservation_periodic_64(exact):
     """Periodic block: the cell sum of every RHS component telescopes to zero,
     so mass, momentum and energy are conserved to round-off."""
-    print("case: conservation NS tgv 64^3", flush=True)
+    print("case: conservation NS tgv 64^3 exact=%d" % exact, flush=True)
     blocks = cases.taylor_green(64, noise=1e-3, seed=7)
-    g = solver.Case(solver.Params(), blocks)
+    g = solver.Case(solver.Params(exact=exact), blocks)
     q0 = g.get_q(0)
     g.steps(2, 0.5)
     q1 = g.get_q(0)
@@ -146,12 +150,13 @@ def test_crweno5_rhs_tgv(exact, model):
     g.close()
 
 
-def test_crweno5_long_lines_shu_osher():
+@pytest.mark.parametrize("exact", [1, 0])
+def test_crweno5_long_lines_shu_osher(exact):
     """x-lines of 300 cells: past the 261-cell limit of the reference's stack
     arrays (crweno5.cpp:69-72); pinned to the oracle, which has no cap."""
-    print("case: crweno5 shu-osher 300x8x8 euler 2 steps", flush=True)
+    print("case: crweno5 shu-osher 300x8x8 euler 2 steps exact=%d" % exact, flush=True)
     blocks = cases.shu_osher(300, 8, 8, seed=4)
-    sp, op = both_params(model=solver.EULER3D, mach=1.0, scheme=(solver.CRWENO5,) * 3)
+    sp, op = both_params(model=solver.EULER3D, mach=1.0, scheme=(solver.CRWENO5,) * 3, exact=exact)
     g = solver.Case(sp, blocks)
     o = O.OracleCase(op, blocks)
     g.steps(2, 0.5)
@@ -162,10 +167,11 @@ def test_crweno5_long_lines_shu_osher():
     g.close()
 
 
-def test_mixed_schemes_per_direction():
-    print("case: x crweno5, y weno5z, z weno5js, NS, 2 steps", flush=True)
+@pytest.mark.parametrize("exact", [1, 0])
+def test_mixed_schemes_per_direction(exact):
+    print("case: x crweno5, y weno5z, z weno5js, NS, 2 steps exact=%d" % exact, flush=True)
     blocks = cases.taylor_green(dims=(18, 16, 14), noise=1e-3, seed=32)
-    sp, op = both_params(scheme=(solver.CRWENO5, solver.WENO5Z, solver.WENO5JS))
+    sp, op = both_params(scheme=(solver.CRWENO5, solver.WENO5Z, solver.WENO5JS), exact=exact)
     g = solver.Case(sp, blocks)
     o = O.OracleCase(op, blocks)
     g.steps(2, 0.5)
