@@ -97,6 +97,15 @@ int nuc3d_stage(nuc3d_case *c, int stage, double cfl, void *stream);
 /* nsteps full TVD-RK3 steps (block::solve, block.cpp:569-594). */
 int nuc3d_steps(nuc3d_case *c, int nsteps, double cfl, void *stream);
 
+/* nsteps full TVD-RK3 steps, like nuc3d_steps (the stage loop of
+ * block::solve, block.cpp:569-594), launched as a captured HIP graph of two
+ * steps (all kernels of six stages, no host work between them); an odd last
+ * step runs as ordinary launches.  The graph is captured on first use and
+ * again when cfl changes; it runs on `stream` in order with the caller's other
+ * work.  Results are bit-identical to nuc3d_steps.  With kernel timing on
+ * (nuc3d_timing) the steps run as ordinary launches. */
+int nuc3d_step_graph(nuc3d_case *c, int nsteps, double cfl, void *stream);
+
 /* Wait for the case's work on its streams. */
 int nuc3d_sync(nuc3d_case *c);
 

@@ -110,6 +110,10 @@ struct nuc3d_case {
     int lampar = 0;
     bool dirty = true;                 // descriptors / ghosts / eigenvalues need refreshing
     bool faces_set = false;
+    // captured two-step graphs, one per parity of the eigenvalue buffers
+    hipStream_t gstream = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    double gcfl[2] = {-1.0, -1.0};
     // kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -413,6 +417,9 @@ int nuc3d_case_destroy(nuc3d_case *c)
 {
     if (!c) return NUC3D_OK;
     (void)hipDeviceSynchronize();
+    for (auto &g : c->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (c->gstream) (void)hipStreamDestroy(c->gstream);
     for (auto &l : c->ev)
         for (auto &pr : l) c->ev_pool.push_back(pr.first), c->ev_pool.push_back(pr.second);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -560,6 +567,62 @@ int nuc3d_steps(nuc3d_case *c, int nsteps, double cfl, void *stream)
             int rc = nuc3d_stage(c, st, cfl, stream);
             if (rc) return rc;
         }
+    return NUC3D_OK;
+}
+
+int nuc3d_step_graph(nuc3d_case *c, int nsteps, double cfl, void *stream)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    if (nsteps < 0) return fail(NUC3D_ERR_BAD_ARG, "negative step count");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "graph steps start at stage 0");
+    // with kernel timing on, every launch is bracketed by events: run eagerly
+    if (c->timing) return nuc3d_steps(c, nsteps, cfl, stream);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = refresh(c, s);
+    if (rc) return rc;
+    int n = 0;
+    for (; n + 2 <= nsteps; n += 2) {
+        const int par = c->lampar;
+        if (!c->gexec[par] || c->gcfl[par] != cfl) {
+            if (c->gexec[par]) {
+                HIPCHK(hipGraphExecDestroy(c->gexec[par]));
+                c->gexec[par] = nullptr;
+            }
+            if (!c->gstream) HIPCHK(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+            // capture two steps: the eigenvalue buffers swap every step, so two
+            // steps return every buffer index to where it started
+            HIPCHK(hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal));
+            const double cfl0 = c->K.cfl;
+            c->K.cfl = cfl;
+            int lp = par;
+            for (int st = 0; st < 6 && rc == 0; st++) {
+                const int stage = st % 3;
+                const int bin = stage, bun = 0, bout = stage == 2 ? 0 : stage + 1;
+                rc = launch_rhs(c, 1, stage, bin, bun, bout, c->dlam + 3 * c->nb * lp,
+                                c->dlam + 3 * c->nb * (1 - lp), c->gstream);
+                if (stage == 2) lp ^= 1;
+            }
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(c->gstream, &g);
+            c->K.cfl = cfl0;
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return hipfail(e, "hipStreamEndCapture");
+            const hipError_t ei = hipGraphInstantiate(&c->gexec[par], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                c->gexec[par] = nullptr;
+                return hipfail(ei, "hipGraphInstantiate");
+            }
+            c->gcfl[par] = cfl;
+        }
+        HIPCHK(hipGraphLaunch(c->gexec[par], s));
+        c->K.cfl = cfl;
+        c->cur = 0;
+    }
+    if (n < nsteps) return nuc3d_steps(c, nsteps - n, cfl, stream);
     return NUC3D_OK;
 }
 

@@ -97,6 +97,7 @@ def lib():
         "nuc3d_rhs": [vp, vp],
         "nuc3d_stage": [vp, ctypes.c_int, ctypes.c_double, vp],
         "nuc3d_steps": [vp, ctypes.c_int, ctypes.c_double, vp],
+        "nuc3d_step_graph": [vp, ctypes.c_int, ctypes.c_double, vp],
         "nuc3d_sync": [vp],
         "nuc3d_get_dt": [vp, dp],
         "nuc3d_get_max_eigen": [vp, ctypes.c_int, dp],
@@ -200,6 +201,11 @@ class Case:
 
     def steps(self, n, cfl):
         _check(lib().nuc3d_steps(self.h, n, cfl, self.stream), "steps")
+
+    def steps_graph(self, n, cfl):
+        """n steps replayed from a captured two-step HIP graph (bit-identical
+        to steps())."""
+        _check(lib().nuc3d_step_graph(self.h, n, cfl, self.stream), "step_graph")
 
     def sync(self):
         _check(lib().nuc3d_sync(self.h), "sync")
