@@ -131,6 +131,51 @@ def taylor_green(n=64, phys: Optional[Physics] = None, pressure="baseline", nois
     return [BlockSpec(dims, periodic_faces(), jac, met, dxyz, q)]
 
 
+def wavy_metrics(dims, amp=0.15) -> Tuple[np.ndarray, np.ndarray, np.ndarray, Tuple[np.ndarray, ...]]:
+    """A smoothly deformed, triply periodic curvilinear mesh on [0,2pi]^3:
+        x = X + a sin(Y) sin(Z),  y = Y + a sin(Z) sin(X),  z = Z + a sin(X) sin(Y)
+    with (X, Y, Z) the uniform cell centres.  Every metric term, cross terms
+    included, is non-zero and periodic.  Returns jac, met, dxyz (the nine
+    derivatives per unit index, as block::getJacobians expects them,
+    block.cpp:237-325) and the physical cell centres (x, y, z)."""
+    nx, ny, nz = dims
+    h = (2.0 * np.pi / nx, 2.0 * np.pi / ny, 2.0 * np.pi / nz)
+    X, Y, Z = cell_centres(dims, (0.0, 0.0, 0.0), h)
+    sX, cX, sY, cY, sZ, cZ = np.sin(X), np.cos(X), np.sin(Y), np.cos(Y), np.sin(Z), np.cos(Z)
+    x = X + amp * sY * sZ
+    y = Y + amp * sZ * sX
+    z = Z + amp * sX * sY
+    one = np.ones_like(X)
+    dx = np.stack([
+        h[0] * one, h[1] * amp * cY * sZ, h[2] * amp * sY * cZ,       # x_xi, x_eta, x_zeta
+        h[0] * amp * sZ * cX, h[1] * one, h[2] * amp * cZ * sX,       # y_xi, y_eta, y_zeta
+        h[0] * amp * cX * sY, h[1] * amp * sX * cY, h[2] * one,       # z_xi, z_eta, z_zeta
+    ])
+    jac, met = metrics_from_jacobian_matrix(dx)
+    return jac, met, dx, (x, y, z)
+
+
+def taylor_green_wavy(dims=(24, 20, 18), amp=0.15, phys: Optional[Physics] = None, noise=1e-3,
+                      seed=0) -> List[BlockSpec]:
+    """The Taylor-Green field (reference pressure form, block.cpp:683-690)
+    evaluated at the physical centres of the wavy periodic mesh of
+    wavy_metrics, with seeded noise on rho and p."""
+    phys = phys or Physics()
+    jac, met, dxyz, (x, y, z) = wavy_metrics(dims, amp)
+    g, ma = phys.gamma, phys.mach
+    rho = np.ones_like(x)
+    u = np.sin(x) * np.cos(y) * np.cos(z)
+    v = -np.cos(x) * np.sin(y) * np.cos(z)
+    w = 0.1 * np.sin(x + y) * np.cos(z)
+    p = 1.0 / (g * ma * ma) + ((np.cos(2.0 * z) + 2.0) * (np.cos(2.0 * x) + np.cos(2.0 * y)) - 2.0) / 16.0
+    if noise:
+        rng = np.random.default_rng(seed)
+        rho = rho * (1.0 + noise * rng.uniform(-1, 1, rho.shape))
+        p = p * (1.0 + noise * rng.uniform(-1, 1, p.shape))
+    q = conserved_over_j(rho, u, v, w, p, g, jac)
+    return [BlockSpec(tuple(dims), periodic_faces(), jac, met, dxyz, q)]
+
+
 def shu_osher(nx=512, ny=64, nz=64, seed=0, phys: Optional[Physics] = None) -> List[BlockSpec]:
     """3-D Shu-Osher shock/entropy-wave problem on x in [-5,5], periodic in y and
     z with h_y = h_z = h_x; inflow (post-shock state) at x-, outflow at x+."""

@@ -147,7 +147,8 @@ __device__ __forceinline__ double face_interior(const double *s, bool plus, bool
 __device__ __forceinline__ void crweno_row(const double *f, int st, bool plus, double &a, double &b, double &c,
                                            double &d)
 {
-    NUC3D_WENO_FP
+    // row coefficients in the reference order in both variants: the tridiagonal
+    // solve amplifies their rounding
     // f points at cell i of face i+1; plus: cells i-2..i+2, minus: i+3..i-1
     double f0, f1, f2, f3, f4;
     if (plus) { f0 = f[-2 * st]; f1 = f[-st]; f2 = f[0]; f3 = f[st]; f4 = f[2 * st]; }

@@ -6,7 +6,7 @@ The reference is compiled from its own sources by oracle/Makefile (target
 the MPI halo exchange with one block per rank).  Each fixture holds the inputs
 (face topology, metrics, Q) and the reference's outputs (RHS of one
 evaluation, or Q and dt after n RK3 steps).  Run in this container only:
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [fixture names]
 """
 import json
 import os
@@ -33,6 +33,14 @@ def walls(dims, faces, seed):
     return [b]
 
 
+def wavy_walls(dims, faces, seed):
+    b = cases.taylor_green_wavy(dims, seed=seed)[0]
+    b.faces = faces
+    return [b]
+
+
+ONLY = set(sys.argv[1:])  # fixture names to (re)write; all when empty
+
 CASES = [
     # name, params, blocks, mode, nsteps, cfl
     ("tgv_ns_weno5js_rhs", dict(model=O.NS), lambda: tgv((16, 12, 10), 11), "rhs", 0, 0.5),
@@ -54,6 +62,13 @@ CASES = [
     ("walls_ns_rhs", dict(model=O.NS, t_wall=1.2),
      lambda: walls((16, 12, 10), [(-1, 2, None), (-1, 1, None), (-1, 2, None), (-1, 3, None), (0, 5, None),
                                   (0, 4, None)], 16), "rhs", 0, 0.5),
+    ("wavy_ns_weno5js_steps2", dict(model=O.NS), lambda: cases.taylor_green_wavy((20, 16, 14), seed=18), "steps",
+     2, 0.5),
+    ("wavy_ns_crweno5_rhs", dict(model=O.NS, scheme=(O.CRWENO5, O.WENO5Z, O.CRWENO5)),
+     lambda: cases.taylor_green_wavy((18, 16, 14), seed=19), "rhs", 0, 0.5),
+    ("wavy_walls_ns_steps2", dict(model=O.NS, t_wall=1.2),
+     lambda: wavy_walls((16, 14, 12), [(-1, 3, None), (-1, 1, None), (-1, 2, None), (-1, 3, None),
+                                       (-1, 0, [1.0, 0.1, 0.02, -0.01, 71.0]), (-1, 2, None)], 20), "steps", 2, 0.5),
     ("walls_ns_steps2", dict(model=O.NS, t_wall=1.2),
      lambda: walls((16, 12, 10), [(-1, 3, None), (-1, 3, None), (-1, 0, [1.0, 0.1, 0.0, 0.0, 71.0]),
                                   (-1, 1, None), (-1, 2, None), (-1, 3, None)], 17), "steps", 2, 0.5),
@@ -74,6 +89,8 @@ def main():
     if not O.ref_available():
         raise SystemExit("reference harness missing: make -C oracle ref")
     for name, pk, mk, mode, nsteps, cfl in CASES:
+        if ONLY and name not in ONLY:
+            continue
         params = O.Params(**pk)
         blocks = mk()
         d = tempfile.mkdtemp(prefix="golden_")

@@ -180,3 +180,22 @@ def test_mixed_schemes_per_direction(exact):
     print("  rel err %.3e" % err, flush=True)
     assert err <= TOL
     g.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+@pytest.mark.parametrize("scheme", [solver.WENO5JS, solver.WENO5Z, solver.CRWENO5])
+def test_curvilinear_multiblock(scheme, exact):
+    """Wavy periodic mesh (every metric term non-zero) cut into 2 blocks, NS, 2 steps."""
+    print("case: wavy mesh 2 blocks NS scheme=%d exact=%d" % (scheme, exact), flush=True)
+    spec = cases.taylor_green_wavy((28, 16, 14), seed=33)[0]
+    blocks = cases.split_block_x(spec, 2)
+    sp, op = both_params(scheme=(scheme,) * 3, exact=exact)
+    g = solver.Case(sp, blocks)
+    o = O.OracleCase(op, blocks)
+    g.steps(2, 0.5)
+    o.steps(2, 0.5)
+    for b in range(2):
+        err = relerr(g.get_q(b), o.get_q(b))
+        print("  block %d rel err %.3e" % (b, err), flush=True)
+        assert err <= TOL
+    g.close()

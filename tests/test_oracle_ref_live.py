@@ -81,3 +81,10 @@ def test_reference_builds_agree_under_parity_measure(name):
         outs.append(O.run_ref(d, 1, "steps", fx.meta["nsteps"], fx.meta["cfl"], optimised=opt)[0]["q"])
     assert max(relerr_components(outs[1], outs[0])) > 1e-12
     assert relerr(outs[1], outs[0]) < 1e-13
+
+
+def test_live_curvilinear_multiblock_two_ranks():
+    """All nine metric terms non-zero (wavy periodic mesh), cut into two
+    blocks exchanged over MPI."""
+    spec = cases.taylor_green_wavy((24, 12, 10), seed=25)[0]
+    compare(O.Params(model=O.NS, scheme=(O.WENO5Z,) * 3), cases.split_block_x(spec, 2), "steps", 2)
