@@ -51,6 +51,16 @@ __device__ __forceinline__ double sq(double x) { return x * x; }
 #define NUC3D_WENO_FP _Pragma("clang fp contract(fast)")
 #endif
 
+// Smoothness indicators (weno5js.cpp:75-82); zq: the + inner beta2 of
+// weno5z.cpp:80.  smooth5_ref keeps the reference's rounding in every build.
+__device__ __forceinline__ void smooth5_ref(double f0, double f1, double f2, double f3, double f4, bool zq,
+                                            double &b0, double &b1, double &b2)
+{
+    b0 = G0 * sq(f0 - 2.0 * f1 + f2) + G1 * sq(f0 - 4.0 * f1 + 3.0 * f2);
+    b1 = G0 * sq(f1 - 2.0 * f2 + f3) + G1 * sq(f1 - f3);
+    b2 = G0 * sq(zq ? (f2 - 2.0 * f3 + f3) : (f2 - 2.0 * f3 + f4)) + G1 * sq(3.0 * f2 - 4.0 * f3 + f4);
+}
+
 __device__ __forceinline__ void smooth5(double f0, double f1, double f2, double f3, double f4, bool zq,
                                         double &b0, double &b1, double &b2)
 {
@@ -154,7 +164,7 @@ __device__ __forceinline__ void crweno_row(const double *f, int st, bool plus, d
     if (plus) { f0 = f[-2 * st]; f1 = f[-st]; f2 = f[0]; f3 = f[st]; f4 = f[2 * st]; }
     else { f0 = f[3 * st]; f1 = f[2 * st]; f2 = f[st]; f3 = f[0]; f4 = f[-st]; }
     double b0, b1, b2;
-    smooth5(f0, f1, f2, f3, f4, false, b0, b1, b2);
+    smooth5_ref(f0, f1, f2, f3, f4, false, b0, b1, b2);
     const double a0 = R0 / sq(EPS + b0), a1 = R1 / sq(EPS + b1), a2 = R2 / sq(EPS + b2);
     const double as = a0 + a1 + a2;
     const double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
