@@ -78,6 +78,21 @@ int nuc3d_set_faces(nuc3d_case *c, int b, const int *type6, const int *id6, cons
  * x_xi,x_eta,x_zeta,y_xi,..,z_zeta; may be NULL unless a face is symmetric). */
 int nuc3d_upload_metrics(nuc3d_case *c, int b, const double *jac, const double *met, const double *dxyz);
 
+/* Metrics of block b computed on the GPU from its mesh nodes, as
+ * block::initial does after reading mesh/mesh_<rank>.x (block.cpp:42-98):
+ * cell centres by tiled 5-point Lagrange interpolation (getXYZ_center,
+ * block.cpp:147-235), the nine coordinate derivatives, J and the metric terms
+ * (getJacobians, block.cpp:237-325).  xn, yn, zn: node coordinates,
+ * (nx+1)(ny+1)(nz+1) each, i-fastest.  Cells per direction must be multiples
+ * of 4 (NUC3D_ERR_UNSUPPORTED otherwise; the reference leaves the remainder
+ * cells unset).  A non-positive Jacobian is NUC3D_ERR_BAD_ARG (the reference
+ * exits).  centre3 (3N, may be NULL) receives the cell centres, where
+ * block::initialQ evaluates the initial state (block.cpp:523-567). */
+int nuc3d_upload_mesh(nuc3d_case *c, int b, const double *xn, const double *yn, const double *zn, double *centre3);
+/* J (N), metrics (9N) and coordinate derivatives (9N) of block b as held on
+ * the GPU (any pointer may be NULL). */
+int nuc3d_download_metrics(nuc3d_case *c, int b, double *jac, double *met, double *dxyz);
+
 /* State of block b: Q_work = conserved / J (5N), PDEData3d::getQ. */
 int nuc3d_upload_q(nuc3d_case *c, int b, const double *q5);
 int nuc3d_download_q(nuc3d_case *c, int b, double *q5);

@@ -155,6 +155,19 @@ def wavy_metrics(dims, amp=0.15) -> Tuple[np.ndarray, np.ndarray, np.ndarray, Tu
     return jac, met, dx, (x, y, z)
 
 
+def wavy_nodes(dims, amp=0.15, lo=(0.0, 0.0, 0.0), extent=(2.0 * np.pi,) * 3):
+    """Node coordinates ((nz+1, ny+1, nx+1) each) of a smoothly deformed box:
+    X + a sin(Y) sin(Z), Y + a sin(Z) sin(X), Z + a sin(X) sin(Y) on the uniform
+    nodes (X, Y, Z) of [lo, lo + extent] -- the mesh-file input of
+    block::initial (block.cpp:42-70)."""
+    nx, ny, nz = dims
+    x = lo[0] + np.arange(nx + 1) * (extent[0] / nx)
+    y = lo[1] + np.arange(ny + 1) * (extent[1] / ny)
+    z = lo[2] + np.arange(nz + 1) * (extent[2] / nz)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    return (X + amp * np.sin(Y) * np.sin(Z), Y + amp * np.sin(Z) * np.sin(X), Z + amp * np.sin(X) * np.sin(Y))
+
+
 def taylor_green_wavy(dims=(24, 20, 18), amp=0.15, phys: Optional[Physics] = None, noise=1e-3,
                       seed=0) -> List[BlockSpec]:
     """The Taylor-Green field (reference pressure form, block.cpp:683-690)

@@ -40,6 +40,8 @@
                                                     unsigned long long *, double *, hipStream_t);
 NUC3D_LAUNCHERS(fast)
 NUC3D_LAUNCHERS(exact)
+extern "C" hipError_t nuc3d_launch_mesh_metrics(nuc3d_block_dev, const double *, const double *, const double *,
+                                                double *, int *, hipStream_t);
 
 namespace {
 
@@ -78,6 +80,22 @@ const Launchers kFast = {nuc3d_launch_sweep_fast, nuc3d_launch_prim_fast, nuc3d_
                          nuc3d_launch_fixup_fast, nuc3d_launch_ghost_fill_fast, nuc3d_launch_crweno_fast};
 const Launchers kExact = {nuc3d_launch_sweep_exact, nuc3d_launch_prim_exact, nuc3d_launch_visc_flux_exact, nuc3d_launch_visc_div_exact, nuc3d_launch_eigen_exact,
                           nuc3d_launch_fixup_exact, nuc3d_launch_ghost_fill_exact, nuc3d_launch_crweno_exact};
+
+// temporary device buffers of one call, freed on return
+struct OpBuffers {
+    std::vector<void *> p;
+    ~OpBuffers()
+    {
+        for (void *q : p) (void)hipFree(q);
+    }
+    double *get(size_t n)
+    {
+        void *q = nullptr;
+        if (hipMalloc(&q, sizeof(double) * std::max<size_t>(n, 1)) != hipSuccess) return nullptr;
+        p.push_back(q);
+        return static_cast<double *>(q);
+    }
+};
 
 constexpr int KSEG = NUC3D_KSEG;  // planes per k segment of a tile
 
@@ -481,6 +499,50 @@ int nuc3d_upload_metrics(nuc3d_case *c, int b, const double *jac, const double *
     return NUC3D_OK;
 }
 
+int nuc3d_upload_mesh(nuc3d_case *c, int b, const double *xn, const double *yn, const double *zn, double *centre3)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!xn || !yn || !zn) return fail(NUC3D_ERR_BAD_ARG, "null node coordinates");
+    const nuc3d_block_dev &B = c->hb[b];
+    for (int d = 0; d < 3; d++)
+        if (B.n[d] % 4)
+            return fail(NUC3D_ERR_UNSUPPORTED,
+                        "mesh metrics need cells per direction divisible by 4 (tiles of block.cpp:163-165)");
+    const long long N = (long long)B.n[0] * B.n[1] * B.n[2];
+    const long long NN = (long long)(B.n[0] + 1) * (B.n[1] + 1) * (B.n[2] + 1);
+    OpBuffers T;
+    double *dn = T.get(3 * NN), *dc = centre3 ? T.get(3 * N) : nullptr;
+    int *dbad = reinterpret_cast<int *>(T.get(1));
+    if (!dn || !dbad || (centre3 && !dc)) return fail(NUC3D_ERR_OOM, "device allocation failed");
+    HIPCHK(hipMemcpy(dn, xn, sizeof(double) * NN, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dn + NN, yn, sizeof(double) * NN, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dn + 2 * NN, zn, sizeof(double) * NN, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(dbad, 0, sizeof(int)));
+    HIPCHK(nuc3d_launch_mesh_metrics(B, dn, dn + NN, dn + 2 * NN, dc, dbad, nullptr));
+    int bad = 0;
+    HIPCHK(hipMemcpy(&bad, dbad, sizeof(int), hipMemcpyDeviceToHost));
+    if (centre3) HIPCHK(hipMemcpy(centre3, dc, sizeof(double) * 3 * N, hipMemcpyDeviceToHost));
+    c->dirty = true;
+    if (bad) return fail(NUC3D_ERR_BAD_ARG, "non-positive Jacobian in the mesh of this block");
+    return NUC3D_OK;
+}
+
+int nuc3d_download_metrics(nuc3d_case *c, int b, double *jac, double *met, double *dxyz)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    nuc3d_block_dev &B = c->hb[b];
+    const long long N = (long long)B.n[0] * B.n[1] * B.n[2];
+    HIPCHK(hipDeviceSynchronize());
+    if (jac && (rc = get_field(c, b, jac, B.jac))) return rc;
+    for (int q = 0; q < 9; q++) {
+        if (met && (rc = get_field(c, b, met + q * N, B.met[q]))) return rc;
+        if (dxyz && (rc = get_field(c, b, dxyz + q * N, B.dxyz[q]))) return rc;
+    }
+    return NUC3D_OK;
+}
+
 int nuc3d_upload_q(nuc3d_case *c, int b, const double *q5)
 {
     int rc = check_block(c, b);
@@ -711,21 +773,6 @@ extern "C" hipError_t nuc3d_launch_op_rk3(int, double, long long, const double *
 
 namespace {
 
-// device buffers of one operator call, freed on return
-struct OpBuffers {
-    std::vector<void *> p;
-    ~OpBuffers()
-    {
-        for (void *q : p) (void)hipFree(q);
-    }
-    double *get(size_t n)
-    {
-        void *q = nullptr;
-        if (hipMalloc(&q, sizeof(double) * std::max<size_t>(n, 1)) != hipSuccess) return nullptr;
-        p.push_back(q);
-        return static_cast<double *>(q);
-    }
-};
 
 int op_device_ready()
 {

@@ -10,6 +10,7 @@
 // performance path.
 #define NUC3D_EXACT 1
 #include "nuc3d_device.h"
+#include "nuc3d_types.h"
 
 using namespace nuc3d_dev;
 
@@ -145,6 +146,82 @@ __global__ __launch_bounds__(256) void k_op_rk3(int stage, double dt, long long 
     ui[e] = v;
 }
 
+// Cell centres, coordinate derivatives, J and metric terms of one block from
+// its mesh nodes (block::getXYZ_center / getJacobians, block.cpp:147-477):
+// within each tile of 4x4x4 cells the five nodes per direction carry a
+// fourth-degree Lagrange polynomial, evaluated (LAG) or differentiated per
+// unit index (DLAG) at the cell centres 0.5..3.5 -- the weights of the
+// reference's tables (block.h:36-47).  One thread per cell, the reference's
+// accumulation order (zeta outermost, block.cpp:212-234), no contraction.
+__constant__ double LAG[4][5] = {
+    {35.0 / 128.0, 35.0 / 32.0, -35.0 / 64.0, 7.0 / 32.0, -5.0 / 128.0},
+    {-5.0 / 128.0, 15.0 / 32.0, 45.0 / 64.0, -5.0 / 32.0, 3.0 / 128.0},
+    {3.0 / 128.0, -5.0 / 32.0, 45.0 / 64.0, 15.0 / 32.0, -5.0 / 128.0},
+    {-5.0 / 128.0, 7.0 / 32.0, -35.0 / 64.0, 35.0 / 32.0, 35.0 / 128.0}};
+__constant__ double DLAG[4][5] = {
+    {-11.0 / 12.0, 17.0 / 24.0, 3.0 / 8.0, -5.0 / 24.0, 1.0 / 24.0},
+    {1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0, 0.0},
+    {0.0, 1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0},
+    {-1.0 / 24.0, 5.0 / 24.0, -3.0 / 8.0, -17.0 / 24.0, 11.0 / 12.0}};
+
+__device__ __forceinline__ double tile_sum(const double *f, long long sy, long long sz, const double *wx,
+                                           const double *wy, const double *wz)
+{
+    double vz = 0.0;
+    for (int k = 0; k <= 4; k++) {
+        double ve = 0.0;
+        for (int j = 0; j <= 4; j++) {
+            double vx = 0.0;
+#pragma unroll
+            for (int i = 0; i <= 4; i++) vx += wx[i] * f[i + j * sy + k * sz];
+            ve += wy[j] * vx;
+        }
+        vz += wz[k] * ve;
+    }
+    return vz;
+}
+
+__global__ __launch_bounds__(256) void k_mesh_metrics(nuc3d_block_dev B, const double *__restrict__ xn,
+                                                      const double *__restrict__ yn, const double *__restrict__ zn,
+                                                      double *__restrict__ centre, int *__restrict__ bad)
+{
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const long long N = (long long)nx * ny * nz;
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= N) return;
+    const int i = (int)(e % nx), j = (int)((e / nx) % ny), k = (int)(e / ((long long)nx * ny));
+    const long long sy = nx + 1, sz = (long long)(nx + 1) * (ny + 1);
+    const long long base = (i / 4) * 4 + ((j / 4) * 4) * sy + ((k / 4) * 4) * sz;
+    const int li = i % 4, lj = j % 4, lk = k % 4;
+    const double *node[3] = {xn, yn, zn};
+    double dv[9];
+    for (int c = 0; c < 3; c++) {
+        if (centre) centre[c * N + e] = tile_sum(node[c] + base, sy, sz, LAG[li], LAG[lj], LAG[lk]);
+        dv[3 * c + 0] = tile_sum(node[c] + base, sy, sz, DLAG[li], LAG[lj], LAG[lk]);
+        dv[3 * c + 1] = tile_sum(node[c] + base, sy, sz, LAG[li], DLAG[lj], LAG[lk]);
+        dv[3 * c + 2] = tile_sum(node[c] + base, sy, sz, LAG[li], LAG[lj], DLAG[lk]);
+    }
+    const double x_xi = dv[0], x_eta = dv[1], x_zeta = dv[2];
+    const double y_xi = dv[3], y_eta = dv[4], y_zeta = dv[5];
+    const double z_xi = dv[6], z_eta = dv[7], z_zeta = dv[8];
+    const double jacob = 1.0 / (x_xi * (y_eta * z_zeta - y_zeta * z_eta) + x_eta * (y_zeta * z_xi - y_xi * z_zeta) +
+                                x_zeta * (y_xi * z_eta - y_eta * z_xi));
+    const long long x = i + B.sy * j + B.sz * k;
+    B.met[0][x] = (y_eta * z_zeta - y_zeta * z_eta) * jacob;
+    B.met[1][x] = (x_zeta * z_eta - x_eta * z_zeta) * jacob;
+    B.met[2][x] = (x_eta * y_zeta - x_zeta * y_eta) * jacob;
+    B.met[3][x] = (y_zeta * z_xi - y_xi * z_zeta) * jacob;
+    B.met[4][x] = (x_xi * z_zeta - x_zeta * z_xi) * jacob;
+    B.met[5][x] = (x_zeta * y_xi - x_xi * y_zeta) * jacob;
+    B.met[6][x] = (y_xi * z_eta - y_eta * z_xi) * jacob;
+    B.met[7][x] = (x_eta * z_xi - x_xi * z_eta) * jacob;
+    B.met[8][x] = (x_xi * y_eta - x_eta * y_xi) * jacob;
+#pragma unroll
+    for (int c = 0; c < 9; c++) B.dxyz[c][x] = dv[c];
+    B.jac[x] = jacob;
+    if (!(jacob > 0.0)) atomicOr(bad, 1);  // block.cpp:310-314 exits on J < 0
+}
+
 unsigned grid_of(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -183,5 +260,13 @@ extern "C" hipError_t nuc3d_launch_op_rk3(int stage, double dt, long long n, con
                                           double *ui, hipStream_t st)
 {
     hipLaunchKernelGGL(k_op_rk3, dim3(grid_of(n, 256)), dim3(256), 0, st, stage, dt, n, rhs, un, ui);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t nuc3d_launch_mesh_metrics(nuc3d_block_dev B, const double *xn, const double *yn,
+                                                const double *zn, double *centre, int *bad, hipStream_t st)
+{
+    const long long N = (long long)B.n[0] * B.n[1] * B.n[2];
+    hipLaunchKernelGGL(k_mesh_metrics, dim3(grid_of(N, 256)), dim3(256), 0, st, B, xn, yn, zn, centre, bad);
     return hipGetLastError();
 }

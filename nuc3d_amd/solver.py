@@ -92,6 +92,8 @@ def lib():
         "nuc3d_set_faces": [vp, ctypes.c_int, ip, ip, dp],
         "nuc3d_upload_metrics": [vp, ctypes.c_int, dp, dp, dp],
         "nuc3d_upload_q": [vp, ctypes.c_int, dp],
+        "nuc3d_upload_mesh": [vp, ctypes.c_int, dp, dp, dp, dp],
+        "nuc3d_download_metrics": [vp, ctypes.c_int, dp, dp, dp],
         "nuc3d_download_q": [vp, ctypes.c_int, dp],
         "nuc3d_download_rhs": [vp, ctypes.c_int, dp],
         "nuc3d_rhs": [vp, vp],
@@ -198,6 +200,22 @@ class Case:
 
     def stage(self, s, cfl):
         _check(lib().nuc3d_stage(self.h, s, cfl, self.stream), "stage")
+
+    def upload_mesh(self, b, xn, yn, zn):
+        """Metrics of block b computed on the GPU from node coordinates of shape
+        (nz+1, ny+1, nx+1); returns the cell centres (3, nz, ny, nx)."""
+        xn, yn, zn = (np.ascontiguousarray(a, dtype=np.float64) for a in (xn, yn, zn))
+        nz, ny, nx = (d - 1 for d in xn.shape)
+        centre = np.empty((3, nz, ny, nx))
+        _check(lib().nuc3d_upload_mesh(self.h, b, _dp(xn), _dp(yn), _dp(zn), _dp(centre)), "upload_mesh")
+        return centre
+
+    def get_metrics(self, b):
+        """(jac, met, dxyz) of block b as held on the GPU."""
+        nx, ny, nz = self.blocks[b].dims
+        jac, met, dxyz = np.empty((nz, ny, nx)), np.empty((9, nz, ny, nx)), np.empty((9, nz, ny, nx))
+        _check(lib().nuc3d_download_metrics(self.h, b, _dp(jac), _dp(met), _dp(dxyz)), "download_metrics")
+        return jac, met, dxyz
 
     def steps(self, n, cfl):
         _check(lib().nuc3d_steps(self.h, n, cfl, self.stream), "steps")

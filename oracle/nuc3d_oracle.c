@@ -1027,3 +1027,86 @@ int orc_steps(orc_case *C, int nsteps, double cfl, double *dts)
     }
     return 0;
 }
+
+/* ---------------------------------------------------------------------------
+ * Cell centres, coordinate derivatives, J and metric terms from mesh nodes:
+ * block::getXYZ_center + interpolation_lag (block.cpp:147-235) and
+ * getJacobians + interpolation_derlag (block.cpp:237-477).  Cells are grouped
+ * in tiles of 4 per direction; within a tile the five nodes 0..4 carry a
+ * fourth-degree Lagrange polynomial, evaluated (LAG) and differentiated
+ * (DLAG, per unit index) at the cell centres 0.5, 1.5, 2.5, 3.5.  These are
+ * the weights of the reference's tables (block.h:36-47); tests check them
+ * against the Lagrange formula in exact rational arithmetic. */
+static const double LAG[4][5] = {
+    {35.0 / 128.0, 35.0 / 32.0, -35.0 / 64.0, 7.0 / 32.0, -5.0 / 128.0},
+    {-5.0 / 128.0, 15.0 / 32.0, 45.0 / 64.0, -5.0 / 32.0, 3.0 / 128.0},
+    {3.0 / 128.0, -5.0 / 32.0, 45.0 / 64.0, 15.0 / 32.0, -5.0 / 128.0},
+    {-5.0 / 128.0, 7.0 / 32.0, -35.0 / 64.0, 35.0 / 32.0, 35.0 / 128.0}};
+static const double DLAG[4][5] = {
+    {-11.0 / 12.0, 17.0 / 24.0, 3.0 / 8.0, -5.0 / 24.0, 1.0 / 24.0},
+    {1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0, 0.0},
+    {0.0, 1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0},
+    {-1.0 / 24.0, 5.0 / 24.0, -3.0 / 8.0, -17.0 / 24.0, 11.0 / 12.0}};
+
+/* triple weighted sum over the 5x5x5 nodes of a tile, zeta outermost, in
+ * the accumulation order of block.cpp:212-234 */
+static double tile_sum(const double *f, long sy, long sz, const double *wx, const double *wy, const double *wz)
+{
+    double vz = 0.0;
+    int i, j, k;
+    for (k = 0; k <= 4; k++) {
+        double ve = 0.0;
+        for (j = 0; j <= 4; j++) {
+            double vx = 0.0;
+            for (i = 0; i <= 4; i++) vx += wx[i] * f[i + j * sy + k * sz];
+            ve += wy[j] * vx;
+        }
+        vz += wz[k] * ve;
+    }
+    return vz;
+}
+
+int orc_mesh_metrics(int nx, int ny, int nz, const double *xn, const double *yn, const double *zn, double *centre,
+                     double *dxyz, double *jac, double *met)
+{
+    const long sy = nx + 1, sz = (long)(nx + 1) * (ny + 1), N = (long)nx * ny * nz;
+    const double *node[3] = {xn, yn, zn};
+    int i, j, k, c, d;
+    if (nx < 4 || ny < 4 || nz < 4 || nx % 4 || ny % 4 || nz % 4) return -1;
+    for (k = 0; k < nz; k++)
+        for (j = 0; j < ny; j++)
+            for (i = 0; i < nx; i++) {
+                const long x = i + (long)j * nx + (long)k * nx * ny;
+                const long base = (i / 4) * 4 + ((j / 4) * 4) * sy + ((k / 4) * 4) * sz;
+                const int li = i % 4, lj = j % 4, lk = k % 4;
+                double dv[9];
+                for (c = 0; c < 3; c++) {
+                    centre[c * N + x] = tile_sum(node[c] + base, sy, sz, LAG[li], LAG[lj], LAG[lk]);
+                    for (d = 0; d < 3; d++) {
+                        dv[3 * c + d] = tile_sum(node[c] + base, sy, sz, d == 0 ? DLAG[li] : LAG[li],
+                                                 d == 1 ? DLAG[lj] : LAG[lj], d == 2 ? DLAG[lk] : LAG[lk]);
+                        dxyz[(3 * c + d) * N + x] = dv[3 * c + d];
+                    }
+                }
+                {
+                    const double x_xi = dv[0], x_eta = dv[1], x_zeta = dv[2];
+                    const double y_xi = dv[3], y_eta = dv[4], y_zeta = dv[5];
+                    const double z_xi = dv[6], z_eta = dv[7], z_zeta = dv[8];
+                    const double jacob = 1.0 / (x_xi * (y_eta * z_zeta - y_zeta * z_eta) +
+                                                x_eta * (y_zeta * z_xi - y_xi * z_zeta) +
+                                                x_zeta * (y_xi * z_eta - y_eta * z_xi));
+                    met[0 * N + x] = (y_eta * z_zeta - y_zeta * z_eta) * jacob;
+                    met[1 * N + x] = (x_zeta * z_eta - x_eta * z_zeta) * jacob;
+                    met[2 * N + x] = (x_eta * y_zeta - x_zeta * y_eta) * jacob;
+                    met[3 * N + x] = (y_zeta * z_xi - y_xi * z_zeta) * jacob;
+                    met[4 * N + x] = (x_xi * z_zeta - x_zeta * z_xi) * jacob;
+                    met[5 * N + x] = (x_zeta * y_xi - x_xi * y_zeta) * jacob;
+                    met[6 * N + x] = (y_xi * z_eta - y_eta * z_xi) * jacob;
+                    met[7 * N + x] = (x_eta * z_xi - x_xi * z_eta) * jacob;
+                    met[8 * N + x] = (x_xi * y_eta - x_eta * y_xi) * jacob;
+                    if (jacob < 0.0) return -2; /* block.cpp:310-314 */
+                    jac[x] = jacob;
+                }
+            }
+    return 0;
+}

@@ -83,6 +83,14 @@ void orc_split_point(const orc_params *p, const double prim5[5], double jac,
 /* TVD-RK3 on flat arrays (TVD-RK3.cpp:37-137). */
 void orc_rk3(int stage, long n, const double *rhs, const double *un, double *ui, double dt);
 
+/* Cell centres (3N), coordinate derivatives (9N: x_xi,x_eta,x_zeta,y_xi..z_zeta),
+ * J (N) and metric terms (9N) of an nx x ny x nz block from its node
+ * coordinates ((nx+1)(ny+1)(nz+1) each, i-fastest): block.cpp:147-477.
+ * Returns -1 unless every dimension is a multiple of 4 (the tile size),
+ * -2 at a negative Jacobian (where the reference exits). */
+int orc_mesh_metrics(int nx, int ny, int nz, const double *xn, const double *yn, const double *zn, double *centre,
+                     double *dxyz, double *jac, double *met);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,8 @@ REF_BIN_O2 = os.path.join(HERE, "_ref", "ref_harness_O2")
 # GPU definitions (nuc3d_amd/shim/): the caller of the operator-level ABI under
 # test, checked against REF_BIN
 REF_BIN_GPUOPS = os.path.join(HERE, "_ref", "ref_harness_gpuops")
+# the reference's block::initial on a mesh file (metrics and initial state)
+REF_BIN_METRICS = os.path.join(HERE, "_ref", "ref_metrics")
 
 WENO5JS, WENO5Z, CRWENO5 = 0, 1, 2
 CD6, CD8 = 0, 1
@@ -120,6 +122,7 @@ def _lib():
     lib.orc_split_point.restype = None
     lib.orc_rk3.argtypes = [ctypes.c_int, ctypes.c_long, dp, dp, dp, ctypes.c_double]
     lib.orc_rk3.restype = None
+    lib.orc_mesh_metrics.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp]
     return lib
 
 
@@ -260,6 +263,20 @@ def rk3(stage, rhs, un, ui, dt):
     return ui
 
 
+def mesh_metrics(xn, yn, zn):
+    """Cell centres, coordinate derivatives, J and metrics from node
+    coordinates of shape (nz+1, ny+1, nx+1) (block.cpp:147-477).
+    Returns centre (3,nz,ny,nx), dxyz (9,...), jac (...), met (9,...)."""
+    xn, yn, zn = (np.ascontiguousarray(a, dtype=np.float64) for a in (xn, yn, zn))
+    nz, ny, nx = (d - 1 for d in xn.shape)
+    shape = (nz, ny, nx)
+    centre, dxyz, jac, met = np.empty((3,) + shape), np.empty((9,) + shape), np.empty(shape), np.empty((9,) + shape)
+    rc = lib().orc_mesh_metrics(nx, ny, nz, _dp(xn), _dp(yn), _dp(zn), _dp(centre), _dp(dxyz), _dp(jac), _dp(met))
+    if rc:
+        raise ValueError("orc_mesh_metrics failed (%d)" % rc)
+    return centre, dxyz, jac, met
+
+
 # --------------------------------------------------------------------------
 # reference harness (oracle/_ref), only where it was built from the reference
 
@@ -337,3 +354,60 @@ def read_ref_output(path: str, rank: int):
     walls = raw[o:o + int(ns)]
     return dict(rhs=rhs, q=q, prim=np.concatenate([w, w0]), dt_local=dt_local, dt_global=dt_global,
                 max_eigen=np.array([e0, e1, e2]), dts=dts, walls=walls)
+
+
+def write_mesh_case(path: str, params: Params, nodes, faces, benchmark: int = 2):
+    """A case the reference builds from mesh files: mesh/mesh_<rank>.x (node
+    counts, then x, y, z one value per line, block.cpp:42-70), the inp/ files
+    of write_ref_case, inp/IOController.in with `Benchmark` (the initial
+    condition, block.cpp:523-567), inp/PostProc.in with no statistics, and the
+    flowData/ directory its geometry output goes to.  nodes: per block (xn, yn, zn) of shape (nz+1, ny+1, nx+1)."""
+    dummy = []
+    for (xn, yn, zn), fc in zip(nodes, faces):
+        nz, ny, nx = (d - 1 for d in xn.shape)
+        z = np.zeros((nz, ny, nx))
+        dummy.append(Block((nx, ny, nz), fc, z, np.zeros((9, nz, ny, nx)), np.zeros((9, nz, ny, nx)),
+                           np.zeros((5, nz, ny, nx))))
+    write_ref_case(path, params, dummy)
+    os.makedirs(os.path.join(path, "mesh"), exist_ok=True)
+    os.makedirs(os.path.join(path, "flowData"), exist_ok=True)
+    with open(os.path.join(path, "inp", "IOController.in"), "w") as f:
+        f.write("startStep 0\nendStep 1\nsaveStep 1000\npostStep 1000\nBenchmark %d\n"
+                "cfl 0.5\ntl 0.0\ndt 0.01\nBinary no\nTecplot no\nPostProc no\n" % benchmark)
+    with open(os.path.join(path, "inp", "PostProc.in"), "w") as f:  # postproc.cpp:29-72: no statistics
+        f.write("AverageType 0\nAverageStartStep 0\nSimutaneousData 0\nAveragedSaveType 0\n"
+                "SimutaneousSaveType 0\n")
+    for r, (xn, yn, zn) in enumerate(nodes):
+        nz1, ny1, nx1 = xn.shape
+        with open(os.path.join(path, "mesh", "mesh_%d.x" % r), "w") as f:
+            f.write("%d %d %d\n" % (nx1, ny1, nz1))
+            for a in (xn, yn, zn):
+                f.write("\n".join(repr(float(v)) for v in np.asarray(a, dtype=np.float64).ravel()))
+                f.write("\n")
+
+
+def run_ref_metrics(path: str, nranks: int, timeout: float = 600.0):
+    """Run the reference's block::initial (oracle/_ref/ref_metrics) in `path`;
+    per rank: dict(jac, met, dxyz, centre, q)."""
+    if nranks == 1:
+        cmd = [REF_BIN_METRICS]
+    else:
+        mpiexec = shutil.which("mpiexec", path="/opt/conda/bin") or "/opt/conda/bin/mpiexec"
+        cmd = [mpiexec, "-n", str(nranks), REF_BIN_METRICS]
+    subprocess.run(cmd, cwd=path, check=True, timeout=timeout, stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    out = []
+    for r in range(nranks):
+        fn = os.path.join(path, "metrics_%d.bin" % r)
+        dims = np.fromfile(fn, dtype=np.int32, count=3)
+        nx, ny, nz = (int(d) for d in dims)
+        raw = np.fromfile(fn, dtype=np.float64, offset=12)
+        N = nx * ny * nz
+        sh = lambda a, c: a.reshape(c, nz, ny, nx)
+        o = 0
+        jac = raw[o:o + N].reshape(nz, ny, nx); o += N
+        met = sh(raw[o:o + 9 * N], 9); o += 9 * N
+        dxyz = sh(raw[o:o + 9 * N], 9); o += 9 * N
+        centre = sh(raw[o:o + 3 * N], 3); o += 3 * N
+        q = sh(raw[o:o + 5 * N], 5)
+        out.append(dict(jac=jac, met=met, dxyz=dxyz, centre=centre, q=q))
+    return out

@@ -85,14 +85,38 @@ def faces_to_array(faces):
     return t, i, v
 
 
+def mesh_case(name, params, dims, amp):
+    """Blocks whose metrics and initial state come from the reference's own
+    block::initial on a mesh file (oracle/_ref/ref_metrics, Benchmark 2 =
+    Taylor-Green); the node coordinates are kept in the fixture."""
+    nodes = cases.wavy_nodes(dims, amp)
+    d = tempfile.mkdtemp(prefix="golden_mesh_")
+    O.write_mesh_case(d, params, [nodes], [cases.periodic_faces()], benchmark=2)
+    m = O.run_ref_metrics(d, 1)[0]
+    blk = cases.BlockSpec(tuple(dims), cases.periodic_faces(), m["jac"], m["met"], m["dxyz"], m["q"])
+    return [blk], {"b0_nodes": np.stack(nodes), "b0_centre": m["centre"]}
+
+
+MESH_CASES = [
+    # name, params, dims, amplitude, mode, nsteps, cfl
+    ("mesh_wavy_tgv_ns_steps2", dict(model=O.NS), (16, 12, 8), 0.2, "steps", 2, 0.5),
+]
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("reference harness missing: make -C oracle ref")
-    for name, pk, mk, mode, nsteps, cfl in CASES:
+    todo = [(n, pk, mk, mode, ns, cfl, None) for n, pk, mk, mode, ns, cfl in CASES]
+    todo += [(n, pk, (dims, amp), mode, ns, cfl, "mesh") for n, pk, dims, amp, mode, ns, cfl in MESH_CASES]
+    for name, pk, mk, mode, nsteps, cfl, kind in todo:
         if ONLY and name not in ONLY:
             continue
         params = O.Params(**pk)
-        blocks = mk()
+        extra = {}
+        if kind == "mesh":
+            blocks, extra = mesh_case(name, params, *mk)
+        else:
+            blocks = mk()
         d = tempfile.mkdtemp(prefix="golden_")
         O.write_ref_case(d, params, blocks)
         out = O.run_ref(d, len(blocks), mode, nsteps, cfl)
@@ -109,6 +133,7 @@ def main():
             })
             if mode == "rhs":
                 arrays["b%d_out_rhs" % b] = o["rhs"]
+        arrays.update(extra)
         arrays["out_dts"] = out[0]["dts"]
         meta = dict(name=name, params={k: (list(v) if isinstance(v, tuple) else v) for k, v in
                                        vars(params).items()},

@@ -1,0 +1,73 @@
+"""Metrics from mesh nodes on the GPU (nuc3d_upload_mesh) against the oracle
+and against the reference's own block::initial (golden fixture
+mesh_wavy_tgv_ns_steps2): cell centres, coordinate derivatives, J and metric
+terms bit for bit, then the run from those metrics to the parity bar."""
+import numpy as np
+import pytest
+
+from nuc3d_amd import cases, solver
+from oracle import oracle as O
+from tests.golden import all_fixtures
+from tests.parity import TOL, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def case_for(dims, faces=None):
+    jac, met, dxyz = cases.cartesian_metrics(dims, (1.0, 1.0, 1.0))
+    nx, ny, nz = dims
+    q = cases.conserved_over_j(*(np.ones((nz, ny, nx)),) * 4, 100.0 * np.ones((nz, ny, nx)), 1.4, jac)
+    q[1:4] = 0.0
+    return cases.BlockSpec(dims, faces or cases.periodic_faces(), jac, met, dxyz, q)
+
+
+@pytest.mark.parametrize("dims,amp", [((16, 12, 8), 0.15), ((8, 24, 16), 0.3)])
+def test_mesh_metrics_match_oracle(dims, amp):
+    print("case: mesh metrics dims=%s amp=%.2f" % (dims, amp), flush=True)
+    nodes = cases.wavy_nodes(dims, amp)
+    g = solver.Case(solver.Params(), [case_for(dims)])
+    centre = g.upload_mesh(0, *nodes)
+    jac, met, dxyz = g.get_metrics(0)
+    oc, od, oj, om = O.mesh_metrics(*nodes)
+    np.testing.assert_array_equal(centre, oc)
+    np.testing.assert_array_equal(dxyz, od)
+    np.testing.assert_array_equal(jac, oj)
+    np.testing.assert_array_equal(met, om)
+    g.close()
+
+
+def test_mesh_fixture_from_reference_block_initial():
+    fx = [f for f in all_fixtures() if f.name == "mesh_wavy_tgv_ns_steps2"][0]
+    print("case: %s" % fx.name, flush=True)
+    nodes = fx.arrays["b0_nodes"]
+    pk = fx.params()
+    for k in ("scheme", "diff", "width"):
+        pk[k] = tuple(pk[k])
+    blk = fx.blocks()[0]
+    g = solver.Case(solver.Params(**pk), [case_for(blk.dims)])
+    centre = g.upload_mesh(0, nodes[0], nodes[1], nodes[2])
+    jac, met, dxyz = g.get_metrics(0)
+    np.testing.assert_array_equal(centre, fx.arrays["b0_centre"])
+    np.testing.assert_array_equal(jac, blk.jac)
+    np.testing.assert_array_equal(met, blk.met)
+    np.testing.assert_array_equal(dxyz, blk.dxyz)
+    g.set_q(0, blk.q)   # the reference's Taylor-Green initial state (block::initialQ)
+    g.steps(fx.meta["nsteps"], fx.meta["cfl"])
+    err = relerr(g.get_q(0), fx.out(0, "q"))
+    print("  q rel err %.3e" % err, flush=True)
+    assert err <= TOL
+    g.close()
+
+
+def test_mesh_errors():
+    dims = (8, 8, 12)
+    g = solver.Case(solver.Params(), [case_for(dims)])
+    xn, yn, zn = cases.wavy_nodes(dims, 0.1)
+    with pytest.raises(solver.Nuc3dError, match=r"\(-1\).*Jacobian"):
+        g.upload_mesh(0, -xn, yn, zn)
+    g.close()
+    g = solver.Case(solver.Params(), [case_for((10, 8, 8))])
+    xn, yn, zn = cases.wavy_nodes((10, 8, 8), 0.1)
+    with pytest.raises(solver.Nuc3dError, match=r"\(-4\).*divisible by 4"):
+        g.upload_mesh(0, xn, yn, zn)
+    g.close()

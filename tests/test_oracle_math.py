@@ -196,3 +196,60 @@ def test_multiblock_equals_single_block():
     many.steps(1, 0.5)
     got = np.concatenate([many.get_q(b) for b in range(3)], axis=3)
     np.testing.assert_array_equal(got, one.get_q(0))
+
+
+def quartic_nodes(dims, seed):
+    """Node coordinates that are quartic in each index direction inside every
+    4-cell tile (one global quartic per direction plus cross terms of degree
+    <= 4 per direction), so the tiled 5-point interpolation is exact."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims
+    K, J, I = np.meshgrid(np.arange(nz + 1.0), np.arange(ny + 1.0), np.arange(nx + 1.0), indexing="ij")
+    a = rng.uniform(-1, 1, 6) * np.array([1e-2, 1e-3, 1e-4, 1e-3, 1e-3, 1e-4])
+
+    def f(i, j, k):
+        return (i + a[0] * i ** 2 + a[1] * i ** 3 + a[2] * i ** 4 + a[3] * i * j + a[4] * j * k ** 2
+                + a[5] * i ** 2 * j ** 2 * k)
+
+    def grad(i, j, k):
+        return (1 + 2 * a[0] * i + 3 * a[1] * i ** 2 + 4 * a[2] * i ** 3 + a[3] * j + 2 * a[5] * i * j ** 2 * k,
+                a[3] * i + a[4] * k ** 2 + 2 * a[5] * i ** 2 * j * k,
+                2 * a[4] * j * k + a[5] * i ** 2 * j ** 2)
+
+    xn = 0.3 * f(I, J, K)
+    yn = 0.2 * f(J, K, I) + 0.05 * I
+    zn = 0.25 * f(K, I, J)
+    return (xn, yn, zn), f, grad
+
+
+def test_mesh_metrics_exact_on_quartic_meshes():
+    # the 5-point Lagrange weights (value and derivative at the cell centres)
+    # are fixed uniquely by exactness on polynomials of degree 4
+    dims = (8, 12, 8)
+    (xn, yn, zn), f, grad = quartic_nodes(dims, 3)
+    centre, dxyz, jac, met = O.mesh_metrics(xn, yn, zn)
+    nx, ny, nz = dims
+    K, J, I = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
+    np.testing.assert_allclose(centre[0], 0.3 * f(I, J, K), rtol=1e-13)
+    np.testing.assert_allclose(centre[1], 0.2 * f(J, K, I) + 0.05 * I, rtol=1e-13)
+    gi, gj, gk = grad(I, J, K)
+    np.testing.assert_allclose(dxyz[0], 0.3 * gi, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(dxyz[1], 0.3 * gj, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(dxyz[2], 0.3 * gk, rtol=1e-12, atol=1e-14)
+    # y = 0.2 f(J,K,I) + 0.05 I: d/dxi = 0.2 df/dk(J,K,I) + 0.05
+    gj2, gk2, gi2 = grad(J, K, I)
+    np.testing.assert_allclose(dxyz[3], 0.2 * gi2 + 0.05, rtol=1e-12, atol=1e-14)
+    # J and metrics are the inverse of the derivative matrix
+    D = dxyz.reshape(3, 3, -1).transpose(2, 0, 1)
+    M = met.reshape(3, 3, -1).transpose(2, 0, 1)
+    eye = np.einsum("nij,njk->nik", M, D)
+    np.testing.assert_allclose(eye, np.broadcast_to(np.eye(3), eye.shape), atol=1e-12)
+    np.testing.assert_allclose(jac.ravel(), 1.0 / np.linalg.det(D), rtol=1e-12)
+
+
+def test_mesh_metrics_rejects_partial_tiles_and_folded_cells():
+    xn, yn, zn = quartic_nodes((8, 8, 8), 4)[0]
+    with pytest.raises(ValueError, match="-1"):
+        O.mesh_metrics(xn[:, :, :8], yn[:, :, :8], zn[:, :, :8])  # nx = 7
+    with pytest.raises(ValueError, match="-2"):
+        O.mesh_metrics(-xn, yn, zn)  # mirrored: negative Jacobian

@@ -1,6 +1,7 @@
 """The oracle against the reference compiled from its own sources
 (oracle/_ref/ref_harness), on seeded cases beyond the fixtures.  Runs where
 that build exists (this container); skipped elsewhere."""
+import os
 import tempfile
 
 import numpy as np
@@ -88,3 +89,47 @@ def test_live_curvilinear_multiblock_two_ranks():
     blocks exchanged over MPI."""
     spec = cases.taylor_green_wavy((24, 12, 10), seed=25)[0]
     compare(O.Params(model=O.NS, scheme=(O.WENO5Z,) * 3), cases.split_block_x(spec, 2), "steps", 2)
+
+
+needs_metrics = pytest.mark.skipif(not os.path.exists(O.REF_BIN_METRICS), reason="oracle/_ref/ref_metrics absent")
+
+
+@needs_metrics
+@pytest.mark.parametrize("dims,amp", [((16, 12, 8), 0.15), ((12, 8, 16), 0.3)])
+def test_live_mesh_metrics_and_tgv_initial_state(dims, amp):
+    """Cell centres, coordinate derivatives, J and metrics from a mesh file
+    (the reference's own block::initial) equal the oracle bit for bit; its
+    Taylor-Green initial state (Benchmark 2) equals conserved_over_j of the
+    reference-pressure field at those centres."""
+    nodes = cases.wavy_nodes(dims, amp)
+    d = tempfile.mkdtemp(prefix="nuc3d_mesh_")
+    O.write_mesh_case(d, O.Params(model=O.NS), [nodes], [cases.periodic_faces()], benchmark=2)
+    ref = O.run_ref_metrics(d, 1)[0]
+    centre, dxyz, jac, met = O.mesh_metrics(*nodes)
+    np.testing.assert_array_equal(centre, ref["centre"])
+    np.testing.assert_array_equal(dxyz, ref["dxyz"])
+    np.testing.assert_array_equal(jac, ref["jac"])
+    np.testing.assert_array_equal(met, ref["met"])
+    x, y, z = centre
+    g, ma = 1.4, 0.1
+    rho = np.ones_like(x)
+    u = np.sin(x) * np.cos(y) * np.cos(z)
+    v = -np.cos(x) * np.sin(y) * np.cos(z)
+    p = 1.0 / (g * ma * ma) + ((np.cos(2.0 * z) + 2.0) * (np.cos(2.0 * x) + np.cos(2.0 * y)) - 2.0) / 16.0
+    q = cases.conserved_over_j(rho, u, v, np.zeros_like(x), p, g, jac)
+    np.testing.assert_allclose(q, ref["q"], rtol=1e-14, atol=1e-14 * np.abs(ref["q"]).max())
+
+
+@needs_metrics
+def test_live_mesh_metrics_two_ranks():
+    nodes = [cases.wavy_nodes((8, 8, 8), 0.2, lo=(x0, 0.0, 0.0), extent=(np.pi, 2 * np.pi, 2 * np.pi))
+             for x0 in (0.0, np.pi)]
+    d = tempfile.mkdtemp(prefix="nuc3d_mesh_")
+    faces = [[(1, 1, None), (1, 0, None), (0, 3, None), (0, 2, None), (0, 5, None), (0, 4, None)],
+             [(0, 1, None), (0, 0, None), (1, 3, None), (1, 2, None), (1, 5, None), (1, 4, None)]]
+    O.write_mesh_case(d, O.Params(model=O.NS), nodes, faces)
+    ref = O.run_ref_metrics(d, 2)
+    for r in range(2):
+        centre, dxyz, jac, met = O.mesh_metrics(*nodes[r])
+        np.testing.assert_array_equal(jac, ref[r]["jac"])
+        np.testing.assert_array_equal(met, ref[r]["met"])
