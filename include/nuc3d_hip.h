@@ -35,7 +35,8 @@ enum {
     NUC3D_ERR_HIP = -3,         /* a HIP runtime call failed */
     NUC3D_ERR_UNSUPPORTED = -4, /* configuration outside what this build implements */
     NUC3D_ERR_NO_DEVICE = -5,   /* no GPU visible */
-    NUC3D_ERR_ORDER = -6        /* RK stages called out of order */
+    NUC3D_ERR_ORDER = -6,       /* RK stages called out of order */
+    NUC3D_ERR_IO = -7           /* a restart file could not be written or read */
 };
 
 /* Schemes and models, named as in inp/Method.in and inp/PhysModel.in
@@ -99,6 +100,15 @@ int nuc3d_download_q(nuc3d_case *c, int b, double *q5);
 /* RHS of the last nuc3d_rhs call (5N), PDEData3d::getRHS. */
 int nuc3d_download_rhs(nuc3d_case *c, int b, double *rhs5);
 
+/* Restart files of the reference (block::outputQ_binary / inputQ_binary,
+ * block.cpp:754-828): one float64 `time`, then the five fields of Q_work
+ * (conserved / J), i-fastest, nothing else.  The reference names them
+ * flowData/NUC3d_step_<step>_ID_<rank>.bin.  nuc3d_save_q writes block b's
+ * current state; nuc3d_load_q reads one (its size must match the block) and
+ * uploads it like nuc3d_upload_q.  Both run between full steps. */
+int nuc3d_save_q(nuc3d_case *c, int b, const char *path, double time);
+int nuc3d_load_q(nuc3d_case *c, int b, const char *path, double *time);
+
 /* One right-hand-side evaluation of every block on the current state
  * (EulerData3D::solve / NaiverStokesData3d::solve, including the halo
  * exchange and boundary conditions). */
@@ -120,6 +130,16 @@ int nuc3d_steps(nuc3d_case *c, int nsteps, double cfl, void *stream);
  * work.  Results are bit-identical to nuc3d_steps.  With kernel timing on
  * (nuc3d_timing) the steps run as ordinary launches. */
 int nuc3d_step_graph(nuc3d_case *c, int nsteps, double cfl, void *stream);
+
+/* Diagnostics of postproc::solveQ (postproc.cpp:743-848) on the current state,
+ * the values block::Post writes to enstrophy.dat: enstrophy = sum over the
+ * cells of all blocks of |vorticity|^2/2 and kinetic = sum of rho|u|^2/2, with
+ * velocity gradients by the case's derivative schemes and closures.  For
+ * block b, fields5 (5N, may be NULL) receives the vorticity (3 fields), the
+ * enstrophy density and the Q-criterion.  Sums are taken per workgroup and
+ * added in a fixed order, so they are reproducible; their rounding differs
+ * from the reference's cell-by-cell sum by a few ulps per cell added. */
+int nuc3d_diagnostics(nuc3d_case *c, double *enstrophy, double *kinetic, int b, double *fields5, void *stream);
 
 /* Wait for the case's work on its streams. */
 int nuc3d_sync(nuc3d_case *c);

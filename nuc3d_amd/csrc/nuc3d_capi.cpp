@@ -21,6 +21,8 @@
                                                    nuc3d_kparams, int, int, int, int, double *,                      \
                                                    const unsigned long long *, unsigned long long *,                 \
                                                    unsigned long long *, hipStream_t);                               \
+    extern "C" hipError_t nuc3d_launch_diag_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, nuc3d_kparams,  \
+                                                  int, int, double *, double *, hipStream_t);                        \
     extern "C" hipError_t nuc3d_launch_prim_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, nuc3d_kparams,  \
                                                   int, hipStream_t);                                                 \
     extern "C" hipError_t nuc3d_launch_visc_flux_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int,           \
@@ -68,6 +70,7 @@ int hipfail(hipError_t e, const char *what)
 struct Launchers {
     decltype(&nuc3d_launch_sweep_fast) sweep;
     decltype(&nuc3d_launch_prim_fast) prim;
+    decltype(&nuc3d_launch_diag_fast) diag;
     decltype(&nuc3d_launch_visc_flux_fast) vflux;
     decltype(&nuc3d_launch_visc_div_fast) vdiv;
     decltype(&nuc3d_launch_eigen_fast) eigen;
@@ -76,9 +79,9 @@ struct Launchers {
     decltype(&nuc3d_launch_crweno_fast) crweno;
 };
 
-const Launchers kFast = {nuc3d_launch_sweep_fast, nuc3d_launch_prim_fast, nuc3d_launch_visc_flux_fast, nuc3d_launch_visc_div_fast, nuc3d_launch_eigen_fast,
+const Launchers kFast = {nuc3d_launch_sweep_fast, nuc3d_launch_prim_fast, nuc3d_launch_diag_fast, nuc3d_launch_visc_flux_fast, nuc3d_launch_visc_div_fast, nuc3d_launch_eigen_fast,
                          nuc3d_launch_fixup_fast, nuc3d_launch_ghost_fill_fast, nuc3d_launch_crweno_fast};
-const Launchers kExact = {nuc3d_launch_sweep_exact, nuc3d_launch_prim_exact, nuc3d_launch_visc_flux_exact, nuc3d_launch_visc_div_exact, nuc3d_launch_eigen_exact,
+const Launchers kExact = {nuc3d_launch_sweep_exact, nuc3d_launch_prim_exact, nuc3d_launch_diag_exact, nuc3d_launch_visc_flux_exact, nuc3d_launch_visc_div_exact, nuc3d_launch_eigen_exact,
                           nuc3d_launch_fixup_exact, nuc3d_launch_ghost_fill_exact, nuc3d_launch_crweno_exact};
 
 // temporary device buffers of one call, freed on return
@@ -306,7 +309,9 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     K.zquirk = p->zquirk;
     K.nblocks = nblocks;
     const bool ns = p->model == NUC3D_NAVIERSTOKES3D;
-    const int nfields = 15 + 5 + 1 + 9 + 9 + (ns ? 16 : 0);
+    // Q x 3, RHS, J, metrics, dxyz, u v w T (viscous terms and diagnostics),
+    // and for Navier-Stokes the 12 viscous flux components
+    const int nfields = 15 + 5 + 1 + 9 + 9 + 4 + (ns ? 12 : 0);
     c->hb.resize(nblocks);
     std::vector<nuc3d_tile> tiles;
     for (int b = 0; b < nblocks; b++) {
@@ -346,8 +351,8 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         B.jac = next();
         for (int q = 0; q < 9; q++) B.met[q] = next();
         for (int q = 0; q < 9; q++) B.dxyz[q] = next();
+        for (int q = 0; q < 4; q++) B.prim[q] = next();
         if (ns) {
-            for (int q = 0; q < 4; q++) B.prim[q] = next();
             for (int q = 0; q < 12; q++) B.fv[q] = next();
         }
         for (int fc = 0; fc < 6; fc++) {
@@ -543,6 +548,48 @@ int nuc3d_download_metrics(nuc3d_case *c, int b, double *jac, double *met, doubl
     return NUC3D_OK;
 }
 
+int nuc3d_save_q(nuc3d_case *c, int b, const char *path, double time)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!path) return fail(NUC3D_ERR_BAD_ARG, "null path");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "restart files are written between full steps");
+    const nuc3d_block_dev &B = c->hb[b];
+    const size_t N = (size_t)B.n[0] * B.n[1] * B.n[2];
+    std::vector<double> q(5 * N);
+    if ((rc = nuc3d_download_q(c, b, q.data()))) return rc;
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return fail(NUC3D_ERR_IO, std::string("cannot open ") + path + " for writing");
+    const bool ok = std::fwrite(&time, sizeof(double), 1, fp) == 1 && std::fwrite(q.data(), sizeof(double), 5 * N, fp) == 5 * N;
+    if (std::fclose(fp) != 0 || !ok) return fail(NUC3D_ERR_IO, std::string("writing ") + path + " failed");
+    return NUC3D_OK;
+}
+
+int nuc3d_load_q(nuc3d_case *c, int b, const char *path, double *time)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!path) return fail(NUC3D_ERR_BAD_ARG, "null path");
+    const nuc3d_block_dev &B = c->hb[b];
+    const size_t N = (size_t)B.n[0] * B.n[1] * B.n[2];
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return fail(NUC3D_ERR_IO, std::string("cannot open ") + path);
+    std::fseek(fp, 0, SEEK_END);
+    const long size = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    if (size != (long)(sizeof(double) * (1 + 5 * N))) {
+        std::fclose(fp);
+        return fail(NUC3D_ERR_BAD_ARG, std::string(path) + " does not hold time + 5 fields of this block");
+    }
+    double t = 0.0;
+    std::vector<double> q(5 * N);
+    const bool ok = std::fread(&t, sizeof(double), 1, fp) == 1 && std::fread(q.data(), sizeof(double), 5 * N, fp) == 5 * N;
+    std::fclose(fp);
+    if (!ok) return fail(NUC3D_ERR_IO, std::string("reading ") + path + " failed");
+    if (time) *time = t;
+    return nuc3d_upload_q(c, b, q.data());
+}
+
 int nuc3d_upload_q(nuc3d_case *c, int b, const double *q5)
 {
     int rc = check_block(c, b);
@@ -685,6 +732,35 @@ int nuc3d_step_graph(nuc3d_case *c, int nsteps, double cfl, void *stream)
         c->cur = 0;
     }
     if (n < nsteps) return nuc3d_steps(c, nsteps - n, cfl, stream);
+    return NUC3D_OK;
+}
+
+int nuc3d_diagnostics(nuc3d_case *c, double *enstrophy, double *kinetic, int b, double *fields5, void *stream)
+{
+    if (!c || !enstrophy || !kinetic) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    if (fields5 && (b < 0 || b >= c->nb)) return fail(NUC3D_ERR_BAD_ARG, "block index out of range");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = refresh(c, s);
+    if (rc) return rc;
+    const long long nwg = (long long)c->ntiles * NUC3D_KSEG;
+    long long N = 0;
+    if (fields5) N = (long long)c->hb[b].n[0] * c->hb[b].n[1] * c->hb[b].n[2];
+    OpBuffers T;
+    double *dpart = T.get(2 * nwg), *dfields = fields5 ? T.get(5 * N) : nullptr;
+    if (!dpart || (fields5 && !dfields)) return fail(NUC3D_ERR_OOM, "device allocation failed");
+    HIPCHK(c->L->prim(c->db, c->dtiles, c->ntiles, c->K, c->cur, s));
+    HIPCHK(c->L->diag(c->db, c->dtiles, c->ntiles, c->K, c->cur, fields5 ? b : -1, dfields, dpart, s));
+    std::vector<double> part(2 * nwg);
+    HIPCHK(hipMemcpyAsync(part.data(), dpart, sizeof(double) * 2 * nwg, hipMemcpyDeviceToHost, s));
+    if (fields5) HIPCHK(hipMemcpyAsync(fields5, dfields, sizeof(double) * 5 * N, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double e = 0.0, k = 0.0;
+    for (long long w = 0; w < nwg; w++) {
+        e += part[2 * w];
+        k += part[2 * w + 1];
+    }
+    *enstrophy = e;
+    *kinetic = k;
     return NUC3D_OK;
 }
 

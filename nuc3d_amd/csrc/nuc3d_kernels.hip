@@ -920,6 +920,72 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
 }
 
 // ---------------------------------------------------------------------------
+// Diagnostics of postproc::solveQ (postproc.cpp:743-848), after k_prim has
+// written u, v, w, T with their ghosts: velocity gradients (the derivative
+// scheme with closures, chain rule in the order of postproc.cpp:1824-1951),
+// vorticity (wy-vz, uz-wx, vx-uy), enstrophy density |w|^2/2, Q-criterion,
+// and per workgroup the sums of |w|^2/2 and rho|u|^2/2 (partial[2 wg],
+// partial[2 wg + 1]; the host adds them in order).  One workgroup per
+// (tile, plane) as k_prim.  For block bf, fields (5N, compact i-fastest)
+// receives the vorticity, enstrophy density and Q-criterion.
+__global__ __launch_bounds__(256) void KNAME(k_diag)(const nuc3d_block_dev *__restrict__ blocks,
+                                                     const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int bin,
+                                                     int bf, double *__restrict__ fields, double *__restrict__ partial)
+{
+    constexpr int T = NUC3D_TILE;
+    const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const nuc3d_block_dev &B = blocks[tl.b];
+    const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T, k = tl.k0 + blockIdx.y;
+    double ens = 0.0, kin = 0.0;
+    if ((int)blockIdx.y < tl.kn && i < B.n[0] && j < B.n[1]) {
+        const long long x = cidx(B, i, j, k);
+        double g[3][3];
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            const double *f = B.prim[v] + x;
+            const double a = line_diff(B, K, f, 0, i), bb = line_diff(B, K, f, 1, j), cc = line_diff(B, K, f, 2, k);
+            g[v][0] = a * B.met[0][x] + bb * B.met[3][x] + cc * B.met[6][x];
+            g[v][1] = a * B.met[1][x] + bb * B.met[4][x] + cc * B.met[7][x];
+            g[v][2] = a * B.met[2][x] + bb * B.met[5][x] + cc * B.met[8][x];
+        }
+        const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
+        const double vx = g[1][0], vy = g[1][1], vz = g[1][2];
+        const double wx = g[2][0], wy = g[2][1], wz = g[2][2];
+        const double w0 = wy - vz, w1 = uz - wx, w2 = vx - uy;
+        const double q0 = ux * vy + ux * wz + vy * wz - uy * vx - uz * wx - vz * wy;
+        const double rho = B.q[bin][0][x] * B.jac[x];  // physicsModel.cpp:313-353
+        const double u = B.prim[0][x], v = B.prim[1][x], w = B.prim[2][x];
+        ens = 0.5 * (w0 * w0 + w1 * w1 + w2 * w2);
+        kin = 0.5 * rho * (u * u + v * v + w * w);
+        if (tl.b == bf && fields) {
+            const long long N = (long long)B.n[0] * B.n[1] * B.n[2];
+            const long long c = i + (long long)B.n[0] * (j + (long long)B.n[1] * k);
+            fields[c] = w0;
+            fields[N + c] = w1;
+            fields[2 * N + c] = w2;
+            fields[3 * N + c] = (w0 * w0 + w1 * w1 + w2 * w2) * 0.5;
+            fields[4 * N + c] = q0;
+        }
+    }
+    __shared__ double red[2][4];
+    for (int o = 32; o > 0; o >>= 1) {
+        ens += __shfl_xor(ens, o);
+        kin += __shfl_xor(kin, o);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wv] = ens;
+        red[1][wv] = kin;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const long long wg = blockIdx.y * (long long)gridDim.x + blockIdx.x;
+        partial[2 * wg + threadIdx.x] =
+            (red[threadIdx.x][0] + red[threadIdx.x][1]) + (red[threadIdx.x][2] + red[threadIdx.x][3]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Maximum eigenvalue per block and direction of buffer `buf`
 // (physicsModel.cpp:597,673,750), accumulated with atomicMax on the bit
 // patterns of positive doubles.
@@ -1145,5 +1211,14 @@ extern "C" hipError_t KNAME(nuc3d_launch_ghost_fill)(const nuc3d_block_dev *bloc
                                                      hipStream_t st)
 {
     hipLaunchKernelGGL(KNAME(k_ghost_fill), dim3(256, 6, nblocks), dim3(256), 0, st, blocks, kind, buf);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t KNAME(nuc3d_launch_diag)(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles, int ntiles,
+                                               nuc3d_kparams K, int bin, int bf, double *fields, double *partial,
+                                               hipStream_t st)
+{
+    hipLaunchKernelGGL(KNAME(k_diag), dim3(ntiles, NUC3D_KSEG), dim3(256), 0, st, blocks, tiles, K, bin, bf, fields,
+                       partial);
     return hipGetLastError();
 }

@@ -93,6 +93,9 @@ def lib():
         "nuc3d_upload_metrics": [vp, ctypes.c_int, dp, dp, dp],
         "nuc3d_upload_q": [vp, ctypes.c_int, dp],
         "nuc3d_upload_mesh": [vp, ctypes.c_int, dp, dp, dp, dp],
+        "nuc3d_save_q": [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_double],
+        "nuc3d_diagnostics": [vp, dp, dp, ctypes.c_int, dp, vp],
+        "nuc3d_load_q": [vp, ctypes.c_int, ctypes.c_char_p, dp],
         "nuc3d_download_metrics": [vp, ctypes.c_int, dp, dp, dp],
         "nuc3d_download_q": [vp, ctypes.c_int, dp],
         "nuc3d_download_rhs": [vp, ctypes.c_int, dp],
@@ -200,6 +203,24 @@ class Case:
 
     def stage(self, s, cfl):
         _check(lib().nuc3d_stage(self.h, s, cfl, self.stream), "stage")
+
+    def diagnostics(self, b=0):
+        """(enstrophy, kinetic, fields5 of block b) of postproc::solveQ."""
+        e, k = ctypes.c_double(), ctypes.c_double()
+        f = np.empty(self._shape(b, 5))
+        _check(lib().nuc3d_diagnostics(self.h, ctypes.byref(e), ctypes.byref(k), b, _dp(f), self.stream),
+               "diagnostics")
+        return e.value, k.value, f
+
+    def save_q(self, b, path, time):
+        """Restart file of block b in the reference's format (block.cpp:754-778)."""
+        _check(lib().nuc3d_save_q(self.h, b, os.fsencode(path), time), "save_q")
+
+    def load_q(self, b, path):
+        """Loads a restart file (block.cpp:780-810) into block b; returns its time."""
+        t = ctypes.c_double()
+        _check(lib().nuc3d_load_q(self.h, b, os.fsencode(path), ctypes.byref(t)), "load_q")
+        return t.value
 
     def upload_mesh(self, b, xn, yn, zn):
         """Metrics of block b computed on the GPU from node coordinates of shape

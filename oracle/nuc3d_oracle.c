@@ -1110,3 +1110,75 @@ int orc_mesh_metrics(int nx, int ny, int nz, const double *xn, const double *yn,
             }
     return 0;
 }
+
+/* ---------------------------------------------------------------------------
+ * Diagnostics of postproc::solveQ (postproc.cpp:743-848) on the current state:
+ * physicsModel::getPrim (con2prim, physicsModel.cpp:153-161), velocity
+ * gradients by the derivative scheme with halos and closures
+ * (postproc.cpp:1794-1966, the same path as NaiverStokes3d.cpp:86-168),
+ * vorticity w = (wy-vz, uz-wx, vx-uy), enstrophy density |w|^2/2, Q-criterion,
+ * and the sums over all cells of |w|^2/2 and rho|u|^2/2 (per block in cell
+ * order, then over blocks: the MPI_SUM over ranks). */
+int orc_diagnostics(orc_case *C, double *enstrophy, double *kinetic, int bf, double *fields5)
+{
+    int b, d, v, rc, i, j, k;
+    double ens_glb = 0.0, kin_glb = 0.0;
+    for (b = 0; b < C->nb; b++) {
+        oblock *B = &C->B[b];
+        if (!B->rot[0])
+            for (d = 0; d < 3; d++) {
+                for (v = 0; v < 4; v++) B->grad[v][d] = dalloc(B->N);
+                B->rot[d] = dalloc(B->N);
+            }
+        if ((rc = con2prim(C, B))) return rc;
+    }
+    for (v = 0; v < 3; v++) {
+        for (b = 0; b < C->nb; b++) {
+            oblock *B = &C->B[b];
+            for (k = 0; k < B->n[2]; k++)
+                for (j = 0; j < B->n[1]; j++)
+                    for (i = 0; i < B->n[0]; i++)
+                        for (d = 0; d < 3; d++) B->rot[d][ridx(B, d, i, j, k)] = B->W[1 + v][xidx(B, i, j, k)];
+        }
+        for (d = 0; d < 3; d++) {
+            if ((rc = exchange(C, d, v, field_rot))) return rc;
+            for (b = 0; b < C->nb; b++)
+                if ((rc = diff_rot(C, &C->B[b], d, v, C->B[b].rot[d], C->B[b].grad[v][d]))) return rc;
+        }
+    }
+    for (b = 0; b < C->nb; b++) {
+        oblock *B = &C->B[b];
+        double ens = 0.0, kin = 0.0;
+        for (k = 0; k < B->n[2]; k++)
+            for (j = 0; j < B->n[1]; j++)
+                for (i = 0; i < B->n[0]; i++) {
+                    long x = xidx(B, i, j, k), e = ridx(B, 1, i, j, k), z = ridx(B, 2, i, j, k);
+                    double g[3][3];
+                    for (v = 0; v < 3; v++) { /* solveGrad_x/_y/_z, postproc.cpp:1824-1951 */
+                        double a = B->grad[v][0][x], bb = B->grad[v][1][e], cc = B->grad[v][2][z];
+                        g[v][0] = a * B->met[0][x] + bb * B->met[3][x] + cc * B->met[6][x];
+                        g[v][1] = a * B->met[1][x] + bb * B->met[4][x] + cc * B->met[7][x];
+                        g[v][2] = a * B->met[2][x] + bb * B->met[5][x] + cc * B->met[8][x];
+                    }
+                    double ux = g[0][0], uy = g[0][1], uz = g[0][2];
+                    double vx = g[1][0], vy = g[1][1], vz = g[1][2];
+                    double wx = g[2][0], wy = g[2][1], wz = g[2][2];
+                    double w0 = wy - vz, w1 = uz - wx, w2 = vx - uy;
+                    double q0 = ux * vy + ux * wz + vy * wz - uy * vx - uz * wx - vz * wy;
+                    if (b == bf && fields5) {
+                        fields5[0 * B->N + x] = w0;
+                        fields5[1 * B->N + x] = w1;
+                        fields5[2 * B->N + x] = w2;
+                        fields5[3 * B->N + x] = (w0 * w0 + w1 * w1 + w2 * w2) * 0.5;
+                        fields5[4 * B->N + x] = q0;
+                    }
+                    ens += 0.5 * (w0 * w0 + w1 * w1 + w2 * w2);
+                    kin += 0.5 * B->W[0][x] * (B->W[1][x] * B->W[1][x] + B->W[2][x] * B->W[2][x] + B->W[3][x] * B->W[3][x]);
+                }
+        ens_glb += ens;
+        kin_glb += kin;
+    }
+    *enstrophy = ens_glb;
+    *kinetic = kin_glb;
+    return 0;
+}

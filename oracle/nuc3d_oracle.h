@@ -83,6 +83,12 @@ void orc_split_point(const orc_params *p, const double prim5[5], double jac,
 /* TVD-RK3 on flat arrays (TVD-RK3.cpp:37-137). */
 void orc_rk3(int stage, long n, const double *rhs, const double *un, double *ui, double dt);
 
+/* postproc::solveQ diagnostics of the current state (postproc.cpp:743-848):
+ * sums over all cells of all blocks of |vorticity|^2/2 and rho|u|^2/2; for
+ * block bf, fields5 (5N, may be NULL) receives the vorticity (3 fields), the
+ * enstrophy density and the Q-criterion. */
+int orc_diagnostics(orc_case *c, double *enstrophy, double *kinetic, int bf, double *fields5);
+
 /* Cell centres (3N), coordinate derivatives (9N: x_xi,x_eta,x_zeta,y_xi..z_zeta),
  * J (N) and metric terms (9N) of an nx x ny x nz block from its node
  * coordinates ((nx+1)(ny+1)(nz+1) each, i-fastest): block.cpp:147-477.

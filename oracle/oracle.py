@@ -123,6 +123,7 @@ def _lib():
     lib.orc_rk3.argtypes = [ctypes.c_int, ctypes.c_long, dp, dp, dp, ctypes.c_double]
     lib.orc_rk3.restype = None
     lib.orc_mesh_metrics.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp]
+    lib.orc_diagnostics.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_int, dp]
     return lib
 
 
@@ -225,6 +226,13 @@ class OracleCase:
         dts = np.empty(max(n, 1))
         self._check(lib().orc_steps(self.h, n, cfl, _dp(dts)), "steps")
         return dts[:n]
+
+    def diagnostics(self, bi=0):
+        """(enstrophy, kinetic, fields5 of block bi) of postproc::solveQ."""
+        e, k = ctypes.c_double(), ctypes.c_double()
+        f = np.empty(self._shape(bi, 5))
+        self._check(lib().orc_diagnostics(self.h, ctypes.byref(e), ctypes.byref(k), bi, _dp(f)), "diagnostics")
+        return e.value, k.value, f
 
 
 def recon_line(scheme, upwind, ext, w, typeL=0, typeR=0, zquirk=1):
@@ -356,12 +364,15 @@ def read_ref_output(path: str, rank: int):
                 max_eigen=np.array([e0, e1, e2]), dts=dts, walls=walls)
 
 
-def write_mesh_case(path: str, params: Params, nodes, faces, benchmark: int = 2):
+def write_mesh_case(path: str, params: Params, nodes, faces, benchmark: int = 2, start_step: int = 0):
     """A case the reference builds from mesh files: mesh/mesh_<rank>.x (node
     counts, then x, y, z one value per line, block.cpp:42-70), the inp/ files
     of write_ref_case, inp/IOController.in with `Benchmark` (the initial
     condition, block.cpp:523-567), inp/PostProc.in with no statistics, and the
-    flowData/ directory its geometry output goes to.  nodes: per block (xn, yn, zn) of shape (nz+1, ny+1, nx+1)."""
+    flowData/ directory its geometry output goes to.  With start_step > 0 the
+    reference reads its initial state from
+    flowData/NUC3d_step_<start_step>_ID_<rank>.bin instead (block.cpp:92-95).
+    nodes: per block (xn, yn, zn) of shape (nz+1, ny+1, nx+1)."""
     dummy = []
     for (xn, yn, zn), fc in zip(nodes, faces):
         nz, ny, nx = (d - 1 for d in xn.shape)
@@ -372,8 +383,9 @@ def write_mesh_case(path: str, params: Params, nodes, faces, benchmark: int = 2)
     os.makedirs(os.path.join(path, "mesh"), exist_ok=True)
     os.makedirs(os.path.join(path, "flowData"), exist_ok=True)
     with open(os.path.join(path, "inp", "IOController.in"), "w") as f:
-        f.write("startStep 0\nendStep 1\nsaveStep 1000\npostStep 1000\nBenchmark %d\n"
-                "cfl 0.5\ntl 0.0\ndt 0.01\nBinary no\nTecplot no\nPostProc no\n" % benchmark)
+        f.write("startStep %d\nendStep %d\nsaveStep 1000\npostStep 1000\nBenchmark %d\n"
+                "cfl 0.5\ntl 0.0\ndt 0.01\nBinary no\nTecplot no\nPostProc no\n"
+                % (start_step, start_step + 1, benchmark))
     with open(os.path.join(path, "inp", "PostProc.in"), "w") as f:  # postproc.cpp:29-72: no statistics
         f.write("AverageType 0\nAverageStartStep 0\nSimutaneousData 0\nAveragedSaveType 0\n"
                 "SimutaneousSaveType 0\n")
@@ -411,3 +423,21 @@ def run_ref_metrics(path: str, nranks: int, timeout: float = 600.0):
         q = sh(raw[o:o + 5 * N], 5)
         out.append(dict(jac=jac, met=met, dxyz=dxyz, centre=centre, q=q))
     return out
+
+
+def run_ref_post(path: str, nranks: int, timeout: float = 600.0):
+    """The reference's postproc::solveQ on the case written by write_ref_case in
+    `path` (ref_harness post): returns (enstrophy, kinetic) as it prints them
+    to enstrophy.dat (12 significant digits, postproc.cpp:833-842)."""
+    with open(os.path.join(path, "inp", "PostProc.in"), "w") as f:
+        f.write("AverageType 0\nAverageStartStep 0\nSimutaneousData 1\nAveragedSaveType 0\n"
+                "SimutaneousSaveType 0\n")
+    with open(os.path.join(path, "inp", "IOController.in"), "w") as f:  # IOcontroller.cpp:39-62 exits without it
+        f.write("startStep 0\nendStep 1\nsaveStep 1000\npostStep 1\nBenchmark 0\n"
+                "cfl 0.5\ntl 0.0\ndt 0.01\nBinary no\nTecplot no\nPostProc yes\n")
+    fn = os.path.join(path, "enstrophy.dat")
+    if os.path.exists(fn):
+        os.remove(fn)
+    run_ref(path, nranks, "post", timeout=timeout)
+    last = open(fn).read().split("\n")[-2].split()
+    return float(last[2]), float(last[3])

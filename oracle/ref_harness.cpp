@@ -14,6 +14,8 @@
 // inp/PhysModel.in, inp/Method.in, bc_topo/BC_Topo_<rank>.in, in_<rank>.bin):
 //   ref_harness rhs                 one RHS evaluation -> out_<rank>.bin
 //   ref_harness steps <n> <cfl>     n TVD-RK3 steps    -> out_<rank>.bin
+//   ref_harness post                postproc::solveQ   -> enstrophy.dat (+ out_<rank>.bin)
+//                                   (needs inp/PostProc.in with SimutaneousData 1)
 //
 // in_<rank>.bin : int32 nx,ny,nz, then float64 J[N], met[9N], dxyz[9N], Q[5N]
 //                 (met: xi_x..zeta_z; dxyz: x_xi,x_eta,x_zeta,y_xi..z_zeta;
@@ -36,6 +38,8 @@
 #include "euler3d.h"
 #include "fieldOperator.h"
 #include "physicsModel.h"
+#include "IOcontroller.h"
+#include "postproc.hpp"
 
 using namespace nuc3d;
 
@@ -110,6 +114,19 @@ int main(int argc, char **argv)
         std::vector<double> dts, walls;
         if (mode == "rhs") {
             flux->solve(pde, op, bufs, phys, mpi, bc);
+        } else if (mode == "post") {
+            // block::Post (block.cpp:600-616): getPrim, then postproc::solvePost,
+            // which with "SimutaneousData 1" in inp/PostProc.in runs solveQ and
+            // appends step, time, enstrophy and kinetic energy to enstrophy.dat
+            // on rank 0 (postproc.cpp:269-296, 725-848)
+            VectorField prim, acoust, xyz;
+            for (int c = 0; c < 5; c++) prim.push_back(Field(nx, ny, nz));
+            for (int c = 0; c < 3; c++) acoust.push_back(Field(nx, ny, nz));
+            phys.getPrim(flux->jacobian, pde.getQ(), prim, acoust);
+            IOController io;
+            postproc post(nx, ny, nz);
+            post.solvePost(prim, acoust, xyz, flux->xi_xyz, flux->eta_xyz, flux->zeta_xyz, phys, op, bufs, mpi, bc,
+                           io, 0, 0.0);
         } else {
             for (int s = 0; s < nsteps; s++) {
                 MPI_Barrier(MPI_COMM_WORLD);

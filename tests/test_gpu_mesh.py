@@ -71,3 +71,38 @@ def test_mesh_errors():
     with pytest.raises(solver.Nuc3dError, match=r"\(-4\).*divisible by 4"):
         g.upload_mesh(0, xn, yn, zn)
     g.close()
+
+
+def test_restart_file_read_by_the_reference(tmp_path):
+    """A restart file written from the GPU state is read back by the
+    reference's own block::initial -> inputQ_binary (block.cpp:92-95, 780-810)
+    bit for bit; nuc3d_load_q restores the state and time."""
+    import os
+    dims = (16, 12, 8)
+    nodes = cases.wavy_nodes(dims, 0.2)
+    g = solver.Case(solver.Params(), [case_for(dims)])
+    centre = g.upload_mesh(0, *nodes)
+    jac, _, _ = g.get_metrics(0)
+    x, y, z = centre
+    rho = 1.0 + 0.1 * np.sin(x) * np.cos(y)
+    q = cases.conserved_over_j(rho, np.sin(y), np.cos(z), 0.2 * np.sin(x), 71.0 + np.cos(x + z), 1.4, jac)
+    g.set_q(0, q)
+    g.steps(2, 0.5)
+    d = str(tmp_path)
+    os.makedirs(os.path.join(d, "flowData"), exist_ok=True)
+    path = os.path.join(d, "flowData", "NUC3d_step_2_ID_0.bin")
+    g.save_q(0, path, 0.125)
+    want = g.get_q(0)
+    raw = np.fromfile(path, dtype=np.float64)
+    assert raw.size == 1 + want.size and raw[0] == 0.125
+    np.testing.assert_array_equal(raw[1:].reshape(want.shape), want)
+    g2 = solver.Case(solver.Params(), [case_for(dims)])
+    g2.upload_mesh(0, *nodes)
+    assert g2.load_q(0, path) == 0.125
+    np.testing.assert_array_equal(g2.get_q(0), want)
+    g.close()
+    g2.close()
+    if os.path.exists(O.REF_BIN_METRICS):
+        O.write_mesh_case(d, O.Params(model=O.NS), [nodes], [cases.periodic_faces()], start_step=2)
+        ref = O.run_ref_metrics(d, 1)[0]
+        np.testing.assert_array_equal(ref["q"], want)

@@ -133,3 +133,18 @@ def test_live_mesh_metrics_two_ranks():
         centre, dxyz, jac, met = O.mesh_metrics(*nodes[r])
         np.testing.assert_array_equal(jac, ref[r]["jac"])
         np.testing.assert_array_equal(met, ref[r]["met"])
+
+
+@pytest.mark.parametrize("nblocks", [1, 2])
+def test_live_diagnostics_enstrophy_kinetic(nblocks):
+    """Enstrophy and kinetic-energy sums of postproc::solveQ (the reference's
+    enstrophy.dat, 12 significant digits) on a curvilinear case."""
+    spec = cases.taylor_green_wavy((16, 12, 10), seed=26)[0]
+    blocks = [spec] if nblocks == 1 else cases.split_block_x(spec, 2)
+    p = O.Params(model=O.NS)
+    d = tempfile.mkdtemp(prefix="nuc3d_post_")
+    O.write_ref_case(d, p, blocks)
+    ens_ref, kin_ref = O.run_ref_post(d, len(blocks))
+    ens, kin, _ = O.OracleCase(p, blocks).diagnostics()
+    assert abs(ens - ens_ref) <= 1e-11 * abs(ens_ref)
+    assert abs(kin - kin_ref) <= 1e-11 * abs(kin_ref)
