@@ -141,6 +141,31 @@ int nuc3d_step_graph(nuc3d_case *c, int nsteps, double cfl, void *stream);
  * from the reference's cell-by-cell sum by a few ulps per cell added. */
 int nuc3d_diagnostics(nuc3d_case *c, double *enstrophy, double *kinetic, int b, double *fields5, void *stream);
 
+/* The host time loop of singleBlock::solvePDE (singleBlock.cpp:51-71) with
+ * the IOController cadence (IOcontroller.cpp:181-212):
+ *   while (time <= tl && step < end_step):
+ *       one TVD-RK3 step (block::solve, block.cpp:569-594): step += 1,
+ *       time += dt, residual as nuc3d_get_residual;
+ *       verbose: the status line of block::printStatus (block.cpp:596-599);
+ *       post_step > 0 and step % post_step == 0: nuc3d_diagnostics, appending
+ *           "step time enstrophy kinetic" (12 significant digits) to
+ *           <dir>/enstrophy.dat, as postproc::solveQ does (postproc.cpp:833-842);
+ *       save_step > 0 and step % save_step == 0: a restart file per block,
+ *           <dir>/flowData/NUC3d_step_<step>_ID_<block>.bin (block.cpp:754-778).
+ * dir NULL writes no files.  `out` (may be NULL) receives the final step,
+ * time, dt and residual. */
+typedef struct {
+    int start_step, end_step, save_step, post_step; /* startStep endStep saveStep postStep */
+    double cfl, tl, time;                           /* cfl, tl, and the time at start_step */
+    const char *dir;
+    int verbose;
+} nuc3d_run_config;
+typedef struct {
+    int step;
+    double time, dt, residual;
+} nuc3d_run_status;
+int nuc3d_run(nuc3d_case *c, const nuc3d_run_config *cfg, nuc3d_run_status *out, void *stream);
+
 /* Wait for the case's work on its streams. */
 int nuc3d_sync(nuc3d_case *c);
 
@@ -149,6 +174,13 @@ int nuc3d_sync(nuc3d_case *c);
  * (EulerFlux::maxEigen, physicsModel.cpp:756-758). */
 int nuc3d_get_dt(nuc3d_case *c, double *dt_global);
 int nuc3d_get_max_eigen(nuc3d_case *c, int b, double *eig3);
+/* Residual of the last full step, PDEData3d::setRES (PDEData3d.cpp:90-122):
+ * per block res = |sqrt(sum (Q - Q_n)^2 / 5N) - the same of the step before|
+ * / dt^2 over the new state Q and the step's start Q_n, and res_global = the
+ * maximum over blocks (the reference's MPI_MAX over ranks, one block per
+ * rank).  res_blocks (nblocks, may be NULL) receives the per-block values.
+ * 0 before the first step. */
+int nuc3d_get_residual(nuc3d_case *c, double *res_global, double *res_blocks);
 /* Number of e<0 energy fix-ups applied so far (physicsModel.cpp:324-332). */
 int nuc3d_get_fixups(nuc3d_case *c, long long *count);
 

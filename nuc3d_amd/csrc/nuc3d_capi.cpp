@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -131,6 +132,12 @@ struct nuc3d_case {
     int lampar = 0;
     bool dirty = true;                 // descriptors / ghosts / eigenvalues need refreshing
     bool faces_set = false;
+    // residual partials of the last two steps (PDEData3d::setRES): one per
+    // work item of the kernel that finishes stage 2, and that item's block
+    double *dresid = nullptr;
+    int nwork = 0;
+    std::vector<int> work_block;
+    long long steps_done = 0;
     // captured two-step graphs, one per parity of the eigenvalue buffers
     hipStream_t gstream = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -238,29 +245,31 @@ int refresh(nuc3d_case *c, hipStream_t s)
 // viscous fluxes (Navier-Stokes), then the three sweeps; the z sweep
 // completes the RHS (mode 0) or applies RK stage `stage` (mode 1)
 int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, const unsigned long long *lcur,
-               unsigned long long *lnext, hipStream_t s)
+               unsigned long long *lnext, double *resid, hipStream_t s)
 {
     const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
+    nuc3d_kparams K = c->K;
+    K.resid = (mode == 1 && stage == 2) ? resid : nullptr;
     if (ns) {
         {
             Timed t(c, s, &c->ev[3]);
-            HIPCHK(c->L->prim(c->db, c->dtiles, c->ntiles, c->K, bin, s));
+            HIPCHK(c->L->prim(c->db, c->dtiles, c->ntiles, K, bin, s));
         }
         Timed t(c, s, &c->ev[4]);
-        HIPCHK(c->L->vflux(c->db, c->dtiles, c->ntiles, c->K, bin, s));
+        HIPCHK(c->L->vflux(c->db, c->dtiles, c->ntiles, K, bin, s));
     }
     for (int d = 0; d < 3; d++) {
         Timed t(c, s, &c->ev[d]);
         if (c->p.scheme[d] == NUC3D_CRWENO5)
-            HIPCHK(c->L->crweno(d, ns, mode, c->db, c->dlines[d], c->nlines[d], c->nmax[d], c->K, stage, bin, bun,
+            HIPCHK(c->L->crweno(d, ns, mode, c->db, c->dlines[d], c->nlines[d], c->nmax[d], K, stage, bin, bun,
                                 bout, c->ddt, lcur, lnext, c->dcount, c->dscratch, s));
         else
-            HIPCHK(c->L->sweep(d, c->p.scheme[d], ns, mode, c->db, c->dsweep[d], c->nsweep[d], c->K, stage, bin,
+            HIPCHK(c->L->sweep(d, c->p.scheme[d], ns, mode, c->db, c->dsweep[d], c->nsweep[d], K, stage, bin,
                                bun, bout, c->ddt, lcur, lnext, c->dcount, s));
     }
     if (ns) {
         Timed t(c, s, &c->ev[5]);
-        HIPCHK(c->L->vdiv(mode, c->db, c->dtiles, c->ntiles, c->K, stage, bin, bun, bout, c->ddt, lcur, lnext,
+        HIPCHK(c->L->vdiv(mode, c->db, c->dtiles, c->ntiles, K, stage, bin, bun, bout, c->ddt, lcur, lnext,
                           c->dcount, s));
     }
     return NUC3D_OK;
@@ -417,6 +426,21 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
             return fail(NUC3D_ERR_OOM, "device allocation of sweep tiles failed");
         }
     }
+    // work items of the kernel that finishes stage 2, and their blocks
+    if (ns) {
+        for (const auto &t : tiles) c->work_block.push_back(t.b);
+    } else if (p->scheme[2] == NUC3D_CRWENO5) {
+        for (int b = 0; b < nblocks; b++)
+            for (long long l = 0; l < (long long)c->hb[b].n[0] * c->hb[b].n[1]; l++) c->work_block.push_back(b);
+    } else {
+        for (const auto &t : sw[2]) c->work_block.push_back(t.x);
+    }
+    c->nwork = (int)c->work_block.size();
+    if (hipMalloc(&c->dresid, sizeof(double) * 2 * c->nwork) != hipSuccess ||
+        hipMemset(c->dresid, 0, sizeof(double) * 2 * c->nwork) != hipSuccess) {
+        nuc3d_case_destroy(c);
+        return fail(NUC3D_ERR_OOM, "device allocation of residual partials failed");
+    }
     if (hipMalloc(&c->db, sizeof(nuc3d_block_dev) * nblocks) != hipSuccess ||
         hipMalloc(&c->dtiles, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess ||
         hipMalloc(&c->ddt, sizeof(double) * 4) != hipSuccess ||
@@ -455,6 +479,7 @@ int nuc3d_case_destroy(nuc3d_case *c)
     (void)hipFree(c->ddt);
     (void)hipFree(c->dlam);
     (void)hipFree(c->dcount);
+    (void)hipFree(c->dresid);
     delete c;
     return NUC3D_OK;
 }
@@ -642,7 +667,7 @@ int nuc3d_rhs(nuc3d_case *c, void *stream)
         HIPCHK(hipMemsetAsync(lam, 0, sizeof(unsigned long long) * 3 * c->nb, s));
         HIPCHK(c->L->eigen(c->db, c->dtiles, c->ntiles, c->K, c->cur, lam, s));
     }
-    return launch_rhs(c, 0, 0, c->cur, 0, 0, c->dlam, lam, s);
+    return launch_rhs(c, 0, 0, c->cur, 0, 0, c->dlam, lam, nullptr, s);
 }
 
 int nuc3d_stage(nuc3d_case *c, int stage, double cfl, void *stream)
@@ -660,10 +685,14 @@ int nuc3d_stage(nuc3d_case *c, int stage, double cfl, void *stream)
     (void)ns;
     unsigned long long *lcur = c->dlam + 3 * c->nb * c->lampar;
     unsigned long long *lnext = c->dlam + 3 * c->nb * (1 - c->lampar);
-    if ((rc = launch_rhs(c, 1, stage, bin, bun, bout, lcur, lnext, s))) return rc;
+    if ((rc = launch_rhs(c, 1, stage, bin, bun, bout, lcur, lnext, c->dresid + (long long)c->lampar * c->nwork, s)))
+        return rc;
     c->cur = bout;
     c->next_stage = (stage + 1) % 3;
-    if (stage == 2) c->lampar ^= 1;
+    if (stage == 2) {
+        c->lampar ^= 1;
+        c->steps_done++;
+    }
     return NUC3D_OK;
 }
 
@@ -708,7 +737,7 @@ int nuc3d_step_graph(nuc3d_case *c, int nsteps, double cfl, void *stream)
                 const int stage = st % 3;
                 const int bin = stage, bun = 0, bout = stage == 2 ? 0 : stage + 1;
                 rc = launch_rhs(c, 1, stage, bin, bun, bout, c->dlam + 3 * c->nb * lp,
-                                c->dlam + 3 * c->nb * (1 - lp), c->gstream);
+                                c->dlam + 3 * c->nb * (1 - lp), c->dresid + (long long)lp * c->nwork, c->gstream);
                 if (stage == 2) lp ^= 1;
             }
             hipGraph_t g = nullptr;
@@ -730,6 +759,7 @@ int nuc3d_step_graph(nuc3d_case *c, int nsteps, double cfl, void *stream)
         HIPCHK(hipGraphLaunch(c->gexec[par], s));
         c->K.cfl = cfl;
         c->cur = 0;
+        c->steps_done += 2;
     }
     if (n < nsteps) return nuc3d_steps(c, nsteps - n, cfl, stream);
     return NUC3D_OK;
@@ -764,6 +794,49 @@ int nuc3d_diagnostics(nuc3d_case *c, double *enstrophy, double *kinetic, int b, 
     return NUC3D_OK;
 }
 
+int nuc3d_run(nuc3d_case *c, const nuc3d_run_config *cfg, nuc3d_run_status *out, void *stream)
+{
+    if (!c || !cfg) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    if (cfg->save_step < 0 || cfg->post_step < 0) return fail(NUC3D_ERR_BAD_ARG, "negative save or post interval");
+    int step = cfg->start_step;
+    double time = cfg->time, dt = 0.0, res = 0.0;
+    int rc;
+    const std::string dir = cfg->dir ? cfg->dir : "";
+    while (time <= cfg->tl && step < cfg->end_step) {  // IOController::ifsolve
+        if ((rc = nuc3d_steps(c, 1, cfg->cfl, stream))) return rc;
+        if ((rc = nuc3d_get_dt(c, &dt))) return rc;
+        if ((rc = nuc3d_get_residual(c, &res, nullptr))) return rc;
+        step++;
+        time += dt;
+        if (cfg->verbose)
+            std::printf("========step = %d\n time = %.6g, dt = %.6g, residual = %.6g\n", step, time, dt, res);
+        if (cfg->post_step > 0 && step % cfg->post_step == 0) {  // ifpost
+            double ens = 0.0, kin = 0.0;
+            if ((rc = nuc3d_diagnostics(c, &ens, &kin, 0, nullptr, stream))) return rc;
+            if (cfg->dir) {
+                FILE *fp = std::fopen((dir + "/enstrophy.dat").c_str(), "a");
+                if (!fp) return fail(NUC3D_ERR_IO, "cannot append to " + dir + "/enstrophy.dat");
+                std::fprintf(fp, "%d %.12g %.12g %.12g\n", step, time, ens, kin);
+                std::fclose(fp);
+            }
+        }
+        if (cfg->save_step > 0 && step % cfg->save_step == 0 && cfg->dir) {  // ifsave
+            for (int b = 0; b < c->nb; b++) {
+                char name[96];
+                std::snprintf(name, sizeof name, "/flowData/NUC3d_step_%d_ID_%d.bin", step, b);
+                if ((rc = nuc3d_save_q(c, b, (dir + name).c_str(), time))) return rc;
+            }
+        }
+    }
+    if (out) {
+        out->step = step;
+        out->time = time;
+        out->dt = dt;
+        out->residual = res;
+    }
+    return NUC3D_OK;
+}
+
 int nuc3d_sync(nuc3d_case *c)
 {
     if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
@@ -791,6 +864,38 @@ int nuc3d_get_max_eigen(nuc3d_case *c, int b, double *eig3)
     unsigned long long v[3];
     HIPCHK(hipMemcpy(v, c->dlam + 3 * c->nb * half + 3 * b, sizeof v, hipMemcpyDeviceToHost));
     for (int d = 0; d < 3; d++) std::memcpy(&eig3[d], &v[d], sizeof(double));
+    return NUC3D_OK;
+}
+
+int nuc3d_get_residual(nuc3d_case *c, double *res_global, double *res_blocks)
+{
+    if (!c || !res_global) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    HIPCHK(hipDeviceSynchronize());
+    std::vector<double> res(c->nb, 0.0);
+    if (c->steps_done > 0) {
+        std::vector<double> part(2 * (size_t)c->nwork);
+        HIPCHK(hipMemcpy(part.data(), c->dresid, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
+        double dt = 0.0;
+        HIPCHK(hipMemcpy(&dt, c->ddt, sizeof(double), hipMemcpyDeviceToHost));
+        // the last step wrote the half of the eigenvalue parity it started with
+        const int last = 1 - c->lampar, prev = c->lampar;
+        std::vector<double> s_last(c->nb, 0.0), s_prev(c->nb, 0.0);
+        for (int w = 0; w < c->nwork; w++) {
+            s_last[c->work_block[w]] += part[(size_t)last * c->nwork + w];
+            s_prev[c->work_block[w]] += part[(size_t)prev * c->nwork + w];
+        }
+        for (int b = 0; b < c->nb; b++) {
+            const double n5 = 5.0 * c->hb[b].n[0] * c->hb[b].n[1] * c->hb[b].n[2];
+            const double rms = std::sqrt(s_last[b] / n5);
+            const double rms_last = c->steps_done > 1 ? std::sqrt(s_prev[b] / n5) : 0.0;
+            res[b] = std::fabs(rms - rms_last) / (dt * dt);
+        }
+    }
+    double g = 0.0;
+    for (int b = 0; b < c->nb; b++) g = res[b] > g ? res[b] : g;  // MPI_MAX over ranks
+    *res_global = g;
+    if (res_blocks)
+        for (int b = 0; b < c->nb; b++) res_blocks[b] = res[b];
     return NUC3D_OK;
 }
 

@@ -251,6 +251,22 @@ __device__ __forceinline__ void block_max3_atomic(double *m3, unsigned long long
     }
 }
 
+// Per-workgroup sum of v, stored at *dst by thread 0 (the residual partials
+// of one work item; the host adds them per block in a fixed order).
+__device__ __forceinline__ void block_sum_store(double v, double *dst)
+{
+    __shared__ double red[4];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+        *dst = t;
+    }
+}
+
 // dt_global = cfl * MIN_b min_d 1/lambda_{b,d} (euler3d.cpp:429-435, PDEData3d.cpp:78-79)
 __device__ __forceinline__ double global_dt(const unsigned long long *lam, int nb, double cfl)
 {
@@ -287,7 +303,7 @@ __device__ __forceinline__ double stage_dt(const nuc3d_kparams &K, int stage, do
 __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
                                                const PhysParams &P, int i, int j, int k, long long x, const double *r,
                                                const double *u5, const double *un5, double jac, int stage, int bout,
-                                               double dt, double *lmax, unsigned long long &nfix)
+                                               double dt, double *lmax, unsigned long long &nfix, double &r2)
 {
     double qn[5];
 #pragma unroll
@@ -301,6 +317,11 @@ __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, co
         else
             qn[c] = 1.0 / 3.0 * un5[c] + 2.0 / 3.0 * (rhs * dt + u);
     }
+    // residual of the step: the new state before the fix-up of the next
+    // con2prim, against u_n (PDEData3d.cpp:70-122)
+    if (stage == 2)
+#pragma unroll
+        for (int c = 0; c < 5; c++) r2 += (qn[c] - un5[c]) * (qn[c] - un5[c]);
     if (fixup_energy(qn, jac, P)) nfix++;
 #pragma unroll
     for (int c = 0; c < 5; c++) B.q[bout][c][x] = qn[c];
@@ -323,7 +344,7 @@ __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, co
 __device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
                                           const PhysParams &P, int i, int j, int k, long long x, const double *r,
                                           int stage, int bin, int bun, int bout, double dt, double *lmax,
-                                          unsigned long long &nfix)
+                                          unsigned long long &nfix, double &r2)
 {
     double u5[5], un5[5];
 #pragma unroll
@@ -331,7 +352,7 @@ __device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const n
         u5[c] = B.q[bin][c][x];
         un5[c] = stage == 0 ? 0.0 : B.q[bun][c][x];
     }
-    rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, B.jac[x], stage, bout, dt, lmax, nfix);
+    rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, B.jac[x], stage, bout, dt, lmax, nfix, r2);
 }
 
 // Cell (i,j,k) receives the flux difference dF of direction DIR:
@@ -342,7 +363,7 @@ template <int DIR, bool NS, int MODE>
 __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
                                             const nuc3d_kparams &K, const PhysParams &P, int i, int j, int k,
                                             const double *dF, int stage, int bin, int bun, int bout, double dt,
-                                            double *lmax, unsigned long long &nfix)
+                                            double *lmax, unsigned long long &nfix, double &r2)
 {
     const long long x = cidx(B, i, j, k);
     double r[5];
@@ -353,7 +374,7 @@ __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const
         for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
         return;
     }
-    rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix);
+    rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
 }
 
 }  // namespace
@@ -385,7 +406,8 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     __shared__ double S[10][NSL * NL];
     __shared__ double HF[5][NF * NL];
 
-    const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const int witem = xcd_tile(blockIdx.x, gridDim.x);
+    const int4 t = tiles[witem];
     const nuc3d_block_dev &B = blocks[t.x];
     const PhysParams P = phys_of(K);
     const int a0 = t.y, l0 = t.z, cc = t.w;
@@ -440,6 +462,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
 
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
+    double r2 = 0.0;  // residual partial (stage 2)
     const double dt = (DIR == 2 && MODE == 1) ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
     if (line < nlines && f >= 1 && f <= nc) {
         int i, j, k;
@@ -447,11 +470,13 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
         double dF[5];
 #pragma unroll
         for (int c = 0; c < 5; c++) dF[c] = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
-        finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix);
+        finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
     }
     if (DIR == 2 && MODE == 1) {
         if (nfix) atomicAdd(counters, nfix);
         if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * t.x);
+        // for Navier-Stokes k_visc_div finishes the stage and holds the partials
+        if (!NS && stage == 2 && K.resid) block_sum_store(r2, K.resid + witem);
     }
 }
 
@@ -478,7 +503,8 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 {
     constexpr int H = NUC3D_HALO;
     extern __shared__ double lds[];
-    const int4 L = lines[xcd_tile(blockIdx.x, gridDim.x)];
+    const int witem = xcd_tile(blockIdx.x, gridDim.x);
+    const int4 L = lines[witem];
     const nuc3d_block_dev &B = blocks[L.x];
     const PhysParams P = phys_of(K);
     const int n = B.n[DIR], w = K.width[DIR], M = n - 2 * w + 1;
@@ -560,6 +586,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
     __syncthreads();
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
+    double r2 = 0.0;  // residual partial (stage 2)
     const double dt = (DIR == 2 && MODE == 1) ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
     for (int a = threadIdx.x; a < n; a += 64) {
         int i, j, k;
@@ -567,7 +594,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
         double dF[5];
 #pragma unroll
         for (int c = 0; c < 5; c++) dF[c] = HF[c * (n + 1) + a + 1] - HF[c * (n + 1) + a];
-        finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix);
+        finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
     }
     if (DIR == 2 && MODE == 1) {
         if (nfix) atomicAdd(counters, nfix);
@@ -578,6 +605,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
                 if (threadIdx.x == 0 && v > 0.0)
                     atomicMax(lam_next + 3 * L.x + d, (unsigned long long)__double_as_longlong(v));
             }
+            if (!NS && K.resid) block_sum_store(r2, K.resid + witem);
         }
     }
 }
@@ -835,6 +863,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
     const double dt = MODE == 1 ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
+    double r2 = 0.0;  // residual partial (stage 2)
 
     double zr[2 * H + 1][4];  // fv_zeta (components 1..4) of the own column, planes p-6..p
 #pragma unroll
@@ -910,12 +939,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
 #pragma unroll
             for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
         } else {
-            rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix);
+            rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
         }
     }
     if (MODE == 1) {
         if (nfix) atomicAdd(counters, nfix);
         if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * tl.b);
+        if (stage == 2 && K.resid) block_sum_store(r2, K.resid + xcd_tile(blockIdx.x, gridDim.x));
     }
 }
 

@@ -40,4 +40,5 @@ struct nuc3d_kparams {
     int width[3];
     int zquirk;
     int nblocks;
+    double *resid;  // stage 2: per-work-item partial sums of (Q - Q_n)^2, or null
 };

@@ -95,6 +95,8 @@ def lib():
         "nuc3d_upload_mesh": [vp, ctypes.c_int, dp, dp, dp, dp],
         "nuc3d_save_q": [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_double],
         "nuc3d_diagnostics": [vp, dp, dp, ctypes.c_int, dp, vp],
+        "nuc3d_get_residual": [vp, dp, dp],
+        "nuc3d_run": [vp, ctypes.POINTER(_RunConfig), ctypes.POINTER(_RunStatus), vp],
         "nuc3d_load_q": [vp, ctypes.c_int, ctypes.c_char_p, dp],
         "nuc3d_download_metrics": [vp, ctypes.c_int, dp, dp, dp],
         "nuc3d_download_q": [vp, ctypes.c_int, dp],
@@ -140,6 +142,17 @@ def _check(rc, what):
     if rc != 0:
         msg = lib().nuc3d_last_error().decode(errors="replace")
         raise Nuc3dError(f"{what} failed ({rc}): {msg}")
+
+
+class _RunConfig(ctypes.Structure):
+    _fields_ = [("start_step", ctypes.c_int), ("end_step", ctypes.c_int), ("save_step", ctypes.c_int),
+                ("post_step", ctypes.c_int), ("cfl", ctypes.c_double), ("tl", ctypes.c_double),
+                ("time", ctypes.c_double), ("dir", ctypes.c_char_p), ("verbose", ctypes.c_int)]
+
+
+class _RunStatus(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_int), ("time", ctypes.c_double), ("dt", ctypes.c_double),
+                ("residual", ctypes.c_double)]
 
 
 class Case:
@@ -203,6 +216,23 @@ class Case:
 
     def stage(self, s, cfl):
         _check(lib().nuc3d_stage(self.h, s, cfl, self.stream), "stage")
+
+    def residual(self):
+        """(global, per block) residual of the last step (PDEData3d::setRES)."""
+        g = ctypes.c_double()
+        per = np.empty(len(self.blocks))
+        _check(lib().nuc3d_get_residual(self.h, ctypes.byref(g), _dp(per)), "get_residual")
+        return g.value, per
+
+    def run(self, end_step, cfl=0.5, tl=1e30, save_step=0, post_step=0, start_step=0, time=0.0, dir=None,
+            verbose=False):
+        """singleBlock::solvePDE with the IOController cadence; returns
+        dict(step, time, dt, residual)."""
+        cfg = _RunConfig(start_step, end_step, save_step, post_step, cfl, tl, time,
+                         os.fsencode(dir) if dir else None, 1 if verbose else 0)
+        st = _RunStatus()
+        _check(lib().nuc3d_run(self.h, ctypes.byref(cfg), ctypes.byref(st), self.stream), "run")
+        return dict(step=st.step, time=st.time, dt=st.dt, residual=st.residual)
 
     def diagnostics(self, b=0):
         """(enstrophy, kinetic, fields5 of block b) of postproc::solveQ."""

@@ -439,6 +439,7 @@ typedef struct {
     double *df[3][5];              /* flux differences, rotated */
     double maxeig[3];
     double dt_local;
+    double res_last, res_local; /* PDEData3d::setRES: previous rms and residual */
     /* NS */
     double *grad[4][3];            /* d(u,v,w,T)/d(xi,eta,zeta), rotated of that direction */
     double *rot[3];                /* setGrad transposes of one scalar */
@@ -453,6 +454,7 @@ struct orc_case {
     int nb, bf;
     oblock *B;
     double dt_global;
+    double res_global;
     long fixups;
     double *work; /* line scratch */
     double *ext, *lout;
@@ -1014,8 +1016,23 @@ int orc_stage(orc_case *C, int stage, double cfl)
     }
     for (b = 0; b < C->nb; b++)
         for (c = 0; c < 5; c++) orc_rk3(stage, C->B[b].N, C->B[b].rhs[c], C->B[b].qn[c], C->B[b].q[c], C->dt_global);
+    if (stage == 2) { /* setRES, PDEData3d.cpp:90-122; the static res_last is per process = per block */
+        C->res_global = 0.0;
+        for (b = 0; b < C->nb; b++) {
+            oblock *B = &C->B[b];
+            double res_this = 0.0;
+            long x;
+            for (c = 0; c < 5; c++)
+                for (x = 0; x < B->N; x++) res_this += pow((B->q[c][x] - B->qn[c][x]), 2);
+            B->res_local = fabs(sqrt(res_this / (B->N * 5)) - B->res_last) / (C->dt_global * C->dt_global);
+            B->res_last = sqrt(res_this / (B->N * 5));
+            if (B->res_local > C->res_global) C->res_global = B->res_local; /* MPI_MAX */
+        }
+    }
     return 0;
 }
+
+double orc_get_residual(const orc_case *C, int b) { return b < 0 ? C->res_global : C->B[b].res_local; }
 
 int orc_steps(orc_case *C, int nsteps, double cfl, double *dts)
 {

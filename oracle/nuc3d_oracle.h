@@ -67,6 +67,9 @@ int orc_rhs(orc_case *c);
 int orc_stage(orc_case *c, int stage, double cfl);
 /* nsteps full TVD-RK3 steps (block.cpp:569-594); dt of each step to dts (may be NULL). */
 int orc_steps(orc_case *c, int nsteps, double cfl, double *dts);
+/* residual of the last step (PDEData3d::setRES): block b, or the maximum over
+ * blocks for b < 0 */
+double orc_get_residual(const orc_case *c, int b);
 
 /* ---- line-level operators (one line, ghosts included) ----
  * ext has n + 2w entries: ext[w+i] = cell i, ext[w-1-g] = left ghost g, ext[w+n+g] = right ghost g.

@@ -124,6 +124,8 @@ def _lib():
     lib.orc_rk3.restype = None
     lib.orc_mesh_metrics.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp]
     lib.orc_diagnostics.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_int, dp]
+    lib.orc_get_residual.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_get_residual.restype = ctypes.c_double
     return lib
 
 
@@ -226,6 +228,11 @@ class OracleCase:
         dts = np.empty(max(n, 1))
         self._check(lib().orc_steps(self.h, n, cfl, _dp(dts)), "steps")
         return dts[:n]
+
+    def residual(self, bi=-1):
+        """Residual of the last step (PDEData3d::setRES): block bi or, for
+        bi < 0, the maximum over blocks."""
+        return lib().orc_get_residual(self.h, bi)
 
     def diagnostics(self, bi=0):
         """(enstrophy, kinetic, fields5 of block bi) of postproc::solveQ."""
@@ -359,9 +366,10 @@ def read_ref_output(path: str, rank: int):
     w0 = sh(raw[o:o + 3 * N], 3); o += 3 * N
     dt_local, dt_global, e0, e1, e2, ns = raw[o:o + 6]; o += 6
     dts = raw[o:o + int(ns)]; o += int(ns)
-    walls = raw[o:o + int(ns)]
+    walls = raw[o:o + int(ns)]; o += int(ns)
+    res = raw[o:o + int(ns)]
     return dict(rhs=rhs, q=q, prim=np.concatenate([w, w0]), dt_local=dt_local, dt_global=dt_global,
-                max_eigen=np.array([e0, e1, e2]), dts=dts, walls=walls)
+                max_eigen=np.array([e0, e1, e2]), dts=dts, walls=walls, res=res)
 
 
 def write_mesh_case(path: str, params: Params, nodes, faces, benchmark: int = 2, start_step: int = 0):

@@ -22,7 +22,8 @@
 //                  Q = conserved/J, i-fastest)
 // out_<rank>.bin: float64 RHS[5N], Q[5N], W[5N] (rho,u,v,w,p), W0[3N] (T,e,c),
 //                 dt_local, dt_global, maxEigen[3], nsteps, dt[nsteps],
-//                 wall[nsteps] (MPI_Wtime around each step, max over ranks)
+//                 wall[nsteps] (MPI_Wtime around each step, max over ranks),
+//                 res[nsteps] (PDEData3d::getRes after each step)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -111,7 +112,7 @@ int main(int argc, char **argv)
         }
         for (int c = 0; c < 5; c++) load(pde.getQ()[c], &buf[(19 + c) * N]);
 
-        std::vector<double> dts, walls;
+        std::vector<double> dts, walls, res;
         if (mode == "rhs") {
             flux->solve(pde, op, bufs, phys, mpi, bc);
         } else if (mode == "post") {
@@ -139,6 +140,7 @@ int main(int argc, char **argv)
                 MPI_Allreduce(&w, &wmax, 1, MPI_DOUBLE, MPI_MAX, MPI_COMM_WORLD);
                 dts.push_back(pde.getDt());
                 walls.push_back(wmax);
+                res.push_back(pde.getRes());
             }
         }
 
@@ -153,6 +155,7 @@ int main(int argc, char **argv)
         std::fwrite(tail, sizeof(double), 6, fp);
         if (!dts.empty()) std::fwrite(dts.data(), sizeof(double), dts.size(), fp);
         if (!walls.empty()) std::fwrite(walls.data(), sizeof(double), walls.size(), fp);
+        if (!res.empty()) std::fwrite(res.data(), sizeof(double), res.size(), fp);
         std::fclose(fp);
     }
     return 0;

@@ -148,3 +148,19 @@ def test_live_diagnostics_enstrophy_kinetic(nblocks):
     ens, kin, _ = O.OracleCase(p, blocks).diagnostics()
     assert abs(ens - ens_ref) <= 1e-11 * abs(ens_ref)
     assert abs(kin - kin_ref) <= 1e-11 * abs(kin_ref)
+
+
+@pytest.mark.parametrize("nranks", [1, 2])
+def test_live_residual_per_step(nranks):
+    """PDEData3d::setRES after every step (function-static res_last per
+    process, MPI_MAX over ranks) equals the oracle's bit for bit."""
+    spec = cases.taylor_green(dims=(16, 12, 10), noise=1e-3, seed=27)[0]
+    blocks = [spec] if nranks == 1 else cases.split_block_x(spec, 2)
+    p = O.Params(model=O.NS)
+    d = tempfile.mkdtemp(prefix="nuc3d_res_")
+    O.write_ref_case(d, p, blocks)
+    ref = O.run_ref(d, nranks, "steps", 3, 0.5)
+    oc = O.OracleCase(p, blocks)
+    for s in range(3):
+        oc.steps(1, 0.5)
+        assert oc.residual() == ref[0]["res"][s], (s, oc.residual(), ref[0]["res"][s])
