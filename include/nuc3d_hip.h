@@ -42,6 +42,12 @@ enum {
 /* Schemes and models, named as in inp/Method.in and inp/PhysModel.in
  * (fieldOperator.cpp:127-369, physicsModel.cpp:13-112). */
 enum { NUC3D_WENO5JS = 0, NUC3D_WENO5Z = 1, NUC3D_CRWENO5 = 2 };
+/* Eighth-order central flux of each split part at face m from cells m-4..m+3,
+ * (533(f3+f4) - 139(f2+f5) + 29(f1+f6) - 3(f0+f7))/840.  No counterpart in the
+ * reference: the smooth-region branch of the hybrid reconstruction.  Needs
+ * width 4 (four ghost layers, four closure faces at physical faces, which keep
+ * the reference's closures). */
+enum { NUC3D_CENTRAL8 = 3 };
 enum { NUC3D_CD6 = 0, NUC3D_CD8 = 1 };
 enum { NUC3D_LF = 0, NUC3D_AUSM = 1, NUC3D_AUSMP = 2 };
 enum { NUC3D_EULER3D = 0, NUC3D_NAVIERSTOKES3D = 1 };

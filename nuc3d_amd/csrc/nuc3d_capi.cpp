@@ -290,10 +290,18 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     if (nblocks < 1 || nblocks > NUC3D_MAX_BLOCKS) return fail(NUC3D_ERR_BAD_ARG, "nblocks out of range");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(NUC3D_ERR_NO_DEVICE, "no HIP device");
+    int G = 3;
     for (int d = 0; d < 3; d++) {
-        if (p->scheme[d] != NUC3D_WENO5JS && p->scheme[d] != NUC3D_WENO5Z && p->scheme[d] != NUC3D_CRWENO5)
+        if (p->scheme[d] != NUC3D_WENO5JS && p->scheme[d] != NUC3D_WENO5Z && p->scheme[d] != NUC3D_CRWENO5 &&
+            p->scheme[d] != NUC3D_CENTRAL8)
             return fail(NUC3D_ERR_UNSUPPORTED, "scheme not available in this build");
-        if (p->width[d] != 3) return fail(NUC3D_ERR_UNSUPPORTED, "ghost width must be 3 for WENO5 schemes");
+        // the central scheme reaches four cells past the face: four ghost layers
+        // and four closure faces at physical faces (bufferWidth 4, as cd8)
+        const int wneed = p->scheme[d] == NUC3D_CENTRAL8 ? 4 : 3;
+        if (p->width[d] != wneed)
+            return fail(NUC3D_ERR_UNSUPPORTED, wneed == 3 ? "ghost width must be 3 for WENO5 schemes"
+                                                          : "ghost width must be 4 for the central scheme");
+        if (wneed > G) G = wneed;   // block.cpp:506-507: the buffer is the widest scheme's
         if (p->diff[d] != NUC3D_CD6)
             return fail(NUC3D_ERR_UNSUPPORTED, "cd8 needs ghost width 4 (centraldifference8th.cpp:34,49)");
     }
@@ -302,7 +310,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     nuc3d_case *c = new nuc3d_case();
     c->p = *p;
     c->nb = nblocks;
-    c->G = 3;
+    c->G = G;
     c->L = p->exact ? &kExact : &kFast;
     (void)hipGetDevice(&c->device);
     nuc3d_kparams &K = c->K;

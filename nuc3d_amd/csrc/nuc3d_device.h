@@ -37,7 +37,7 @@ constexpr double CD6_0 = 3.0 / 4.0, CD6_1 = -3.0 / 20.0, CD6_2 = 1.0 / 60.0;
 constexpr double CD8_0 = 4.0 / 5.0, CD8_1 = -1.0 / 5.0, CD8_2 = 4.0 / 105.0, CD8_3 = -1.0 / 280.0;
 constexpr double CD4_0 = 2.0 / 3.0, CD4_1 = -1.0 / 12.0;
 
-enum { SCH_WENO5JS = 0, SCH_WENO5Z = 1, SCH_CRWENO5 = 2 };
+enum { SCH_WENO5JS = 0, SCH_WENO5Z = 1, SCH_CRWENO5 = 2, SCH_CENTRAL8 = 3, SCH_HYBRID = 4 };
 enum { RIE_LF = 0, RIE_AUSM = 1, RIE_AUSMP = 2 };
 enum { BC_INLET = 0, BC_OUTLET = 1, BC_WALL = 2, BC_SYMM = 3 };
 
@@ -137,11 +137,23 @@ __device__ __forceinline__ double third_order(double f0, double f1, double f2)
     return w0 * q20 + w1 * q21;
 }
 
+// Eighth-order central flux of one split part at face m from cells m-4..m+3
+// (s[-1..6]): the face point value of the degree-7 polynomial with the eight
+// cell averages, (533 (f3+f4) - 139 (f2+f5) + 29 (f1+f6) - 3 (f0+f7)) / 840.
+// No counterpart in the reference: the smooth-region branch of the hybrid
+// reconstruction.
+__device__ __forceinline__ double central8(const double *s)
+{
+    return ((533.0 * (s[2] + s[3]) - 139.0 * (s[1] + s[4])) + (29.0 * (s[0] + s[5]) - 3.0 * (s[-1] + s[6]))) / 840.0;
+}
+
 // Interior face value for the + (upwind>0) or - stencil.  s[0..5] holds the
-// split flux at cells m-3..m+2 of face m (between cells m-1 and m).
+// split flux at cells m-3..m+2 of face m (between cells m-1 and m); the
+// central scheme also reads s[-1] and s[6] (cells m-4 and m+3).
 template <int SCHEME>
 __device__ __forceinline__ double face_interior(const double *s, bool plus, bool zq)
 {
+    if (SCHEME == SCH_CENTRAL8) return central8(s);
     if (plus) {
         if (SCHEME == SCH_WENO5Z) return weno5z(s[0], s[1], s[2], s[3], s[4], zq);
         return weno5js(s[0], s[1], s[2], s[3], s[4]);

@@ -383,8 +383,9 @@ __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const
 // Direction sweep.  A workgroup owns 16 lines of one block along direction
 // DIR and 15 consecutive cells of each line (16 faces).  It computes the split
 // fluxes of the 21 cells of every line that its faces reach (3-cell halo,
-// physicsModel.cpp:528-754) into LDS, reconstructs the 16 faces per line
-// (one face per thread: WENO5-JS / WENO5-Z or the physical closures), and
+// physicsModel.cpp:528-754; 23 cells and a 4-cell halo for the eighth-order
+// central scheme) into LDS, reconstructs the 16 faces per line (one face per
+// thread: WENO5-JS / WENO5-Z / central or the physical closures), and
 // differences them into the RHS (euler3d.cpp:143-241, :368-427):
 //   DIR 0: rhs  = dF/dxi
 //   DIR 1: rhs += dG/deta
@@ -402,7 +403,9 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
                                                       const unsigned long long *lam_cur,
                                                       unsigned long long *lam_next, unsigned long long *counters)
 {
-    constexpr int NL = 16, NF = 16, NC = NF - 1, H = NUC3D_HALO, NSL = NC + 2 * H;  // 21 slots per line
+    // 3-cell halo (21 slots per line) for WENO5; the central scheme reaches 4 cells (23 slots)
+    constexpr int H = SCHEME == SCH_CENTRAL8 ? 4 : NUC3D_HALO;
+    constexpr int NL = 16, NF = 16, NC = NF - 1, NSL = NC + 2 * H, NST = 2 * H;
     __shared__ double S[10][NSL * NL];
     __shared__ double HF[5][NF * NL];
 
@@ -449,13 +452,14 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
         const int m = a0 + f;
 #pragma unroll
         for (int c = 0; c < 5; c++) {
-            double sp[6], sn[6];
+            // slots f..f+NST-1 = cells m-H..m+H-1; the six-cell WENO stencil starts at cell m-3
+            double sp[NST], sn[NST];
 #pragma unroll
-            for (int q = 0; q < 6; q++) {
+            for (int q = 0; q < NST; q++) {
                 sp[q] = S[c][sidx(line, f + q)];
                 sn[q] = S[5 + c][sidx(line, f + q)];
             }
-            HF[c][hidx(line, f)] = face_value<SCHEME>(B, K, DIR, m, sp, sn);
+            HF[c][hidx(line, f)] = face_value<SCHEME>(B, K, DIR, m, sp + (H - 3), sn + (H - 3));
         }
     }
     __syncthreads();
@@ -1151,6 +1155,9 @@ extern "C" hipError_t KNAME(nuc3d_launch_sweep)(int dir, int scheme, int ns, int
     } else if (scheme == SCH_WENO5Z) {
         if (ns) sweep_dispatch<SCH_WENO5Z, true>(NUC3D_A);
         else sweep_dispatch<SCH_WENO5Z, false>(NUC3D_A);
+    } else if (scheme == SCH_CENTRAL8) {
+        if (ns) sweep_dispatch<SCH_CENTRAL8, true>(NUC3D_A);
+        else sweep_dispatch<SCH_CENTRAL8, false>(NUC3D_A);
     } else {
         return hipErrorInvalidValue;
     }

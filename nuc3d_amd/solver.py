@@ -18,6 +18,7 @@ import numpy as np
 from . import LIB_PATH
 
 WENO5JS, WENO5Z, CRWENO5 = 0, 1, 2
+CENTRAL8 = 3   # eight-cell central face value, width 4; no counterpart in the reference (the hybrid's smooth scheme)
 CD6, CD8 = 0, 1
 RECON_BOUNDARY = 3   # physical-face reconstruction closure (operator level)
 DIFF_BOUNDARY = 2    # physical-face derivative closure (operator level)

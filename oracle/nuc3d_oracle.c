@@ -96,6 +96,14 @@ static void stencil5(const double *ext, int W, int m, int upwind, double *f)
         for (z = 0; z < 5; z++) f[z] = ext[W + m + 2 - z];
 }
 
+/* Eighth-order central face value from cells m-4..m+3 (c[0..7]): the face
+ * point value of the degree-7 polynomial with the eight cell averages.  Not
+ * in the reference; it is the smooth-region scheme of the learned hybrid. */
+static double central8_face(const double *c)
+{
+    return ((533.0 * (c[3] + c[4]) - 139.0 * (c[2] + c[5])) + (29.0 * (c[1] + c[6]) - 3.0 * (c[0] + c[7]))) / 840.0;
+}
+
 /* ReconstructionboundaryScheme stencils: reconstructionBoundaryScheme.cpp:322-343 */
 static double third_order(double f0, double f1, double f2)
 {
@@ -219,6 +227,19 @@ static int recon_line_W(int scheme, int upwind, int n, int w, int W, const doubl
     double f[5];
     int m;
     if (n < 2 * w + 1 || w < 2 || w > 4 || W < w) return -1;
+    if (scheme == ORC_CENTRAL8) {
+        /* no counterpart in the reference: the symmetric eight-cell face value,
+         * the same for both split parts; interior and halo faces alike,
+         * physical faces keep the reference's closure (width 4) */
+        if (w != 4) return -1;
+        for (m = 0; m <= n; m++) {
+            if ((m < w && typeL == -1) || (m > n - w && typeR == -1)) continue;
+            out[m] = central8_face(ext + W + m - 4);
+        }
+        if (typeL == -1) recon_exterior(1, upwind, n, w, W, ext, out);
+        if (typeR == -1) recon_exterior(0, upwind, n, w, W, ext, out);
+        return 0;
+    }
     if (scheme == ORC_CRWENO5) {
         crweno5_inner(upwind, n, w, W, ext, out, work, work + n, work + 2 * n, work + 3 * n);
     } else {

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-enum { ORC_WENO5JS = 0, ORC_WENO5Z = 1, ORC_CRWENO5 = 2 };
+enum { ORC_WENO5JS = 0, ORC_WENO5Z = 1, ORC_CRWENO5 = 2, ORC_CENTRAL8 = 3 };
 enum { ORC_CD6 = 0, ORC_CD8 = 1 };
 enum { ORC_LF = 0, ORC_AUSM = 1, ORC_AUSMP = 2 };
 enum { ORC_EULER = 0, ORC_NS = 1 };

@@ -27,6 +27,7 @@ REF_BIN_GPUOPS = os.path.join(HERE, "_ref", "ref_harness_gpuops")
 REF_BIN_METRICS = os.path.join(HERE, "_ref", "ref_metrics")
 
 WENO5JS, WENO5Z, CRWENO5 = 0, 1, 2
+CENTRAL8 = 3   # not a reference scheme: the smooth-region face value of the learned hybrid (width 4)
 CD6, CD8 = 0, 1
 LF, AUSM, AUSMP = 0, 1, 2
 EULER, NS = 0, 1

@@ -199,3 +199,41 @@ def test_curvilinear_multiblock(scheme, exact):
         print("  block %d rel err %.3e" % (b, err), flush=True)
         assert err <= TOL
     g.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+@pytest.mark.parametrize("walls", [False, True])
+def test_central8_two_blocks(walls, exact):
+    """The eighth-order central scheme (the smooth-region scheme of the learned
+    hybrid, four ghost layers): interior and halo faces central, physical faces
+    the reference's closures; NS, 2 blocks, 2 steps."""
+    print("case: central8 NS 2 blocks walls=%s exact=%d" % (walls, exact), flush=True)
+    spec = cases.taylor_green(dims=(24, 14, 12), noise=1e-3, seed=34)[0]
+    if walls:
+        spec.faces = [(-1, 2, None), (-1, 2, None), (-1, 3, None), (-1, 3, None), (0, 5, None), (0, 4, None)]
+    blocks = cases.split_block_x(spec, 2)
+    sp, op = both_params(scheme=(solver.CENTRAL8,) * 3, width=(4, 4, 4), t_wall=1.2, exact=exact)
+    g = solver.Case(sp, blocks)
+    o = O.OracleCase(op, blocks)
+    g.steps(2, 0.5)
+    o.steps(2, 0.5)
+    for b in range(2):
+        err = relerr(g.get_q(b), o.get_q(b))
+        print("  block %d rel err %.3e" % (b, err), flush=True)
+        assert err <= TOL
+    g.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+def test_central8_x_weno5js_yz(exact):
+    """Central in x only (four ghost layers everywhere, WENO5-JS closures of
+    width 3 in y and z): one RHS and 2 steps against the oracle."""
+    print("case: central8 x, weno5js y z, Euler exact=%d" % exact, flush=True)
+    blocks = cases.taylor_green(dims=(20, 16, 12), noise=1e-3, seed=35)
+    sp, op = both_params(model=solver.EULER3D, scheme=(solver.CENTRAL8, solver.WENO5JS, solver.WENO5JS),
+                         width=(4, 3, 3), exact=exact)
+    g, o = run_rhs(sp, op, blocks)
+    err = relerr(g.get_rhs(0), o.get_rhs(0))
+    print("  rhs rel err %.3e" % err, flush=True)
+    assert err <= TOL
+    g.close()

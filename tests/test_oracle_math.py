@@ -57,6 +57,71 @@ def test_convergence_order_smooth(scheme, zquirk, order):
     assert rates[-1] >= order, (errs, rates)
 
 
+def _poly_averages(coef, n, w, h=1.0):
+    """exact cell averages of sum_k coef[k] x^k on cells [-w, n+w) of width h"""
+    xl = (np.arange(-w, n + w)) * h
+    xr = xl + h
+    avg = np.zeros(n + 2 * w)
+    for k, a in enumerate(coef):
+        avg += a * (xr ** (k + 1) - xl ** (k + 1)) / ((k + 1) * h)
+    return avg
+
+
+@pytest.mark.parametrize("upwind", [1, -1])
+def test_central8_polynomial_exactness(upwind):
+    # the eight-cell face value is the point value of the degree-7 polynomial
+    # through eight cell averages: exact up to degree 7, not at degree 8 (a
+    # wrong weight or a shifted stencil breaks the first, a lower-order
+    # formula the second)
+    n, w = 12, 4
+    rng = np.random.default_rng(3)
+    for deg in range(8):
+        coef = rng.standard_normal(deg + 1)
+        avg = _poly_averages(coef, n, w, 0.1)   # cell i spans [0.1 i, 0.1 (i+1)]: face m at 0.1 m
+        got = O.recon_line(O.CENTRAL8, upwind, avg, w)
+        want = np.polyval(coef[::-1], np.arange(n + 1) * 0.1)
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * max(1.0, np.abs(want).max()))
+    coef = np.zeros(9)
+    coef[8] = 1.0
+    avg = _poly_averages(coef, n, w, 0.1)
+    got = O.recon_line(O.CENTRAL8, upwind, avg, w)
+    want = (np.arange(n + 1) * 0.1) ** 8
+    # degree 8 leaves the truncation error 64 h^8 = 6.4e-7 at every face
+    assert np.abs(got - want).max() > 1e-10
+
+
+def test_central8_closed_form_and_eighth_order():
+    n, w = 14, 4
+    f = np.random.default_rng(4).standard_normal(n + 2 * w)
+    out = O.recon_line(O.CENTRAL8, 1, f, w)
+    m = 6
+    s = f[w + m - 4: w + m + 4]
+    want = (-3 * s[0] + 29 * s[1] - 139 * s[2] + 533 * s[3] + 533 * s[4] - 139 * s[5] + 29 * s[6] - 3 * s[7]) / 840
+    np.testing.assert_allclose(out[m], want, rtol=1e-14)
+    # both split parts use the same symmetric stencil
+    np.testing.assert_array_equal(out, O.recon_line(O.CENTRAL8, -1, f, w))
+    errs = []
+    for n in (16, 32, 64):
+        h = 2 * np.pi / n
+        xc = (np.arange(-w, n + w) + 0.5) * h
+        avg = (np.cos(xc - h / 2) - np.cos(xc + h / 2)) / h
+        errs.append(np.abs(O.recon_line(O.CENTRAL8, 1, avg, w) - np.sin(np.arange(n + 1) * h)).max())
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert rates[-1] >= 7.7, (errs, rates)
+
+
+@pytest.mark.parametrize("upwind", [1, -1])
+def test_central8_keeps_physical_face_closures(upwind):
+    n, w = 14, 4
+    f = np.random.default_rng(5).standard_normal(n + 2 * w) + 2.0
+    c8 = O.recon_line(O.CENTRAL8, upwind, f, w, -1, -1)
+    js = O.recon_line(O.WENO5JS, upwind, f, w, -1, -1)
+    np.testing.assert_array_equal(c8[:w], js[:w])
+    np.testing.assert_array_equal(c8[n - w + 1:], js[n - w + 1:])
+    inner = O.recon_line(O.CENTRAL8, upwind, f, w)
+    np.testing.assert_array_equal(c8[w:n - w + 1], inner[w:n - w + 1])
+
+
 def test_weno5z_quirk_degrades_order():
     # with the +inner beta2 of weno5z.cpp:80, beta2 no longer measures the
     # curvature of its stencil and WENO5-Z drops to about third order
@@ -161,10 +226,11 @@ def test_rk3_amplification_factor():
 
 
 @pytest.mark.parametrize("model", [O.EULER, O.NS])
-@pytest.mark.parametrize("scheme", [O.WENO5JS, O.WENO5Z, O.CRWENO5])
+@pytest.mark.parametrize("scheme", [O.WENO5JS, O.WENO5Z, O.CRWENO5, O.CENTRAL8])
 def test_periodic_rhs_telescopes(model, scheme):
     blocks = cases.taylor_green(dims=(16, 12, 14), noise=1e-2, seed=8)
-    oc = O.OracleCase(O.Params(model=model, scheme=(scheme,) * 3), blocks)
+    w = 4 if scheme == O.CENTRAL8 else 3
+    oc = O.OracleCase(O.Params(model=model, scheme=(scheme,) * 3, width=(w,) * 3), blocks)
     oc.rhs()
     r = oc.get_rhs(0)
     for c in range(5):
