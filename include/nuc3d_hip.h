@@ -48,6 +48,12 @@ enum { NUC3D_WENO5JS = 0, NUC3D_WENO5Z = 1, NUC3D_CRWENO5 = 2 };
  * width 4 (four ghost layers, four closure faces at physical faces, which keep
  * the reference's closures). */
 enum { NUC3D_CENTRAL8 = 3 };
+/* Hybrid reconstruction, no counterpart in the reference: per interface the
+ * NUC3D_CENTRAL8 face value in smooth flow and WENO5-JS elsewhere, as decided
+ * by the learned sensor set with nuc3d_set_sensor (below); one decision per
+ * face for all five components and both split parts, so each face flux is
+ * computed once and differenced.  Needs width 4, like NUC3D_CENTRAL8. */
+enum { NUC3D_HYBRID = 4 };
 enum { NUC3D_CD6 = 0, NUC3D_CD8 = 1 };
 enum { NUC3D_LF = 0, NUC3D_AUSM = 1, NUC3D_AUSMP = 2 };
 enum { NUC3D_EULER3D = 0, NUC3D_NAVIERSTOKES3D = 1 };
@@ -201,6 +207,67 @@ int nuc3d_get_fixups(nuc3d_case *c, long long *count);
 #define NUC3D_NTIMING 6
 int nuc3d_timing(nuc3d_case *c, int enable);
 int nuc3d_timing_read(nuc3d_case *c, double *ms, int *launches);
+
+/* ---- Learned sensor of the hybrid reconstruction (no counterpart in the
+ * reference) ----
+ * A multilayer perceptron 30 -> 32 -> 32 -> 1 (ReLU) with BF16 weights and
+ * activations and FP32 accumulation on the matrix cores.  Its input at face m
+ * are rho, u, v, w, p on the six cells m-3..m+2, each centred on its stencil
+ * mean and divided by (stencil range + 0.01 S), S = mean rho for rho, mean p
+ * for p and the sound speed of the means for the velocities.  A face takes
+ * WENO5-JS when the logit reaches the threshold, when the stencil's mean rho
+ * or p is not positive, or when the central face value of rho or p is not
+ * positive (the positivity fallback); otherwise the central value.
+ * Parameters are FP32, NUC3D_SENSOR_NPARAM of them, in the order
+ *   W1[32][32] (hidden x input; inputs 30 and 31 are unused and kept zero),
+ *   b1[32], W2[32][32], b2[32], w3[32], b3.
+ * The training label of a face (the teacher) is 1 when, for some primitive,
+ * the central face value and the WENO5-JS value of either upwind side differ
+ * by more than tau S.  Training runs on the GPU: minibatch Adam with the
+ * loss wt (softplus(z) - y z), wt = pos_weight for y = 1, averaged over the
+ * batch.  A sensor handle owns the parameters, the optimiser state and a
+ * device-resident dataset of (30 features, label) samples. */
+#define NUC3D_SENSOR_NPARAM 2145
+typedef struct nuc3d_sensor nuc3d_sensor;
+int nuc3d_sensor_create(long long capacity, nuc3d_sensor **out);
+int nuc3d_sensor_destroy(nuc3d_sensor *s);
+/* parameters in and out (set_params also resets the optimiser state) */
+int nuc3d_sensor_set_params(nuc3d_sensor *s, const float *params);
+int nuc3d_sensor_get_params(nuc3d_sensor *s, float *params);
+/* Append samples of every block and direction of the case's current state
+ * (between full steps): the faces whose index L in the face array of a
+ * direction ((n_d+1) x n_{d+1} x n_{d+2}, face index fastest, blocks and
+ * directions in order) has L % stride == offset.  A face whose eight-cell
+ * stencil m-4..m+3 meets a physical face or leaves the ghost frame is stored
+ * with label -1 (not trained on).  *added (may be NULL) receives the count. */
+int nuc3d_sensor_sample(nuc3d_sensor *s, nuc3d_case *c, double tau, int stride, int offset, long long *added);
+int nuc3d_sensor_count(nuc3d_sensor *s, long long *count, long long *positives);
+int nuc3d_sensor_clear(nuc3d_sensor *s);
+/* dataset rows: 32 bf16 bit patterns (features 0..29, then two zeros) and a label */
+int nuc3d_sensor_upload(nuc3d_sensor *s, const unsigned short *x32, const float *y, long long n);
+int nuc3d_sensor_download(nuc3d_sensor *s, long long first, long long n, unsigned short *x32, float *y);
+/* Gradient of the mean loss over samples [first, first+n) (NUC3D_SENSOR_NPARAM
+ * values) and the loss, by the training kernels. */
+int nuc3d_sensor_grad(nuc3d_sensor *s, long long first, long long n, float pos_weight, float *grad, double *loss);
+/* logits of samples [first, first+n) by the network as the hybrid sweep runs it */
+int nuc3d_sensor_infer(nuc3d_sensor *s, long long first, long long n, float *logits);
+/* `steps` Adam steps (lr, beta 0.9 / 0.999) on minibatches of `batch` samples
+ * drawn from the dataset by a hash of (seed, step, position); losses (steps
+ * values, may be NULL) receives the loss of every step. */
+int nuc3d_sensor_train(nuc3d_sensor *s, int steps, int batch, float lr, float pos_weight, unsigned long long seed,
+                       float *losses);
+/* The network the case's hybrid directions use (NUC3D_SENSOR_NPARAM FP32
+ * parameters, rounded to BF16 as in training) and the logit threshold. */
+int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold);
+/* Record every hybrid decision of later RHS evaluations (enable = 1), or stop
+ * (0).  Per block and hybrid direction d the flags have the layout of the
+ * face array ((n_d+1) x n_{d+1} x n_{d+2}, face index fastest): 0 central,
+ * 1 WENO5-JS by the network, 2 WENO5-JS by the fallback, 3 physical-face
+ * closure; the last evaluation's decisions are kept. */
+int nuc3d_record_hybrid(nuc3d_case *c, int enable);
+int nuc3d_download_hybrid_flags(nuc3d_case *c, int b, int d, unsigned char *flags);
+/* Faces decided so far: central, WENO5-JS by the network, WENO5-JS by the fallback. */
+int nuc3d_get_hybrid_counts(nuc3d_case *c, long long *counts3);
 
 /* 1 when this build implements the CRWENO5 line solves (no counterpart in the
  * reference: a capability query for callers). */

@@ -43,6 +43,14 @@
                                                     unsigned long long *, double *, hipStream_t);
 NUC3D_LAUNCHERS(fast)
 NUC3D_LAUNCHERS(exact)
+extern "C" hipError_t nuc3d_launch_sensor_sample(const nuc3d_block_dev *, int, int, int, double, double, long long,
+                                                 long long, long long, unsigned *, float *, unsigned long long *,
+                                                 hipStream_t);
+extern "C" hipError_t nuc3d_launch_sensor_fwdbwd(const unsigned *, const float *, long long, long long, long long, int,
+                                                 unsigned long long, long long, const float *, float, float, int,
+                                                 float *, float *, float *, hipStream_t);
+extern "C" hipError_t nuc3d_launch_sensor_adam(float *, float *, float *, const float *, float, int, float *,
+                                               hipStream_t);
 extern "C" hipError_t nuc3d_launch_mesh_metrics(nuc3d_block_dev, const double *, const double *, const double *,
                                                 double *, int *, hipStream_t);
 
@@ -142,6 +150,9 @@ struct nuc3d_case {
     hipStream_t gstream = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     double gcfl[2] = {-1.0, -1.0};
+    // hybrid reconstruction: the network (inference copy) and the recorded decisions
+    nuc3d_sensor_dev *dsensor = nullptr;
+    std::vector<void *> dflags;
     // kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -250,6 +261,9 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
     const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
     nuc3d_kparams K = c->K;
     K.resid = (mode == 1 && stage == 2) ? resid : nullptr;
+    for (int d = 0; d < 3; d++)
+        if (c->p.scheme[d] == NUC3D_HYBRID && !K.sensor)
+            return fail(NUC3D_ERR_BAD_ARG, "the hybrid scheme needs a sensor (nuc3d_set_sensor)");
     if (ns) {
         {
             Timed t(c, s, &c->ev[3]);
@@ -293,14 +307,16 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     int G = 3;
     for (int d = 0; d < 3; d++) {
         if (p->scheme[d] != NUC3D_WENO5JS && p->scheme[d] != NUC3D_WENO5Z && p->scheme[d] != NUC3D_CRWENO5 &&
-            p->scheme[d] != NUC3D_CENTRAL8)
+            p->scheme[d] != NUC3D_CENTRAL8 && p->scheme[d] != NUC3D_HYBRID)
             return fail(NUC3D_ERR_UNSUPPORTED, "scheme not available in this build");
-        // the central scheme reaches four cells past the face: four ghost layers
-        // and four closure faces at physical faces (bufferWidth 4, as cd8)
-        const int wneed = p->scheme[d] == NUC3D_CENTRAL8 ? 4 : 3;
+        // the central scheme (alone or in the hybrid) reaches four cells past the
+        // face: four ghost layers and four closure faces at physical faces
+        // (bufferWidth 4, as cd8)
+        const bool wide = p->scheme[d] == NUC3D_CENTRAL8 || p->scheme[d] == NUC3D_HYBRID;
+        const int wneed = wide ? 4 : 3;
         if (p->width[d] != wneed)
             return fail(NUC3D_ERR_UNSUPPORTED, wneed == 3 ? "ghost width must be 3 for WENO5 schemes"
-                                                          : "ghost width must be 4 for the central scheme");
+                                                          : "ghost width must be 4 for the central and hybrid schemes");
         if (wneed > G) G = wneed;   // block.cpp:506-507: the buffer is the widest scheme's
         if (p->diff[d] != NUC3D_CD6)
             return fail(NUC3D_ERR_UNSUPPORTED, "cd8 needs ghost width 4 (centraldifference8th.cpp:34,49)");
@@ -488,6 +504,8 @@ int nuc3d_case_destroy(nuc3d_case *c)
     (void)hipFree(c->dlam);
     (void)hipFree(c->dcount);
     (void)hipFree(c->dresid);
+    (void)hipFree(c->dsensor);
+    for (void *f : c->dflags) (void)hipFree(f);
     delete c;
     return NUC3D_OK;
 }
@@ -943,6 +961,336 @@ int nuc3d_timing_read(nuc3d_case *c, double *ms, int *launches)
         if (ms) ms[l] = acc;
         if (launches) launches[l] = cnt;
     }
+    return NUC3D_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Learned sensor of the hybrid reconstruction (no counterpart in the reference)
+
+}  // extern "C"
+
+struct nuc3d_sensor {
+    long long cap = 0, count = 0;
+    unsigned *dX = nullptr;        // [cap][16] bf16 feature pairs
+    float *dY = nullptr;           // [cap] labels 1 / 0 / -1
+    float *dparams = nullptr, *dmom = nullptr, *dvel = nullptr;  // [NPARAM]
+    float *dgrad = nullptr;        // [NPARAM + 1], the loss last
+    float *dpartial = nullptr;     // per-workgroup partial gradients
+    long long npartial = 0;
+    unsigned long long *dpos = nullptr;  // positive labels in the dataset
+    int t = 0;                     // Adam steps taken
+};
+
+namespace {
+
+constexpr int NPAR = NUC3D_SENSOR_NPARAM;
+
+unsigned short host_bf16(float x)
+{
+    unsigned u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+float host_bf16_float(unsigned short b)
+{
+    const unsigned u = (unsigned)b << 16;
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+}
+
+int sensor_partials(nuc3d_sensor *s, long long nsamp)
+{
+    const long long nwg = (nsamp + 127) / 128;
+    if (nwg <= s->npartial) return NUC3D_OK;
+    (void)hipFree(s->dpartial);
+    s->dpartial = nullptr;
+    s->npartial = 0;
+    if (hipMalloc(&s->dpartial, sizeof(float) * (NPAR + 1) * nwg) != hipSuccess)
+        return fail(NUC3D_ERR_OOM, "device allocation of sensor partials failed");
+    s->npartial = nwg;
+    return NUC3D_OK;
+}
+
+int check_range(nuc3d_sensor *s, long long first, long long n)
+{
+    if (!s) return fail(NUC3D_ERR_BAD_ARG, "null sensor");
+    if (first < 0 || n < 1 || first + n > s->count) return fail(NUC3D_ERR_BAD_ARG, "sample range outside the dataset");
+    return NUC3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nuc3d_sensor_create(long long capacity, nuc3d_sensor **out)
+{
+    if (!out) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    *out = nullptr;
+    if (capacity < 1) return fail(NUC3D_ERR_BAD_ARG, "capacity must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(NUC3D_ERR_NO_DEVICE, "no HIP device");
+    nuc3d_sensor *s = new nuc3d_sensor();
+    s->cap = capacity;
+    if (hipMalloc(&s->dX, sizeof(unsigned) * 16 * capacity) != hipSuccess ||
+        hipMalloc(&s->dY, sizeof(float) * capacity) != hipSuccess ||
+        hipMalloc(&s->dparams, sizeof(float) * NPAR) != hipSuccess ||
+        hipMalloc(&s->dmom, sizeof(float) * NPAR) != hipSuccess ||
+        hipMalloc(&s->dvel, sizeof(float) * NPAR) != hipSuccess ||
+        hipMalloc(&s->dgrad, sizeof(float) * (NPAR + 1)) != hipSuccess ||
+        hipMalloc(&s->dpos, sizeof(unsigned long long)) != hipSuccess) {
+        nuc3d_sensor_destroy(s);
+        return fail(NUC3D_ERR_OOM, "device allocation of the sensor dataset failed");
+    }
+    if (hipMemset(s->dparams, 0, sizeof(float) * NPAR) != hipSuccess ||
+        hipMemset(s->dmom, 0, sizeof(float) * NPAR) != hipSuccess ||
+        hipMemset(s->dvel, 0, sizeof(float) * NPAR) != hipSuccess ||
+        hipMemset(s->dpos, 0, sizeof(unsigned long long)) != hipSuccess) {
+        nuc3d_sensor_destroy(s);
+        return fail(NUC3D_ERR_HIP, "initialising the sensor failed");
+    }
+    *out = s;
+    return NUC3D_OK;
+}
+
+int nuc3d_sensor_destroy(nuc3d_sensor *s)
+{
+    if (!s) return NUC3D_OK;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(s->dX);
+    (void)hipFree(s->dY);
+    (void)hipFree(s->dparams);
+    (void)hipFree(s->dmom);
+    (void)hipFree(s->dvel);
+    (void)hipFree(s->dgrad);
+    (void)hipFree(s->dpartial);
+    (void)hipFree(s->dpos);
+    delete s;
+    return NUC3D_OK;
+}
+
+int nuc3d_sensor_set_params(nuc3d_sensor *s, const float *params)
+{
+    if (!s || !params) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    std::vector<float> p(params, params + NPAR);
+    for (int o = 0; o < 32; o++)
+        for (int i = NUC3D_SENSOR_NFEAT; i < 32; i++) p[o * 32 + i] = 0.0f;  // padding inputs of W1
+    HIPCHK(hipMemcpy(s->dparams, p.data(), sizeof(float) * NPAR, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(s->dmom, 0, sizeof(float) * NPAR));
+    HIPCHK(hipMemset(s->dvel, 0, sizeof(float) * NPAR));
+    s->t = 0;
+    return NUC3D_OK;
+}
+
+int nuc3d_sensor_get_params(nuc3d_sensor *s, float *params)
+{
+    if (!s || !params) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    HIPCHK(hipMemcpy(params, s->dparams, sizeof(float) * NPAR, hipMemcpyDeviceToHost));
+    return NUC3D_OK;
+}
+
+int nuc3d_sensor_clear(nuc3d_sensor *s)
+{
+    if (!s) return fail(NUC3D_ERR_BAD_ARG, "null sensor");
+    s->count = 0;
+    HIPCHK(hipMemset(s->dpos, 0, sizeof(unsigned long long)));
+    return NUC3D_OK;
+}
+
+int nuc3d_sensor_sample(nuc3d_sensor *s, nuc3d_case *c, double tau, int stride, int offset, long long *added)
+{
+    if (!s || !c) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    if (stride < 1 || offset < 0 || offset >= stride) return fail(NUC3D_ERR_BAD_ARG, "need stride >= 1, 0 <= offset < stride");
+    if (!(tau > 0.0)) return fail(NUC3D_ERR_BAD_ARG, "tau must be positive");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "sample between full steps");
+    HIPCHK(hipDeviceSynchronize());
+    int rc = refresh(c, nullptr);
+    if (rc) return rc;
+    long long total = 0;
+    std::vector<long long> ns;
+    for (int b = 0; b < c->nb; b++)
+        for (int d = 0; d < 3; d++) {
+            const nuc3d_block_dev &B = c->hb[b];
+            const long long nf = (long long)(B.n[d] + 1) * B.n[(d + 1) % 3] * B.n[(d + 2) % 3];
+            const long long n = offset < nf ? (nf - offset + stride - 1) / stride : 0;
+            ns.push_back(n);
+            total += n;
+        }
+    if (s->count + total > s->cap) return fail(NUC3D_ERR_BAD_ARG, "sensor dataset capacity exceeded");
+    long long pos = s->count;
+    int e = 0;
+    for (int b = 0; b < c->nb; b++)
+        for (int d = 0; d < 3; d++, e++) {
+            HIPCHK(nuc3d_launch_sensor_sample(c->db, b, d, c->cur, c->p.gamma, tau, stride, offset, ns[e],
+                                              s->dX + 16 * pos, s->dY + pos, s->dpos, nullptr));
+            pos += ns[e];
+        }
+    HIPCHK(hipDeviceSynchronize());
+    s->count = pos;
+    if (added) *added = total;
+    return NUC3D_OK;
+}
+
+int nuc3d_sensor_count(nuc3d_sensor *s, long long *count, long long *positives)
+{
+    if (!s) return fail(NUC3D_ERR_BAD_ARG, "null sensor");
+    if (count) *count = s->count;
+    if (positives) {
+        unsigned long long v = 0;
+        HIPCHK(hipMemcpy(&v, s->dpos, sizeof v, hipMemcpyDeviceToHost));
+        *positives = (long long)v;
+    }
+    return NUC3D_OK;
+}
+
+int nuc3d_sensor_upload(nuc3d_sensor *s, const unsigned short *x32, const float *y, long long n)
+{
+    if (!s || !x32 || !y) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    if (n < 0 || s->count + n > s->cap) return fail(NUC3D_ERR_BAD_ARG, "sensor dataset capacity exceeded");
+    HIPCHK(hipMemcpy(s->dX + 16 * s->count, x32, sizeof(unsigned) * 16 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->dY + s->count, y, sizeof(float) * n, hipMemcpyHostToDevice));
+    unsigned long long pos = 0;
+    HIPCHK(hipMemcpy(&pos, s->dpos, sizeof pos, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < n; i++) pos += y[i] > 0.5f;
+    HIPCHK(hipMemcpy(s->dpos, &pos, sizeof pos, hipMemcpyHostToDevice));
+    s->count += n;
+    return NUC3D_OK;
+}
+
+int nuc3d_sensor_download(nuc3d_sensor *s, long long first, long long n, unsigned short *x32, float *y)
+{
+    int rc = check_range(s, first, n);
+    if (rc) return rc;
+    if (x32) HIPCHK(hipMemcpy(x32, s->dX + 16 * first, sizeof(unsigned) * 16 * n, hipMemcpyDeviceToHost));
+    if (y) HIPCHK(hipMemcpy(y, s->dY + first, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return NUC3D_OK;
+}
+
+int nuc3d_sensor_grad(nuc3d_sensor *s, long long first, long long n, float pos_weight, float *grad, double *loss)
+{
+    int rc = check_range(s, first, n);
+    if (rc) return rc;
+    if ((rc = sensor_partials(s, n))) return rc;
+    HIPCHK(nuc3d_launch_sensor_fwdbwd(s->dX, s->dY, s->count, first, n, 0, 0ull, 0, s->dparams, pos_weight,
+                                      1.0f / (float)n, 1, s->dpartial, s->dgrad, nullptr, nullptr));
+    std::vector<float> g(NPAR + 1);
+    HIPCHK(hipMemcpy(g.data(), s->dgrad, sizeof(float) * (NPAR + 1), hipMemcpyDeviceToHost));
+    if (grad) std::memcpy(grad, g.data(), sizeof(float) * NPAR);
+    if (loss) *loss = g[NPAR];
+    return NUC3D_OK;
+}
+
+int nuc3d_sensor_infer(nuc3d_sensor *s, long long first, long long n, float *logits)
+{
+    int rc = check_range(s, first, n);
+    if (rc) return rc;
+    if (!logits) return fail(NUC3D_ERR_BAD_ARG, "null logits");
+    OpBuffers ob;
+    float *dz = reinterpret_cast<float *>(ob.get((size_t)(n + 1) / 2));
+    if (!dz) return fail(NUC3D_ERR_OOM, "device allocation failed");
+    HIPCHK(nuc3d_launch_sensor_fwdbwd(s->dX, s->dY, s->count, first, n, 0, 0ull, 0, s->dparams, 1.0f, 1.0f, 0,
+                                      nullptr, nullptr, dz, nullptr));
+    HIPCHK(hipMemcpy(logits, dz, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return NUC3D_OK;
+}
+
+int nuc3d_sensor_train(nuc3d_sensor *s, int steps, int batch, float lr, float pos_weight, unsigned long long seed,
+                       float *losses)
+{
+    if (!s) return fail(NUC3D_ERR_BAD_ARG, "null sensor");
+    if (steps < 0 || batch < 1 || !(lr > 0.0f)) return fail(NUC3D_ERR_BAD_ARG, "need steps >= 0, batch >= 1, lr > 0");
+    if (s->count < 1) return fail(NUC3D_ERR_BAD_ARG, "empty dataset");
+    int rc = sensor_partials(s, batch);
+    if (rc) return rc;
+    OpBuffers ob;
+    float *dl = losses ? reinterpret_cast<float *>(ob.get((size_t)(steps + 2) / 2)) : nullptr;
+    if (losses && !dl) return fail(NUC3D_ERR_OOM, "device allocation failed");
+    for (int i = 0; i < steps; i++) {
+        HIPCHK(nuc3d_launch_sensor_fwdbwd(s->dX, s->dY, s->count, 0, batch, 1, seed, s->t, s->dparams, pos_weight,
+                                          1.0f / (float)batch, 1, s->dpartial, s->dgrad, nullptr, nullptr));
+        s->t++;
+        HIPCHK(nuc3d_launch_sensor_adam(s->dparams, s->dmom, s->dvel, s->dgrad, lr, s->t, dl ? dl + i : nullptr,
+                                        nullptr));
+    }
+    HIPCHK(hipDeviceSynchronize());
+    if (losses && steps) HIPCHK(hipMemcpy(losses, dl, sizeof(float) * steps, hipMemcpyDeviceToHost));
+    return NUC3D_OK;
+}
+
+int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold)
+{
+    if (!c || !params) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    nuc3d_sensor_dev h;
+    std::memset(&h, 0, sizeof h);
+    for (int o = 0; o < 32; o++)
+        for (int i = 0; i < 32; i++) {
+            h.w1[o * 32 + i] = i < NUC3D_SENSOR_NFEAT ? host_bf16(params[o * 32 + i]) : 0;
+            h.w2[o * 32 + i] = host_bf16(params[1056 + o * 32 + i]);
+        }
+    for (int u = 0; u < 32; u++) {
+        h.b1[u] = params[1024 + u];
+        h.b2[u] = params[2080 + u];
+        h.w3[u] = host_bf16_float(host_bf16(params[2112 + u]));
+    }
+    h.b3 = params[2144];
+    h.thr = threshold;
+    if (!c->dsensor) {
+        void *m = nullptr;
+        if (hipMalloc(&m, sizeof h) != hipSuccess) return fail(NUC3D_ERR_OOM, "device allocation of the sensor failed");
+        c->dsensor = static_cast<nuc3d_sensor_dev *>(m);
+    }
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(c->dsensor, &h, sizeof h, hipMemcpyHostToDevice));
+    c->K.sensor = c->dsensor;
+    return NUC3D_OK;
+}
+
+int nuc3d_record_hybrid(nuc3d_case *c, int enable)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    HIPCHK(hipDeviceSynchronize());
+    for (void *f : c->dflags) (void)hipFree(f);
+    c->dflags.clear();
+    for (int b = 0; b < c->nb; b++)
+        for (int d = 0; d < 3; d++) {
+            nuc3d_block_dev &B = c->hb[b];
+            B.hflag[d] = nullptr;
+            if (!enable || c->p.scheme[d] != NUC3D_HYBRID) continue;
+            const size_t nf = (size_t)(B.n[d] + 1) * B.n[(d + 1) % 3] * B.n[(d + 2) % 3];
+            void *m = nullptr;
+            if (hipMalloc(&m, nf) != hipSuccess) return fail(NUC3D_ERR_OOM, "device allocation of decisions failed");
+            c->dflags.push_back(m);
+            HIPCHK(hipMemset(m, 0xff, nf));
+            B.hflag[d] = static_cast<unsigned char *>(m);
+        }
+    HIPCHK(hipMemcpy(c->db, c->hb.data(), sizeof(nuc3d_block_dev) * c->nb, hipMemcpyHostToDevice));
+    // captured graphs hold the descriptors by address: they stay valid
+    return NUC3D_OK;
+}
+
+int nuc3d_download_hybrid_flags(nuc3d_case *c, int b, int d, unsigned char *flags)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (d < 0 || d > 2 || !flags) return fail(NUC3D_ERR_BAD_ARG, "bad direction or null flags");
+    const nuc3d_block_dev &B = c->hb[b];
+    if (!B.hflag[d]) return fail(NUC3D_ERR_BAD_ARG, "decisions are not recorded for this block and direction");
+    const size_t nf = (size_t)(B.n[d] + 1) * B.n[(d + 1) % 3] * B.n[(d + 2) % 3];
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(flags, B.hflag[d], nf, hipMemcpyDeviceToHost));
+    return NUC3D_OK;
+}
+
+int nuc3d_get_hybrid_counts(nuc3d_case *c, long long *counts3)
+{
+    if (!c || !counts3) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    HIPCHK(hipDeviceSynchronize());
+    unsigned long long v[4];
+    HIPCHK(hipMemcpy(v, c->dcount, sizeof v, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; i++) counts3[i] = (long long)v[1 + i];
     return NUC3D_OK;
 }
 

@@ -16,6 +16,7 @@
 // the maximum eigenvalues of the new state, from which the next step's dt is
 // taken (euler3d.cpp:429-435, PDEData3d.cpp:74-80).
 #include "nuc3d_device.h"
+#include "nuc3d_sensor.h"
 #include "nuc3d_types.h"
 
 #if NUC3D_EXACT
@@ -62,9 +63,10 @@ __device__ __forceinline__ long long dstride(const nuc3d_block_dev &B, int d)
     return d == 0 ? 1LL : (d == 1 ? B.sy : B.sz);
 }
 
-// split fluxes of cell x of buffer qb in direction d; returns the local eigenvalue
+// split fluxes of cell x of buffer qb in direction d; returns the local
+// eigenvalue.  prim (may be null) receives rho, u, v, w, p.
 __device__ __forceinline__ double cell_split(const nuc3d_block_dev &B, double *const *qb, long long x, int d,
-                                             const PhysParams &P, double *fp, double *fn)
+                                             const PhysParams &P, double *fp, double *fn, double *prim = nullptr)
 {
     double Q[5];
 #pragma unroll
@@ -72,6 +74,13 @@ __device__ __forceinline__ double cell_split(const nuc3d_block_dev &B, double *c
     double jac = B.jac[x];
     double rho, u, v, w, p, cs;
     con2prim(Q, jac, P, rho, u, v, w, p, cs);
+    if (prim) {
+        prim[0] = rho;
+        prim[1] = u;
+        prim[2] = v;
+        prim[3] = w;
+        prim[4] = p;
+    }
     return split_cell(Q, jac, B.met[3 * d][x], B.met[3 * d + 1][x], B.met[3 * d + 2][x], u, v, w, p, cs, P.riemann,
                       fp, fn);
 }
@@ -121,11 +130,16 @@ __device__ __forceinline__ void bc_ghost(const nuc3d_block_dev &B, double *const
 // Split fluxes of cell (i,j,k) in direction d, where the cell may lie in the
 // ghost frame.  Beyond a physical face only the slot next to the face is
 // defined (+ flux at -1, - flux at n); other slots are zero and never read.
+// prim (may be null) receives the cell's rho, u, v, w, p (zero beyond a
+// physical face).
 __device__ __forceinline__ void split_any(const nuc3d_block_dev &B, double *const *qb, int i, int j, int k, int d,
-                                          const PhysParams &P, double *fp, double *fn)
+                                          const PhysParams &P, double *fp, double *fn, double *prim = nullptr)
 {
     const int c3[3] = {i, j, k};
     const int t = c3[d], n = B.n[d];
+    if (prim && ((t < 0 && B.ftype[2 * d] == -1) || (t >= n && B.ftype[2 * d + 1] == -1)))
+#pragma unroll
+        for (int c = 0; c < 5; c++) prim[c] = 0.0;
     if (t < 0 && B.ftype[2 * d] == -1) {
 #pragma unroll
         for (int c = 0; c < 5; c++) fp[c] = fn[c] = 0.0;
@@ -148,7 +162,7 @@ __device__ __forceinline__ void split_any(const nuc3d_block_dev &B, double *cons
         }
         return;
     }
-    cell_split(B, qb, cidx(B, i, j, k), d, P, fp, fn);
+    cell_split(B, qb, cidx(B, i, j, k), d, P, fp, fn, prim);
 }
 
 // Face value of direction d at face m from the six-slot stencils sp (F+) and
@@ -484,6 +498,212 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Hybrid direction sweep (no counterpart in the reference): the tile, staging
+// and differencing of k_sweep with a 4-cell halo, and per interface a choice
+// between the eighth-order central face value and WENO5-JS made by the
+// learned sensor (nuc3d_sensor.h), one decision per face for all five
+// components and both split parts, so the flux of a face is still computed
+// once and differenced (conservation is kept).
+//   1. split fluxes and primitives of 23 cells x 16 lines into LDS;
+//   2. every thread: the 30 features of its face, the positivity check of
+//      the central face values of rho and p;
+//   3. every wavefront: the network on its 64 faces, four MFMA tiles of 16;
+//   4. central faces and physical-face closures by their own thread; the
+//      WENO5-JS faces are compacted into a list and their (face, component)
+//      items spread over the whole workgroup;
+//   5. flux differences, and the RK stage as in k_sweep.
+// Decisions are counted (counters[1] central, [2] WENO5-JS by the network,
+// [3] WENO5-JS by the fallback) and, where B.hflag[DIR] is set, recorded per
+// face: 0 central, 1 network, 2 fallback, 3 physical-face closure.
+template <int DIR, bool NS, int MODE>
+__global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_dev *__restrict__ blocks,
+                                                             const int4 *__restrict__ tiles, nuc3d_kparams K,
+                                                             int stage, int bin, int bun, int bout, double *dtbuf,
+                                                             const unsigned long long *lam_cur,
+                                                             unsigned long long *lam_next,
+                                                             unsigned long long *counters)
+{
+    using namespace nuc3d_sensor;
+    constexpr int H = 4;
+    constexpr int NL = 16, NF = 16, NC = NF - 1, NSL = NC + 2 * H, NST = 2 * H;
+    __shared__ double S[10][NSL * NL];
+    __shared__ double HF[5][NF * NL];
+    // primitives of the staged cells, then (once the features are taken) the
+    // features of the 256 faces, bf16 packed in pairs
+    __shared__ __align__(16) union {
+        double pr[5][NSL * NL];
+        unsigned xf[256][16];
+    } U;
+    __shared__ float ZL[256];
+    __shared__ unsigned short LST[256];
+    __shared__ int WCNT[4];
+
+    const int witem = xcd_tile(blockIdx.x, gridDim.x);
+    const int4 t = tiles[witem];
+    const nuc3d_block_dev &B = blocks[t.x];
+    const PhysParams P = phys_of(K);
+    const int a0 = t.y, l0 = t.z, cc = t.w;
+    const int nc = min(NC, B.n[DIR] - a0);
+    const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
+    double *const *qin = B.q[bin];
+
+    auto sidx = [](int line, int slot) { return DIR == 0 ? line * NSL + slot : slot * NL + line; };
+    auto hidx = [](int line, int f) { return DIR == 0 ? line * NF + f : f * NL + line; };
+    auto cell = [&](int line, int a, int &i, int &j, int &k) {
+        if (DIR == 0) { i = a; j = l0 + line; k = cc; }
+        else if (DIR == 1) { i = l0 + line; j = a; k = cc; }
+        else { i = l0 + line; j = cc; k = a; }
+    };
+
+    const int nslots = nc + 2 * H;
+    for (int e = threadIdx.x; e < NSL * NL; e += 256) {
+        const int line = DIR == 0 ? e / NSL : e % NL;
+        const int sl = DIR == 0 ? e % NSL : e / NL;
+        if (line < nlines && sl < nslots) {
+            int i, j, k;
+            cell(line, a0 - H + sl, i, j, k);
+            double fp[5], fn[5], pr[5];
+            split_any(B, qin, i, j, k, DIR, P, fp, fn, pr);
+            const int o = sidx(line, sl);
+#pragma unroll
+            for (int c = 0; c < 5; c++) {
+                S[c][o] = fp[c];
+                S[5 + c][o] = fn[c];
+                U.pr[c][o] = pr[c];
+            }
+        }
+    }
+    __syncthreads();
+
+    const int line = DIR == 0 ? threadIdx.x / NF : threadIdx.x % NL;
+    const int f = DIR == 0 ? threadIdx.x % NF : threadIdx.x / NL;
+    const int n = B.n[DIR], w = K.width[DIR];
+    const int m = a0 + f;
+    const bool active = line < nlines && f <= nc;
+    const bool closure = (B.ftype[2 * DIR] == -1 && m < w) || (B.ftype[2 * DIR + 1] == -1 && m > n - w);
+    const bool decided = active && !closure;
+    unsigned xw[16];
+    bool posok = false;
+    if (decided) {
+        // cells m-3..m+2 are slots f+1..f+6; the central values use m-4..m+3
+        auto get = [&](int v, int j) { return U.pr[v][sidx(line, f + 1 + j)]; };
+        const bool valid = features(get, K.gamma, xw);
+        auto c8 = [&](int v) {
+            return central8_ref(U.pr[v][sidx(line, f)], U.pr[v][sidx(line, f + 1)], U.pr[v][sidx(line, f + 2)],
+                                U.pr[v][sidx(line, f + 3)], U.pr[v][sidx(line, f + 4)], U.pr[v][sidx(line, f + 5)],
+                                U.pr[v][sidx(line, f + 6)], U.pr[v][sidx(line, f + 7)]);
+        };
+        posok = valid && c8(0) > 0.0 && c8(4) > 0.0;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) xw[i] = 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; i++) U.xf[threadIdx.x][i] = xw[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    {
+        Regs R;
+        load_regs(K.sensor, R);
+        const int g = lane >> 4;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int tf = 64 * wv + 16 * q + (lane & 15);
+            const uint4 v4 = *reinterpret_cast<const uint4 *>(&U.xf[tf][4 * g]);
+            const unsigned wds[4] = {v4.x, v4.y, v4.z, v4.w};
+            u16x8 xv;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                xv[2 * e] = (unsigned short)(wds[e] & 0xffffu);
+                xv[2 * e + 1] = (unsigned short)(wds[e] >> 16);
+            }
+            Acts A;
+            const float z = forward16(R, __builtin_bit_cast(bf16x8, xv), A);
+            if (g == 0) ZL[tf] = z;
+        }
+    }
+    __syncthreads();
+    const bool weno = decided && (!posok || ZL[threadIdx.x] >= K.sensor->thr);
+    const unsigned long long bal = __ballot(weno);
+    // a tile owns faces 1..nc (and face 0 of the first tile): the face shared
+    // with the previous tile is counted there
+    const bool owned = f >= 1 || a0 == 0;
+    const unsigned long long bcen = __ballot(owned && decided && !weno);
+    const unsigned long long bnet = __ballot(owned && weno && posok);
+    const unsigned long long bfb = __ballot(owned && decided && !posok);
+    if (lane == 0) {
+        WCNT[wv] = __popcll(bal);
+        const int ncen = __popcll(bcen), nfb = __popcll(bfb), nnet = __popcll(bnet);
+        if (ncen) atomicAdd(counters + 1, (unsigned long long)ncen);
+        if (nnet) atomicAdd(counters + 2, (unsigned long long)nnet);
+        if (nfb) atomicAdd(counters + 3, (unsigned long long)nfb);
+    }
+    if (active && B.hflag[DIR]) {
+        int i, j, k;
+        cell(line, m, i, j, k);
+        long long idx;
+        if (DIR == 0) idx = m + (long long)(B.n[0] + 1) * (j + (long long)B.n[1] * k);
+        else if (DIR == 1) idx = m + (long long)(B.n[1] + 1) * (k + (long long)B.n[2] * i);
+        else idx = m + (long long)(B.n[2] + 1) * (i + (long long)B.n[0] * j);
+        B.hflag[DIR][idx] = closure ? 3 : (weno ? (posok ? 1 : 2) : 0);
+    }
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        base += q < wv ? WCNT[q] : 0;
+        total += WCNT[q];
+    }
+    if (weno) LST[base + __popcll(bal & ((1ull << lane) - 1ull))] = (unsigned short)threadIdx.x;
+    if (active && !weno) {
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            double sp[NST], sn[NST];
+#pragma unroll
+            for (int q = 0; q < NST; q++) {
+                sp[q] = S[c][sidx(line, f + q)];
+                sn[q] = S[5 + c][sidx(line, f + q)];
+            }
+            // slots f..f+7 = cells m-4..m+3; the six-cell stencil starts at m-3
+            HF[c][hidx(line, f)] = closure ? face_value<SCH_WENO5JS>(B, K, DIR, m, sp + 1, sn + 1)
+                                           : central8(sp + 1) + central8(sn + 1);
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 5 * total; e += 256) {
+        const int tf = LST[e / 5], c = e % 5;
+        const int ln = DIR == 0 ? tf / NF : tf % NL, ff = DIR == 0 ? tf % NF : tf / NL;
+        double sp[6], sn[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            sp[q] = S[c][sidx(ln, ff + 1 + q)];
+            sn[q] = S[5 + c][sidx(ln, ff + 1 + q)];
+        }
+        HF[c][hidx(ln, ff)] = face_interior<SCH_WENO5JS>(sp, true, false) + face_interior<SCH_WENO5JS>(sn, false, false);
+    }
+    __syncthreads();
+
+    double lmax[3] = {0.0, 0.0, 0.0};
+    unsigned long long nfix = 0;
+    double r2 = 0.0;
+    const double dt = (DIR == 2 && MODE == 1) ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
+    if (line < nlines && f >= 1 && f <= nc) {
+        int i, j, k;
+        cell(line, a0 + f - 1, i, j, k);
+        double dF[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) dF[c] = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
+        finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
+    }
+    if (DIR == 2 && MODE == 1) {
+        if (nfix) atomicAdd(counters, nfix);
+        if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * t.x);
+        if (!NS && stage == 2 && K.resid) block_sum_store(r2, K.resid + witem);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // CRWENO5 along direction DIR (crweno5.cpp:44-401): one wavefront per line.
@@ -1119,8 +1339,12 @@ static void launch_sweep(const nuc3d_block_dev *blocks, const int4 *tiles, int n
                          int bin, int bun, int bout, double *dtbuf, const unsigned long long *lam_cur,
                          unsigned long long *lam_next, unsigned long long *counters, hipStream_t st)
 {
-    hipLaunchKernelGGL((KNAME(k_sweep)<DIR, SCHEME, NS, MODE>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K,
-                       stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
+    if constexpr (SCHEME == SCH_HYBRID)
+        hipLaunchKernelGGL((KNAME(k_sweep_hybrid)<DIR, NS, MODE>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K,
+                           stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
+    else
+        hipLaunchKernelGGL((KNAME(k_sweep)<DIR, SCHEME, NS, MODE>), dim3(ntiles), dim3(256), 0, st, blocks, tiles,
+                           K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
 }
 
 template <int SCHEME, bool NS>
@@ -1158,6 +1382,10 @@ extern "C" hipError_t KNAME(nuc3d_launch_sweep)(int dir, int scheme, int ns, int
     } else if (scheme == SCH_CENTRAL8) {
         if (ns) sweep_dispatch<SCH_CENTRAL8, true>(NUC3D_A);
         else sweep_dispatch<SCH_CENTRAL8, false>(NUC3D_A);
+    } else if (scheme == SCH_HYBRID) {
+        if (!K.sensor) return hipErrorInvalidValue;
+        if (ns) sweep_dispatch<SCH_HYBRID, true>(NUC3D_A);
+        else sweep_dispatch<SCH_HYBRID, false>(NUC3D_A);
     } else {
         return hipErrorInvalidValue;
     }

@@ -27,6 +27,27 @@ struct nuc3d_block_dev {
     int ftype[6];        // -1: physical face, else neighbour block index
     int fid[6];          // BC id (0 inlet, 1 outlet, 2 wall, 3 symmetric) or neighbour face
     double fval[6][5];   // inlet primitive state per face
+    unsigned char *hflag[3];  // hybrid decisions per face of each direction, or null (not recorded)
+};
+
+// Learned sensor of the hybrid reconstruction: 30 features -> 32 -> 32 -> 1.
+#define NUC3D_SENSOR_NFEAT 30
+#define NUC3D_SENSOR_NH 32
+// master parameters (FP32): W1[32][32] (inputs 30, 31 unused), b1[32],
+// W2[32][32], b2[32], w3[32], b3
+#ifndef NUC3D_SENSOR_NPARAM
+#define NUC3D_SENSOR_NPARAM 2145
+#endif
+static_assert(NUC3D_SENSOR_NPARAM == 32 * 32 + 32 + 32 * 32 + 32 + 32 + 1, "sensor parameter count");
+
+// Inference copy of the network: BF16 weight bits (round to nearest even),
+// FP32 biases, w3 as the FP32 values of its BF16 rounding, and the logit
+// threshold at and above which a face takes WENO5-JS.
+struct nuc3d_sensor_dev {
+    unsigned short w1[32 * 32];  // [hidden][input]
+    unsigned short w2[32 * 32];  // [hidden][hidden]
+    float b1[32], b2[32], w3[32];
+    float b3, thr;
 };
 
 struct nuc3d_tile {
@@ -41,4 +62,5 @@ struct nuc3d_kparams {
     int zquirk;
     int nblocks;
     double *resid;  // stage 2: per-work-item partial sums of (Q - Q_n)^2, or null
+    const nuc3d_sensor_dev *sensor;  // hybrid reconstruction: the network, or null
 };

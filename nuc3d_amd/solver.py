@@ -19,6 +19,8 @@ from . import LIB_PATH
 
 WENO5JS, WENO5Z, CRWENO5 = 0, 1, 2
 CENTRAL8 = 3   # eight-cell central face value, width 4; no counterpart in the reference (the hybrid's smooth scheme)
+HYBRID = 4     # per-face CENTRAL8 or WENO5-JS chosen by the learned sensor, width 4; no counterpart in the reference
+SENSOR_NPARAM = 2145
 CD6, CD8 = 0, 1
 RECON_BOUNDARY = 3   # physical-face reconstruction closure (operator level)
 DIFF_BOUNDARY = 2    # physical-face derivative closure (operator level)
@@ -87,6 +89,8 @@ def lib():
         raise Nuc3dError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
     L = ctypes.CDLL(LIB_PATH)
     vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    fp, llp = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_longlong)
+    u16p = ctypes.POINTER(ctypes.c_uint16)
     sig = {
         "nuc3d_case_create": [ctypes.POINTER(_CParams), ctypes.c_int, ip, ctypes.POINTER(vp)],
         "nuc3d_case_destroy": [vp],
@@ -112,6 +116,23 @@ def lib():
         "nuc3d_get_fixups": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "nuc3d_timing": [vp, ctypes.c_int],
         "nuc3d_timing_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
+        "nuc3d_sensor_create": [ctypes.c_longlong, ctypes.POINTER(vp)],
+        "nuc3d_sensor_destroy": [vp],
+        "nuc3d_sensor_set_params": [vp, fp],
+        "nuc3d_sensor_get_params": [vp, fp],
+        "nuc3d_sensor_sample": [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, llp],
+        "nuc3d_sensor_count": [vp, llp, llp],
+        "nuc3d_sensor_clear": [vp],
+        "nuc3d_sensor_upload": [vp, u16p, fp, ctypes.c_longlong],
+        "nuc3d_sensor_download": [vp, ctypes.c_longlong, ctypes.c_longlong, u16p, fp],
+        "nuc3d_sensor_grad": [vp, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_float, fp, dp],
+        "nuc3d_sensor_infer": [vp, ctypes.c_longlong, ctypes.c_longlong, fp],
+        "nuc3d_sensor_train": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_ulonglong,
+                               fp],
+        "nuc3d_set_sensor": [vp, fp, ctypes.c_float],
+        "nuc3d_record_hybrid": [vp, ctypes.c_int],
+        "nuc3d_download_hybrid_flags": [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte)],
+        "nuc3d_get_hybrid_counts": [vp, llp],
         "nuc3d_op_interpolate": [ctypes.c_int] * 8 + [dp, dp, dp, vp],
         "nuc3d_op_differentiate": [ctypes.c_int] * 6 + [dp, dp, dp, vp],
         "nuc3d_op_rk3": [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_longlong,
@@ -300,6 +321,35 @@ class Case:
 
     TIMING_CLASSES = ("x_sweep", "y_sweep", "z_sweep", "primitives", "viscous_flux", "viscous_div_update")
 
+    def set_sensor(self, params, threshold=0.0):
+        """The learned sensor of the hybrid directions: SENSOR_NPARAM FP32
+        parameters and the logit threshold at and above which a face takes
+        WENO5-JS."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == SENSOR_NPARAM
+        _check(lib().nuc3d_set_sensor(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), float(threshold)),
+               "set_sensor")
+
+    def record_hybrid(self, enable=True):
+        _check(lib().nuc3d_record_hybrid(self.h, 1 if enable else 0), "record_hybrid")
+
+    def hybrid_flags(self, b, d):
+        """Decisions of the last RHS evaluation for direction d of block b, in
+        the face-array layout (n_{d+2}, n_{d+1}, n_d + 1): 0 central, 1 WENO5-JS
+        by the network, 2 WENO5-JS by the fallback, 3 physical-face closure."""
+        n = self.blocks[b].dims
+        shape = (n[(d + 2) % 3], n[(d + 1) % 3], n[d] + 1)
+        out = np.empty(shape, dtype=np.uint8)
+        _check(lib().nuc3d_download_hybrid_flags(self.h, b, d, out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))),
+               "download_hybrid_flags")
+        return out
+
+    def hybrid_counts(self):
+        """Faces decided so far: (central, WENO5-JS by the network, WENO5-JS by the fallback)."""
+        v = (ctypes.c_longlong * 3)()
+        _check(lib().nuc3d_get_hybrid_counts(self.h, v), "get_hybrid_counts")
+        return tuple(v)
+
     def timing_read(self):
         """{class: (total_ms, launches)} since the last read."""
         ms = (ctypes.c_double * len(self.TIMING_CLASSES))()
@@ -352,3 +402,92 @@ def op_rk3(stage, dt, rhs, u_n, u_i):
     arr = lambda xs: (P * len(xs))(*[_dp(x) for x in xs])
     _check(lib().nuc3d_op_rk3(stage, dt, len(u_i), u_i[0].size, arr(rs), arr(ns), arr(u_i), None), "op_rk3")
     return u_i
+
+
+# ---------------------------------------------------------------------------
+# learned sensor of the hybrid reconstruction (no counterpart in the reference)
+
+class Sensor:
+    """The sensor network's parameters, Adam state and a device-resident
+    dataset of (30 bf16 features, label) samples; sampling from cases and
+    training run on the GPU (include/nuc3d_hip.h)."""
+
+    def __init__(self, capacity):
+        h = ctypes.c_void_p()
+        _check(lib().nuc3d_sensor_create(int(capacity), ctypes.byref(h)), "sensor_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nuc3d_sensor_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _fp(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == SENSOR_NPARAM
+        _check(lib().nuc3d_sensor_set_params(self.h, self._fp(p)), "sensor_set_params")
+
+    def params(self):
+        p = np.empty(SENSOR_NPARAM, dtype=np.float32)
+        _check(lib().nuc3d_sensor_get_params(self.h, self._fp(p)), "sensor_get_params")
+        return p
+
+    def sample(self, case, tau, stride=1, offset=0):
+        n = ctypes.c_longlong()
+        _check(lib().nuc3d_sensor_sample(self.h, case.h, float(tau), int(stride), int(offset), ctypes.byref(n)),
+               "sensor_sample")
+        return n.value
+
+    def count(self):
+        n, pos = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().nuc3d_sensor_count(self.h, ctypes.byref(n), ctypes.byref(pos)), "sensor_count")
+        return n.value, pos.value
+
+    def clear(self):
+        _check(lib().nuc3d_sensor_clear(self.h), "sensor_clear")
+
+    def upload(self, xbits, y):
+        x = np.ascontiguousarray(xbits, dtype=np.uint16)
+        yy = np.ascontiguousarray(y, dtype=np.float32)
+        assert x.shape == (yy.size, 32)
+        _check(lib().nuc3d_sensor_upload(self.h, x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), self._fp(yy),
+                                         yy.size), "sensor_upload")
+
+    def download(self, first=0, n=None):
+        if n is None:
+            n = self.count()[0] - first
+        x = np.empty((n, 32), dtype=np.uint16)
+        y = np.empty(n, dtype=np.float32)
+        _check(lib().nuc3d_sensor_download(self.h, first, n, x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                           self._fp(y)), "sensor_download")
+        return x, y
+
+    def grad(self, first, n, pos_weight=1.0):
+        g = np.empty(SENSOR_NPARAM, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().nuc3d_sensor_grad(self.h, first, n, float(pos_weight), self._fp(g), ctypes.byref(loss)),
+               "sensor_grad")
+        return loss.value, g
+
+    def infer(self, first=0, n=None):
+        if n is None:
+            n = self.count()[0] - first
+        z = np.empty(n, dtype=np.float32)
+        _check(lib().nuc3d_sensor_infer(self.h, first, n, self._fp(z)), "sensor_infer")
+        return z
+
+    def train(self, steps, batch=8192, lr=1e-3, pos_weight=1.0, seed=0):
+        losses = np.empty(max(steps, 1), dtype=np.float32)
+        _check(lib().nuc3d_sensor_train(self.h, int(steps), int(batch), float(lr), float(pos_weight), int(seed),
+                                        self._fp(losses)), "sensor_train")
+        return losses[:steps]

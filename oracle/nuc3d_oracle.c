@@ -222,11 +222,29 @@ static void crweno5_inner(int upwind, int n, int w, int W, const double *ext, do
  * (fieldOperator.cpp:371-413): inner faces w..n-w, then the left and right
  * face groups from the halo (scheme face variants) or the exterior closure. */
 static int recon_line_W(int scheme, int upwind, int n, int w, int W, const double *ext,
-                        int typeL, int typeR, int zq, double *out, double *work)
+                        int typeL, int typeR, int zq, double *out, double *work, const unsigned char *mask)
 {
     double f[5];
     int m;
     if (n < 2 * w + 1 || w < 2 || w > 4 || W < w) return -1;
+    if (scheme == ORC_HYBRID) {
+        /* no counterpart in the reference: per face WENO5-JS (mask 1, 2 or no
+         * mask) or the eighth-order central value (mask 0); physical faces keep
+         * the reference's closure (width 4) */
+        if (w != 4) return -1;
+        for (m = 0; m <= n; m++) {
+            if ((m < w && typeL == -1) || (m > n - w && typeR == -1)) continue;
+            if (!mask || mask[m] == 1 || mask[m] == 2) {
+                stencil5(ext, W, m, upwind, f);
+                out[m] = weno5js_face(f);
+            } else {
+                out[m] = central8_face(ext + W + m - 4);
+            }
+        }
+        if (typeL == -1) recon_exterior(1, upwind, n, w, W, ext, out);
+        if (typeR == -1) recon_exterior(0, upwind, n, w, W, ext, out);
+        return 0;
+    }
     if (scheme == ORC_CENTRAL8) {
         /* no counterpart in the reference: the symmetric eight-cell face value,
          * the same for both split parts; interior and halo faces alike,
@@ -309,7 +327,7 @@ int orc_recon_line(int scheme, int upwind, int n, int w, const double *ext, int 
                    double *out)
 {
     double *work = (double *)malloc(sizeof(double) * 4 * (size_t)(n + 1));
-    int rc = recon_line_W(scheme, upwind, n, w, w, ext, typeL, typeR, zquirk, out, work);
+    int rc = recon_line_W(scheme, upwind, n, w, w, ext, typeL, typeR, zquirk, out, work, NULL);
     free(work);
     return rc;
 }
@@ -468,6 +486,7 @@ typedef struct {
     double *fv[3][5], *dfv[3][5];
     /* halo slabs per component x face: (bfsize, n2, n3) of the face's direction */
     double *bsend[5][6], *brecv[5][6];
+    unsigned char *hmask[3];       /* hybrid decisions per face of direction d, or NULL */
 } oblock;
 
 struct orc_case {
@@ -586,11 +605,28 @@ void orc_case_destroy(orc_case *C)
             free(B->rot[d]);
         }
         free(B->miu); free(B->coef); free(B->rtmp);
+        for (d = 0; d < 3; d++) free(B->hmask[d]);
         for (c = 0; c < 5; c++)
             for (f = 0; f < 6; f++) { free(B->bsend[c][f]); free(B->brecv[c][f]); }
     }
     free(C->B); free(C->work); free(C->ext); free(C->lout);
     free(C);
+}
+
+int orc_set_hybrid_mask(orc_case *C, int b, int d, const unsigned char *mask)
+{
+    if (!C || b < 0 || b >= C->nb || d < 0 || d > 2) return -1;
+    oblock *B = &C->B[b];
+    int n1, n2, n3;
+    rdims(B, d, &n1, &n2, &n3);
+    free(B->hmask[d]);
+    B->hmask[d] = NULL;
+    if (!mask) return 0;
+    size_t nf = (size_t)(n1 + 1) * n2 * n3;
+    B->hmask[d] = (unsigned char *)malloc(nf);
+    if (!B->hmask[d]) return -2;
+    memcpy(B->hmask[d], mask, nf);
+    return 0;
 }
 
 int orc_set_faces(orc_case *C, int b, const int *type6, const int *id6, const double *val30)
@@ -851,8 +887,10 @@ static int solve_inv(orc_case *C)
                     for (a3 = 0; a3 < n3; a3++)
                         for (a2 = 0; a2 < n2; a2++) {
                             gather_line(F, bl, br, n1, n2, C->bf, a2, a3, C->ext);
+                            const unsigned char *mk =
+                                B->hmask[d] ? B->hmask[d] + (long)(n1 + 1) * (a2 + (long)n2 * a3) : NULL;
                             rc = recon_line_W(C->P.scheme[d], cur_sign, n1, w, C->bf, C->ext, tL, tR, C->P.zquirk,
-                                              C->lout, C->work);
+                                              C->lout, C->work, mk);
                             if (rc) return rc;
                             long ob = (long)(n1 + 1) * (a2 + (long)n2 * a3);
                             if (s == 0)

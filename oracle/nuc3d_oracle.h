@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-enum { ORC_WENO5JS = 0, ORC_WENO5Z = 1, ORC_CRWENO5 = 2, ORC_CENTRAL8 = 3 };
+enum { ORC_WENO5JS = 0, ORC_WENO5Z = 1, ORC_CRWENO5 = 2, ORC_CENTRAL8 = 3, ORC_HYBRID = 4 };
 enum { ORC_CD6 = 0, ORC_CD8 = 1 };
 enum { ORC_LF = 0, ORC_AUSM = 1, ORC_AUSMP = 2 };
 enum { ORC_EULER = 0, ORC_NS = 1 };
@@ -60,6 +60,14 @@ double orc_get_dt_local(const orc_case *c, int b);
 double orc_get_dt_global(const orc_case *c);
 int orc_get_max_eigen(const orc_case *c, int b, double *eig3);
 long orc_get_fixups(const orc_case *c);                    /* con2prim e<0 fix-ups */
+
+/* Decisions of the hybrid reconstruction (ORC_HYBRID, no counterpart in the
+ * reference) for direction d of block b, in the layout of the reconstructed
+ * flux of that direction: (n_d+1) x n_{d+1} x n_{d+2}, face index fastest.
+ * Faces flagged 1 or 2 take WENO5-JS, 0 the eighth-order central value;
+ * physical-face closures ignore the flag.  NULL (the default) is all WENO5-JS.
+ * The array is copied. */
+int orc_set_hybrid_mask(orc_case *c, int b, int d, const unsigned char *mask);
 
 /* One RHS evaluation of every block: EulerData3D::solve / NaiverStokesData3d::solve. */
 int orc_rhs(orc_case *c);

@@ -28,6 +28,7 @@ REF_BIN_METRICS = os.path.join(HERE, "_ref", "ref_metrics")
 
 WENO5JS, WENO5Z, CRWENO5 = 0, 1, 2
 CENTRAL8 = 3   # not a reference scheme: the smooth-region face value of the learned hybrid (width 4)
+HYBRID = 4     # not a reference scheme: per-face WENO5-JS or CENTRAL8 from a decision mask (width 4)
 CD6, CD8 = 0, 1
 LF, AUSM, AUSMP = 0, 1, 2
 EULER, NS = 0, 1
@@ -127,6 +128,7 @@ def _lib():
     lib.orc_diagnostics.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_int, dp]
     lib.orc_get_residual.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_get_residual.restype = ctypes.c_double
+    lib.orc_set_hybrid_mask.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
     return lib
 
 
@@ -189,6 +191,19 @@ class OracleCase:
     def set_q(self, bi, q):
         q = np.ascontiguousarray(q, dtype=np.float64)
         self._check(lib().orc_set_q(self.h, bi, _dp(q)), "set_q")
+
+    def set_hybrid_mask(self, bi, d, mask):
+        """Decisions of the hybrid scheme for direction d of block bi: uint8 in the
+        layout of that direction's face array (n_{d+2}, n_{d+1}, n_d + 1); 1 or 2
+        take WENO5-JS, 0 the central value.  None: all WENO5-JS."""
+        if mask is None:
+            self._check(lib().orc_set_hybrid_mask(self.h, bi, d, None), "set_hybrid_mask")
+            return
+        mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        n = self.blocks[bi].dims
+        assert mask.size == (n[d] + 1) * n[(d + 1) % 3] * n[(d + 2) % 3]
+        self._check(lib().orc_set_hybrid_mask(self.h, bi, d, mask.ctypes.data_as(ctypes.c_void_p)),
+                    "set_hybrid_mask")
 
     def get_q(self, bi=0):
         out = np.empty(self._shape(bi, 5))

@@ -1,0 +1,224 @@
+"""The CPU restatement of the learned sensor (oracle/sensor.py) and of the
+hybrid reconstruction (oracle ORC_HYBRID), pinned by what does not depend on
+this restatement: torch's own bf16 rounding, Adam and autograd, finite
+differences, closed forms of the features and the teacher on constant,
+linear and step data, and the identities all-WENO5 mask = WENO5-JS and
+all-central mask = the central scheme."""
+import numpy as np
+import pytest
+import torch
+
+from nuc3d_amd import cases
+from oracle import oracle as O
+from oracle import sensor as S
+
+
+def test_bf16_rounding_matches_torch():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.normal(0, 1, 20000), rng.normal(0, 1e-3, 5000), rng.normal(0, 1e3, 5000)])
+    x = x.astype(np.float32)
+    # exact ties: values halfway between two bf16 numbers
+    base = S.bf16_round(rng.normal(0, 1, 1000).astype(np.float32))
+    ulp = (base.view(np.uint32) & 0xFFFF0000).view(np.float32)
+    x = np.concatenate([x, base + np.ldexp(np.ones_like(base), np.frexp(np.abs(base))[1] - 9)]).astype(np.float32)
+    want = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    np.testing.assert_array_equal(S.bf16_bits(x), want)
+    del ulp
+
+
+def _stencil(vals):
+    """P6 for one face from (5, 6) values."""
+    return [[np.array(vals[v][j]) for j in range(6)] for v in range(5)]
+
+
+def test_features_closed_forms():
+    g = 1.4
+    rho, p = 1.3, 2.0
+    # constant stencil: all features zero up to the rounding of the mean, valid
+    x, ok = S.features(_stencil([[rho] * 6, [0.2] * 6, [0.0] * 6, [-0.1] * 6, [p] * 6]), g)
+    assert ok and np.abs(S.bf16_value(x)).max() < 1e-13
+    # linear ramp in u with slope s: x_j = s (j - 2.5) / (5|s| + 0.01 c)
+    s = 0.05
+    u = [0.3 + s * j for j in range(6)]
+    x, ok = S.features(_stencil([[rho] * 6, u, [0.0] * 6, [0.0] * 6, [p] * 6]), g)
+    c = np.sqrt(g * p / rho)
+    want = [s * (j - 2.5) / (5 * s + 0.01 * c) for j in range(6)]
+    np.testing.assert_allclose(S.bf16_value(x[6:12]), want, rtol=2 ** -8)
+    assert not x[:6].any() and not x[12:].any()
+    # non-positive mean density: invalid, zero features
+    x, ok = S.features(_stencil([[-1.0] * 6, u, [0.0] * 6, [0.0] * 6, [p] * 6]), g)
+    assert not ok and not x.any()
+
+
+def test_teacher_closed_forms():
+    g, tau = 1.4, 1e-3
+    P8 = lambda vals: [[np.array(vals[v][j]) for j in range(8)] for v in range(5)]
+    const = P8([[1.0] * 8, [0.1] * 8, [0.0] * 8, [0.0] * 8, [1.0] * 8])
+    assert S.teacher(const, g, tau) == 0
+    # linear data: every WENO5 smoothness indicator is equal, so WENO5-JS is its
+    # linear fifth-order scheme, exact like the central scheme
+    lin = P8([[1.0 + 0.1 * j for j in range(8)], [0.1 * j for j in range(8)], [0.0] * 8, [0.0] * 8,
+              [1.0 + 0.2 * j for j in range(8)]])
+    assert S.teacher(lin, g, tau) == 0
+    # a density step: the central value overshoots, WENO5-JS does not
+    step = P8([[1.0] * 4 + [3.0] * 4, [0.0] * 8, [0.0] * 8, [0.0] * 8, [1.0] * 8])
+    assert S.teacher(step, g, tau) == 1
+    # a smooth wave of tiny amplitude stays below tau
+    k = 0.3
+    wave = P8([[1.0 + 1e-5 * np.sin(k * j) for j in range(8)], [0.0] * 8, [0.0] * 8, [0.0] * 8, [1.0] * 8])
+    assert S.teacher(wave, g, tau) == 0
+
+
+def test_weno_restatement_matches_oracle_line():
+    """sensor.weno5js equals the C oracle's WENO5-JS face values bit for bit."""
+    rng = np.random.default_rng(1)
+    f = rng.normal(0, 1, 40)
+    ext = np.concatenate([f[-3:], f, f[:3]])
+    got = O.recon_line(O.WENO5JS, 1, ext, 3)
+    m = np.arange(1, 40)
+    want = S.weno5js(ext[m], ext[m + 1], ext[m + 2], ext[m + 3], ext[m + 4])
+    np.testing.assert_array_equal(got[1:40], want)
+
+
+def _random_batch(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 0.5, (n, 32)).astype(np.float32)
+    x[:, 30:] = 0.0
+    y = (rng.uniform(size=n) < 0.3).astype(np.float32)
+    y[::17] = -1.0
+    return x, y
+
+
+def test_backward_matches_torch_autograd():
+    x, y = _random_batch(300, 2)
+    params = S.init_params(3)
+    params[S.OB1:S.OW2] = np.random.default_rng(4).normal(0, 0.1, 32)
+    params[S.OB2:S.OW3] = np.random.default_rng(5).normal(0, 0.1, 32)
+    pw = 3.0
+    loss, g = S.loss_grad(x, y, params, pos_weight=pw, quantize=False)
+    P = torch.tensor(params.astype(np.float64), requires_grad=True)
+    X = torch.tensor(x.astype(np.float64))
+    Y = torch.tensor(y.astype(np.float64))
+    W1 = P[S.OW1:S.OB1].reshape(32, 32)
+    W1 = torch.cat([W1[:, :30], torch.zeros(32, 2, dtype=torch.float64)], dim=1)
+    h1 = torch.relu(X @ W1.T + P[S.OB1:S.OW2])
+    h2 = torch.relu(h1 @ P[S.OW2:S.OB2].reshape(32, 32).T + P[S.OB2:S.OW3])
+    z = h2 @ P[S.OW3:S.OB3] + P[S.OB3]
+    wt = torch.tensor(np.where(y < 0, 0.0, np.where(y > 0.5, pw, 1.0)) / len(y), dtype=torch.float64)
+    L = torch.sum(wt * torch.nn.functional.binary_cross_entropy_with_logits(z, (Y > 0.5).double(), reduction="none"))
+    L.backward()
+    assert abs(loss - L.item()) <= 1e-12 * abs(L.item())
+    np.testing.assert_allclose(g, P.grad.numpy(), rtol=1e-9, atol=1e-12)
+
+
+def test_backward_matches_finite_differences():
+    x, y = _random_batch(64, 6)
+    params = S.init_params(7).astype(np.float64)
+    _, g = S.loss_grad(x, y, params, pos_weight=2.0, quantize=False)
+    rng = np.random.default_rng(8)
+    idx = list(rng.choice(np.arange(S.NP), 40, replace=False)) + [S.OB3, S.OW3 + 5, S.OB1 + 3]
+    idx = [i for i in idx if not (i < S.OB1 and i % 32 >= 30)]
+    h = 1e-6
+    for i in idx:
+        pp, pm = params.copy(), params.copy()
+        pp[i] += h
+        pm[i] -= h
+        # unpack() rounds to float32, so difference in float64 by hand
+        lp = _loss64(x, y, pp, 2.0)
+        lm = _loss64(x, y, pm, 2.0)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-8, (i, fd, g[i])
+
+
+def _loss64(x, y, p, pw):
+    W1 = p[S.OW1:S.OB1].reshape(32, 32).copy()
+    W1[:, 30:] = 0.0
+    h1 = np.maximum(x.astype(np.float64) @ W1.T + p[S.OB1:S.OW2], 0.0)
+    h2 = np.maximum(h1 @ p[S.OW2:S.OB2].reshape(32, 32).T + p[S.OB2:S.OW3], 0.0)
+    z = h2 @ p[S.OW3:S.OB3] + p[S.OB3]
+    wt = np.where(y < 0, 0.0, np.where(y > 0.5, pw, 1.0)) / len(y)
+    yy = (y > 0.5).astype(np.float64)
+    return float(np.sum(wt * (np.maximum(z, 0.0) + np.log1p(np.exp(-np.abs(z))) - yy * z)))
+
+
+def test_adam_matches_torch():
+    params = S.init_params(9)
+    rng = np.random.default_rng(10)
+    m = np.zeros(S.NP)
+    v = np.zeros(S.NP)
+    P = torch.tensor(params.astype(np.float64), requires_grad=True)
+    opt = torch.optim.Adam([P], lr=1e-3, betas=(0.9, 0.999), eps=1e-8)
+    p = params.astype(np.float64)
+    for t in range(1, 4):
+        g = rng.normal(0, 1, S.NP)
+        g[S.OW1:S.OB1][(np.arange(1024) % 32) >= 30] = 0.0
+        new, m, v = S.adam(p, g, m, v, t, 1e-3)
+        P.grad = torch.tensor(g)
+        opt.step()
+        np.testing.assert_allclose(new, P.detach().numpy().astype(np.float32), rtol=1e-6, atol=1e-7)
+        p = P.detach().numpy().copy()
+
+
+def test_sample_block_shapes_and_labels():
+    blocks = cases.taylor_green(12, noise=1e-2, seed=1)
+    b = blocks[0]
+    x, y = S.sample_block(b.q, b.jac, 1.4, 1e-3, 0, periodic=True, ghosts=3)
+    assert x.shape == (13 * 12 * 12, 32) and y.shape == (13 * 12 * 12,)
+    yy = y.reshape(12, 12, 13)
+    # faces 0 and n reach the fourth ghost layer, which a width-3 block lacks
+    assert (yy[:, :, 0] == -1).all() and (yy[:, :, 12] == -1).all()
+    assert set(np.unique(yy[:, :, 1:12])) <= {0.0, 1.0}
+    x4, y4 = S.sample_block(b.q, b.jac, 1.4, 1e-3, 0, periodic=True, ghosts=4)
+    assert (y4 >= 0).all()
+    xs, ys = S.sample_block(b.q, b.jac, 1.4, 1e-3, 0, periodic=True, stride=5, offset=2, ghosts=4)
+    np.testing.assert_array_equal(xs, x4[2::5])
+    np.testing.assert_array_equal(ys, y4[2::5])
+
+
+# ---- the oracle's hybrid reconstruction (decision mask) ----
+
+def _rhs(params, blocks, masks=None):
+    o = O.OracleCase(params, blocks)
+    if masks:
+        for d, m in masks.items():
+            o.set_hybrid_mask(0, d, m)
+    o.rhs()
+    return o.get_rhs(0)
+
+
+def _mask(dims, d, value):
+    n = dims
+    return np.full((n[(d + 2) % 3], n[(d + 1) % 3], n[d] + 1), value, dtype=np.uint8)
+
+
+@pytest.mark.parametrize("model", [O.EULER, O.NS])
+def test_hybrid_mask_extremes(model):
+    blocks = cases.taylor_green(dims=(14, 12, 10), noise=1e-3, seed=11)
+    dims = blocks[0].dims
+    hy = O.Params(model=model, scheme=(O.HYBRID,) * 3, width=(4, 4, 4))
+    weno = _rhs(O.Params(model=model), blocks)
+    allw = _rhs(hy, blocks, {d: _mask(dims, d, 1) for d in range(3)})
+    np.testing.assert_array_equal(allw, weno)
+    nomask = _rhs(hy, blocks)
+    np.testing.assert_array_equal(nomask, weno)
+    c8 = _rhs(O.Params(model=model, scheme=(O.CENTRAL8,) * 3, width=(4, 4, 4)), blocks)
+    allc = _rhs(hy, blocks, {d: _mask(dims, d, 0) for d in range(3)})
+    np.testing.assert_array_equal(allc, c8)
+
+
+def test_hybrid_mixed_mask_conserves():
+    """Random per-face decisions on a periodic Euler block: every face flux is
+    computed once and differenced, so each conserved variable's RHS sums to
+    zero over the block (round-off)."""
+    blocks = cases.taylor_green(dims=(14, 12, 10), noise=1e-2, seed=12)
+    dims = blocks[0].dims
+    rng = np.random.default_rng(13)
+    masks = {d: (rng.uniform(size=_mask(dims, d, 0).shape) < 0.5).astype(np.uint8) for d in range(3)}
+    for m in masks.values():
+        m[..., -1] = m[..., 0]   # faces 0 and n are one face of the periodic block
+    r = _rhs(O.Params(model=O.EULER, scheme=(O.HYBRID,) * 3, width=(4, 4, 4)), blocks, masks)
+    for c in range(5):
+        assert abs(r[c].sum()) <= 1e-12 * np.abs(r[c]).sum()
+    # and it lies between the two pure schemes face by face: differs from both
+    weno = _rhs(O.Params(model=O.EULER), blocks)
+    assert np.abs(r - weno).max() > 0
