@@ -43,7 +43,7 @@ enum {
  * (fieldOperator.cpp:127-369, physicsModel.cpp:13-112). */
 enum { NUC3D_WENO5JS = 0, NUC3D_WENO5Z = 1, NUC3D_CRWENO5 = 2 };
 /* Eighth-order central flux of each split part at face m from cells m-4..m+3,
- * (533(f3+f4) - 139(f2+f5) + 29(f1+f6) - 3(f0+f7))/840.  No counterpart in the
+ * (533(f3+f4) - 139(f2+f5) + 29(f1+f6) - 3(f0+f7)) * (1/840).  No counterpart in the
  * reference: the smooth-region branch of the hybrid reconstruction.  Needs
  * width 4 (four ghost layers, four closure faces at physical faces, which keep
  * the reference's closures). */
@@ -259,14 +259,14 @@ int nuc3d_sensor_train(nuc3d_sensor *s, int steps, int batch, float lr, float po
 /* The network the case's hybrid directions use (NUC3D_SENSOR_NPARAM FP32
  * parameters, rounded to BF16 as in training) and the logit threshold. */
 int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold);
-/* Record every hybrid decision of later RHS evaluations (enable = 1), or stop
- * (0).  Per block and hybrid direction d the flags have the layout of the
- * face array ((n_d+1) x n_{d+1} x n_{d+2}, face index fastest): 0 central,
- * 1 WENO5-JS by the network, 2 WENO5-JS by the fallback, 3 physical-face
- * closure; the last evaluation's decisions are kept. */
-int nuc3d_record_hybrid(nuc3d_case *c, int enable);
+/* The hybrid decisions are made by nuc3d_rhs and by stage 0 of every step
+ * (from the stage-0 state); stages 1 and 2 of the step reuse them.  The last
+ * decisions of block b and hybrid direction d, in the layout of the face array
+ * ((n_d+1) x n_{d+1} x n_{d+2}, face index fastest): 0 central, 1 WENO5-JS by
+ * the network, 2 WENO5-JS by the fallback, 3 physical-face closure. */
 int nuc3d_download_hybrid_flags(nuc3d_case *c, int b, int d, unsigned char *flags);
-/* Faces decided so far: central, WENO5-JS by the network, WENO5-JS by the fallback. */
+/* Faces decided so far (counted where decisions are made): central, WENO5-JS
+ * by the network, WENO5-JS by the fallback. */
 int nuc3d_get_hybrid_counts(nuc3d_case *c, long long *counts3);
 
 /* 1 when this build implements the CRWENO5 line solves (no counterpart in the

@@ -135,6 +135,7 @@ struct nuc3d_case {
     double *ddt = nullptr;             // dt of the current step
     unsigned long long *dlam = nullptr;   // [2][nb][3] eigenvalue maxima (bit patterns)
     unsigned long long *dcount = nullptr; // [4] fix-up counter
+    unsigned long long *dhcount = nullptr; // hybrid decision counts, NUC3D_HCOUNT_SLOTS x 16
     int cur = 0;                       // state buffer holding Q_work
     int next_stage = 0;
     int lampar = 0;
@@ -341,6 +342,8 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     for (int d = 0; d < 3; d++) K.width[d] = p->width[d];
     K.zquirk = p->zquirk;
     K.nblocks = nblocks;
+    K.sensor = nullptr;
+    K.hcount = nullptr;
     const bool ns = p->model == NUC3D_NAVIERSTOKES3D;
     // Q x 3, RHS, J, metrics, dxyz, u v w T (viscous terms and diagnostics),
     // and for Navier-Stokes the 12 viscous flux components
@@ -391,6 +394,19 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         for (int fc = 0; fc < 6; fc++) {
             B.ftype[fc] = -1;
             B.fid[fc] = NUC3D_BC_OUTLET;
+        }
+        // hybrid decisions per face: made at stage 0, reused by stages 1 and 2
+        for (int d = 0; d < 3; d++) {
+            if (p->scheme[d] != NUC3D_HYBRID) continue;
+            const size_t nf = (size_t)(B.n[d] + 1) * B.n[(d + 1) % 3] * B.n[(d + 2) % 3];
+            void *fm = nullptr;
+            if (hipMalloc(&fm, nf) != hipSuccess || hipMemset(fm, 0xff, nf) != hipSuccess) {
+                (void)hipFree(fm);
+                nuc3d_case_destroy(c);
+                return fail(NUC3D_ERR_OOM, "device allocation of hybrid decisions failed");
+            }
+            c->dflags.push_back(fm);
+            B.hflag[d] = static_cast<unsigned char *>(fm);
         }
         for (int k0 = 0; k0 < B.n[2]; k0 += KSEG)
             for (int j0 = 0; j0 < B.n[1]; j0 += NUC3D_TILE)
@@ -469,14 +485,17 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         hipMalloc(&c->dtiles, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess ||
         hipMalloc(&c->ddt, sizeof(double) * 4) != hipSuccess ||
         hipMalloc(&c->dlam, sizeof(unsigned long long) * 6 * nblocks) != hipSuccess ||
-        hipMalloc(&c->dcount, sizeof(unsigned long long) * 4) != hipSuccess) {
+        hipMalloc(&c->dcount, sizeof(unsigned long long) * 4) != hipSuccess ||
+        hipMalloc(&c->dhcount, sizeof(unsigned long long) * 16 * NUC3D_HCOUNT_SLOTS) != hipSuccess) {
         nuc3d_case_destroy(c);
         return fail(NUC3D_ERR_OOM, "device allocation of control arrays failed");
     }
+    c->K.hcount = c->dhcount;
     if (hipMemcpy(c->dtiles, tiles.data(), sizeof(nuc3d_tile) * tiles.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->ddt, 0, sizeof(double) * 4) != hipSuccess ||
         hipMemset(c->dlam, 0, sizeof(unsigned long long) * 6 * nblocks) != hipSuccess ||
-        hipMemset(c->dcount, 0, sizeof(unsigned long long) * 4) != hipSuccess) {
+        hipMemset(c->dcount, 0, sizeof(unsigned long long) * 4) != hipSuccess ||
+        hipMemset(c->dhcount, 0, sizeof(unsigned long long) * 16 * NUC3D_HCOUNT_SLOTS) != hipSuccess) {
         nuc3d_case_destroy(c);
         return fail(NUC3D_ERR_HIP, "initialising control arrays failed");
     }
@@ -503,6 +522,7 @@ int nuc3d_case_destroy(nuc3d_case *c)
     (void)hipFree(c->ddt);
     (void)hipFree(c->dlam);
     (void)hipFree(c->dcount);
+    (void)hipFree(c->dhcount);
     (void)hipFree(c->dresid);
     (void)hipFree(c->dsensor);
     for (void *f : c->dflags) (void)hipFree(f);
@@ -1237,6 +1257,25 @@ int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold)
     }
     h.b3 = params[2144];
     h.thr = threshold;
+    // lane-arranged copies (the layout of load_regs_lanes)
+    for (int l = 0; l < 64; l++) {
+        const int g = l / 16, i = l % 16;
+        unsigned short c[8];
+        for (int e = 0; e < 8; e++) c[e] = host_bf16(h.w3[e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4)]);
+        std::memcpy(&h.lw3[l], c, 16);
+        for (int mt = 0; mt < 2; mt++) {
+            unsigned short a[8], b[8];
+            for (int e = 0; e < 8; e++) {
+                a[e] = h.w1[(16 * mt + i) * 32 + 8 * g + e];
+                b[e] = h.w2[(16 * mt + i) * 32 + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4))];
+            }
+            std::memcpy(&h.lw1[mt][l], a, 16);
+            std::memcpy(&h.lw2[mt][l], b, 16);
+            const int u = 16 * mt + 4 * g;
+            h.lb1[mt][l] = make_float4(h.b1[u], h.b1[u + 1], h.b1[u + 2], h.b1[u + 3]);
+            h.lb2[mt][l] = make_float4(h.b2[u], h.b2[u + 1], h.b2[u + 2], h.b2[u + 3]);
+        }
+    }
     if (!c->dsensor) {
         void *m = nullptr;
         if (hipMalloc(&m, sizeof h) != hipSuccess) return fail(NUC3D_ERR_OOM, "device allocation of the sensor failed");
@@ -1248,36 +1287,13 @@ int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold)
     return NUC3D_OK;
 }
 
-int nuc3d_record_hybrid(nuc3d_case *c, int enable)
-{
-    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
-    HIPCHK(hipDeviceSynchronize());
-    for (void *f : c->dflags) (void)hipFree(f);
-    c->dflags.clear();
-    for (int b = 0; b < c->nb; b++)
-        for (int d = 0; d < 3; d++) {
-            nuc3d_block_dev &B = c->hb[b];
-            B.hflag[d] = nullptr;
-            if (!enable || c->p.scheme[d] != NUC3D_HYBRID) continue;
-            const size_t nf = (size_t)(B.n[d] + 1) * B.n[(d + 1) % 3] * B.n[(d + 2) % 3];
-            void *m = nullptr;
-            if (hipMalloc(&m, nf) != hipSuccess) return fail(NUC3D_ERR_OOM, "device allocation of decisions failed");
-            c->dflags.push_back(m);
-            HIPCHK(hipMemset(m, 0xff, nf));
-            B.hflag[d] = static_cast<unsigned char *>(m);
-        }
-    HIPCHK(hipMemcpy(c->db, c->hb.data(), sizeof(nuc3d_block_dev) * c->nb, hipMemcpyHostToDevice));
-    // captured graphs hold the descriptors by address: they stay valid
-    return NUC3D_OK;
-}
-
 int nuc3d_download_hybrid_flags(nuc3d_case *c, int b, int d, unsigned char *flags)
 {
     int rc = check_block(c, b);
     if (rc) return rc;
     if (d < 0 || d > 2 || !flags) return fail(NUC3D_ERR_BAD_ARG, "bad direction or null flags");
     const nuc3d_block_dev &B = c->hb[b];
-    if (!B.hflag[d]) return fail(NUC3D_ERR_BAD_ARG, "decisions are not recorded for this block and direction");
+    if (!B.hflag[d]) return fail(NUC3D_ERR_BAD_ARG, "direction d does not use the hybrid scheme");
     const size_t nf = (size_t)(B.n[d] + 1) * B.n[(d + 1) % 3] * B.n[(d + 2) % 3];
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(flags, B.hflag[d], nf, hipMemcpyDeviceToHost));
@@ -1288,9 +1304,13 @@ int nuc3d_get_hybrid_counts(nuc3d_case *c, long long *counts3)
 {
     if (!c || !counts3) return fail(NUC3D_ERR_BAD_ARG, "null argument");
     HIPCHK(hipDeviceSynchronize());
-    unsigned long long v[4];
-    HIPCHK(hipMemcpy(v, c->dcount, sizeof v, hipMemcpyDeviceToHost));
-    for (int i = 0; i < 3; i++) counts3[i] = (long long)v[1 + i];
+    std::vector<unsigned long long> v(16 * NUC3D_HCOUNT_SLOTS);
+    HIPCHK(hipMemcpy(v.data(), c->dhcount, sizeof(unsigned long long) * v.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; i++) {
+        unsigned long long t = 0;
+        for (int sl = 0; sl < NUC3D_HCOUNT_SLOTS; sl++) t += v[16 * sl + i];
+        counts3[i] = (long long)t;
+    }
     return NUC3D_OK;
 }
 

@@ -139,12 +139,14 @@ __device__ __forceinline__ double third_order(double f0, double f1, double f2)
 
 // Eighth-order central flux of one split part at face m from cells m-4..m+3
 // (s[-1..6]): the face point value of the degree-7 polynomial with the eight
-// cell averages, (533 (f3+f4) - 139 (f2+f5) + 29 (f1+f6) - 3 (f0+f7)) / 840.
+// cell averages, (533 (f3+f4) - 139 (f2+f5) + 29 (f1+f6) - 3 (f0+f7)) / 840,
+// the division taken as a product with the FP64 rounding of 1/840.
 // No counterpart in the reference: the smooth-region branch of the hybrid
 // reconstruction.
 __device__ __forceinline__ double central8(const double *s)
 {
-    return ((533.0 * (s[2] + s[3]) - 139.0 * (s[1] + s[4])) + (29.0 * (s[0] + s[5]) - 3.0 * (s[-1] + s[6]))) / 840.0;
+    return ((533.0 * (s[2] + s[3]) - 139.0 * (s[1] + s[4])) + (29.0 * (s[0] + s[5]) - 3.0 * (s[-1] + s[6]))) *
+           (1.0 / 840.0);
 }
 
 // Interior face value for the + (upwind>0) or - stencil.  s[0..5] holds the

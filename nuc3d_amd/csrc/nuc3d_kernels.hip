@@ -510,14 +510,18 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
 //   2. every thread: the 30 features of its face, the positivity check of
 //      the central face values of rho and p;
 //   3. every wavefront: the network on its 64 faces, four MFMA tiles of 16;
+//   (2 and 3 run where the decisions are made (DECIDE): in a single RHS
+//   evaluation and in stage 0 of a step; stages 1 and 2 read the step's
+//   decisions back from B.hflag[DIR]);
 //   4. central faces and physical-face closures by their own thread; the
 //      WENO5-JS faces are compacted into a list and their (face, component)
 //      items spread over the whole workgroup;
 //   5. flux differences, and the RK stage as in k_sweep.
-// Decisions are counted (counters[1] central, [2] WENO5-JS by the network,
-// [3] WENO5-JS by the fallback) and, where B.hflag[DIR] is set, recorded per
-// face: 0 central, 1 network, 2 fallback, 3 physical-face closure.
-template <int DIR, bool NS, int MODE>
+// Decisions are counted when made (K.hcount: central, WENO5-JS by the
+// network, WENO5-JS by the fallback; one atomic per workgroup and count,
+// spread over slots) and kept per face in B.hflag[DIR]: 0 central, 1 network,
+// 2 fallback, 3 physical-face closure.
+template <int DIR, bool NS, int MODE, bool DECIDE>
 __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_dev *__restrict__ blocks,
                                                              const int4 *__restrict__ tiles, nuc3d_kparams K,
                                                              int stage, int bin, int bun, int bout, double *dtbuf,
@@ -529,16 +533,19 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     constexpr int H = 4;
     constexpr int NL = 16, NF = 16, NC = NF - 1, NSL = NC + 2 * H, NST = 2 * H;
     __shared__ double S[10][NSL * NL];
-    __shared__ double HF[5][NF * NL];
-    // primitives of the staged cells, then (once the features are taken) the
-    // features of the 256 faces, bf16 packed in pairs
+    // one area, three lives: the FP32 roundings of rho, u, v, w, p of the
+    // staged cells; then the bf16 features of the 256 faces; then the face fluxes
     __shared__ __align__(16) union {
-        double pr[5][NSL * NL];
+        float pr[5][NSL * NL];
         unsigned xf[256][16];
+        double hf[5][NF * NL];
     } U;
+    auto &PR = U.pr;
+    auto &HF = U.hf;
     __shared__ float ZL[256];
     __shared__ unsigned short LST[256];
     __shared__ int WCNT[4];
+    __shared__ int WSTAT[4][3];
 
     const int witem = xcd_tile(blockIdx.x, gridDim.x);
     const int4 t = tiles[witem];
@@ -571,7 +578,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
             for (int c = 0; c < 5; c++) {
                 S[c][o] = fp[c];
                 S[5 + c][o] = fn[c];
-                U.pr[c][o] = pr[c];
+                if (DECIDE) PR[c][o] = (float)pr[c];
             }
         }
     }
@@ -584,49 +591,65 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     const bool active = line < nlines && f <= nc;
     const bool closure = (B.ftype[2 * DIR] == -1 && m < w) || (B.ftype[2 * DIR + 1] == -1 && m > n - w);
     const bool decided = active && !closure;
-    unsigned xw[16];
-    bool posok = false;
-    if (decided) {
-        // cells m-3..m+2 are slots f+1..f+6; the central values use m-4..m+3
-        auto get = [&](int v, int j) { return U.pr[v][sidx(line, f + 1 + j)]; };
-        const bool valid = features(get, K.gamma, xw);
-        auto c8 = [&](int v) {
-            return central8_ref(U.pr[v][sidx(line, f)], U.pr[v][sidx(line, f + 1)], U.pr[v][sidx(line, f + 2)],
-                                U.pr[v][sidx(line, f + 3)], U.pr[v][sidx(line, f + 4)], U.pr[v][sidx(line, f + 5)],
-                                U.pr[v][sidx(line, f + 6)], U.pr[v][sidx(line, f + 7)]);
-        };
-        posok = valid && c8(0) > 0.0 && c8(4) > 0.0;
-    } else {
-#pragma unroll
-        for (int i = 0; i < 16; i++) xw[i] = 0u;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 16; i++) U.xf[threadIdx.x][i] = xw[i];
-    __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the face's index in the decision array of direction DIR
+    long long fidx = 0;
     {
-        Regs R;
-        load_regs(K.sensor, R);
-        const int g = lane >> 4;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int tf = 64 * wv + 16 * q + (lane & 15);
-            const uint4 v4 = *reinterpret_cast<const uint4 *>(&U.xf[tf][4 * g]);
-            const unsigned wds[4] = {v4.x, v4.y, v4.z, v4.w};
-            u16x8 xv;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                xv[2 * e] = (unsigned short)(wds[e] & 0xffffu);
-                xv[2 * e + 1] = (unsigned short)(wds[e] >> 16);
-            }
-            Acts A;
-            const float z = forward16(R, __builtin_bit_cast(bf16x8, xv), A);
-            if (g == 0) ZL[tf] = z;
-        }
+        int i, j, k;
+        cell(line, m, i, j, k);
+        if (DIR == 0) fidx = m + (long long)(B.n[0] + 1) * (j + (long long)B.n[1] * k);
+        else if (DIR == 1) fidx = m + (long long)(B.n[1] + 1) * (k + (long long)B.n[2] * i);
+        else fidx = m + (long long)(B.n[2] + 1) * (i + (long long)B.n[0] * j);
     }
-    __syncthreads();
-    const bool weno = decided && (!posok || ZL[threadIdx.x] >= K.sensor->thr);
+    bool posok = false, weno = false;
+    if constexpr (DECIDE) {
+        // every face: its 30 features and the positivity fallback (the stencil's
+        // mean rho and p and the central face values of rho and p over cells
+        // m-4..m+3 = slots f..f+7 must be positive)
+        unsigned xw[16];
+        if (decided) {
+            auto getf = [&](int v, int j) { return PR[v][sidx(line, f + 1 + j)]; };
+            const bool valid = features(getf, (float)K.gamma, xw);
+            auto c8 = [&](int v) {
+                return central8f(PR[v][sidx(line, f)], PR[v][sidx(line, f + 1)], PR[v][sidx(line, f + 2)],
+                                 PR[v][sidx(line, f + 3)], PR[v][sidx(line, f + 4)], PR[v][sidx(line, f + 5)],
+                                 PR[v][sidx(line, f + 6)], PR[v][sidx(line, f + 7)]);
+            };
+            posok = valid && c8(0) > 0.0f && c8(4) > 0.0f;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) xw[i] = 0u;
+        }
+        __syncthreads();  // the primitives are dead: the area takes the features
+        {
+            uint4 *row = reinterpret_cast<uint4 *>(U.xf[threadIdx.x]);
+#pragma unroll
+            for (int i = 0; i < 4; i++) row[i] = make_uint4(xw[4 * i], xw[4 * i + 1], xw[4 * i + 2], xw[4 * i + 3]);
+        }
+        __syncthreads();
+        // the network: wavefront wv evaluates its 64 faces in four MFMA tiles of
+        // 16; lane group g takes features 8g..8g+7 (words 4g..4g+3 of a row)
+        {
+            Regs R;
+            load_regs_lanes(K.sensor, R);
+            const int g = lane >> 4;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int tf = 64 * wv + 16 * q + (lane & 15);
+                const uint4 v4 = *reinterpret_cast<const uint4 *>(&U.xf[tf][4 * g]);
+                Acts A;
+                const float z = forward16(R, __builtin_bit_cast(bf16x8, v4), A);
+                if (g == 0) ZL[tf] = z;
+            }
+        }
+        __syncthreads();  // the features are dead: the area takes the face fluxes
+        weno = decided && (!posok || ZL[threadIdx.x] >= K.sensor->thr);
+    } else {
+        // stages 1 and 2 of a step take the decisions of its stage 0
+        const unsigned char fl = decided ? B.hflag[DIR][fidx] : (unsigned char)0;
+        weno = decided && (fl == 1 || fl == 2);
+        posok = fl != 2;
+    }
     const unsigned long long bal = __ballot(weno);
     // a tile owns faces 1..nc (and face 0 of the first tile): the face shared
     // with the previous tile is counted there
@@ -636,20 +659,11 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     const unsigned long long bfb = __ballot(owned && decided && !posok);
     if (lane == 0) {
         WCNT[wv] = __popcll(bal);
-        const int ncen = __popcll(bcen), nfb = __popcll(bfb), nnet = __popcll(bnet);
-        if (ncen) atomicAdd(counters + 1, (unsigned long long)ncen);
-        if (nnet) atomicAdd(counters + 2, (unsigned long long)nnet);
-        if (nfb) atomicAdd(counters + 3, (unsigned long long)nfb);
+        WSTAT[wv][0] = __popcll(bcen);
+        WSTAT[wv][1] = __popcll(bnet);
+        WSTAT[wv][2] = __popcll(bfb);
     }
-    if (active && B.hflag[DIR]) {
-        int i, j, k;
-        cell(line, m, i, j, k);
-        long long idx;
-        if (DIR == 0) idx = m + (long long)(B.n[0] + 1) * (j + (long long)B.n[1] * k);
-        else if (DIR == 1) idx = m + (long long)(B.n[1] + 1) * (k + (long long)B.n[2] * i);
-        else idx = m + (long long)(B.n[2] + 1) * (i + (long long)B.n[0] * j);
-        B.hflag[DIR][idx] = closure ? 3 : (weno ? (posok ? 1 : 2) : 0);
-    }
+    if (DECIDE && active) B.hflag[DIR][fidx] = closure ? 3 : (weno ? (posok ? 1 : 2) : 0);
     __syncthreads();
     int base = 0, total = 0;
 #pragma unroll
@@ -657,8 +671,15 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
         base += q < wv ? WCNT[q] : 0;
         total += WCNT[q];
     }
+    if (DECIDE && threadIdx.x < 3 && K.hcount) {
+        const int v = WSTAT[0][threadIdx.x] + WSTAT[1][threadIdx.x] + WSTAT[2][threadIdx.x] + WSTAT[3][threadIdx.x];
+        if (v) atomicAdd(K.hcount + 16 * (witem % NUC3D_HCOUNT_SLOTS) + threadIdx.x, (unsigned long long)v);
+    }
     if (weno) LST[base + __popcll(bal & ((1ull << lane) - 1ull))] = (unsigned short)threadIdx.x;
-    if (active && !weno) {
+    // slots f..f+7 = cells m-4..m+3; the six-cell stencil starts at m-3.
+    // Closure faces (next to physical faces only) and central faces are
+    // separate branches, so waves without closure faces skip that code.
+    if (active && closure) {
 #pragma unroll
         for (int c = 0; c < 5; c++) {
             double sp[NST], sn[NST];
@@ -667,14 +688,25 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
                 sp[q] = S[c][sidx(line, f + q)];
                 sn[q] = S[5 + c][sidx(line, f + q)];
             }
-            // slots f..f+7 = cells m-4..m+3; the six-cell stencil starts at m-3
-            HF[c][hidx(line, f)] = closure ? face_value<SCH_WENO5JS>(B, K, DIR, m, sp + 1, sn + 1)
-                                           : central8(sp + 1) + central8(sn + 1);
+            HF[c][hidx(line, f)] = face_value<SCH_WENO5JS>(B, K, DIR, m, sp + 1, sn + 1);
+        }
+    } else if (decided && !weno) {
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            double sp[NST], sn[NST];
+#pragma unroll
+            for (int q = 0; q < NST; q++) {
+                sp[q] = S[c][sidx(line, f + q)];
+                sn[q] = S[5 + c][sidx(line, f + q)];
+            }
+            HF[c][hidx(line, f)] = central8(sp + 1) + central8(sn + 1);
         }
     }
     __syncthreads();
+    // items component-major, so that neighbouring lanes read one component's
+    // LDS array at neighbouring faces (no bank conflicts between components)
     for (int e = threadIdx.x; e < 5 * total; e += 256) {
-        const int tf = LST[e / 5], c = e % 5;
+        const int c = e / total, tf = LST[e - c * total];
         const int ln = DIR == 0 ? tf / NF : tf % NL, ff = DIR == 0 ? tf % NF : tf / NL;
         double sp[6], sn[6];
 #pragma unroll
@@ -1337,12 +1369,16 @@ __global__ __launch_bounds__(256) void KNAME(k_ghost_fill)(const nuc3d_block_dev
 template <int DIR, int SCHEME, bool NS, int MODE>
 static void launch_sweep(const nuc3d_block_dev *blocks, const int4 *tiles, int ntiles, nuc3d_kparams K, int stage,
                          int bin, int bun, int bout, double *dtbuf, const unsigned long long *lam_cur,
-                         unsigned long long *lam_next, unsigned long long *counters, hipStream_t st)
+                         unsigned long long *lam_next, unsigned long long *counters, hipStream_t st, bool decide)
 {
-    if constexpr (SCHEME == SCH_HYBRID)
-        hipLaunchKernelGGL((KNAME(k_sweep_hybrid)<DIR, NS, MODE>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K,
-                           stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
-    else
+    if constexpr (SCHEME == SCH_HYBRID) {
+        if (decide)
+            hipLaunchKernelGGL((KNAME(k_sweep_hybrid)<DIR, NS, MODE, true>), dim3(ntiles), dim3(256), 0, st, blocks,
+                               tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
+        else
+            hipLaunchKernelGGL((KNAME(k_sweep_hybrid)<DIR, NS, MODE, false>), dim3(ntiles), dim3(256), 0, st, blocks,
+                               tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
+    } else
         hipLaunchKernelGGL((KNAME(k_sweep)<DIR, SCHEME, NS, MODE>), dim3(ntiles), dim3(256), 0, st, blocks, tiles,
                            K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
 }
@@ -1353,7 +1389,10 @@ static void sweep_dispatch(int dir, int mode, const nuc3d_block_dev *blocks, con
                            const unsigned long long *lam_cur, unsigned long long *lam_next,
                            unsigned long long *counters, hipStream_t st)
 {
-#define NUC3D_A blocks, tiles, ntiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, st
+    // hybrid decisions: made by a single RHS evaluation and by stage 0 of a
+    // step, reused by stages 1 and 2
+    const bool decide = mode == 0 || stage == 0;
+#define NUC3D_A blocks, tiles, ntiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, st, decide
     if (dir == 0)
         launch_sweep<0, SCHEME, NS, 0>(NUC3D_A);
     else if (dir == 1)
@@ -1383,7 +1422,7 @@ extern "C" hipError_t KNAME(nuc3d_launch_sweep)(int dir, int scheme, int ns, int
         if (ns) sweep_dispatch<SCH_CENTRAL8, true>(NUC3D_A);
         else sweep_dispatch<SCH_CENTRAL8, false>(NUC3D_A);
     } else if (scheme == SCH_HYBRID) {
-        if (!K.sensor) return hipErrorInvalidValue;
+        if (!K.sensor || !blocks) return hipErrorInvalidValue;
         if (ns) sweep_dispatch<SCH_HYBRID, true>(NUC3D_A);
         else sweep_dispatch<SCH_HYBRID, false>(NUC3D_A);
     } else {

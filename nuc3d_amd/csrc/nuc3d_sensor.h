@@ -6,7 +6,8 @@
 //   input   30 features: rho, u, v, w, p on the six cells m-3..m+2 around
 //           face m, each variable centred on its stencil mean and divided by
 //           (stencil range + 0.01 S), with S = mean rho for rho, mean p for p
-//           and the sound speed of the means for the velocities;
+//           and the sound speed of the means for the velocities; computed in
+//           FP32 from the FP32 roundings of the primitives;
 //   layers  30 -> 32 -> 32 -> 1, ReLU, BF16 weights and activations, FP32
 //           accumulation on the matrix cores (v_mfma_f32_16x16x32_bf16);
 //   output  a logit; WENO5-JS when it reaches the threshold, when the stencil
@@ -37,7 +38,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int NFEAT = NUC3D_SENSOR_NFEAT;   // 30
 constexpr int KIN = 32;                     // features padded to one MFMA K step
 constexpr int NH = NUC3D_SENSOR_NH;         // 32
-constexpr double FEAT_EPS = 0.01;
+constexpr float FEAT_EPS = 0.01f;
 
 // float -> bf16 bits, round to nearest even (finite inputs)
 __device__ __forceinline__ unsigned short bf16_bits(float x)
@@ -47,6 +48,14 @@ __device__ __forceinline__ unsigned short bf16_bits(float x)
     return (unsigned short)(u >> 16);
 }
 __device__ __forceinline__ float bf16_float(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+// two floats -> two bf16 (round to nearest even, v_cvt_pk_bf16_f32), a in the low half
+__device__ __forceinline__ unsigned pack_bf16(float a, float b)
+{
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 v = {(__bf16)a, (__bf16)b};
+    return __builtin_bit_cast(unsigned, v);
+}
 
 // WENO5-JS face value in the reference's operation order (weno5js.cpp:75-108)
 // whatever the kernel variant: the teacher's labels must not depend on it.
@@ -81,25 +90,36 @@ __device__ __forceinline__ double weno5js_ref(double f0, double f1, double f2, d
 __device__ __forceinline__ double central8_ref(double c0, double c1, double c2, double c3, double c4, double c5,
                                                double c6, double c7)
 {
-    return ((533.0 * (c3 + c4) - 139.0 * (c2 + c5)) + (29.0 * (c1 + c6) - 3.0 * (c0 + c7))) / 840.0;
+    return ((533.0 * (c3 + c4) - 139.0 * (c2 + c5)) + (29.0 * (c1 + c6) - 3.0 * (c0 + c7))) * (1.0 / 840.0);
 }
 
-// Stencil statistics of face m from get(v, j) = primitive v (rho,u,v,w,p) at
-// cell m-3+j, j = 0..5: per-variable means and the scales S.  Returns false
-// when the mean density or pressure is not positive.
-template <class Get>
-__device__ __forceinline__ bool stencil_scales(const Get &get, double gamma, double *mean, double *S)
+// The same in FP32 (the positivity check of the central face values of rho and p)
+__device__ __forceinline__ float central8f(float c0, float c1, float c2, float c3, float c4, float c5, float c6,
+                                           float c7)
 {
+    return ((533.0f * (c3 + c4) - 139.0f * (c2 + c5)) + (29.0f * (c1 + c6) - 3.0f * (c0 + c7))) * (1.0f / 840.0f);
+}
+
+// ---- features, FP32 from the FP32 roundings of the FP64 primitives ----
+// getf(v, j) = (float) primitive v (rho,u,v,w,p) at cell m-3+j, j = 0..5.
+template <class Get>
+__device__ __forceinline__ float stencil_mean(const Get &getf, int v)
+{
+    float s = getf(v, 0);
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
-        double s = get(v, 0);
-#pragma unroll
-        for (int j = 1; j < 6; j++) s = s + get(v, j);
-        mean[v] = s / 6.0;
-    }
-    const double rb = mean[0], pb = mean[4];
-    if (!(rb > 0.0 && pb > 0.0)) return false;
-    const double cb = sqrt(gamma * pb / rb);
+    for (int j = 1; j < 6; j++) s = s + getf(v, j);
+    return s * (1.0f / 6.0f);
+}
+
+// Scales S of the five variables: mean rho, the sound speed of the means
+// (three velocities) and mean p.  False when mean rho or mean p is not
+// positive (the stencil is invalid; the face takes WENO5-JS).
+template <class Get>
+__device__ __forceinline__ bool feature_scales(const Get &getf, float gamma, float *S)
+{
+    const float rb = stencil_mean(getf, 0), pb = stencil_mean(getf, 4);
+    if (!(rb > 0.0f && pb > 0.0f)) return false;
+    const float cb = sqrtf(gamma * pb / rb);
     S[0] = rb;
     S[1] = cb;
     S[2] = cb;
@@ -108,43 +128,64 @@ __device__ __forceinline__ bool stencil_scales(const Get &get, double gamma, dou
     return true;
 }
 
-// The 30 features as bf16 bits, packed two per word (feature 2i in the low
-// half of word i; words 15 is the zero padding of features 30, 31).  Returns
-// false (and zero features) for an invalid stencil.
+// mean and 1 / (range + 0.01 S) of variable v
 template <class Get>
-__device__ __forceinline__ bool features(const Get &get, double gamma, unsigned *xw)
+__device__ __forceinline__ void feature_norm(const Get &getf, int v, float Sv, float &mean, float &inv)
 {
-    double mean[5], S[5];
-    const bool ok = stencil_scales(get, gamma, mean, S);
-    unsigned short x[KIN];
+    mean = stencil_mean(getf, v);
+    float lo = getf(v, 0), hi = lo;
+#pragma unroll
+    for (int j = 1; j < 6; j++) {
+        lo = fminf(lo, getf(v, j));
+        hi = fmaxf(hi, getf(v, j));
+    }
+    inv = 1.0f / ((hi - lo) + FEAT_EPS * Sv);
+}
+
+// All 30 features as bf16 bits, packed two per word (feature 2i in the low
+// half of word i; word 15 is the zero padding of features 30, 31).  Feature
+// 6v+j = bf16((P_v(j) - mean_v) / (range_v + 0.01 S_v)), the division taken
+// as a product with the reciprocal.  Returns false (and zero features) for an
+// invalid stencil.
+template <class Get>
+__device__ __forceinline__ bool features(const Get &getf, float gamma, unsigned *xw)
+{
+    float S[5];
+    const bool ok = feature_scales(getf, gamma, S);
+    float x[KIN];
 #pragma unroll
     for (int v = 0; v < 5; v++) {
-        double lo = get(v, 0), hi = lo;
+        float mean = 0.0f, inv = 0.0f;
+        if (ok) feature_norm(getf, v, S[v], mean, inv);
 #pragma unroll
-        for (int j = 1; j < 6; j++) {
-            lo = fmin(lo, get(v, j));
-            hi = fmax(hi, get(v, j));
-        }
-        const double den = (hi - lo) + FEAT_EPS * S[v];
-#pragma unroll
-        for (int j = 0; j < 6; j++) x[6 * v + j] = ok ? bf16_bits((float)((get(v, j) - mean[v]) / den)) : 0;
+        for (int j = 0; j < 6; j++) x[6 * v + j] = ok ? (getf(v, j) - mean) * inv : 0.0f;
     }
-    x[30] = x[31] = 0;
+    x[30] = x[31] = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 16; i++) xw[i] = (unsigned)x[2 * i] | ((unsigned)x[2 * i + 1] << 16);
+    for (int i = 0; i < 16; i++) xw[i] = pack_bf16(x[2 * i], x[2 * i + 1]);
     return ok;
 }
 
 // Teacher label of face m from get8(v, j) = primitive v at cell m-4+j,
-// j = 0..7: 1 when, for some primitive, the central face value differs from
-// the WENO5-JS value of either upwind side by more than tau S.  -1 for an
-// invalid stencil.
+// j = 0..7 (FP64): 1 when, for some primitive, the central face value differs
+// from the WENO5-JS value of either upwind side by more than tau S, with S =
+// mean rho (rho), mean p (p) and sqrt(gamma mean p / mean rho) (velocities)
+// over cells m-3..m+2.  -1 when mean rho or mean p is not positive.
 template <class Get8>
 __device__ __forceinline__ int teacher(const Get8 &get8, double gamma, double tau)
 {
-    double mean[5], S[5];
-    auto get6 = [&](int v, int j) { return get8(v, j + 1); };
-    if (!stencil_scales(get6, gamma, mean, S)) return -1;
+    double mean[5];
+#pragma unroll
+    for (int v = 0; v < 5; v += 4) {
+        double sm = get8(v, 1);
+#pragma unroll
+        for (int j = 2; j < 7; j++) sm = sm + get8(v, j);
+        mean[v] = sm / 6.0;
+    }
+    const double rb = mean[0], pb = mean[4];
+    if (!(rb > 0.0 && pb > 0.0)) return -1;
+    const double cb = sqrt(gamma * pb / rb);
+    const double S[5] = {rb, cb, cb, cb, pb};
     double dev = 0.0;
 #pragma unroll
     for (int v = 0; v < 5; v++) {
@@ -158,19 +199,28 @@ __device__ __forceinline__ int teacher(const Get8 &get8, double gamma, double ta
     return dev > tau ? 1 : 0;
 }
 
-// Per-lane weights of the network (the A operands and the per-row terms).
+// Per-lane weights of the network: the A operands of the three layers and
+// the accumulator initial values (the biases).
 struct Regs {
     bf16x8 w1[2];      // W1 rows 16mt + l%16, inputs 8g..8g+7
     bf16x8 w2[2];      // W2 rows 16mt + l%16, hidden inputs of the permuted K slots
-    float b1[2][4];    // b1 of hidden 16mt + 4g + r
-    float b2[2][4];
-    float w3[2][4];    // w3 (bf16 values) of hidden 16mt + 4g + r
+    bf16x8 w3;         // w3 at the permuted K slots, the same in all 16 rows
+    f32x4 b1[2];       // b1 of hidden 16mt + 4g + r, r = 0..3
+    f32x4 b2[2];
+    float w3f[2][4];   // w3 (bf16 values) of hidden 16mt + 4g + r
     float b3;
 };
+
+// hidden unit of permuted K slot e of lane group g
+__device__ __forceinline__ int kslot_unit(int g, int e) { return e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4); }
 
 __device__ __forceinline__ void load_regs(const nuc3d_sensor_dev *__restrict__ S, Regs &R)
 {
     const int l = threadIdx.x & 63, g = l >> 4, i = l & 15;
+    u16x8 c;
+#pragma unroll
+    for (int e = 0; e < 8; e++) c[e] = bf16_bits(S->w3[kslot_unit(g, e)]);
+    R.w3 = __builtin_bit_cast(bf16x8, c);
 #pragma unroll
     for (int mt = 0; mt < 2; mt++) {
         const unsigned short *r1 = S->w1 + (16 * mt + i) * KIN;
@@ -179,7 +229,7 @@ __device__ __forceinline__ void load_regs(const nuc3d_sensor_dev *__restrict__ S
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             a[e] = r1[8 * g + e];
-            b[e] = e < 4 ? r2[4 * g + e] : r2[16 + 4 * g + (e - 4)];
+            b[e] = r2[kslot_unit(g, e)];
         }
         R.w1[mt] = __builtin_bit_cast(bf16x8, a);
         R.w2[mt] = __builtin_bit_cast(bf16x8, b);
@@ -187,8 +237,24 @@ __device__ __forceinline__ void load_regs(const nuc3d_sensor_dev *__restrict__ S
         for (int r = 0; r < 4; r++) {
             R.b1[mt][r] = S->b1[16 * mt + 4 * g + r];
             R.b2[mt][r] = S->b2[16 * mt + 4 * g + r];
-            R.w3[mt][r] = S->w3[16 * mt + 4 * g + r];
+            R.w3f[mt][r] = S->w3[16 * mt + 4 * g + r];
         }
+    }
+    R.b3 = S->b3;
+}
+
+// The inference registers from the lane-arranged copy (eight 16-byte loads per lane).
+__device__ __forceinline__ void load_regs_lanes(const nuc3d_sensor_dev *__restrict__ S, Regs &R)
+{
+    const int l = threadIdx.x & 63;
+    R.w3 = __builtin_bit_cast(bf16x8, S->lw3[l]);
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) {
+        R.w1[mt] = __builtin_bit_cast(bf16x8, S->lw1[mt][l]);
+        R.w2[mt] = __builtin_bit_cast(bf16x8, S->lw2[mt][l]);
+        const float4 b1 = S->lb1[mt][l], b2 = S->lb2[mt][l];
+        R.b1[mt][0] = b1.x; R.b1[mt][1] = b1.y; R.b1[mt][2] = b1.z; R.b1[mt][3] = b1.w;
+        R.b2[mt][0] = b2.x; R.b2[mt][1] = b2.y; R.b2[mt][2] = b2.z; R.b2[mt][3] = b2.w;
     }
     R.b3 = S->b3;
 }
@@ -196,48 +262,37 @@ __device__ __forceinline__ void load_regs(const nuc3d_sensor_dev *__restrict__ S
 // Activations of one 16-sample tile kept for the backward pass (per lane:
 // sample l%16, hidden units 16mt + 4g + r).
 struct Acts {
-    float h1[2][4];    // layer-1 pre-activations
-    float h2[2][4];    // layer-2 pre-activations
+    f32x4 h1[2];       // layer-1 pre-activations
+    f32x4 h2[2];       // layer-2 pre-activations
     bf16x8 a1;         // relu(h1) as bf16, in the permuted K order
-    float a2[2][4];    // relu(h2)
+    bf16x8 a2;         // relu(h2) as bf16, in the permuted K order
 };
 
 // Forward pass of one tile of 16 samples by one wavefront: x is the lane's B
-// operand (features 8g..8g+7 of sample l%16).  Returns the logit of sample
-// l%16 (the same in the four lanes that share it).
+// operand (features 8g..8g+7 of sample l%16).  Each layer is one MFMA
+// K step with the bias as the accumulator's initial value; the output
+// layer's A operand repeats w3 in all 16 rows, so every lane receives the
+// logit of its sample l%16.  Between layers: ReLU, then bf16.
 __device__ __forceinline__ float forward16(const Regs &R, bf16x8 x, Acts &A)
 {
-    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-    f32x4 c[2];
-    c[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w1[0], x, zero, 0, 0, 0);
-    c[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w1[1], x, zero, 0, 0, 0);
-    u16x8 a1;
-#pragma unroll
-    for (int mt = 0; mt < 2; mt++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const float h = c[mt][r] + R.b1[mt][r];
-            A.h1[mt][r] = h;
-            a1[4 * mt + r] = bf16_bits(fmaxf(h, 0.0f));
-        }
-    A.a1 = __builtin_bit_cast(bf16x8, a1);
-    f32x4 d[2];
-    d[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w2[0], A.a1, zero, 0, 0, 0);
-    d[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w2[1], A.a1, zero, 0, 0, 0);
-    float z = 0.0f;
-#pragma unroll
-    for (int mt = 0; mt < 2; mt++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const float h = d[mt][r] + R.b2[mt][r];
-            A.h2[mt][r] = h;
-            const float a = fmaxf(h, 0.0f);
-            A.a2[mt][r] = a;
-            z = z + R.w3[mt][r] * a;
-        }
-    z = z + __shfl_xor(z, 16);
-    z = z + __shfl_xor(z, 32);
-    return z + R.b3;
+    A.h1[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w1[0], x, R.b1[0], 0, 0, 0);
+    A.h1[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w1[1], x, R.b1[1], 0, 0, 0);
+    uint4 u;
+    u.x = pack_bf16(fmaxf(A.h1[0][0], 0.0f), fmaxf(A.h1[0][1], 0.0f));
+    u.y = pack_bf16(fmaxf(A.h1[0][2], 0.0f), fmaxf(A.h1[0][3], 0.0f));
+    u.z = pack_bf16(fmaxf(A.h1[1][0], 0.0f), fmaxf(A.h1[1][1], 0.0f));
+    u.w = pack_bf16(fmaxf(A.h1[1][2], 0.0f), fmaxf(A.h1[1][3], 0.0f));
+    A.a1 = __builtin_bit_cast(bf16x8, u);
+    A.h2[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w2[0], A.a1, R.b2[0], 0, 0, 0);
+    A.h2[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w2[1], A.a1, R.b2[1], 0, 0, 0);
+    u.x = pack_bf16(fmaxf(A.h2[0][0], 0.0f), fmaxf(A.h2[0][1], 0.0f));
+    u.y = pack_bf16(fmaxf(A.h2[0][2], 0.0f), fmaxf(A.h2[0][3], 0.0f));
+    u.z = pack_bf16(fmaxf(A.h2[1][0], 0.0f), fmaxf(A.h2[1][1], 0.0f));
+    u.w = pack_bf16(fmaxf(A.h2[1][2], 0.0f), fmaxf(A.h2[1][3], 0.0f));
+    A.a2 = __builtin_bit_cast(bf16x8, u);
+    const f32x4 b3 = {R.b3, R.b3, R.b3, R.b3};
+    const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w3, A.a2, b3, 0, 0, 0);
+    return z[0];
 }
 
 }  // namespace nuc3d_sensor

@@ -80,9 +80,9 @@ __global__ __launch_bounds__(256) void k_sensor_sample(const nuc3d_block_dev *__
                 double cs;
                 con2prim(Q, B.jac[x], P, pr[0][j], pr[1][j], pr[2][j], pr[3][j], pr[4][j], cs);
             }
-            auto get = [&](int v, int j) { return pr[v][j + 1]; };
+            auto getf = [&](int v, int j) { return (float)pr[v][j + 1]; };
             auto get8 = [&](int v, int j) { return pr[v][j]; };
-            if (features(get, gamma, xw)) {
+            if (features(getf, (float)gamma, xw)) {
                 const int lab = teacher(get8, gamma, tau);
                 y = (float)lab;
                 pos = lab == 1;
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(128) void k_sensor_fwdbwd(const unsigned *__restric
 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
     Regs R;
-    load_regs(&SW, R);
+    load_regs(&SW, R);  // the same operands as load_regs_lanes gives the sweep
     // A operand of dA1^T = W2^T dH2^T: row = input unit 16mt + j, k = output unit 8g+e
     bf16x8 w2t[2];
 #pragma unroll
@@ -197,8 +197,8 @@ __global__ __launch_bounds__(128) void k_sensor_fwdbwd(const unsigned *__restric
         for (int mt = 0; mt < 2; mt++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const float dh2 = A.h2[mt][r] > 0.0f ? dz * R.w3[mt][r] : 0.0f;
-                gw3[mt][r] += dz * A.a2[mt][r];
+                const float dh2 = A.h2[mt][r] > 0.0f ? dz * R.w3f[mt][r] : 0.0f;
+                gw3[mt][r] += dz * bf16_float(__builtin_bit_cast(u16x8, A.a2)[4 * mt + r]);
                 gb2[mt][r] += dh2;
                 const unsigned short hb = bf16_bits(dh2);
                 dh2b[4 * mt + r] = hb;

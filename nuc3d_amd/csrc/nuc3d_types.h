@@ -48,6 +48,14 @@ struct nuc3d_sensor_dev {
     unsigned short w2[32 * 32];  // [hidden][hidden]
     float b1[32], b2[32], w3[32];
     float b3, thr;
+    float pad[2];
+    // the same values arranged per lane of a wavefront as the MFMA operands
+    // of the inference (nuc3d_sensor.h, load_regs_lanes): lane l of lane group
+    // g = l/16 holds W1 row 16mt + l%16, inputs 8g..8g+7; W2 row 16mt + l%16
+    // and w3, both at hidden inputs 4g..4g+3 and 16+4g..16+4g+3; b1, b2 of
+    // hidden 16mt + 4g .. 16mt + 4g + 3
+    uint4 lw1[2][64], lw2[2][64], lw3[64];
+    float4 lb1[2][64], lb2[2][64];
 };
 
 struct nuc3d_tile {
@@ -63,4 +71,9 @@ struct nuc3d_kparams {
     int nblocks;
     double *resid;  // stage 2: per-work-item partial sums of (Q - Q_n)^2, or null
     const nuc3d_sensor_dev *sensor;  // hybrid reconstruction: the network, or null
+    // hybrid decision counts (central, WENO5-JS by the network, by the
+    // fallback), spread over NUC3D_HCOUNT_SLOTS slots of 16 words so that
+    // the atomics of different workgroups do not meet on one address
+    unsigned long long *hcount;
 };
+#define NUC3D_HCOUNT_SLOTS 256

@@ -130,7 +130,6 @@ def lib():
         "nuc3d_sensor_train": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_ulonglong,
                                fp],
         "nuc3d_set_sensor": [vp, fp, ctypes.c_float],
-        "nuc3d_record_hybrid": [vp, ctypes.c_int],
         "nuc3d_download_hybrid_flags": [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte)],
         "nuc3d_get_hybrid_counts": [vp, llp],
         "nuc3d_op_interpolate": [ctypes.c_int] * 8 + [dp, dp, dp, vp],
@@ -330,11 +329,9 @@ class Case:
         _check(lib().nuc3d_set_sensor(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), float(threshold)),
                "set_sensor")
 
-    def record_hybrid(self, enable=True):
-        _check(lib().nuc3d_record_hybrid(self.h, 1 if enable else 0), "record_hybrid")
-
     def hybrid_flags(self, b, d):
-        """Decisions of the last RHS evaluation for direction d of block b, in
+        """Last decisions (of nuc3d_rhs, or of the current step's stage 0) for
+        direction d of block b, in
         the face-array layout (n_{d+2}, n_{d+1}, n_d + 1): 0 central, 1 WENO5-JS
         by the network, 2 WENO5-JS by the fallback, 3 physical-face closure."""
         n = self.blocks[b].dims

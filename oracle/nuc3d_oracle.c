@@ -97,11 +97,13 @@ static void stencil5(const double *ext, int W, int m, int upwind, double *f)
 }
 
 /* Eighth-order central face value from cells m-4..m+3 (c[0..7]): the face
- * point value of the degree-7 polynomial with the eight cell averages.  Not
- * in the reference; it is the smooth-region scheme of the learned hybrid. */
+ * point value of the degree-7 polynomial with the eight cell averages, the
+ * division by 840 taken as a product with its FP64 reciprocal.  Not in the
+ * reference; it is the smooth-region scheme of the learned hybrid. */
 static double central8_face(const double *c)
 {
-    return ((533.0 * (c[3] + c[4]) - 139.0 * (c[2] + c[5])) + (29.0 * (c[1] + c[6]) - 3.0 * (c[0] + c[7]))) / 840.0;
+    return ((533.0 * (c[3] + c[4]) - 139.0 * (c[2] + c[5])) + (29.0 * (c[1] + c[6]) - 3.0 * (c[0] + c[7]))) *
+           (1.0 / 840.0);
 }
 
 /* ReconstructionboundaryScheme stencils: reconstructionBoundaryScheme.cpp:322-343 */

@@ -8,15 +8,18 @@ this build's own definition:
   features  rho, u, v, w, p on the six cells m-3..m+2 of face m, each centred
             on its stencil mean and divided by (range + 0.01 S), S = mean rho
             (rho), mean p (p), sqrt(gamma mean p / mean rho) (velocities);
-            FP64, then float32, then bf16 (round to nearest even);
+            FP32 on the FP32 roundings of the primitives, then bf16 (round
+            to nearest even);
   teacher   1 when, for some primitive, the eighth-order central face value
             and the WENO5-JS value (weno5js.cpp:75-108 order) of either upwind
             side differ by more than tau S;
-  network   30 -> 32 -> 32 -> 1, ReLU, bf16 weights and activations, FP32
-            accumulation; loss wt (softplus(z) - y z).
+  network   30 -> 32 -> 32 -> 1, ReLU, bf16 weights and activations (both
+            hidden layers), FP32 accumulation with the biases as initial
+            values; loss wt (softplus(z) - y z).
 
-Every FP64 step of the features and the teacher is written in the same
-operation order as the device code, so features and labels are bit-exact.
+Every step of the features (FP32) and the teacher (FP64) is written in the
+same operation order as the device code, so features and labels are
+bit-exact.
 The network is evaluated here in float64 from the same bf16-rounded operands
 (the GPU accumulates in FP32 on the matrix cores), so logits and gradients
 agree to a tolerance, not bit for bit.  quantize=False gives the plain
@@ -92,50 +95,76 @@ def weno5js(f0, f1, f2, f3, f4):
 
 
 def central8(c0, c1, c2, c3, c4, c5, c6, c7):
-    return ((533.0 * (c3 + c4) - 139.0 * (c2 + c5)) + (29.0 * (c1 + c6) - 3.0 * (c0 + c7))) / 840.0
+    """Eighth-order central face value (FP64), the division by 840 taken as a
+    product with its FP64 reciprocal."""
+    return ((533.0 * (c3 + c4) - 139.0 * (c2 + c5)) + (29.0 * (c1 + c6) - 3.0 * (c0 + c7))) * (1.0 / 840.0)
+
+
+_F = np.float32
+
+
+def central8f(c0, c1, c2, c3, c4, c5, c6, c7):
+    """The same in FP32 (the positivity check of the hybrid)."""
+    return ((_F(533.0) * (c3 + c4) - _F(139.0) * (c2 + c5)) + (_F(29.0) * (c1 + c6) - _F(3.0) * (c0 + c7))) * \
+        (_F(1.0) / _F(840.0))
 
 
 def stencil_scales(P6, gamma):
-    """P6[v][j] (v = rho,u,v,w,p; j = cells m-3..m+2; trailing axes are
-    faces): means, scales S and the validity mask."""
-    mean = []
-    for v in range(5):
+    """FP64 means of rho and p over the stencil P6[v][j] (v = rho,u,v,w,p;
+    j = cells m-3..m+2), the scales S and the validity mask (the teacher's)."""
+    mean = {}
+    for v in (0, 4):
         s = P6[v][0]
         for j in range(1, 6):
             s = s + P6[v][j]
-        mean.append(s / 6.0)
+        mean[v] = s / 6.0
     rb, pb = mean[0], mean[4]
     ok = (rb > 0.0) & (pb > 0.0)
     with np.errstate(invalid="ignore", divide="ignore"):
         cb = np.sqrt(gamma * pb / rb)
-    S = [rb, cb, cb, cb, pb]
-    return mean, S, ok
+    return [rb, cb, cb, cb, pb], ok
 
 
 def features(P6, gamma):
-    """bf16 bit patterns (..., 32) of the features of stencils P6 (5, 6, ...),
-    and the validity mask; invalid stencils get zero features."""
-    mean, S, ok = stencil_scales(P6, gamma)
-    shape = np.shape(P6[0][0])
-    x = np.zeros(shape + (32,), dtype=np.uint16)
-    with np.errstate(invalid="ignore", divide="ignore"):
+    """bf16 bit patterns (..., 32) of the features of stencils P6 (5, 6, ...)
+    of FP64 primitives, and the validity mask; invalid stencils get zero
+    features.  FP32 arithmetic on the FP32 roundings of the primitives:
+    mean = sum * (1/6), S = (mean rho, c, c, c, mean p) with
+    c = sqrt(gamma mean p / mean rho), x = (P - mean) * (1 / (range + 0.01 S))."""
+    P = [[np.asarray(P6[v][j], dtype=np.float64).astype(_F) for j in range(6)] for v in range(5)]
+    g = _F(gamma)
+    sixth = _F(1.0) / _F(6.0)
+
+    def mean_of(v):
+        s = P[v][0]
+        for j in range(1, 6):
+            s = s + P[v][j]
+        return s * sixth
+
+    rb, pb = mean_of(0), mean_of(4)
+    ok = (rb > 0) & (pb > 0)
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        cb = np.sqrt(g * pb / rb)
+        S = [rb, cb, cb, cb, pb]
+        shape = np.shape(P[0][0])
+        x = np.zeros(shape + (32,), dtype=np.uint16)
         for v in range(5):
-            lo = P6[v][0]
-            hi = P6[v][0]
+            mean = mean_of(v)
+            lo, hi = P[v][0], P[v][0]
             for j in range(1, 6):
-                lo = np.minimum(lo, P6[v][j])
-                hi = np.maximum(hi, P6[v][j])
-            den = (hi - lo) + FEAT_EPS * S[v]
+                lo = np.minimum(lo, P[v][j])
+                hi = np.maximum(hi, P[v][j])
+            inv = _F(1.0) / ((hi - lo) + _F(0.01) * S[v])
             for j in range(6):
-                val = ((P6[v][j] - mean[v]) / den).astype(np.float32)
-                x[..., 6 * v + j] = np.where(ok, bf16_bits(np.where(ok, val, 0.0)), 0)
+                val = (P[v][j] - mean) * inv
+                x[..., 6 * v + j] = np.where(ok, bf16_bits(np.where(ok, val, _F(0.0))), 0)
     return x, ok
 
 
 def teacher(P8, gamma, tau):
     """Labels of stencils P8 (5, 8, ...) over cells m-4..m+3: 1 / 0, -1 where
     the stencil is invalid."""
-    _, S, ok = stencil_scales([P8[v][1:7] for v in range(5)], gamma)
+    S, ok = stencil_scales([P8[v][1:7] for v in range(5)], gamma)
     dev = np.zeros(np.shape(P8[0][0]))
     with np.errstate(invalid="ignore", divide="ignore"):
         for v in range(5):
@@ -211,7 +240,9 @@ def decisions(q, jac, gamma, d, periodic, params, thr):
     P8 = face_stencil_prims(q, jac, gamma, d, periodic)
     x, ok = features([P8[v][1:7] for v in range(5)], gamma)
     with np.errstate(invalid="ignore"):
-        posok = ok & (central8(*P8[0]) > 0.0) & (central8(*P8[4]) > 0.0)
+        r32 = [np.asarray(c, dtype=np.float64).astype(_F) for c in P8[0]]
+        p32 = [np.asarray(c, dtype=np.float64).astype(_F) for c in P8[4]]
+        posok = ok & (central8f(*r32) > 0) & (central8f(*p32) > 0)
     z, _ = forward(x.reshape(-1, 32), params)
     z = z.reshape(x.shape[:-1])
     flag = np.where(~posok, 2, np.where(z >= thr, 1, 0))
@@ -262,7 +293,7 @@ def forward(xbits, params, quantize=True):
     h1 = x @ W1q.T + b1.astype(np.float64)
     a1 = _q(np.maximum(h1, 0.0).astype(np.float32), quantize) if quantize else np.maximum(h1, 0.0)
     h2 = a1 @ W2q.T + b2.astype(np.float64)
-    a2 = np.maximum(h2, 0.0)
+    a2 = _q(np.maximum(h2, 0.0).astype(np.float32), quantize) if quantize else np.maximum(h2, 0.0)
     z = a2 @ w3q + float(b3)
     return z, (x, h1, a1, h2, a2, W2q, w3q)
 

@@ -204,7 +204,6 @@ def test_hybrid_rhs_parity_periodic(model, exact):
     params = mixed_params(4)
     g = solver.Case(sp, blocks)
     g.set_sensor(params, 0.0)
-    g.record_hybrid(True)
     g.rhs()
     flags = [g.hybrid_flags(0, d) for d in range(3)]
     assert all((f == 0).any() and (f == 1).any() for f in flags)
@@ -233,15 +232,22 @@ def test_hybrid_steps_walls_two_blocks(exact):
     params = mixed_params(5)
     g = solver.Case(sp, blocks)
     g.set_sensor(params, 0.0)
-    g.record_hybrid(True)
     o = O.OracleCase(op, blocks)
     for step in range(2):
         for st in range(3):
+            counts = g.hybrid_counts()
             g.stage(st, 0.5)
-            for b in range(2):
-                for d in range(3):
-                    f = g.hybrid_flags(b, d)
-                    o.set_hybrid_mask(b, d, f)
+            flags = {(b, d): g.hybrid_flags(b, d) for b in range(2) for d in range(3)}
+            if st == 0:
+                step_flags = flags
+                assert sum(g.hybrid_counts()) > sum(counts)
+            else:
+                # stages 1 and 2 reuse the decisions of stage 0 (and count none)
+                for k in flags:
+                    np.testing.assert_array_equal(flags[k], step_flags[k])
+                assert g.hybrid_counts() == counts
+            for (b, d), f in flags.items():
+                o.set_hybrid_mask(b, d, f)
             o.stage(st, 0.5)
             for b in range(2):
                 err = relerr(g.get_q(b), o.get_q(b))

@@ -34,9 +34,10 @@ def _stencil(vals):
 def test_features_closed_forms():
     g = 1.4
     rho, p = 1.3, 2.0
-    # constant stencil: all features zero up to the rounding of the mean, valid
+    # constant stencil: all features zero up to the FP32 rounding of the mean
+    # (a few ulps of the value over 0.01 S), valid
     x, ok = S.features(_stencil([[rho] * 6, [0.2] * 6, [0.0] * 6, [-0.1] * 6, [p] * 6]), g)
-    assert ok and np.abs(S.bf16_value(x)).max() < 1e-13
+    assert ok and np.abs(S.bf16_value(x)).max() < 1e-4
     # linear ramp in u with slope s: x_j = s (j - 2.5) / (5|s| + 0.01 c)
     s = 0.05
     u = [0.3 + s * j for j in range(6)]
@@ -44,7 +45,7 @@ def test_features_closed_forms():
     c = np.sqrt(g * p / rho)
     want = [s * (j - 2.5) / (5 * s + 0.01 * c) for j in range(6)]
     np.testing.assert_allclose(S.bf16_value(x[6:12]), want, rtol=2 ** -8)
-    assert not x[:6].any() and not x[12:].any()
+    assert np.abs(S.bf16_value(x[:6])).max() < 1e-4 and np.abs(S.bf16_value(x[12:])).max() < 1e-4
     # non-positive mean density: invalid, zero features
     x, ok = S.features(_stencil([[-1.0] * 6, u, [0.0] * 6, [0.0] * 6, [p] * 6]), g)
     assert not ok and not x.any()
