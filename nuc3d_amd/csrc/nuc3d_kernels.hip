@@ -15,6 +15,8 @@
 // on-device form of MPICommunicator.cpp:55-115).  Stage 2 also accumulates
 // the maximum eigenvalues of the new state, from which the next step's dt is
 // taken (euler3d.cpp:429-435, PDEData3d.cpp:74-80).
+#include <type_traits>
+
 #include "nuc3d_device.h"
 #include "nuc3d_sensor.h"
 #include "nuc3d_types.h"
@@ -533,16 +535,31 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     constexpr int H = 4;
     constexpr int NL = 16, NF = 16, NC = NF - 1, NSL = NC + 2 * H, NST = 2 * H;
     __shared__ double S[10][NSL * NL];
-    // one area, three lives: the FP32 roundings of rho, u, v, w, p of the
-    // staged cells; then the bf16 features of the 256 faces; then the face fluxes
-    __shared__ __align__(16) union {
-        float pr[5][NSL * NL];
-        unsigned xf[256][16];
-        double hf[5][NF * NL];
-    } U;
-    auto &PR = U.pr;
-    auto &HF = U.hf;
-    __shared__ float ZL[256];
+    // Where decisions are made, one area has three lives: the FP32 roundings
+    // of rho, u, v, w, p of the staged cells; then the bf16 features of the
+    // 256 faces; then the face fluxes.  Where they are reused, the area holds
+    // the face fluxes only, and the kernel fits four workgroups per CU.
+    struct LdsDecide {
+        union {
+            float pr[5][NSL * NL];
+            unsigned xf[256][16];
+            double hf[5][NF * NL];
+        } u;
+        float zl[256];
+    };
+    struct LdsReuse {
+        union {
+            double hf[5][NF * NL];
+        } u;
+    };
+    __shared__ __align__(16) std::conditional_t<DECIDE, LdsDecide, LdsReuse> U;
+    auto &HF = U.u.hf;
+    float(*PR)[NSL * NL] = nullptr;
+    float *ZL = nullptr;
+    if constexpr (DECIDE) {
+        PR = U.u.pr;
+        ZL = U.zl;
+    }
     __shared__ unsigned short LST[256];
     __shared__ int WCNT[4];
     __shared__ int WSTAT[4][3];
@@ -578,7 +595,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
             for (int c = 0; c < 5; c++) {
                 S[c][o] = fp[c];
                 S[5 + c][o] = fn[c];
-                if (DECIDE) PR[c][o] = (float)pr[c];
+                if constexpr (DECIDE) PR[c][o] = (float)pr[c];
             }
         }
     }
@@ -622,7 +639,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
         }
         __syncthreads();  // the primitives are dead: the area takes the features
         {
-            uint4 *row = reinterpret_cast<uint4 *>(U.xf[threadIdx.x]);
+            uint4 *row = reinterpret_cast<uint4 *>(U.u.xf[threadIdx.x]);
 #pragma unroll
             for (int i = 0; i < 4; i++) row[i] = make_uint4(xw[4 * i], xw[4 * i + 1], xw[4 * i + 2], xw[4 * i + 3]);
         }
@@ -636,7 +653,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int tf = 64 * wv + 16 * q + (lane & 15);
-                const uint4 v4 = *reinterpret_cast<const uint4 *>(&U.xf[tf][4 * g]);
+                const uint4 v4 = *reinterpret_cast<const uint4 *>(&U.u.xf[tf][4 * g]);
                 Acts A;
                 const float z = forward16(R, __builtin_bit_cast(bf16x8, v4), A);
                 if (g == 0) ZL[tf] = z;
