@@ -178,6 +178,17 @@ typedef struct {
 } nuc3d_run_status;
 int nuc3d_run(nuc3d_case *c, const nuc3d_run_config *cfg, nuc3d_run_status *out, void *stream);
 
+/* Batches of independent cases (no counterpart in the reference, where each
+ * case is its own MPI job).  Block b belongs to group `group` (0 by default):
+ * each group is one case, with dt = cfl * min over its own blocks
+ * (PDEData3d.cpp:74-80 per case), and its own Mach and Reynolds numbers
+ * (mach, reynolds <= 0 keep the case's).  Blocks of different groups must not
+ * be connected.  All groups advance together in the same launches.  Set
+ * between full steps; nuc3d_run needs a single group. */
+int nuc3d_set_block_group(nuc3d_case *c, int b, int group, double mach, double reynolds);
+/* dt of group `group` in the current step (nuc3d_get_dt gives group 0's). */
+int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt);
+
 /* Wait for the case's work on its streams. */
 int nuc3d_sync(nuc3d_case *c);
 

@@ -302,3 +302,40 @@ def split_block_x(spec: BlockSpec, nparts: int) -> List[BlockSpec]:
         out.append(BlockSpec((b - a, ny, nz), faces, spec.jac[:, :, a:b].copy(), spec.met[:, :, :, a:b].copy(),
                              spec.dxyz[:, :, :, a:b].copy(), spec.q[:, :, :, a:b].copy()))
     return out
+
+
+@dataclass
+class BatchMember:
+    block: BlockSpec
+    group: int
+    mach: float
+    reynolds: float
+    kind: str
+
+
+def batch_cases(n_cases=48, lo=32, hi=160, seed=0, step=4) -> List[BatchMember]:
+    """A batch of independent cases of seeded uneven sizes: cubes of n^3 cells
+    with n in [lo, hi] (multiples of `step`), half Taylor-Green vortices on
+    [0,2pi]^3 at Ma 0.1 with Re drawn from [400, 3200], half spherical blasts
+    on the unit cube (p=10, rho=1 inside r<0.2; p=1, rho=0.125 outside) at the
+    blast's own normalisation (Ma 1) and Re 1e4.  Every case is one periodic
+    block of its own group (its own dt)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(lo // step, hi // step + 1, n_cases) * step
+    out = []
+    for c in range(n_cases):
+        n = int(sizes[c])
+        if c % 2 == 0:
+            re = float(rng.uniform(400.0, 3200.0))
+            phys = Physics(mach=0.1, reynolds=re)
+            b = taylor_green(n, phys=phys)[0]
+            kind = "tgv"
+            mach = 0.1
+        else:
+            re = 1.0e4
+            b = spherical_blast(n)[0]
+            kind = "blast"
+            mach = 1.0
+        b.faces = periodic_faces(c)
+        out.append(BatchMember(b, c, mach, re, kind))
+    return out

@@ -344,6 +344,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     K.nblocks = nblocks;
     K.sensor = nullptr;
     K.hcount = nullptr;
+    K.ngroups = 1;
     const bool ns = p->model == NUC3D_NAVIERSTOKES3D;
     // Q x 3, RHS, J, metrics, dxyz, u v w T (viscous terms and diagnostics),
     // and for Navier-Stokes the 12 viscous flux components
@@ -361,6 +362,9 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
             }
         }
         B.G = c->G;
+        B.group = 0;
+        B.mach = p->mach;
+        B.reynolds = p->reynolds;
         const long long px = (XLEAD + B.n[0] + B.G + ROWALIGN - 1) / ROWALIGN * ROWALIGN;
         const long long py = B.n[1] + 2 * B.G, pz = B.n[2] + 2 * B.G;
         B.sy = px;
@@ -483,7 +487,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     }
     if (hipMalloc(&c->db, sizeof(nuc3d_block_dev) * nblocks) != hipSuccess ||
         hipMalloc(&c->dtiles, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess ||
-        hipMalloc(&c->ddt, sizeof(double) * 4) != hipSuccess ||
+        hipMalloc(&c->ddt, sizeof(double) * NUC3D_MAX_BLOCKS) != hipSuccess ||
         hipMalloc(&c->dlam, sizeof(unsigned long long) * 6 * nblocks) != hipSuccess ||
         hipMalloc(&c->dcount, sizeof(unsigned long long) * 4) != hipSuccess ||
         hipMalloc(&c->dhcount, sizeof(unsigned long long) * 16 * NUC3D_HCOUNT_SLOTS) != hipSuccess) {
@@ -492,7 +496,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     }
     c->K.hcount = c->dhcount;
     if (hipMemcpy(c->dtiles, tiles.data(), sizeof(nuc3d_tile) * tiles.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(c->ddt, 0, sizeof(double) * 4) != hipSuccess ||
+        hipMemset(c->ddt, 0, sizeof(double) * NUC3D_MAX_BLOCKS) != hipSuccess ||
         hipMemset(c->dlam, 0, sizeof(unsigned long long) * 6 * nblocks) != hipSuccess ||
         hipMemset(c->dcount, 0, sizeof(unsigned long long) * 4) != hipSuccess ||
         hipMemset(c->dhcount, 0, sizeof(unsigned long long) * 16 * NUC3D_HCOUNT_SLOTS) != hipSuccess) {
@@ -548,6 +552,7 @@ int nuc3d_set_faces(nuc3d_case *c, int b, const int *type6, const int *id6, cons
             const int d = f / 2;
             if (N.n[(d + 1) % 3] != B.n[(d + 1) % 3] || N.n[(d + 2) % 3] != B.n[(d + 2) % 3])
                 return fail(NUC3D_ERR_BAD_ARG, "connected faces differ in tangential size");
+            if (N.group != B.group) return fail(NUC3D_ERR_BAD_ARG, "connected faces join blocks of different groups");
         }
         B.ftype[f] = t;
         B.fid[f] = id;
@@ -844,6 +849,7 @@ int nuc3d_run(nuc3d_case *c, const nuc3d_run_config *cfg, nuc3d_run_status *out,
 {
     if (!c || !cfg) return fail(NUC3D_ERR_BAD_ARG, "null argument");
     if (cfg->save_step < 0 || cfg->post_step < 0) return fail(NUC3D_ERR_BAD_ARG, "negative save or post interval");
+    if (c->K.ngroups > 1) return fail(NUC3D_ERR_UNSUPPORTED, "the time loop runs one case (group) at a time");
     int step = cfg->start_step;
     double time = cfg->time, dt = 0.0, res = 0.0;
     int rc;
@@ -895,6 +901,42 @@ int nuc3d_get_dt(nuc3d_case *c, double *dt)
     if (!c || !dt) return fail(NUC3D_ERR_BAD_ARG, "null argument");
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(dt, c->ddt, sizeof(double), hipMemcpyDeviceToHost));
+    return NUC3D_OK;
+}
+
+int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt)
+{
+    if (!c || !dt) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    if (group < 0 || group >= c->K.ngroups) return fail(NUC3D_ERR_BAD_ARG, "group out of range");
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(dt, c->ddt + group, sizeof(double), hipMemcpyDeviceToHost));
+    return NUC3D_OK;
+}
+
+int nuc3d_set_block_group(nuc3d_case *c, int b, int group, double mach, double reynolds)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (group < 0 || group >= c->nb) return fail(NUC3D_ERR_BAD_ARG, "group must be in [0, nblocks)");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "groups change between full steps");
+    nuc3d_block_dev &B = c->hb[b];
+    for (int f = 0; f < 6; f++)
+        if (B.ftype[f] >= 0 && B.ftype[f] != b && c->hb[B.ftype[f]].group != group)
+            return fail(NUC3D_ERR_BAD_ARG, "a connected face joins a block of another group");
+    B.group = group;
+    B.mach = mach > 0.0 ? mach : c->p.mach;
+    B.reynolds = reynolds > 0.0 ? reynolds : c->p.reynolds;
+    int ng = 0;
+    for (const auto &x : c->hb) ng = std::max(ng, x.group + 1);
+    c->K.ngroups = ng;
+    // the captured steps hold the launch parameters by value
+    for (auto &g : c->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    c->gcfl[0] = c->gcfl[1] = -1.0;
+    c->dirty = true;
     return NUC3D_OK;
 }
 

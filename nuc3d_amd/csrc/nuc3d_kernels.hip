@@ -31,12 +31,14 @@ using namespace nuc3d_dev;
 
 namespace {
 
-__device__ __forceinline__ PhysParams phys_of(const nuc3d_kparams &K)
+// physics of block B: the case's, with the block's (its case's) Mach and
+// Reynolds numbers
+__device__ __forceinline__ PhysParams phys_of(const nuc3d_kparams &K, const nuc3d_block_dev &B)
 {
     PhysParams P;
     P.gamma = K.gamma;
-    P.mach = K.mach;
-    P.reynolds = K.reynolds;
+    P.mach = B.mach;
+    P.reynolds = B.reynolds;
     P.prandtl = K.prandtl;
     P.sref = K.sref;
     P.twall = K.twall;
@@ -283,30 +285,40 @@ __device__ __forceinline__ void block_sum_store(double v, double *dst)
     }
 }
 
-// dt_global = cfl * MIN_b min_d 1/lambda_{b,d} (euler3d.cpp:429-435, PDEData3d.cpp:78-79)
-__device__ __forceinline__ double global_dt(const unsigned long long *lam, int nb, double cfl)
+// dt of group g = cfl * MIN over its blocks b of min_d 1/lambda_{b,d}
+// (euler3d.cpp:429-435, PDEData3d.cpp:78-79: the MPI_MIN over the ranks of one
+// case; with one group, every block)
+__device__ __forceinline__ double group_dt(const nuc3d_block_dev *blocks, const unsigned long long *lam, int nb, int g,
+                                          double cfl)
 {
     double m = 0.0;
+    bool first = true;
     for (int b = 0; b < nb; b++) {
+        if (blocks[b].group != g) continue;
         double a = 1.0 / __longlong_as_double((long long)lam[3 * b]);
         double bb = 1.0 / __longlong_as_double((long long)lam[3 * b + 1]);
         double cc = 1.0 / __longlong_as_double((long long)lam[3 * b + 2]);
         double mm = (bb < a) ? bb : a;
         double dl = (cc < mm) ? cc : mm;
-        m = (b == 0 || dl < m) ? dl : m;
+        m = (first || dl < m) ? dl : m;
+        first = false;
     }
     return m * cfl;
 }
 
-// dt of the stage: stage 0 takes cfl * min 1/lambda from the eigenvalue
-// maxima of the stage-0 state (and workgroup 0 publishes it and clears the
-// maxima the coming stage 2 accumulates); stages 1 and 2 reuse it.
-__device__ __forceinline__ double stage_dt(const nuc3d_kparams &K, int stage, double *dtbuf,
-                                           const unsigned long long *lam_cur, unsigned long long *lam_next)
+// dt of the stage for a block of group g: stage 0 takes cfl * min 1/lambda
+// over the group from the eigenvalue maxima of the stage-0 state (and
+// workgroup 0 publishes every group's dt and clears the maxima the coming
+// stage 2 accumulates); stages 1 and 2 reuse it.
+__device__ __forceinline__ double stage_dt(const nuc3d_block_dev *blocks, const nuc3d_kparams &K, int stage, int g,
+                                          double *dtbuf, const unsigned long long *lam_cur,
+                                          unsigned long long *lam_next)
 {
-    if (stage != 0) return dtbuf[0];
-    const double dt = global_dt(lam_cur, K.nblocks, K.cfl);
-    if (blockIdx.x == 0 && threadIdx.x == 0) dtbuf[0] = dt;
+    if (stage != 0) return dtbuf[g];
+    const double dt = group_dt(blocks, lam_cur, K.nblocks, g, K.cfl);
+    if (blockIdx.x == 0)
+        for (int g2 = threadIdx.x; g2 < K.ngroups; g2 += blockDim.x)
+            dtbuf[g2] = g2 == g ? dt : group_dt(blocks, lam_cur, K.nblocks, g2, K.cfl);
     if (blockIdx.x == 0)
         for (int e = threadIdx.x; e < 3 * K.nblocks; e += blockDim.x) lam_next[e] = 0ULL;
     return dt;
@@ -428,7 +440,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     const int witem = xcd_tile(blockIdx.x, gridDim.x);
     const int4 t = tiles[witem];
     const nuc3d_block_dev &B = blocks[t.x];
-    const PhysParams P = phys_of(K);
+    const PhysParams P = phys_of(K, B);
     const int a0 = t.y, l0 = t.z, cc = t.w;
     const int nc = min(NC, B.n[DIR] - a0);
     const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
@@ -483,7 +495,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
     double r2 = 0.0;  // residual partial (stage 2)
-    const double dt = (DIR == 2 && MODE == 1) ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
+    const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
     if (line < nlines && f >= 1 && f <= nc) {
         int i, j, k;
         cell(line, a0 + f - 1, i, j, k);
@@ -567,7 +579,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     const int witem = xcd_tile(blockIdx.x, gridDim.x);
     const int4 t = tiles[witem];
     const nuc3d_block_dev &B = blocks[t.x];
-    const PhysParams P = phys_of(K);
+    const PhysParams P = phys_of(K, B);
     const int a0 = t.y, l0 = t.z, cc = t.w;
     const int nc = min(NC, B.n[DIR] - a0);
     const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
@@ -738,7 +750,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
     double r2 = 0.0;
-    const double dt = (DIR == 2 && MODE == 1) ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
+    const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
     if (line < nlines && f >= 1 && f <= nc) {
         int i, j, k;
         cell(line, a0 + f - 1, i, j, k);
@@ -779,7 +791,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
     const int witem = xcd_tile(blockIdx.x, gridDim.x);
     const int4 L = lines[witem];
     const nuc3d_block_dev &B = blocks[L.x];
-    const PhysParams P = phys_of(K);
+    const PhysParams P = phys_of(K, B);
     const int n = B.n[DIR], w = K.width[DIR], M = n - 2 * w + 1;
     const int ns = n + 2 * H;              // slots: cell a at slot a + H
     double *F = lds;                       // [10][ns]
@@ -860,7 +872,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
     double r2 = 0.0;  // residual partial (stage 2)
-    const double dt = (DIR == 2 && MODE == 1) ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
+    const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
     for (int a = threadIdx.x; a < n; a += 64) {
         int i, j, k;
         cell(a, i, j, k);
@@ -938,7 +950,7 @@ __global__ __launch_bounds__(256) void KNAME(k_prim)(const nuc3d_block_dev *__re
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     if ((int)blockIdx.y >= tl.kn) return;
     const nuc3d_block_dev &B = blocks[tl.b];
-    const PhysParams P = phys_of(K);
+    const PhysParams P = phys_of(K, B);
     const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T, k = tl.k0 + blockIdx.y;
     if (i >= B.n[0] || j >= B.n[1]) return;
     const long long x = cidx(B, i, j, k);
@@ -980,9 +992,9 @@ __device__ __forceinline__ bool halo_cell(int h, int tw, int th, int &li, int &l
 
 // Viscous fluxes of one cell from its u,v,w,T gradients along xi, eta, zeta
 // (dxi/deta/dzeta [4] each) and centre values (NaiverStokes3d.cpp:170-326).
-__device__ __forceinline__ void visc_flux_cell(const nuc3d_kparams &K, const double *dxi, const double *deta,
-                                               const double *dzeta, const double *w4, const double *m, double jac,
-                                               double *out)
+__device__ __forceinline__ void visc_flux_cell(const nuc3d_kparams &K, const PhysParams &P, const double *dxi,
+                                               const double *deta, const double *dzeta, const double *w4,
+                                               const double *m, double jac, double *out)
 {
     const double xi_x = m[0], xi_y = m[1], xi_z = m[2];
     const double eta_x = m[3], eta_y = m[4], eta_z = m[5];
@@ -997,8 +1009,8 @@ __device__ __forceinline__ void visc_flux_cell(const nuc3d_kparams &K, const dou
     }
     const double S = K.sref;
     const double u = w4[0], vv = w4[1], w = w4[2], T0 = w4[3];
-    const double miu0 = (1.0 + S) / (T0 + S) * pow(T0, 1.5) / K.reynolds;  // physicsModel.cpp:226
-    const double coeff0 = miu0 / ((K.gamma - 1.0) * K.mach * K.mach * K.prandtl);
+    const double miu0 = (1.0 + S) / (T0 + S) * pow(T0, 1.5) / P.reynolds;  // physicsModel.cpp:226
+    const double coeff0 = miu0 / ((K.gamma - 1.0) * P.mach * P.mach * K.prandtl);
     const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
     const double vx = g[1][0], vy = g[1][1], vz = g[1][2];
     const double wx = g[2][0], wy = g[2][1], wz = g[2][2];
@@ -1028,6 +1040,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
     __shared__ double Wp[2][4][L][L];  // u,v,w,T of plane k with the xy halo, double-buffered
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
+    const PhysParams P = phys_of(K, B);
     const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
     const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
     const bool incell = ti < tw && tj < th;
@@ -1096,7 +1109,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
             w4[v] = zr[H][v];
         }
         double out[12];
-        visc_flux_cell(K, dxi, deta, dzeta, w4, m + 1, m[0], out);
+        visc_flux_cell(K, P, dxi, deta, dzeta, w4, m + 1, m[0], out);
         const long long x = xc + k * sz;
 #pragma unroll
         for (int e = 0; e < 12; e++) B.fv[e][x] = out[e];
@@ -1123,7 +1136,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
     __shared__ double Vy[2][4][L][T];  // fv_eta, y halo
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
-    const PhysParams P = phys_of(K);
+    const PhysParams P = phys_of(K, B);
     const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
     const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
     const bool incell = ti < tw && tj < th;
@@ -1133,7 +1146,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
     const bool hx = threadIdx.x < 2 * H * T;  // x halo (fv_xi) or y halo (fv_eta)
     const long long xc = cidx(B, i, j, 0), xh = cidx(B, tl.i0 + hi, tl.j0 + hj, 0), sz = B.sz;
     const int kb = tl.k0, ke = tl.k0 + tl.kn;
-    const double dt = MODE == 1 ? stage_dt(K, stage, dtbuf, lam_cur, lam_next) : 0.0;
+    const double dt = MODE == 1 ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
     double r2 = 0.0;  // residual partial (stage 2)
@@ -1299,7 +1312,7 @@ __global__ __launch_bounds__(256) void KNAME(k_eigen)(const nuc3d_block_dev *__r
     constexpr int T = NUC3D_TILE;
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
-    const PhysParams P = phys_of(K);
+    const PhysParams P = phys_of(K, B);
     const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T;
     double lmax[3] = {0.0, 0.0, 0.0};
     if (i < B.n[0] && j < B.n[1]) {
@@ -1331,7 +1344,7 @@ __global__ __launch_bounds__(256) void KNAME(k_fixup)(const nuc3d_block_dev *__r
     constexpr int T = NUC3D_TILE;
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
-    const PhysParams P = phys_of(K);
+    const PhysParams P = phys_of(K, B);
     const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T;
     if (i >= B.n[0] || j >= B.n[1]) return;
     unsigned long long nfix = 0;

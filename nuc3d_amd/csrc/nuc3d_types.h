@@ -27,7 +27,13 @@ struct nuc3d_block_dev {
     int ftype[6];        // -1: physical face, else neighbour block index
     int fid[6];          // BC id (0 inlet, 1 outlet, 2 wall, 3 symmetric) or neighbour face
     double fval[6][5];   // inlet primitive state per face
-    unsigned char *hflag[3];  // hybrid decisions per face of each direction, or null (not recorded)
+    unsigned char *hflag[3];  // hybrid decisions per face of each hybrid direction, else null
+    // batch of independent cases: the block's case (group) takes its own dt
+    // (the minimum over that group's blocks), and its own Mach and Reynolds
+    // numbers (the case's by default)
+    int group;
+    int pad_g;
+    double mach, reynolds;
 };
 
 // Learned sensor of the hybrid reconstruction: 30 features -> 32 -> 32 -> 1.
@@ -69,6 +75,7 @@ struct nuc3d_kparams {
     int width[3];
     int zquirk;
     int nblocks;
+    int ngroups;    // independent cases among the blocks (dt per group)
     double *resid;  // stage 2: per-work-item partial sums of (Q - Q_n)^2, or null
     const nuc3d_sensor_dev *sensor;  // hybrid reconstruction: the network, or null
     // hybrid decision counts (central, WENO5-JS by the network, by the

@@ -113,6 +113,8 @@ def lib():
         "nuc3d_sync": [vp],
         "nuc3d_get_dt": [vp, dp],
         "nuc3d_get_max_eigen": [vp, ctypes.c_int, dp],
+        "nuc3d_set_block_group": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double],
+        "nuc3d_get_dt_group": [vp, ctypes.c_int, dp],
         "nuc3d_get_fixups": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "nuc3d_timing": [vp, ctypes.c_int],
         "nuc3d_timing_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
@@ -303,6 +305,15 @@ class Case:
     def dt(self):
         v = ctypes.c_double()
         _check(lib().nuc3d_get_dt(self.h, ctypes.byref(v)), "get_dt")
+        return v.value
+
+    def set_group(self, b, group, mach=0.0, reynolds=0.0):
+        """Block b belongs to independent case `group` (own dt, Mach and Reynolds numbers)."""
+        _check(lib().nuc3d_set_block_group(self.h, b, group, float(mach), float(reynolds)), "set_block_group")
+
+    def dt_group(self, group):
+        v = ctypes.c_double()
+        _check(lib().nuc3d_get_dt_group(self.h, group, ctypes.byref(v)), "get_dt_group")
         return v.value
 
     def max_eigen(self, bi=0):

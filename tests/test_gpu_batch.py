@@ -1,0 +1,88 @@
+"""Batches of independent cases in one case (block groups): each group takes
+its own dt and its own Mach and Reynolds numbers, and every group's result
+equals the same case run alone, bit for bit, and the oracle run on that case
+alone to the parity bar."""
+import numpy as np
+import pytest
+
+from nuc3d_amd import cases, solver
+from oracle import oracle as O
+from tests.parity import TOL, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(seed=3):
+    return cases.batch_cases(n_cases=4, lo=16, hi=28, seed=seed)
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+def test_batch_equals_cases_run_alone(exact):
+    members = _batch()
+    print("case: batch of %d: %s" % (len(members), ", ".join("%s %d^3 Re %.0f" % (m.kind, m.block.dims[0], m.reynolds)
+                                                            for m in members)), flush=True)
+    p = solver.Params(exact=exact)
+    g = solver.Case(p, [m.block for m in members])
+    for b, m in enumerate(members):
+        g.set_group(b, m.group, m.mach, m.reynolds)
+    g.steps_graph(2, 0.5)
+    g.steps(1, 0.5)
+    for b, m in enumerate(members):
+        blk = cases.BlockSpec(m.block.dims, cases.periodic_faces(0), m.block.jac, m.block.met, m.block.dxyz, m.block.q)
+        alone = solver.Case(solver.Params(exact=exact, mach=m.mach, reynolds=m.reynolds), [blk])
+        alone.steps(3, 0.5)
+        np.testing.assert_array_equal(g.get_q(b), alone.get_q(0))
+        assert g.dt_group(b) == alone.dt()
+        alone.close()
+    # the groups really take different dt (equal-sized blasts share theirs)
+    assert len({g.dt_group(b) for b in range(len(members))}) > 1
+    g.close()
+
+
+def test_batch_against_oracle():
+    members = _batch(seed=4)
+    print("case: batch of %d against the oracle per case" % len(members), flush=True)
+    g = solver.Case(solver.Params(), [m.block for m in members])
+    for b, m in enumerate(members):
+        g.set_group(b, m.group, m.mach, m.reynolds)
+    g.steps(2, 0.5)
+    for b, m in enumerate(members):
+        blk = O.Block(m.block.dims, cases.periodic_faces(0), m.block.jac, m.block.met, m.block.dxyz, m.block.q)
+        o = O.OracleCase(O.Params(mach=m.mach, reynolds=m.reynolds), [blk])
+        o.steps(2, 0.5)
+        err = relerr(g.get_q(b), o.get_q(0))
+        print("  %s %d^3: rel err %.3e" % (m.kind, m.block.dims[0], err), flush=True)
+        assert err <= TOL
+    g.close()
+
+
+def test_batch_hybrid():
+    """The learned hybrid over a batch: each group equals its case run alone."""
+    from nuc3d_amd import train_sensor
+    params, thr = train_sensor.load()
+    members = _batch(seed=5)
+    print("case: hybrid batch of %d" % len(members), flush=True)
+    p = solver.Params(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), exact=0)
+    g = solver.Case(p, [m.block for m in members])
+    g.set_sensor(params, thr)
+    for b, m in enumerate(members):
+        g.set_group(b, m.group, m.mach, m.reynolds)
+    g.steps_graph(2, 0.5)
+    for b, m in enumerate(members):
+        blk = cases.BlockSpec(m.block.dims, cases.periodic_faces(0), m.block.jac, m.block.met, m.block.dxyz, m.block.q)
+        alone = solver.Case(solver.Params(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), exact=0, mach=m.mach,
+                                          reynolds=m.reynolds), [blk])
+        alone.set_sensor(params, thr)
+        alone.steps(2, 0.5)
+        np.testing.assert_array_equal(g.get_q(b), alone.get_q(0))
+        alone.close()
+    g.close()
+
+
+def test_groups_cannot_be_connected():
+    spec = cases.taylor_green(dims=(24, 12, 12), seed=1)[0]
+    blocks = cases.split_block_x(spec, 2)
+    g = solver.Case(solver.Params(), blocks)
+    with pytest.raises(solver.Nuc3dError, match="group"):
+        g.set_group(1, 1)
+    g.close()
