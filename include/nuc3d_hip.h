@@ -189,6 +189,18 @@ int nuc3d_set_block_group(nuc3d_case *c, int b, int group, double mach, double r
 /* dt of group `group` in the current step (nuc3d_get_dt gives group 0's). */
 int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt);
 
+/* Autotuner (no counterpart in the reference).  The pointwise passes
+ * (primitives, viscous fluxes, viscous divergence, diagnostics) run over
+ * 16 x 16 column tiles that march along k in segments of kseg planes, 32 by
+ * default.  For Navier-Stokes cases this times the three passes of one RHS on
+ * one block of every distinct block shape with kseg = 8, 16, 32, 64, 128 (up to
+ * twice the block's depth), on the current state, and keeps the fastest per
+ * shape.  kseg_out (nblocks, may be NULL) receives the choices.  Results do
+ * not change, except the rounding of the diagnostics' partial sums; the
+ * residual history restarts (the next nuc3d_get_residual after one step is 0).
+ * Runs between full steps. */
+int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out);
+
 /* Wait for the case's work on its streams. */
 int nuc3d_sync(nuc3d_case *c);
 

@@ -22,10 +22,10 @@
                                                    nuc3d_kparams, int, int, int, int, double *,                      \
                                                    const unsigned long long *, unsigned long long *,                 \
                                                    unsigned long long *, hipStream_t);                               \
-    extern "C" hipError_t nuc3d_launch_diag_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, nuc3d_kparams,  \
-                                                  int, int, double *, double *, hipStream_t);                        \
-    extern "C" hipError_t nuc3d_launch_prim_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, nuc3d_kparams,  \
-                                                  int, hipStream_t);                                                 \
+    extern "C" hipError_t nuc3d_launch_diag_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, int,            \
+                                                  nuc3d_kparams, int, int, double *, double *, hipStream_t);         \
+    extern "C" hipError_t nuc3d_launch_prim_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, int,            \
+                                                  nuc3d_kparams, int, hipStream_t);                                  \
     extern "C" hipError_t nuc3d_launch_visc_flux_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int,           \
                                                        nuc3d_kparams, int, hipStream_t);                             \
     extern "C" hipError_t nuc3d_launch_visc_div_##SFX(int, const nuc3d_block_dev *, const nuc3d_tile *, int,        \
@@ -126,6 +126,8 @@ struct nuc3d_case {
     nuc3d_block_dev *db = nullptr;     // device descriptors
     nuc3d_tile *dtiles = nullptr;      // 16x16 column tiles with k segments (pointwise kernels)
     int ntiles = 0;
+    std::vector<int> kseg;             // k-segment length of the column tiles per block (nuc3d_autotune)
+    int kmax = NUC3D_KSEG;             // the longest
     int4 *dsweep[3] = {nullptr, nullptr, nullptr};  // direction-sweep tiles
     int nsweep[3] = {0, 0, 0};
     int4 *dlines[3] = {nullptr, nullptr, nullptr};  // whole lines (CRWENO5 directions)
@@ -268,7 +270,7 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
     if (ns) {
         {
             Timed t(c, s, &c->ev[3]);
-            HIPCHK(c->L->prim(c->db, c->dtiles, c->ntiles, K, bin, s));
+            HIPCHK(c->L->prim(c->db, c->dtiles, c->ntiles, c->kmax, K, bin, s));
         }
         Timed t(c, s, &c->ev[4]);
         HIPCHK(c->L->vflux(c->db, c->dtiles, c->ntiles, K, bin, s));
@@ -412,6 +414,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
             c->dflags.push_back(fm);
             B.hflag[d] = static_cast<unsigned char *>(fm);
         }
+        c->kseg.push_back(KSEG);
         for (int k0 = 0; k0 < B.n[2]; k0 += KSEG)
             for (int j0 = 0; j0 < B.n[1]; j0 += NUC3D_TILE)
                 for (int i0 = 0; i0 < B.n[0]; i0 += NUC3D_TILE) {
@@ -823,14 +826,14 @@ int nuc3d_diagnostics(nuc3d_case *c, double *enstrophy, double *kinetic, int b, 
     hipStream_t s = static_cast<hipStream_t>(stream);
     int rc = refresh(c, s);
     if (rc) return rc;
-    const long long nwg = (long long)c->ntiles * NUC3D_KSEG;
+    const long long nwg = (long long)c->ntiles * c->kmax;
     long long N = 0;
     if (fields5) N = (long long)c->hb[b].n[0] * c->hb[b].n[1] * c->hb[b].n[2];
     OpBuffers T;
     double *dpart = T.get(2 * nwg), *dfields = fields5 ? T.get(5 * N) : nullptr;
     if (!dpart || (fields5 && !dfields)) return fail(NUC3D_ERR_OOM, "device allocation failed");
-    HIPCHK(c->L->prim(c->db, c->dtiles, c->ntiles, c->K, c->cur, s));
-    HIPCHK(c->L->diag(c->db, c->dtiles, c->ntiles, c->K, c->cur, fields5 ? b : -1, dfields, dpart, s));
+    HIPCHK(c->L->prim(c->db, c->dtiles, c->ntiles, c->kmax, c->K, c->cur, s));
+    HIPCHK(c->L->diag(c->db, c->dtiles, c->ntiles, c->kmax, c->K, c->cur, fields5 ? b : -1, dfields, dpart, s));
     std::vector<double> part(2 * nwg);
     HIPCHK(hipMemcpyAsync(part.data(), dpart, sizeof(double) * 2 * nwg, hipMemcpyDeviceToHost, s));
     if (fields5) HIPCHK(hipMemcpyAsync(fields5, dfields, sizeof(double) * 5 * N, hipMemcpyDeviceToHost, s));
@@ -1026,6 +1029,147 @@ int nuc3d_timing_read(nuc3d_case *c, double *ms, int *launches)
     return NUC3D_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Autotuner of the column tiles (no counterpart in the reference)
+
+}  // extern "C"
+
+namespace {
+
+// column tiles of the given blocks with their k-segment lengths
+std::vector<nuc3d_tile> column_tiles(const nuc3d_case *c, const std::vector<int> &blocks, const std::vector<int> &kseg)
+{
+    std::vector<nuc3d_tile> tiles;
+    for (int b : blocks) {
+        const nuc3d_block_dev &B = c->hb[b];
+        const int ks = kseg[b];
+        for (int k0 = 0; k0 < B.n[2]; k0 += ks)
+            for (int j0 = 0; j0 < B.n[1]; j0 += NUC3D_TILE)
+                for (int i0 = 0; i0 < B.n[0]; i0 += NUC3D_TILE) {
+                    nuc3d_tile t = {b, i0, j0, k0, std::min(ks, B.n[2] - k0), 0, 0, 0};
+                    tiles.push_back(t);
+                }
+    }
+    return tiles;
+}
+
+// Replace the case's column tiles (and, for Navier-Stokes, the residual work
+// items that follow them; the residual history restarts).
+int set_column_tiles(nuc3d_case *c, const std::vector<int> &kseg)
+{
+    std::vector<int> all(c->nb);
+    for (int b = 0; b < c->nb; b++) all[b] = b;
+    const std::vector<nuc3d_tile> tiles = column_tiles(c, all, kseg);
+    nuc3d_tile *dt = nullptr;
+    if (hipMalloc(&dt, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess) return fail(NUC3D_ERR_OOM, "tile table");
+    HIPCHK(hipMemcpy(dt, tiles.data(), sizeof(nuc3d_tile) * tiles.size(), hipMemcpyHostToDevice));
+    const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
+    if (ns) {
+        double *dr = nullptr;
+        if (hipMalloc(&dr, sizeof(double) * 2 * tiles.size()) != hipSuccess) {
+            (void)hipFree(dt);
+            return fail(NUC3D_ERR_OOM, "residual partials");
+        }
+        HIPCHK(hipMemset(dr, 0, sizeof(double) * 2 * tiles.size()));
+        (void)hipFree(c->dresid);
+        c->dresid = dr;
+        c->work_block.clear();
+        for (const auto &t : tiles) c->work_block.push_back(t.b);
+        c->nwork = (int)tiles.size();
+    }
+    (void)hipFree(c->dtiles);
+    c->dtiles = dt;
+    c->ntiles = (int)tiles.size();
+    c->kseg = kseg;
+    c->kmax = *std::max_element(kseg.begin(), kseg.end());
+    for (auto &g : c->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    c->gcfl[0] = c->gcfl[1] = -1.0;
+    return NUC3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "autotune between full steps");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = refresh(c, s);
+    if (rc) return rc;
+    std::vector<int> kseg = c->kseg;
+    if (c->p.model == NUC3D_NAVIERSTOKES3D) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        std::vector<int> done(c->nb, 0);
+        for (int b = 0; b < c->nb && rc == 0; b++) {
+            if (done[b]) continue;
+            const nuc3d_block_dev &B = c->hb[b];
+            // one representative block per shape; candidates up to the block's depth
+            float best = 1e30f;
+            int bestk = kseg[b];
+            for (int ks : {8, 16, 32, 64, 128}) {
+                if (ks > 2 * B.n[2] && ks != 8) break;
+                std::vector<int> trial = kseg;
+                trial[b] = ks;
+                const std::vector<nuc3d_tile> tiles = column_tiles(c, {b}, trial);
+                nuc3d_tile *dt = nullptr;
+                if (hipMalloc(&dt, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess) {
+                    rc = fail(NUC3D_ERR_OOM, "tile table");
+                    break;
+                }
+                hipError_t e = hipMemcpy(dt, tiles.data(), sizeof(nuc3d_tile) * tiles.size(), hipMemcpyHostToDevice);
+                const int nt = (int)tiles.size();
+                // the pointwise passes of one RHS over the block: primitives,
+                // viscous fluxes, viscous divergence into the RHS (no state change)
+                auto pass = [&]() {
+                    hipError_t r = c->L->prim(c->db, dt, nt, ks, c->K, c->cur, s);
+                    if (r == hipSuccess) r = c->L->vflux(c->db, dt, nt, c->K, c->cur, s);
+                    if (r == hipSuccess)
+                        r = c->L->vdiv(0, c->db, dt, nt, c->K, 0, c->cur, 0, 0, c->ddt, c->dlam, c->dlam, c->dcount, s);
+                    return r;
+                };
+                if (e == hipSuccess) e = pass();
+                if (e == hipSuccess) e = hipEventRecord(e0, s);
+                for (int r = 0; r < 3 && e == hipSuccess; r++) e = pass();
+                if (e == hipSuccess) e = hipEventRecord(e1, s);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                (void)hipFree(dt);
+                if (e != hipSuccess) {
+                    rc = hipfail(e, "autotune");
+                    break;
+                }
+                if (ms < best) {
+                    best = ms;
+                    bestk = ks;
+                }
+            }
+            for (int b2 = b; b2 < c->nb; b2++) {
+                const nuc3d_block_dev &B2 = c->hb[b2];
+                if (B2.n[0] == B.n[0] && B2.n[1] == B.n[1] && B2.n[2] == B.n[2]) {
+                    kseg[b2] = bestk;
+                    done[b2] = 1;
+                }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        if ((rc = set_column_tiles(c, kseg))) return rc;
+    }
+    if (kseg_out)
+        for (int b = 0; b < c->nb; b++) kseg_out[b] = c->kseg[b];
+    return NUC3D_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Learned sensor of the hybrid reconstruction (no counterpart in the reference)

@@ -1496,10 +1496,11 @@ extern "C" hipError_t KNAME(nuc3d_launch_crweno)(int dir, int ns, int mode, cons
 #undef NUC3D_A
 }
 
+// kmax: the longest k segment of the tiles (one workgroup per tile and plane)
 extern "C" hipError_t KNAME(nuc3d_launch_prim)(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles, int ntiles,
-                                               nuc3d_kparams K, int bin, hipStream_t st)
+                                               int kmax, nuc3d_kparams K, int bin, hipStream_t st)
 {
-    hipLaunchKernelGGL(KNAME(k_prim), dim3(ntiles, NUC3D_KSEG), dim3(256), 0, st, blocks, tiles, K, bin);
+    hipLaunchKernelGGL(KNAME(k_prim), dim3(ntiles, kmax), dim3(256), 0, st, blocks, tiles, K, bin);
     return hipGetLastError();
 }
 
@@ -1549,10 +1550,10 @@ extern "C" hipError_t KNAME(nuc3d_launch_ghost_fill)(const nuc3d_block_dev *bloc
 }
 
 extern "C" hipError_t KNAME(nuc3d_launch_diag)(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles, int ntiles,
-                                               nuc3d_kparams K, int bin, int bf, double *fields, double *partial,
-                                               hipStream_t st)
+                                               int kmax, nuc3d_kparams K, int bin, int bf, double *fields,
+                                               double *partial, hipStream_t st)
 {
-    hipLaunchKernelGGL(KNAME(k_diag), dim3(ntiles, NUC3D_KSEG), dim3(256), 0, st, blocks, tiles, K, bin, bf, fields,
+    hipLaunchKernelGGL(KNAME(k_diag), dim3(ntiles, kmax), dim3(256), 0, st, blocks, tiles, K, bin, bf, fields,
                        partial);
     return hipGetLastError();
 }

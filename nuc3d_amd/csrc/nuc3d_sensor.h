@@ -5,9 +5,9 @@
 // smooth flow and WENO5-JS elsewhere.  A small MLP decides:
 //   input   30 features: rho, u, v, w, p on the six cells m-3..m+2 around
 //           face m, each variable centred on its stencil mean and divided by
-//           (stencil range + 0.01 S), with S = mean rho for rho, mean p for p
-//           and the sound speed of the means for the velocities; computed in
-//           FP32 from the FP32 roundings of the primitives;
+//           (stencil range + 0.01 S), with the acoustic scales S of
+//           feature_scales; computed in FP32 from the FP32 roundings of the
+//           primitives;
 //   layers  30 -> 32 -> 32 -> 1, ReLU, BF16 weights and activations, FP32
 //           accumulation on the matrix cores (v_mfma_f32_16x16x32_bf16);
 //   output  a logit; WENO5-JS when it reaches the threshold, when the stencil
@@ -111,8 +111,13 @@ __device__ __forceinline__ float stencil_mean(const Get &getf, int v)
     return s * (1.0f / 6.0f);
 }
 
-// Scales S of the five variables: mean rho, the sound speed of the means
-// (three velocities) and mean p.  False when mean rho or mean p is not
+// Scales S of the five variables from the stencil's means and a velocity
+// scale U = (largest |velocity component| over the stencil) + 0.1 c, with c
+// the sound speed of the means: U for the velocities, rho c U for the
+// pressure and rho U / c for the density (the acoustic relations
+// dp ~ rho c du, drho ~ rho du / c, so that a tolerance tau S is the same
+// disturbance in every variable, and low-Mach flows are not judged on the
+// scale of the mean pressure).  False when mean rho or mean p is not
 // positive (the stencil is invalid; the face takes WENO5-JS).
 template <class Get>
 __device__ __forceinline__ bool feature_scales(const Get &getf, float gamma, float *S)
@@ -120,11 +125,17 @@ __device__ __forceinline__ bool feature_scales(const Get &getf, float gamma, flo
     const float rb = stencil_mean(getf, 0), pb = stencil_mean(getf, 4);
     if (!(rb > 0.0f && pb > 0.0f)) return false;
     const float cb = sqrtf(gamma * pb / rb);
-    S[0] = rb;
-    S[1] = cb;
-    S[2] = cb;
-    S[3] = cb;
-    S[4] = pb;
+    float um = 0.0f;
+#pragma unroll
+    for (int v = 1; v < 4; v++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) um = fmaxf(um, fabsf(getf(v, j)));
+    const float U = um + 0.1f * cb;
+    S[0] = rb * U / cb;
+    S[1] = U;
+    S[2] = U;
+    S[3] = U;
+    S[4] = rb * cb * U;
     return true;
 }
 
@@ -168,9 +179,11 @@ __device__ __forceinline__ bool features(const Get &getf, float gamma, unsigned 
 
 // Teacher label of face m from get8(v, j) = primitive v at cell m-4+j,
 // j = 0..7 (FP64): 1 when, for some primitive, the central face value differs
-// from the WENO5-JS value of either upwind side by more than tau S, with S =
-// mean rho (rho), mean p (p) and sqrt(gamma mean p / mean rho) (velocities)
-// over cells m-3..m+2.  -1 when mean rho or mean p is not positive.
+// from the WENO5-JS value of either upwind side by more than tau S, with the
+// scales S of feature_scales in FP64 over cells m-3..m+2 (mean rho and p,
+// c = sqrt(gamma mean p / mean rho), U = max |velocity component| + 0.1 c;
+// S = rho U / c, U, U, U, rho c U).  -1 when mean rho or mean p is not
+// positive.
 template <class Get8>
 __device__ __forceinline__ int teacher(const Get8 &get8, double gamma, double tau)
 {
@@ -185,7 +198,13 @@ __device__ __forceinline__ int teacher(const Get8 &get8, double gamma, double ta
     const double rb = mean[0], pb = mean[4];
     if (!(rb > 0.0 && pb > 0.0)) return -1;
     const double cb = sqrt(gamma * pb / rb);
-    const double S[5] = {rb, cb, cb, cb, pb};
+    double um = 0.0;
+#pragma unroll
+    for (int v = 1; v < 4; v++)
+#pragma unroll
+        for (int j = 1; j < 7; j++) um = fmax(um, fabs(get8(v, j)));
+    const double U = um + 0.1 * cb;
+    const double S[5] = {rb * U / cb, U, U, U, rb * cb * U};
     double dev = 0.0;
 #pragma unroll
     for (int v = 0; v < 5; v++) {

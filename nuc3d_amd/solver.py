@@ -115,6 +115,7 @@ def lib():
         "nuc3d_get_max_eigen": [vp, ctypes.c_int, dp],
         "nuc3d_set_block_group": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double],
         "nuc3d_get_dt_group": [vp, ctypes.c_int, dp],
+        "nuc3d_autotune": [vp, vp, ip],
         "nuc3d_get_fixups": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "nuc3d_timing": [vp, ctypes.c_int],
         "nuc3d_timing_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
@@ -310,6 +311,13 @@ class Case:
     def set_group(self, b, group, mach=0.0, reynolds=0.0):
         """Block b belongs to independent case `group` (own dt, Mach and Reynolds numbers)."""
         _check(lib().nuc3d_set_block_group(self.h, b, group, float(mach), float(reynolds)), "set_block_group")
+
+    def autotune(self):
+        """Times the pointwise passes per block shape and keeps the fastest k
+        segment of the column tiles; returns the choice per block."""
+        out = (ctypes.c_int * len(self.blocks))()
+        _check(lib().nuc3d_autotune(self.h, self.stream, out), "autotune")
+        return list(out)
 
     def dt_group(self, group):
         v = ctypes.c_double()

@@ -6,13 +6,14 @@ product.  The sensor has no counterpart in the reference; what it restates is
 this build's own definition:
 
   features  rho, u, v, w, p on the six cells m-3..m+2 of face m, each centred
-            on its stencil mean and divided by (range + 0.01 S), S = mean rho
-            (rho), mean p (p), sqrt(gamma mean p / mean rho) (velocities);
-            FP32 on the FP32 roundings of the primitives, then bf16 (round
-            to nearest even);
+            on its stencil mean and divided by (range + 0.01 S), with the
+            acoustic scales S = (rho U / c, U, U, U, rho c U) of the stencil's
+            mean rho and p, c = sqrt(gamma p / rho) and U = max |velocity
+            component| + 0.1 c; FP32 on the FP32 roundings of the
+            primitives, then bf16 (round to nearest even);
   teacher   1 when, for some primitive, the eighth-order central face value
             and the WENO5-JS value (weno5js.cpp:75-108 order) of either upwind
-            side differ by more than tau S;
+            side differ by more than tau S (the same scales in FP64);
   network   30 -> 32 -> 32 -> 1, ReLU, bf16 weights and activations (both
             hidden layers), FP32 accumulation with the biases as initial
             values; loss wt (softplus(z) - y z).
@@ -110,8 +111,10 @@ def central8f(c0, c1, c2, c3, c4, c5, c6, c7):
 
 
 def stencil_scales(P6, gamma):
-    """FP64 means of rho and p over the stencil P6[v][j] (v = rho,u,v,w,p;
-    j = cells m-3..m+2), the scales S and the validity mask (the teacher's)."""
+    """FP64 scales S of the teacher over the stencil P6[v][j] (v = rho,u,v,w,p;
+    j = cells m-3..m+2) and the validity mask: mean rho and p, c = sqrt(gamma
+    mean p / mean rho), U = max |velocity component| + 0.1 c, and
+    S = (rho U / c, U, U, U, rho c U)."""
     mean = {}
     for v in (0, 4):
         s = P6[v][0]
@@ -122,15 +125,22 @@ def stencil_scales(P6, gamma):
     ok = (rb > 0.0) & (pb > 0.0)
     with np.errstate(invalid="ignore", divide="ignore"):
         cb = np.sqrt(gamma * pb / rb)
-    return [rb, cb, cb, cb, pb], ok
+        um = np.zeros(np.shape(rb))
+        for v in (1, 2, 3):
+            for j in range(6):
+                um = np.maximum(um, np.abs(P6[v][j]))
+        U = um + 0.1 * cb
+        S = [rb * U / cb, U, U, U, rb * cb * U]
+    return S, ok
 
 
 def features(P6, gamma):
     """bf16 bit patterns (..., 32) of the features of stencils P6 (5, 6, ...)
     of FP64 primitives, and the validity mask; invalid stencils get zero
     features.  FP32 arithmetic on the FP32 roundings of the primitives:
-    mean = sum * (1/6), S = (mean rho, c, c, c, mean p) with
-    c = sqrt(gamma mean p / mean rho), x = (P - mean) * (1 / (range + 0.01 S))."""
+    mean = sum * (1/6), c = sqrt(gamma mean p / mean rho),
+    U = max |velocity component| + 0.1 c, S = (rho U / c, U, U, U, rho c U),
+    x = (P - mean) * (1 / (range + 0.01 S))."""
     P = [[np.asarray(P6[v][j], dtype=np.float64).astype(_F) for j in range(6)] for v in range(5)]
     g = _F(gamma)
     sixth = _F(1.0) / _F(6.0)
@@ -145,7 +155,12 @@ def features(P6, gamma):
     ok = (rb > 0) & (pb > 0)
     with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
         cb = np.sqrt(g * pb / rb)
-        S = [rb, cb, cb, cb, pb]
+        um = np.zeros(np.shape(rb), dtype=_F)
+        for v in (1, 2, 3):
+            for j in range(6):
+                um = np.maximum(um, np.abs(P[v][j]))
+        U = um + _F(0.1) * cb
+        S = [rb * U / cb, U, U, U, rb * cb * U]
         shape = np.shape(P[0][0])
         x = np.zeros(shape + (32,), dtype=np.uint16)
         for v in range(5):

@@ -86,3 +86,33 @@ def test_groups_cannot_be_connected():
     with pytest.raises(solver.Nuc3dError, match="group"):
         g.set_group(1, 1)
     g.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+def test_autotune_keeps_results(exact):
+    """The autotuner only changes the k segments of the column tiles: steps
+    after it equal steps of an untuned case bit for bit (uneven batch and a
+    two-block connected case)."""
+    members = cases.batch_cases(n_cases=3, lo=16, hi=40, seed=7)
+    spec = cases.taylor_green(dims=(24, 20, 36), noise=1e-3, seed=8)[0]
+    two = cases.split_block_x(spec, 2)
+    for label, blocks, groups in (("batch", [m.block for m in members], members), ("two blocks", two, None)):
+        print("case: autotune, %s, exact=%d" % (label, exact), flush=True)
+        a = solver.Case(solver.Params(exact=exact), blocks)
+        b = solver.Case(solver.Params(exact=exact), blocks)
+        if groups:
+            for i, m in enumerate(groups):
+                a.set_group(i, m.group, m.mach, m.reynolds)
+                b.set_group(i, m.group, m.mach, m.reynolds)
+        ks = a.autotune()
+        print("  k segments %s" % ks, flush=True)
+        assert all(k in (8, 16, 32, 64, 128) for k in ks)
+        a.steps_graph(2, 0.5)
+        b.steps_graph(2, 0.5)
+        for i in range(len(blocks)):
+            np.testing.assert_array_equal(a.get_q(i), b.get_q(i))
+        ea, ka, _ = a.diagnostics(0)
+        eb, kb, _ = b.diagnostics(0)
+        assert abs(ea - eb) <= 1e-12 * abs(eb) and abs(ka - kb) <= 1e-12 * abs(kb)
+        a.close()
+        b.close()

@@ -38,12 +38,14 @@ def test_features_closed_forms():
     # (a few ulps of the value over 0.01 S), valid
     x, ok = S.features(_stencil([[rho] * 6, [0.2] * 6, [0.0] * 6, [-0.1] * 6, [p] * 6]), g)
     assert ok and np.abs(S.bf16_value(x)).max() < 1e-4
-    # linear ramp in u with slope s: x_j = s (j - 2.5) / (5|s| + 0.01 c)
+    # linear ramp in u with slope s: x_j = s (j - 2.5) / (5|s| + 0.01 U), with
+    # U = max |u| + 0.1 c the velocity scale
     s = 0.05
     u = [0.3 + s * j for j in range(6)]
     x, ok = S.features(_stencil([[rho] * 6, u, [0.0] * 6, [0.0] * 6, [p] * 6]), g)
     c = np.sqrt(g * p / rho)
-    want = [s * (j - 2.5) / (5 * s + 0.01 * c) for j in range(6)]
+    U = max(u) + 0.1 * c
+    want = [s * (j - 2.5) / (5 * s + 0.01 * U) for j in range(6)]
     np.testing.assert_allclose(S.bf16_value(x[6:12]), want, rtol=2 ** -8)
     assert np.abs(S.bf16_value(x[:6])).max() < 1e-4 and np.abs(S.bf16_value(x[12:])).max() < 1e-4
     # non-positive mean density: invalid, zero features
