@@ -169,67 +169,90 @@ def time_method(case, steps, warmup, cells, cfl=0.5):
     return cells * steps / elapsed, 1e3 * elapsed / steps, tm
 
 
-def bench_batch(args):
-    """48 independent cases advanced together (one block group each), plain
-    WENO5-JS and the learned hybrid from the same developed states, at CFL 0.2:
-    with the reference's local Lax-Friedrichs splitting the blasts need e<0
-    fix-ups from the first steps at CFL 0.5, and the Navier-Stokes blast of
-    136^3 diverges there (it does on the plain path, as the reference's
-    scheme)."""
-    cfl = 0.2
+def bench_multiblock(args):
+    """Multi-block workloads, plain WENO5-JS and the learned hybrid from the
+    same developed states:
+      batch48    48 independent cases (one block group each) at CFL 0.2: with
+                 the reference's local Lax-Friedrichs splitting the blasts need
+                 e<0 fix-ups from the first steps at CFL 0.5, and the
+                 Navier-Stokes blast of 136^3 diverges there (on the plain path,
+                 which is the reference's scheme);
+      flatplate  the 16-block supersonic flat plate over a bump (60.5 M cells,
+                 curvilinear metrics computed on the GPU from the mesh) at CFL
+                 0.5."""
     import numpy as np
     from nuc3d_amd import cases, solver, train_sensor
 
-    members = cases.batch_cases(n_cases=48, lo=32, hi=160, seed=0)
-    blocks = [m.block for m in members]
-    cells = int(sum(np.prod(b.dims) for b in blocks))
-    log("bench: batch of %d cases, %d cells" % (len(blocks), cells))
     exact = 0 if args.fast else 1
+    if args.workload == "batch48":
+        members = cases.batch_cases(n_cases=48, lo=32, hi=160, seed=0)
+        blocks = [m.block for m in members]
+        phys, cfl = {}, 0.2
+        workload = ("batch of 48 independent cases (24 TGV Re 400-3200, 24 spherical blasts), cubes of seeded "
+                    "sizes 32^3..160^3, one block group each, Navier-Stokes, CFL 0.2")
+    else:
+        members = None
+        blocks = cases.flat_plate_blocks()
+        phys, cfl = dict(mach=2.9, reynolds=2.0e4, t_wall=2.5), 0.5
+        workload = ("supersonic flat plate over a Gaussian bump, Ma 2.9, Re_delta 2e4, isothermal wall T_w 2.5, "
+                    "16 blocks of seeded uneven sizes (60.5 M cells), curvilinear tanh-stretched mesh, "
+                    "Navier-Stokes, CFL 0.5")
+    cells = int(sum(np.prod(b.dims) for b in blocks))
+    log("bench: %s: %d blocks, %d cells" % (args.workload, len(blocks), cells))
 
-    def make(params, bl):
+    def make(params, bl, state=None):
         c = solver.Case(params, bl)
-        for i, m in enumerate(members):
-            c.set_group(i, m.group, m.mach, m.reynolds)
+        if members:
+            for i, m in enumerate(members):
+                c.set_group(i, m.group, m.mach, m.reynolds)
+        else:
+            cases.setup_flat_plate(c, bl)
+        if state is not None:
+            for i, q in enumerate(state):
+                c.set_q(i, q)
         return c
 
-    plain = make(solver.Params(exact=exact), blocks)
+    plain = make(solver.Params(exact=exact, **phys), blocks)
     kseg = plain.autotune()
-    log("bench: autotuned k segments per case: %s" % kseg)
+    log("bench: autotuned k segments per block: %s" % kseg)
     if args.develop:
         plain.steps_graph(args.develop, cfl)
         plain.sync()
-    start = [cases.BlockSpec(b.dims, b.faces, b.jac, b.met, b.dxyz, plain.get_q(i)) for i, b in enumerate(blocks)]
+    start = [plain.get_q(i) for i in range(len(blocks))]
     v_plain, ms_plain, _ = time_method(plain, args.steps, args.warmup, cells, cfl)
-    log("bench: batch plain %.3f ms/step, %.4g cell-updates/s" % (ms_plain, v_plain))
+    log("bench: %s plain %.3f ms/step, %.4g cell-updates/s" % (args.workload, ms_plain, v_plain))
     params, thr = train_sensor.load()
-    hyb = make(solver.Params(exact=exact, scheme=(solver.HYBRID,) * 3, width=(4, 4, 4)), start)
+    hyb = make(solver.Params(exact=exact, scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), **phys), blocks, start)
     hyb.set_sensor(params, thr)
     hyb.autotune()
     v_h, ms_h, _ = time_method(hyb, args.steps, args.warmup, cells, cfl)
     cen, net, fb = hyb.hybrid_counts()
     num = den = 0.0
-    for i, b in enumerate(blocks):
-        rp = plain.get_q(i)[0] * b.jac
-        rh = hyb.get_q(i)[0] * b.jac
+    finite = True
+    for i in range(len(blocks)):
+        jac = plain.get_metrics(i)[0] if members is None else blocks[i].jac
+        rp = plain.get_q(i)[0] * jac
+        rh = hyb.get_q(i)[0] * jac
+        finite = finite and bool(np.isfinite(rp).all() and np.isfinite(rh).all())
         num += float(np.sum((rh - rp) ** 2))
         den += float(np.sum(rp ** 2))
     dev = float(np.sqrt(num / den))
-    log("bench: batch hybrid %.3f ms/step, %.4g cell-updates/s (x%.3f), central %.3f, L2 density deviation %.2e "
-        "of rms" % (ms_h, v_h, v_h / v_plain, cen / max(cen + net + fb, 1), dev))
+    log("bench: %s hybrid %.3f ms/step, %.4g cell-updates/s (x%.3f), central %.3f, L2 density deviation %.2e "
+        "of rms" % (args.workload, ms_h, v_h, v_h / v_plain, cen / max(cen + net + fb, 1), dev))
     plain.close()
     hyb.close()
-    value, method, ms = (v_h, "hybrid", ms_h) if (dev <= 1e-3 and v_h > v_plain) else (v_plain, "plain", ms_plain)
+    ok = finite and dev <= 1e-3
+    value, method, ms = (v_h, "hybrid", ms_h) if (ok and v_h > v_plain) else (v_plain, "plain", ms_plain)
     print(json.dumps({
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "batch of 48 independent cases (24 TGV Re 400-3200, 24 spherical blasts), cubes of "
-                               "seeded sizes 32^3..160^3, one block group each, Navier-Stokes, CFL 0.2, %d plain "
-                               "steps developed: plain WENO5-JS vs learned hybrid" % args.develop,
-                   "cells": cells, "variant": "fast" if args.fast else "exact", "value_method": method},
+        "config": {"workload": workload + ", %d plain steps developed: plain WENO5-JS vs learned hybrid" % args.develop,
+                   "cells": cells, "blocks": len(blocks), "variant": "fast" if args.fast else "exact",
+                   "value_method": method, "column_tile_k_segments": kseg},
         "plain": {"value": v_plain, "ms_per_step": ms_plain},
         "hybrid": {"value": v_h, "ms_per_step": ms_h, "speedup_vs_plain": v_h / v_plain,
-                   "central_fraction": cen / max(cen + net + fb, 1), "l2_over_density_rms": dev},
+                   "central_fraction": cen / max(cen + net + fb, 1), "l2_over_density_rms": dev, "finite": finite},
     }), flush=True)
 
 
@@ -244,9 +267,10 @@ def main():
     ap.add_argument("--fast", action="store_true",
                     help="contracted WENO arithmetic (within 1e-12 of the reference on well-conditioned fields only)")
     ap.add_argument("--no-hybrid", action="store_true")
-    ap.add_argument("--workload", choices=("turbulence", "batch48"), default="turbulence",
+    ap.add_argument("--workload", choices=("turbulence", "batch48", "flatplate"), default="turbulence",
                     help="batch48: 48 independent TGV and blast cases of seeded sizes 32^3..160^3 in one case "
-                         "(block groups), plain WENO5-JS and the learned hybrid")
+                         "(block groups); flatplate: the 16-block supersonic flat plate over a bump (60.5 M "
+                         "cells); both plain WENO5-JS and the learned hybrid")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
@@ -262,8 +286,8 @@ def main():
     import numpy as np
     from nuc3d_amd import cases, solver, train_sensor
 
-    if args.workload == "batch48":
-        return bench_batch(args)
+    if args.workload != "turbulence":
+        return bench_multiblock(args)
     n = args.n
     cells = n ** 3
     log("bench: generating %d^3 isotropic turbulence" % n)

@@ -192,10 +192,11 @@ int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt);
 /* Autotuner (no counterpart in the reference).  The pointwise passes
  * (primitives, viscous fluxes, viscous divergence, diagnostics) run over
  * 16 x 16 column tiles that march along k in segments of kseg planes, 32 by
- * default.  For Navier-Stokes cases this times the three passes of one RHS on
- * one block of every distinct block shape with kseg = 8, 16, 32, 64, 128 (up to
- * twice the block's depth), on the current state, and keeps the fastest per
- * shape.  kseg_out (nblocks, may be NULL) receives the choices.  Results do
+ * default.  For Navier-Stokes cases this takes the distinct block shapes in
+ * turn and, for kseg = 8, 16, 32, 64, 128 (up to twice the block's depth) on
+ * every block of that shape, the other blocks keeping their current choice,
+ * times the three passes of one RHS over the whole case on the current state;
+ * it keeps the fastest per shape.  kseg_out (nblocks, may be NULL) receives the choices.  Results do
  * not change, except the rounding of the diagnostics' partial sums; the
  * residual history restarts (the next nuc3d_get_residual after one step is 0).
  * Runs between full steps. */

@@ -27,6 +27,10 @@ class BlockSpec:
     met: np.ndarray                     # (9, nz, ny, nx): xi_x..zeta_z
     dxyz: np.ndarray                    # (9, nz, ny, nx): x_xi,x_eta,x_zeta,y_xi..z_zeta
     q: np.ndarray                       # (5, nz, ny, nx): conserved / J
+    # node coordinates (xn, yn, zn), each (nz+1, ny+1, nx+1), for blocks whose
+    # metrics are computed on the GPU from the mesh (jac, met, dxyz and q may
+    # then be None until the state is set)
+    nodes: Optional[tuple] = None
 
 
 @dataclass
@@ -339,3 +343,85 @@ def batch_cases(n_cases=48, lo=32, hi=160, seed=0, step=4) -> List[BatchMember]:
         b.faces = periodic_faces(c)
         out.append(BatchMember(b, c, mach, re, kind))
     return out
+
+
+def flat_plate(sizes_x, sizes_z, ny, delta=1.0, lx=None, ly=None, lz=None, bump_height=0.5, bump_width=2.0,
+               beta=2.5, mach=2.9):
+    """Multiblock supersonic flat-plate boundary layer over a Gaussian bump:
+    len(sizes_x) x len(sizes_z) blocks (x index fastest) of sizes_x[i] x ny x
+    sizes_z[k] cells on a curvilinear mesh.  The wall y = h(x, z) carries the
+    bump h = bump_height * delta * exp(-((x-x0)^2 + (z-z0)^2) / (bump_width *
+    delta)^2) at x0 = 0.3 lx, z0 = lz / 2; wall-normal nodes are clustered at
+    the wall by tanh stretching, y = h + (ly - h) (1 - tanh(beta (1 - eta)) /
+    tanh(beta)).  Faces: x- inflow (free stream rho = 1, u = 1, p = 1/(gamma
+    Ma^2)), x+ outflow, y- isothermal wall, y+ outflow, z periodic; blocks are
+    joined face to face.  Every block carries its nodes; metrics and state are
+    set by setup_flat_plate."""
+    nbx, nbz = len(sizes_x), len(sizes_z)
+    nxt, nzt = int(sum(sizes_x)), int(sum(sizes_z))
+    lx = lx if lx is not None else 20.0 * delta
+    ly = ly if ly is not None else 5.0 * delta
+    lz = lz if lz is not None else 10.0 * delta
+    gamma = 1.4
+    inflow = [1.0, 1.0, 0.0, 0.0, 1.0 / (gamma * mach * mach)]
+    x0, z0, w = 0.3 * lx, 0.5 * lz, bump_width * delta
+    xs = np.concatenate([[0], np.cumsum(sizes_x)])
+    zs = np.concatenate([[0], np.cumsum(sizes_z)])
+    eta = np.arange(ny + 1) / ny
+    stretch = 1.0 - np.tanh(beta * (1.0 - eta)) / np.tanh(beta)
+    blocks = []
+    for k in range(nbz):
+        for i in range(nbx):
+            b = k * nbx + i
+            xn = (xs[i] + np.arange(sizes_x[i] + 1)) * (lx / nxt)
+            zn = (zs[k] + np.arange(sizes_z[k] + 1)) * (lz / nzt)
+            Z, E, X = np.meshgrid(zn, stretch, xn, indexing="ij")
+            h = bump_height * delta * np.exp(-((X - x0) ** 2 + (Z - z0) ** 2) / (w * w))
+            Y = h + (ly - h) * E
+            left = (-1, BC_INLET, inflow) if i == 0 else (b - 1, 1, None)
+            right = (-1, BC_OUTLET, None) if i == nbx - 1 else (b + 1, 0, None)
+            back = (k - 1) % nbz * nbx + i
+            front = (k + 1) % nbz * nbx + i
+            faces = [left, right, (-1, BC_WALL, None), (-1, BC_OUTLET, None), (back, 5, None), (front, 4, None)]
+            blocks.append(BlockSpec((int(sizes_x[i]), int(ny), int(sizes_z[k])), faces, None, None, None, None,
+                                    nodes=(X, Y, Z)))
+    return blocks
+
+
+def flat_plate_blocks(seed=3, lo=96, hi=192, ny=192, nbx=4, nbz=4):
+    """The 16-block flat plate of seeded uneven sizes (multiples of 4 in
+    [lo, hi] per x column and z row, ny wall-normal cells): 60.5 M cells with
+    the defaults (seed 3: x columns 176, 104, 112, 116; z rows 112, 176, 180,
+    152)."""
+    rng = np.random.default_rng(seed)
+    sx = rng.integers(lo // 4, hi // 4 + 1, nbx) * 4
+    sz = rng.integers(lo // 4, hi // 4 + 1, nbz) * 4
+    return flat_plate(sx, sz, ny)
+
+
+def flat_plate_state(centre, jac, delta=1.0, mach=2.9, t_wall=2.5, gamma=1.4, y_wall=None):
+    """Initial conserved/J of a flat-plate block from its cell centres
+    (3, nz, ny, nx) and J: u = tanh(3 d / delta) with d the height above the
+    wall at the cell's column, T = T_w + (1 - T_w) u (the wall at T_w, the
+    free stream at 1), uniform p = 1/(gamma Ma^2), rho = gamma Ma^2 p / T."""
+    y = centre[1]
+    d = y - (y_wall if y_wall is not None else y[:, :1, :])
+    d = np.maximum(d, 0.0)
+    u = np.tanh(3.0 * d / delta)
+    T = t_wall + (1.0 - t_wall) * u
+    p = np.full_like(u, 1.0 / (gamma * mach * mach))
+    rho = gamma * mach * mach * p / T
+    z0 = np.zeros_like(u)
+    return conserved_over_j(rho, u, z0, z0, p, gamma, jac)
+
+
+def setup_flat_plate(case, blocks, mach=2.9, t_wall=2.5):
+    """Metrics of every block from its nodes on the GPU (nuc3d_upload_mesh),
+    then the initial state from the cell centres the GPU returns.  The wall
+    height of each column is the height of the node below its first cell."""
+    for b, spec in enumerate(blocks):
+        centre = case.upload_mesh(b, *spec.nodes)
+        jac = case.get_metrics(b)[0]
+        yn = spec.nodes[1]
+        wall = 0.25 * (yn[:-1, :1, :-1] + yn[1:, :1, :-1] + yn[:-1, :1, 1:] + yn[1:, :1, 1:])
+        case.set_q(b, flat_plate_state(centre, jac, mach=mach, t_wall=t_wall, y_wall=wall))

@@ -1112,14 +1112,24 @@ int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out)
         for (int b = 0; b < c->nb && rc == 0; b++) {
             if (done[b]) continue;
             const nuc3d_block_dev &B = c->hb[b];
-            // one representative block per shape; candidates up to the block's depth
+            // every block of this shape takes the candidate, the others keep
+            // their current choice, and the whole case's passes are timed: the
+            // choice is made in the context of the blocks it runs with
+            std::vector<int> same;
+            for (int b2 = b; b2 < c->nb; b2++) {
+                const nuc3d_block_dev &B2 = c->hb[b2];
+                if (B2.n[0] == B.n[0] && B2.n[1] == B.n[1] && B2.n[2] == B.n[2]) same.push_back(b2);
+            }
+            std::vector<int> all(c->nb);
+            for (int b2 = 0; b2 < c->nb; b2++) all[b2] = b2;
             float best = 1e30f;
             int bestk = kseg[b];
             for (int ks : {8, 16, 32, 64, 128}) {
                 if (ks > 2 * B.n[2] && ks != 8) break;
                 std::vector<int> trial = kseg;
-                trial[b] = ks;
-                const std::vector<nuc3d_tile> tiles = column_tiles(c, {b}, trial);
+                for (int b2 : same) trial[b2] = ks;
+                const int kmx = *std::max_element(trial.begin(), trial.end());
+                const std::vector<nuc3d_tile> tiles = column_tiles(c, all, trial);
                 nuc3d_tile *dt = nullptr;
                 if (hipMalloc(&dt, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess) {
                     rc = fail(NUC3D_ERR_OOM, "tile table");
@@ -1130,7 +1140,7 @@ int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out)
                 // the pointwise passes of one RHS over the block: primitives,
                 // viscous fluxes, viscous divergence into the RHS (no state change)
                 auto pass = [&]() {
-                    hipError_t r = c->L->prim(c->db, dt, nt, ks, c->K, c->cur, s);
+                    hipError_t r = c->L->prim(c->db, dt, nt, kmx, c->K, c->cur, s);
                     if (r == hipSuccess) r = c->L->vflux(c->db, dt, nt, c->K, c->cur, s);
                     if (r == hipSuccess)
                         r = c->L->vdiv(0, c->db, dt, nt, c->K, 0, c->cur, 0, 0, c->ddt, c->dlam, c->dlam, c->dcount, s);
@@ -1153,12 +1163,9 @@ int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out)
                     bestk = ks;
                 }
             }
-            for (int b2 = b; b2 < c->nb; b2++) {
-                const nuc3d_block_dev &B2 = c->hb[b2];
-                if (B2.n[0] == B.n[0] && B2.n[1] == B.n[1] && B2.n[2] == B.n[2]) {
-                    kseg[b2] = bestk;
-                    done[b2] = 1;
-                }
+            for (int b2 : same) {
+                kseg[b2] = bestk;
+                done[b2] = 1;
             }
         }
         (void)hipEventDestroy(e0);

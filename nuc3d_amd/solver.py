@@ -200,11 +200,14 @@ class Case:
                 if len(f) > 2 and f[2] is not None:
                     vals[5 * fi:5 * fi + len(f[2])] = f[2]
             _check(L.nuc3d_set_faces(self.h, bi, t, i, _dp(vals)), "set_faces")
-            jac = np.ascontiguousarray(b.jac, dtype=np.float64)
-            met = np.ascontiguousarray(b.met, dtype=np.float64)
-            dxyz = np.ascontiguousarray(b.dxyz, dtype=np.float64)
-            _check(L.nuc3d_upload_metrics(self.h, bi, _dp(jac), _dp(met), _dp(dxyz)), "upload_metrics")
-            self.set_q(bi, b.q)
+            # blocks given by their mesh nodes get metrics later (upload_mesh)
+            if b.jac is not None:
+                jac = np.ascontiguousarray(b.jac, dtype=np.float64)
+                met = np.ascontiguousarray(b.met, dtype=np.float64)
+                dxyz = np.ascontiguousarray(b.dxyz, dtype=np.float64)
+                _check(L.nuc3d_upload_metrics(self.h, bi, _dp(jac), _dp(met), _dp(dxyz)), "upload_metrics")
+            if b.q is not None:
+                self.set_q(bi, b.q)
 
     def close(self):
         if getattr(self, "h", None):
