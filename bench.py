@@ -224,6 +224,7 @@ def bench_multiblock(args):
     params, thr = train_sensor.load()
     hyb = make(solver.Params(exact=exact, scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), **phys), blocks, start)
     hyb.set_sensor(params, thr)
+    hyb.set_sensor_interval(args.sensor_interval)
     hyb.autotune()
     v_h, ms_h, _ = time_method(hyb, args.steps, args.warmup, cells, cfl)
     cen, net, fb = hyb.hybrid_counts()
@@ -267,6 +268,9 @@ def main():
     ap.add_argument("--fast", action="store_true",
                     help="contracted WENO arithmetic (within 1e-12 of the reference on well-conditioned fields only)")
     ap.add_argument("--no-hybrid", action="store_true")
+    ap.add_argument("--sensor-interval", type=int, default=2, choices=(1, 2),
+                    help="steps between the hybrid's decisions (2: stage 0 of every other step; the 256^3 "
+                         "turbulence keeps the same L2 deviation as with 1)")
     ap.add_argument("--workload", choices=("turbulence", "batch48", "flatplate"), default="turbulence",
                     help="batch48: 48 independent TGV and blast cases of seeded sizes 32^3..160^3 in one case "
                          "(block groups); flatplate: the 16-block supersonic flat plate over a bump (60.5 M "
@@ -322,6 +326,7 @@ def main():
         params, thr = train_sensor.load()
         hcase = solver.Case(solver.Params(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), **common), start)
         hcase.set_sensor(params, thr)
+        hcase.set_sensor_interval(args.sensor_interval)
         hcase.autotune()
         log("bench: learned hybrid, %d timed steps" % args.steps)
         v_h, ms_h, tmh = time_method(hcase, args.steps, args.warmup, cells)
@@ -344,6 +349,7 @@ def main():
             "l2_density_deviation": l2, "l2_over_density_rms": l2 / rms, "l2_over_fluctuation_rms": l2 / rms_f,
             "max_density_deviation": float(np.abs(rho_h - rho_p).max()),
             "kernel_ms_per_step": {k: v[0] / 2 for k, v in tmh.items()},
+            "decision_interval_steps": args.sensor_interval,
         }
         log("bench: hybrid %.3f ms/step, %.4g cell-updates/s (x%.3f), central faces %.3f, "
             "L2 density deviation %.3e of rms (%.3e of the fluctuation rms)" % (

@@ -289,6 +289,9 @@ int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold);
  * ((n_d+1) x n_{d+1} x n_{d+2}, face index fastest): 0 central, 1 WENO5-JS by
  * the network, 2 WENO5-JS by the fallback, 3 physical-face closure. */
 int nuc3d_download_hybrid_flags(nuc3d_case *c, int b, int d, unsigned char *flags);
+/* Steps between hybrid decisions: 1 (default, stage 0 of every step) or 2
+ * (stage 0 of every other step; the stages between reuse them). */
+int nuc3d_set_sensor_interval(nuc3d_case *c, int steps);
 /* Faces decided so far (counted where decisions are made): central, WENO5-JS
  * by the network, WENO5-JS by the fallback. */
 int nuc3d_get_hybrid_counts(nuc3d_case *c, long long *counts3);

@@ -156,6 +156,7 @@ struct nuc3d_case {
     // hybrid reconstruction: the network (inference copy) and the recorded decisions
     nuc3d_sensor_dev *dsensor = nullptr;
     std::vector<void *> dflags;
+    int hybrid_interval = 1;           // steps between hybrid decisions (1 or 2)
     // kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -258,12 +259,15 @@ int refresh(nuc3d_case *c, hipStream_t s)
 
 // viscous fluxes (Navier-Stokes), then the three sweeps; the z sweep
 // completes the RHS (mode 0) or applies RK stage `stage` (mode 1)
+// step: index of the step this stage belongs to (hybrid decisions are made at
+// stage 0 of every hybrid_interval-th step, and by every single RHS)
 int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, const unsigned long long *lcur,
-               unsigned long long *lnext, double *resid, hipStream_t s)
+               unsigned long long *lnext, double *resid, hipStream_t s, long long step = 0)
 {
     const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
     nuc3d_kparams K = c->K;
     K.resid = (mode == 1 && stage == 2) ? resid : nullptr;
+    K.hdecide = (mode == 0 || (stage == 0 && step % c->hybrid_interval == 0)) ? 1 : 0;
     for (int d = 0; d < 3; d++)
         if (c->p.scheme[d] == NUC3D_HYBRID && !K.sensor)
             return fail(NUC3D_ERR_BAD_ARG, "the hybrid scheme needs a sensor (nuc3d_set_sensor)");
@@ -739,7 +743,8 @@ int nuc3d_stage(nuc3d_case *c, int stage, double cfl, void *stream)
     (void)ns;
     unsigned long long *lcur = c->dlam + 3 * c->nb * c->lampar;
     unsigned long long *lnext = c->dlam + 3 * c->nb * (1 - c->lampar);
-    if ((rc = launch_rhs(c, 1, stage, bin, bun, bout, lcur, lnext, c->dresid + (long long)c->lampar * c->nwork, s)))
+    if ((rc = launch_rhs(c, 1, stage, bin, bun, bout, lcur, lnext, c->dresid + (long long)c->lampar * c->nwork, s,
+                         c->steps_done)))
         return rc;
     c->cur = bout;
     c->next_stage = (stage + 1) % 3;
@@ -790,8 +795,11 @@ int nuc3d_step_graph(nuc3d_case *c, int nsteps, double cfl, void *stream)
             for (int st = 0; st < 6 && rc == 0; st++) {
                 const int stage = st % 3;
                 const int bin = stage, bun = 0, bout = stage == 2 ? 0 : stage + 1;
+                // steps_done and the eigenvalue parity advance together, so a
+                // graph (kept per parity) sees the same step pattern at every replay
                 rc = launch_rhs(c, 1, stage, bin, bun, bout, c->dlam + 3 * c->nb * lp,
-                                c->dlam + 3 * c->nb * (1 - lp), c->dresid + (long long)lp * c->nwork, c->gstream);
+                                c->dlam + 3 * c->nb * (1 - lp), c->dresid + (long long)lp * c->nwork, c->gstream,
+                                c->steps_done + st / 3);
                 if (stage == 2) lp ^= 1;
             }
             hipGraph_t g = nullptr;
@@ -1477,6 +1485,21 @@ int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold)
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(c->dsensor, &h, sizeof h, hipMemcpyHostToDevice));
     c->K.sensor = c->dsensor;
+    return NUC3D_OK;
+}
+
+int nuc3d_set_sensor_interval(nuc3d_case *c, int steps)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    if (steps != 1 && steps != 2) return fail(NUC3D_ERR_BAD_ARG, "the decision interval is 1 or 2 steps");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "set the interval between full steps");
+    c->hybrid_interval = steps;
+    for (auto &g : c->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    c->gcfl[0] = c->gcfl[1] = -1.0;
     return NUC3D_OK;
 }
 

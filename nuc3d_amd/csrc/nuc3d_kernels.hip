@@ -1419,9 +1419,10 @@ static void sweep_dispatch(int dir, int mode, const nuc3d_block_dev *blocks, con
                            const unsigned long long *lam_cur, unsigned long long *lam_next,
                            unsigned long long *counters, hipStream_t st)
 {
-    // hybrid decisions: made by a single RHS evaluation and by stage 0 of a
-    // step, reused by stages 1 and 2
-    const bool decide = mode == 0 || stage == 0;
+    // hybrid decisions: made where the host asks (a single RHS evaluation, and
+    // stage 0 of a step at the case's decision interval), reused otherwise
+    const bool decide = K.hdecide != 0;
+    (void)mode;
 #define NUC3D_A blocks, tiles, ntiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, st, decide
     if (dir == 0)
         launch_sweep<0, SCHEME, NS, 0>(NUC3D_A);

@@ -135,6 +135,7 @@ def lib():
         "nuc3d_set_sensor": [vp, fp, ctypes.c_float],
         "nuc3d_download_hybrid_flags": [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte)],
         "nuc3d_get_hybrid_counts": [vp, llp],
+        "nuc3d_set_sensor_interval": [vp, ctypes.c_int],
         "nuc3d_op_interpolate": [ctypes.c_int] * 8 + [dp, dp, dp, vp],
         "nuc3d_op_differentiate": [ctypes.c_int] * 6 + [dp, dp, dp, vp],
         "nuc3d_op_rk3": [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_longlong,
@@ -350,6 +351,10 @@ class Case:
         assert p.size == SENSOR_NPARAM
         _check(lib().nuc3d_set_sensor(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), float(threshold)),
                "set_sensor")
+
+    def set_sensor_interval(self, steps):
+        """Steps between hybrid decisions (1 or 2)."""
+        _check(lib().nuc3d_set_sensor_interval(self.h, int(steps)), "set_sensor_interval")
 
     def hybrid_flags(self, b, d):
         """Last decisions (of nuc3d_rhs, or of the current step's stage 0) for
