@@ -451,6 +451,8 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
             const int d2 = d == 0 ? 1 : 0, d3 = d == 2 ? 1 : 2;  // transverse axes: (j,k) / (i,k) / (i,j)
             for (int a3 = 0; a3 < B.n[d3]; a3++)
                 for (int a2 = 0; a2 < B.n[d2]; a2++) ln.push_back(make_int4(b, a2, a3, 0));
+            // groups of NUC3D_CRW_LINES lines of one block, the last one padded
+            while (ln.size() % NUC3D_CRW_LINES) ln.push_back(make_int4(-1, 0, 0, 0));
             c->nmax[d] = std::max(c->nmax[d], B.n[d]);
         }
         if (c->nmax[d] > 1024) {
@@ -458,7 +460,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
             return fail(NUC3D_ERR_UNSUPPORTED, "CRWENO5 lines longer than 1024 cells");
         }
         c->nlines[d] = (int)ln.size();
-        scratch = std::max(scratch, ln.size() * 20 * (size_t)c->nmax[d]);
+        scratch = std::max(scratch, ln.size() / NUC3D_CRW_LINES * 128 * (size_t)c->nmax[d]);
         if (hipMalloc(&c->dlines[d], sizeof(int4) * ln.size()) != hipSuccess ||
             hipMemcpy(c->dlines[d], ln.data(), sizeof(int4) * ln.size(), hipMemcpyHostToDevice) != hipSuccess) {
             nuc3d_case_destroy(c);
@@ -481,8 +483,10 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     if (ns) {
         for (const auto &t : tiles) c->work_block.push_back(t.b);
     } else if (p->scheme[2] == NUC3D_CRWENO5) {
+        const long long L = NUC3D_CRW_LINES;
         for (int b = 0; b < nblocks; b++)
-            for (long long l = 0; l < (long long)c->hb[b].n[0] * c->hb[b].n[1]; l++) c->work_block.push_back(b);
+            for (long long g = 0; g < ((long long)c->hb[b].n[0] * c->hb[b].n[1] + L - 1) / L; g++)
+                c->work_block.push_back(b);
     } else {
         for (const auto &t : sw[2]) c->work_block.push_back(t.x);
     }

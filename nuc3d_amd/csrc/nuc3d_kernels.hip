@@ -767,17 +767,23 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
 }
 
 // ---------------------------------------------------------------------------
-// CRWENO5 along direction DIR (crweno5.cpp:44-401): one wavefront per line.
-// The split fluxes of the whole line and its halo are staged in LDS; faces
-// w..n-w form a tridiagonal system per component and upwind side whose end
-// rows are explicit WENO5-JS and whose inner rows carry the compact weights
-// (crweno5.cpp:130-160, :310-340).  The ten systems (5 components x +/-) are
-// solved side by side by ten lanes with the Thomas recurrence of
-// crweno5.cpp:207-218, coefficients computed on the fly; the solutions for
-// c' and d' live in global scratch.  Faces outside the system use the
-// WENO5-JS face variants or the physical closures.  There is no line-length
-// cap (the reference's stack arrays hold 256 rows, crweno5.cpp:69-72).
-// line: x = block, y, z = transverse coordinates (j,k / i,k / i,j).
+// CRWENO5 along direction DIR (crweno5.cpp:44-401): NUC3D_CRW_LINES lines of
+// one block per wavefront, one lane per tridiagonal system.  Faces w..n-w of
+// a line form a system per component and upwind side whose end rows are
+// explicit WENO5-JS and whose inner rows carry the compact weights
+// (crweno5.cpp:130-160, :310-340); the ten systems of each of the six lines
+// are solved by 60 lanes side by side with the Thomas recurrence of
+// crweno5.cpp:207-218, in the reference's operation order, the row
+// coefficients computed on the fly from split fluxes staged in LDS in chunks
+// of CRW_R rows.  c' and d' go to global scratch (one 64-wide row per system
+// row: coalesced), the back substitution walks the chunks down again and
+// stages the face solutions in LDS, where the cells of the chunk are
+// finished.  Faces outside the system use the WENO5-JS face variants or the
+// physical closures.  Line list: groups of NUC3D_CRW_LINES lines of one
+// block (x = block, y, z = transverse coordinates j,k / i,k / i,j), padded
+// with x = -1.
+constexpr int CRW_R = 32;
+
 template <int DIR, bool NS, int MODE>
 __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__restrict__ blocks,
                                                       const int4 *__restrict__ lines, nuc3d_kparams K, int stage,
@@ -786,100 +792,177 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
                                                       unsigned long long *lam_next, unsigned long long *counters,
                                                       double *scratch, int nmax)
 {
-    constexpr int H = NUC3D_HALO;
-    extern __shared__ double lds[];
+    constexpr int LW = NUC3D_CRW_LINES, R = CRW_R, NQ = R + 5;
+    static_assert(LW * 10 <= 64, "one lane per system");
+    static_assert(NQ >= 16 + 4 + 5, "end cells of both line ends fit one chunk");
+    __shared__ union {
+        double sf[10][LW][NQ + 1];  // split fluxes F+ (0..4), F- (5..9) of the staged cells
+        double xl[64][R + 1];       // face solutions of one chunk, per system lane
+    } U;
+    __shared__ double hfe[LW][5][8];  // explicit faces: 0..w-1 at 0..w-1, n-w+1..n at 4..3+w
+    __shared__ int4 lsh[LW];
     const int witem = xcd_tile(blockIdx.x, gridDim.x);
-    const int4 L = lines[witem];
-    const nuc3d_block_dev &B = blocks[L.x];
+    const int4 *lg = lines + (long long)witem * LW;
+    if (threadIdx.x < LW) lsh[threadIdx.x] = lg[threadIdx.x];
+    const nuc3d_block_dev &B = blocks[lg[0].x];
     const PhysParams P = phys_of(K, B);
     const int n = B.n[DIR], w = K.width[DIR], M = n - 2 * w + 1;
-    const int ns = n + 2 * H;              // slots: cell a at slot a + H
-    double *F = lds;                       // [10][ns]
-    double *HF = lds + 10 * ns;            // [5][n+1]
-    double *cp = scratch + (long long)blockIdx.x * 20 * nmax;  // [10][M] c', then [10][M] d'
-    double *dp = cp + 10 * nmax;
     double *const *qin = B.q[bin];
-    auto cell = [&](int a, int &i, int &j, int &k) {
-        if (DIR == 0) { i = a; j = L.y; k = L.z; }
-        else if (DIR == 1) { i = L.y; j = a; k = L.z; }
-        else { i = L.y; j = L.z; k = a; }
-    };
-    for (int sl = threadIdx.x; sl < ns; sl += 64) {
-        int i, j, k;
-        cell(sl - H, i, j, k);
-        double fp[5], fn[5];
-        split_any(B, qin, i, j, k, DIR, P, fp, fn);
-#pragma unroll
-        for (int c = 0; c < 5; c++) {
-            F[c * ns + sl] = fp[c];
-            F[(5 + c) * ns + sl] = fn[c];
-        }
-    }
     __syncthreads();
-    // explicit faces: 0..w-1 and n-w+1..n (halo face variants or physical closures)
-    for (int e = threadIdx.x; e < 2 * w * 5; e += 64) {
-        const int c = e / (2 * w), q = e % (2 * w);
-        const int m = q < w ? q : n - w + 1 + (q - w);
+    auto cell = [&](int l, int a, int &i, int &j, int &k) {
+        const int4 Ll = lsh[l];
+        if (DIR == 0) { i = a; j = Ll.y; k = Ll.z; }
+        else if (DIR == 1) { i = Ll.y; j = a; k = Ll.z; }
+        else { i = Ll.y; j = Ll.z; k = a; }
+    };
+    // split fluxes of cells cb..cb+nc-1 of every line into U.sf[.][l][q0 + q]
+    // (consecutive lanes take consecutive cells along x, consecutive lines
+    // across: the lines of a group are neighbours in i for DIR 1 and 2)
+    auto stage_cells = [&](int cb, int nc, int q0) {
+        for (int e = threadIdx.x; e < LW * nc; e += 64) {
+            const int l = DIR == 0 ? e / nc : e % LW, q = DIR == 0 ? e % nc : e / LW;
+            if (lsh[l].x < 0) continue;
+            int i, j, k;
+            cell(l, cb + q, i, j, k);
+            double fp[5], fn[5];
+            split_any(B, qin, i, j, k, DIR, P, fp, fn);
+#pragma unroll
+            for (int c = 0; c < 5; c++) {
+                U.sf[c][l][q0 + q] = fp[c];
+                U.sf[5 + c][l][q0 + q] = fn[c];
+            }
+        }
+    };
+
+    // explicit faces 0..w-1 (cells -3..w+1 at q 0..w+4) and n-w+1..n
+    // (cells n-w-2..n+2 at q 16..20+w): halo face variants or physical closures
+    stage_cells(-3, w + 5, 0);
+    stage_cells(n - w - 2, w + 5, 16);
+    __syncthreads();
+    for (int e = threadIdx.x; e < LW * 5 * 2 * w; e += 64) {
+        const int l = e / (10 * w), c = (e / (2 * w)) % 5, t = e % (2 * w);
+        if (lsh[l].x < 0) continue;
+        const int m = t < w ? t : n - w + 1 + (t - w);
+        const int q = t < w ? m : 16 + (t - w);  // q of cell m-3
         double sp[6], sn[6];
 #pragma unroll
         for (int o = 0; o < 6; o++) {
-            sp[o] = F[c * ns + m + o];
-            sn[o] = F[(5 + c) * ns + m + o];
+            sp[o] = U.sf[c][l][q + o];
+            sn[o] = U.sf[5 + c][l][q + o];
         }
-        HF[c * (n + 1) + m] = face_value<SCH_WENO5JS>(B, K, DIR, m, sp, sn);
+        hfe[l][c][t < w ? t : 4 + (t - w)] = face_value<SCH_WENO5JS>(B, K, DIR, m, sp, sn);
     }
-    // the ten tridiagonal systems of faces w..n-w
-    if (threadIdx.x < 10) {
-        const int sys = threadIdx.x, c = sys % 5;
-        const bool plus = sys < 5;
-        const double *f = F + (plus ? c : 5 + c) * ns;
-        double *C = cp + sys * nmax, *D = dp + sys * nmax;
-        double cprev = 0.0, dprev = 0.0;
-        for (int r = 0; r < M; r++) {
-            const int i = w - 1 + r;       // reference loop index, face i+1
-            double a, b, cc, d;
-            if (r == 0 || r == M - 1) {
-                const double *s6 = f + i + 1;   // slots of cells i-2..i+3 (face i+1 = m, slot m-3+H = m)
-                a = 0.0; b = 1.0; cc = 0.0;
-                d = plus ? weno5js(s6[0], s6[1], s6[2], s6[3], s6[4]) : weno5js(s6[5], s6[4], s6[3], s6[2], s6[1]);
-            } else {
-                crweno_row(f + i + H, 1, plus, a, b, cc, d);
+
+    // forward elimination, CRW_R rows per chunk; row r = face w+r, reference
+    // loop index i = w-1+r, stencil cells i-2..i+3 (staged at q = rr..rr+5)
+    const int s = threadIdx.x, ls = s / 10, sys = s % 10, c = sys % 5;
+    const bool plus = sys < 5;
+    const bool active = s < LW * 10 && lsh[ls < LW ? ls : 0].x >= 0;
+    double *Cg = scratch + (long long)witem * 128 * nmax;  // c'[r][64]
+    double *Dg = Cg + 64 * (long long)nmax;                // d'[r][64]
+    double cprev = 0.0, dprev = 0.0;
+    for (int r0 = 0; r0 < M; r0 += R) {
+        const int nr = min(R, M - r0);
+        __syncthreads();
+        stage_cells(w - 3 + r0, nr + 5, 0);
+        __syncthreads();
+        if (active) {
+            const double *f = &U.sf[plus ? c : 5 + c][ls][0];
+            int rr = 0;
+            if (r0 == 0) {  // row 0: explicit face w
+                const double d = plus ? weno5js(f[0], f[1], f[2], f[3], f[4])
+                                      : weno5js(f[5], f[4], f[3], f[2], f[1]);
+                Cg[s] = 0.0;
+                Dg[s] = d;
+                cprev = 0.0;
+                dprev = d;
+                rr = 1;
             }
-            if (r > 0) {
+            const bool last = r0 + nr == M && M > 1;
+            const int rin = last ? nr - 1 : nr;
+            // the row coefficients do not depend on the recurrence: four rows
+            // at a time give the lane four independent chains to interleave
+            for (; rr + 4 <= rin; rr += 4) {
+                double a[4], b[4], cc[4], d[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) crweno_row(f + rr + u + 2, 1, plus, a[u], b[u], cc[u], d[u]);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    cc[u] = cc[u] / (b[u] - cprev * a[u]);
+                    d[u] = (d[u] - dprev * a[u]) / (b[u] - cprev * a[u]);
+                    Cg[(long long)(r0 + rr + u) * 64 + s] = cc[u];
+                    Dg[(long long)(r0 + rr + u) * 64 + s] = d[u];
+                    cprev = cc[u];
+                    dprev = d[u];
+                }
+            }
+            for (; rr < rin; rr++) {
+                double a, b, cc, d;
+                crweno_row(f + rr + 2, 1, plus, a, b, cc, d);
                 cc = cc / (b - cprev * a);
                 d = (d - dprev * a) / (b - cprev * a);
+                Cg[(long long)(r0 + rr) * 64 + s] = cc;
+                Dg[(long long)(r0 + rr) * 64 + s] = d;
+                cprev = cc;
+                dprev = d;
             }
-            C[r] = cc;
-            D[r] = d;
-            cprev = cc;
-            dprev = d;
-        }
-        // back substitution: out[n-w] = d'[M-1]; out[w+r] = d'[r] - c'[r] out[w+r+1]
-        double x = D[M - 1];
-        D[M - 1] = x;
-        for (int r = M - 2; r >= 0; r--) {
-            x = D[r] - C[r] * x;
-            D[r] = x;
+            if (last) {  // row M-1: explicit face n-w
+                const double a = 0.0, b = 1.0;
+                double cc = 0.0;
+                double d = plus ? weno5js(f[rr], f[rr + 1], f[rr + 2], f[rr + 3], f[rr + 4])
+                                : weno5js(f[rr + 5], f[rr + 4], f[rr + 3], f[rr + 2], f[rr + 1]);
+                cc = cc / (b - cprev * a);
+                d = (d - dprev * a) / (b - cprev * a);
+                Cg[(long long)(M - 1) * 64 + s] = cc;
+                Dg[(long long)(M - 1) * 64 + s] = d;
+            }
         }
     }
-    __syncthreads();
-    // combine F^+ and F^- solutions into the inner face fluxes
-    for (int e = threadIdx.x; e < 5 * M; e += 64) {
-        const int c = e / M, r = e % M;
-        HF[c * (n + 1) + w + r] = dp[c * nmax + r] + dp[(5 + c) * nmax + r];
-    }
-    __syncthreads();
+
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
     double r2 = 0.0;  // residual partial (stage 2)
     const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
-    for (int a = threadIdx.x; a < n; a += 64) {
-        int i, j, k;
-        cell(a, i, j, k);
-        double dF[5];
+    // back substitution (out[n-w] = d'[M-1]; out[w+r] = d'[r] - c'[r] out[w+r+1])
+    // by chunks of cells c0..c1-1 from the top; faces c0..c1 at slots c0-c1+R..R
+    double x = 0.0;
+    for (int c1 = n; c1 > 0; c1 -= R) {
+        const int c0 = max(0, c1 - R);
+        __syncthreads();
+        if (active) {
+            if (c1 < n && c1 >= w && c1 <= n - w) U.xl[s][R] = x;  // face c1: the row of the chunk above
+            int fhi = min(c1 - 1, n - w);
+            const int flo = max(c0, w);
+            if (fhi == n - w && fhi >= flo) {
+                x = Dg[(long long)(M - 1) * 64 + s];
+                U.xl[s][fhi - c1 + R] = x;
+                fhi--;
+            }
+#pragma unroll 4
+            for (int f = fhi; f >= flo; f--) {
+                const long long o = (long long)(f - w) * 64 + s;
+                x = Dg[o] - Cg[o] * x;
+                U.xl[s][f - c1 + R] = x;
+            }
+        }
+        __syncthreads();
+        auto hf = [&](int l, int cc, int f) -> double {
+            if (f < w) return hfe[l][cc][f];
+            if (f > n - w) return hfe[l][cc][4 + f - (n - w + 1)];
+            const int sl = f - c1 + R;
+            return U.xl[l * 10 + cc][sl] + U.xl[l * 10 + 5 + cc][sl];
+        };
+        const int nc = c1 - c0;
+        for (int e = threadIdx.x; e < LW * nc; e += 64) {
+            const int l = DIR == 0 ? e / nc : e % LW, a = c0 + (DIR == 0 ? e % nc : e / LW);
+            if (lsh[l].x < 0) continue;
+            int i, j, k;
+            cell(l, a, i, j, k);
+            double dF[5];
 #pragma unroll
-        for (int c = 0; c < 5; c++) dF[c] = HF[c * (n + 1) + a + 1] - HF[c * (n + 1) + a];
-        finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
+            for (int cc = 0; cc < 5; cc++) dF[cc] = hf(l, cc, a + 1) - hf(l, cc, a);
+            finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
+        }
     }
     if (DIR == 2 && MODE == 1) {
         if (nfix) atomicAdd(counters, nfix);
@@ -888,7 +971,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
             for (int d = 0; d < 3; d++) {
                 const double v = warp_max(lmax[d]);
                 if (threadIdx.x == 0 && v > 0.0)
-                    atomicMax(lam_next + 3 * L.x + d, (unsigned long long)__double_as_longlong(v));
+                    atomicMax(lam_next + 3 * lg[0].x + d, (unsigned long long)__double_as_longlong(v));
             }
             if (!NS && K.resid) block_sum_store(r2, K.resid + witem);
         }
@@ -1469,16 +1552,8 @@ static hipError_t launch_crweno(const nuc3d_block_dev *blocks, const int4 *lines
                                 const unsigned long long *lam_cur, unsigned long long *lam_next,
                                 unsigned long long *counters, double *scratch, hipStream_t st)
 {
-    const size_t lds = sizeof(double) * (10 * (nmax + 2 * NUC3D_HALO) + 5 * (nmax + 1));
-    static bool attr_set = false;
-    if (!attr_set && lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&KNAME(k_crweno)<DIR, NS, MODE>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
-    hipLaunchKernelGGL((KNAME(k_crweno)<DIR, NS, MODE>), dim3(nlines), dim3(64), lds, st, blocks, lines, K, stage,
-                       bin, bun, bout, dtbuf, lam_cur, lam_next, counters, scratch, nmax);
+    hipLaunchKernelGGL((KNAME(k_crweno)<DIR, NS, MODE>), dim3(nlines / NUC3D_CRW_LINES), dim3(64), 0, st, blocks,
+                       lines, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, scratch, nmax);
     return hipGetLastError();
 }
 
@@ -1488,7 +1563,7 @@ extern "C" hipError_t KNAME(nuc3d_launch_crweno)(int dir, int ns, int mode, cons
                                                  const unsigned long long *lam_cur, unsigned long long *lam_next,
                                                  unsigned long long *counters, double *scratch, hipStream_t st)
 {
-    if (nmax > 1024) return hipErrorInvalidValue;
+    if (nmax > 1024 || nlines % NUC3D_CRW_LINES) return hipErrorInvalidValue;
 #define NUC3D_A blocks, lines, nlines, nmax, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, scratch, st
     if (dir == 0) return ns ? launch_crweno<0, true, 0>(NUC3D_A) : launch_crweno<0, false, 0>(NUC3D_A);
     if (dir == 1) return ns ? launch_crweno<1, true, 0>(NUC3D_A) : launch_crweno<1, false, 0>(NUC3D_A);

@@ -12,6 +12,7 @@
 #define NUC3D_TILE 16   // tile edge in i and j (one thread per column)
 #define NUC3D_HALO 3    // stencil half-width of WENO5 / cd6
 #define NUC3D_KSEG 32   // planes per k segment of a column tile
+#define NUC3D_CRW_LINES 6  // CRWENO5 lines per wavefront (ten tridiagonal systems each)
 
 struct nuc3d_block_dev {
     int n[3];            // interior cells nx, ny, nz
