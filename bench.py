@@ -50,8 +50,8 @@ def kernel_alg_bytes(cls, ns):
       z sweep:      Q 40 + J 8 + zeta metrics 24 + RHS read and write 80       = 152
                     (Euler: RHS read 40, Q^n 40 at stages 1 and 2, Q_out write
                      40, xi/eta metrics 48 at stage 2 for the eigenvalues)
-      primitives:   Q 40 + J 8 read, u v w T write 32                         = 80
-      viscous flux: u v w T 32 + J 8 + nine metrics 72 read, fv write 96      = 208
+      viscous flux: Q 40 + J 8 + nine metrics 72 read (u v w T computed in
+                    the pass), fv write 96                                    = 216
       viscous div:  fv 96 + RHS 40 + Q 40 + J 8 read, Q_out 40 write,
                     Q^n 40 at stages 1 and 2, nine metrics 72 at stage 2      = 274.7
     """
@@ -61,10 +61,8 @@ def kernel_alg_bytes(cls, ns):
         return 152.0
     if cls == "z_sweep":
         return 112.0 + (0 + 40 + 40) / 3.0 + 40.0 + 48.0 / 3.0
-    if cls == "primitives":
-        return 80.0
     if cls == "viscous_flux":
-        return 208.0
+        return 216.0
     return 96.0 + 40.0 + 40.0 + 8.0 + 40.0 + (0 + 40 + 40) / 3.0 + 72.0 / 3.0
 
 
@@ -118,7 +116,7 @@ def cpu_baseline(n, nsteps=3, ranks=None, timeout=240):
 
 # timing class -> kernel (rocprofv3 names: k_sweep_<variant><DIR,...>, ...)
 KERNEL_OF_CLASS = {"x_sweep": "k_sweep_x", "y_sweep": "k_sweep_y", "z_sweep": "k_sweep_z",
-                   "primitives": "k_prim", "viscous_flux": "k_visc_flux", "viscous_div_update": "k_visc_div"}
+                   "viscous_flux": "k_visc_flux", "viscous_div_update": "k_visc_div"}
 
 
 def read_traffic(cls, n, variant):

@@ -189,13 +189,13 @@ int nuc3d_set_block_group(nuc3d_case *c, int b, int group, double mach, double r
 /* dt of group `group` in the current step (nuc3d_get_dt gives group 0's). */
 int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt);
 
-/* Autotuner (no counterpart in the reference).  The pointwise passes
- * (primitives, viscous fluxes, viscous divergence, diagnostics) run over
+/* Autotuner (no counterpart in the reference).  The column-tile passes
+ * (viscous fluxes with u,v,w,T, viscous divergence, diagnostics) run over
  * 16 x 16 column tiles that march along k in segments of kseg planes, 32 by
  * default.  For Navier-Stokes cases this takes the distinct block shapes in
  * turn and, for kseg = 8, 16, 32, 64, 128 (up to twice the block's depth) on
  * every block of that shape, the other blocks keeping their current choice,
- * times the three passes of one RHS over the whole case on the current state;
+ * times the two viscous passes of one RHS over the whole case on the current state;
  * it keeps the fastest per shape.  kseg_out (nblocks, may be NULL) receives the choices.  Results do
  * not change, except the rounding of the diagnostics' partial sums; the
  * residual history restarts (the next nuc3d_get_residual after one step is 0).
@@ -224,8 +224,9 @@ int nuc3d_get_fixups(nuc3d_case *c, long long *count);
  * with MPI_Wtime in block::solve, block.cpp:576-584).  While enabled, every
  * kernel launch is bracketed by HIP events on its stream, in NUC3D_NTIMING
  * classes: 0 x sweep, 1 y sweep, 2 z sweep (with the RK update for Euler),
- * and for Navier-Stokes 3 primitives u,v,w,T, 4 viscous fluxes, 5 viscous
- * divergence with the RK update.  nuc3d_timing_read returns the summed
+ * and for Navier-Stokes 4 viscous fluxes (u,v,w,T computed in the same
+ * pass), 5 viscous divergence with the RK update; class 3 (a separate
+ * primitives pass) is no longer launched by the stage pipeline.  nuc3d_timing_read returns the summed
  * milliseconds and launch counts per class since the last read (arrays of
  * NUC3D_NTIMING) and resets them. */
 #define NUC3D_NTIMING 6

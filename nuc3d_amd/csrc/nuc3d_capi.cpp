@@ -271,11 +271,7 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
     for (int d = 0; d < 3; d++)
         if (c->p.scheme[d] == NUC3D_HYBRID && !K.sensor)
             return fail(NUC3D_ERR_BAD_ARG, "the hybrid scheme needs a sensor (nuc3d_set_sensor)");
-    if (ns) {
-        {
-            Timed t(c, s, &c->ev[3]);
-            HIPCHK(c->L->prim(c->db, c->dtiles, c->ntiles, c->kmax, K, bin, s));
-        }
+    if (ns) {  // u, v, w, T are computed inside the viscous-flux pass
         Timed t(c, s, &c->ev[4]);
         HIPCHK(c->L->vflux(c->db, c->dtiles, c->ntiles, K, bin, s));
     }
@@ -1140,7 +1136,6 @@ int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out)
                 if (ks > 2 * B.n[2] && ks != 8) break;
                 std::vector<int> trial = kseg;
                 for (int b2 : same) trial[b2] = ks;
-                const int kmx = *std::max_element(trial.begin(), trial.end());
                 const std::vector<nuc3d_tile> tiles = column_tiles(c, all, trial);
                 nuc3d_tile *dt = nullptr;
                 if (hipMalloc(&dt, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess) {
@@ -1149,11 +1144,10 @@ int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out)
                 }
                 hipError_t e = hipMemcpy(dt, tiles.data(), sizeof(nuc3d_tile) * tiles.size(), hipMemcpyHostToDevice);
                 const int nt = (int)tiles.size();
-                // the pointwise passes of one RHS over the block: primitives,
-                // viscous fluxes, viscous divergence into the RHS (no state change)
+                // the column-tile passes of one RHS over the block: viscous
+                // fluxes, viscous divergence into the RHS (no state change)
                 auto pass = [&]() {
-                    hipError_t r = c->L->prim(c->db, dt, nt, kmx, c->K, c->cur, s);
-                    if (r == hipSuccess) r = c->L->vflux(c->db, dt, nt, c->K, c->cur, s);
+                    hipError_t r = c->L->vflux(c->db, dt, nt, c->K, c->cur, s);
                     if (r == hipSuccess)
                         r = c->L->vdiv(0, c->db, dt, nt, c->K, 0, c->cur, 0, 0, c->ddt, c->dlam, c->dlam, c->dcount, s);
                     return r;

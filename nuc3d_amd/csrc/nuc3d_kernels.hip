@@ -979,13 +979,14 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 }
 
 // ---------------------------------------------------------------------------
-// Navier-Stokes viscous terms (NaiverStokes3d.cpp:51-513) in three passes
+// Navier-Stokes viscous terms (NaiverStokes3d.cpp:51-513) in two passes
 // over 16x16 column tiles marching in k.
 //
-// k_prim:      u, v, w, T of every cell (con2prim, physicsModel.cpp:313-353,
-//              T = gamma Ma^2 p / rho, physicsModel.cpp:1029), pushed to the
-//              neighbours' ghost layers like Q.
-// k_visc_flux: cd6 (or exterior closure) derivatives of u, v, w, T along xi
+// k_visc_flux: u, v, w, T from Q/J and J as each plane arrives (con2prim,
+//              physicsModel.cpp:313-353, T = gamma Ma^2 p / rho,
+//              physicsModel.cpp:1029; ghost Q and J are the neighbours' own
+//              values, so the halo's u, v, w, T are theirs bit for bit);
+//              cd6 (or exterior closure) derivatives of u, v, w, T along xi
 //              and eta from a double-buffered LDS plane with its xy halo, and
 //              along zeta from a seven-plane register ring of the thread's own
 //              column (NaiverStokes3d.cpp:86-168); chain rule with the
@@ -1011,21 +1012,36 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 #define NUC3D_VISC_DIV_WAVES 2
 #endif
 
-__device__ __forceinline__ void prim4(const nuc3d_block_dev &B, double *const *qb, long long x, const PhysParams &P,
-                                      double *w4)
+// u, v, w, T of a cell from its conserved state Q/J (q6[0..4]) and J (q6[5])
+__device__ __forceinline__ void prim4_of(const double *q6, const PhysParams &P, double *w4)
 {
-    double Q[5];
-#pragma unroll
-    for (int c = 0; c < 5; c++) Q[c] = qb[c][x];
     double rho, u, v, w, p, cs;
-    con2prim(Q, B.jac[x], P, rho, u, v, w, p, cs);
+    con2prim(q6, q6[5], P, rho, u, v, w, p, cs);
     w4[0] = u;
     w4[1] = v;
     w4[2] = w;
     w4[3] = P.gamma * P.mach * P.mach * p / rho;  // physicsModel.cpp:1029
 }
 
-// one workgroup per (tile, plane): blockIdx.y = plane within the tile's k segment
+__device__ __forceinline__ void load_q6(const nuc3d_block_dev &B, double *const *qb, long long x, double *q6)
+{
+#pragma unroll
+    for (int c = 0; c < 5; c++) q6[c] = qb[c][x];
+    q6[5] = B.jac[x];
+}
+
+__device__ __forceinline__ void prim4(const nuc3d_block_dev &B, double *const *qb, long long x, const PhysParams &P,
+                                      double *w4)
+{
+    double q6[6];
+    load_q6(B, qb, x, q6);
+    prim4_of(q6, P, w4);
+}
+
+// k_prim: u, v, w, T of every cell into B.prim, pushed to the neighbours'
+// ghost layers like Q (for the diagnostics; the stage pipeline computes them
+// inside k_visc_flux).  One workgroup per (tile, plane): blockIdx.y = plane
+// within the tile's k segment.
 __global__ __launch_bounds__(256) void KNAME(k_prim)(const nuc3d_block_dev *__restrict__ blocks,
                                                      const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int bin)
 {
@@ -1117,13 +1133,14 @@ __device__ __forceinline__ void visc_flux_cell(const nuc3d_kparams &K, const Phy
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_FLUX_WAVES))) void KNAME(k_visc_flux)(
-    const nuc3d_block_dev *__restrict__ blocks, const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K)
+    const nuc3d_block_dev *__restrict__ blocks, const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int bin)
 {
     constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
     __shared__ double Wp[2][4][L][L];  // u,v,w,T of plane k with the xy halo, double-buffered
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K, B);
+    double *const *qb = B.q[bin];
     const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
     const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
     const bool incell = ti < tw && tj < th;
@@ -1138,13 +1155,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
     for (int a = 0; a < 2 * H + 1; a++)
 #pragma unroll
         for (int v = 0; v < 4; v++) zr[a][v] = 0.0;
-    double pc[4] = {0.0, 0.0, 0.0, 0.0}, ph[4] = {0.0, 0.0, 0.0, 0.0};  // prefetched plane p / halo plane k
+    double pc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, ph[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // Q/J and J of
+                                                                                        // plane p / halo plane k
     double pm[10];                                                       // prefetched J + metrics of cell k
 #pragma unroll
     for (int e = 0; e < 10; e++) pm[e] = 0.0;
-    if (incell)
-#pragma unroll
-        for (int v = 0; v < 4; v++) pc[v] = B.prim[v][xc + (kb - H) * sz];
+    if (incell) load_q6(B, qb, xc + (kb - H) * sz, pc);
 
     for (int p = kb - H; p < ke + H; p++) {
         const int k = p - H, buf = p & 1;
@@ -1152,26 +1168,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
         for (int a = 0; a < 2 * H; a++)
 #pragma unroll
             for (int v = 0; v < 4; v++) zr[a][v] = zr[a + 1][v];
-#pragma unroll
-        for (int v = 0; v < 4; v++) zr[2 * H][v] = pc[v];
+        prim4_of(pc, P, zr[2 * H]);
         double m[10];
 #pragma unroll
         for (int e = 0; e < 10; e++) m[e] = pm[e];
         if (k >= kb) {
+            double wh[4];
+            prim4_of(ph, P, wh);
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 if (incell) Wp[buf][v][tj + H][ti + H] = zr[H][v];
-                if (hv) Wp[buf][v][hj + H][hi + H] = ph[v];
+                if (hv) Wp[buf][v][hj + H][hi + H] = wh[v];
             }
         }
         // loads for the next plane, in flight during this plane's work
-        if (incell && p + 1 < ke + H)
-#pragma unroll
-            for (int v = 0; v < 4; v++) pc[v] = B.prim[v][xc + (p + 1) * sz];
+        if (incell && p + 1 < ke + H) load_q6(B, qb, xc + (p + 1) * sz, pc);
         if (k + 1 >= kb && k + 1 < ke) {
-            if (hv)
-#pragma unroll
-                for (int v = 0; v < 4; v++) ph[v] = B.prim[v][xh + (k + 1) * sz];
+            if (hv) load_q6(B, qb, xh + (k + 1) * sz, ph);
             if (incell) {
                 pm[0] = B.jac[xc + (k + 1) * sz];
 #pragma unroll
@@ -1583,8 +1596,7 @@ extern "C" hipError_t KNAME(nuc3d_launch_prim)(const nuc3d_block_dev *blocks, co
 extern "C" hipError_t KNAME(nuc3d_launch_visc_flux)(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles,
                                                     int ntiles, nuc3d_kparams K, int bin, hipStream_t st)
 {
-    (void)bin;
-    hipLaunchKernelGGL(KNAME(k_visc_flux), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K);
+    hipLaunchKernelGGL(KNAME(k_visc_flux), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
     return hipGetLastError();
 }
 
