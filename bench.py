@@ -255,6 +255,40 @@ def bench_multiblock(args):
     }), flush=True)
 
 
+def bench_shock(args):
+    """The shock workload of the compact scheme: the 3-D Shu-Osher
+    shock/entropy-wave problem on 1024 x 128 x 128 cells (Euler, Mach 1,
+    inflow/outflow in x, periodic in y and z, CFL 0.5), CRWENO5 (a
+    tridiagonal system per line, component and upwind side) against WENO5-Z
+    from the same initial state; `value` is CRWENO5's throughput."""
+    from nuc3d_amd import cases, solver
+
+    exact = 0 if args.fast else 1
+    nx, ny, nz = 1024, 128, 128
+    blocks = cases.shu_osher(nx, ny, nz, seed=0)
+    cells = nx * ny * nz
+    res = {}
+    for name, sch in (("weno5z", solver.WENO5Z), ("crweno5", solver.CRWENO5)):
+        c = solver.Case(solver.Params(model=solver.EULER3D, mach=1.0, scheme=(sch,) * 3, exact=exact), blocks)
+        v, ms, tm = time_method(c, args.steps, args.warmup, cells)
+        c.close()
+        res[name] = {"value": v, "ms_per_step": ms,
+                     "kernel_ms_per_step": {k: t / 2.0 for k, (t, n) in tm.items() if n}}
+        log("bench: shu_osher %-8s %.3f ms/step, %.4g cell-updates/s  %s" % (
+            name, ms, v, " ".join("%s %.3f" % (k, t / 2.0) for k, (t, n) in tm.items() if n)))
+    print(json.dumps({
+        "metric": "cell-updates/s per RK3 step FP64, Shu-Osher: CRWENO5 (compact) vs WENO5-Z", "value":
+        res["crweno5"]["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["crweno5"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3-D Shu-Osher shock/entropy wave %dx%dx%d, Euler, Mach 1, inflow/outflow in x, "
+                               "periodic y and z, CFL 0.5" % (nx, ny, nz), "cells": cells,
+                   "variant": "fast" if args.fast else "exact"},
+        "crweno5": res["crweno5"], "weno5z": res["weno5z"],
+        "crweno5_over_weno5z": res["crweno5"]["value"] / res["weno5z"]["value"],
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,10 +303,11 @@ def main():
     ap.add_argument("--sensor-interval", type=int, default=2, choices=(1, 2),
                     help="steps between the hybrid's decisions (2: stage 0 of every other step; the 256^3 "
                          "turbulence keeps the same L2 deviation as with 1)")
-    ap.add_argument("--workload", choices=("turbulence", "batch48", "flatplate"), default="turbulence",
+    ap.add_argument("--workload", choices=("turbulence", "batch48", "flatplate", "shu_osher"), default="turbulence",
                     help="batch48: 48 independent TGV and blast cases of seeded sizes 32^3..160^3 in one case "
                          "(block groups); flatplate: the 16-block supersonic flat plate over a bump (60.5 M "
-                         "cells); both plain WENO5-JS and the learned hybrid")
+                         "cells); both plain WENO5-JS and the learned hybrid; shu_osher: CRWENO5 against "
+                         "WENO5-Z on the 1024x128x128 Shu-Osher problem")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
@@ -288,6 +323,8 @@ def main():
     import numpy as np
     from nuc3d_amd import cases, solver, train_sensor
 
+    if args.workload == "shu_osher":
+        return bench_shock(args)
     if args.workload != "turbulence":
         return bench_multiblock(args)
     n = args.n
