@@ -47,20 +47,23 @@ def kernel_alg_bytes(cls, ns):
     written once; ghost layers and halo re-reads not counted):
       x sweep:      Q 40 + J 8 + xi metrics 24 + RHS write 40                 = 112
       y sweep:      Q 40 + J 8 + eta metrics 24 + RHS read and write 80        = 152
-      z sweep:      Q 40 + J 8 + zeta metrics 24 + RHS read and write 80       = 152
-                    (Euler: RHS read 40, Q^n 40 at stages 1 and 2, Q_out write
-                     40, xi/eta metrics 48 at stage 2 for the eigenvalues)
+      z sweep:      Q 40 + J 8 + zeta metrics 24 + RHS read 40, Q^n 40 at
+                    stages 1 and 2, Q_out write 40, xi/eta metrics 48 at
+                    stage 2 for the eigenvalues                               = 194.7
+                    (Navier-Stokes: + the twelve viscous fluxes fv 96, whose
+                     divergence the z sweep takes                            = 290.7)
       viscous flux: Q 40 + J 8 + nine metrics 72 read (u v w T computed in
                     the pass), fv write 96                                    = 216
       viscous div:  fv 96 + RHS 40 + Q 40 + J 8 read, Q_out 40 write,
                     Q^n 40 at stages 1 and 2, nine metrics 72 at stage 2      = 274.7
+                    (single RHS evaluations, and CRWENO5 z lines)
     """
     if cls == "x_sweep":
         return 112.0
-    if cls in ("y_sweep", "z_sweep") and (ns or cls == "y_sweep"):
+    if cls == "y_sweep":
         return 152.0
     if cls == "z_sweep":
-        return 112.0 + (0 + 40 + 40) / 3.0 + 40.0 + 48.0 / 3.0
+        return 112.0 + (0 + 40 + 40) / 3.0 + 40.0 + 48.0 / 3.0 + (96.0 if ns else 0.0)
     if cls == "viscous_flux":
         return 216.0
     return 96.0 + 40.0 + 40.0 + 8.0 + 40.0 + (0 + 40 + 40) / 3.0 + 72.0 / 3.0

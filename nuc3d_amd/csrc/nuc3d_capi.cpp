@@ -257,8 +257,17 @@ int refresh(nuc3d_case *c, hipStream_t s)
     return NUC3D_OK;
 }
 
+// Navier-Stokes stages whose z sweep is a direction sweep (not CRWENO5 lines)
+// finish their cells there: viscous divergence and RK stage in one pass.
+bool visc_in_z_sweep(const nuc3d_case *c)
+{
+    return c->p.model == NUC3D_NAVIERSTOKES3D && c->p.scheme[2] != NUC3D_CRWENO5;
+}
+
 // viscous fluxes (Navier-Stokes), then the three sweeps; the z sweep
-// completes the RHS (mode 0) or applies RK stage `stage` (mode 1)
+// completes the RHS (mode 0) or applies RK stage `stage` (mode 1); for
+// Navier-Stokes the viscous divergence follows in its own pass, except in the
+// stages of visc_in_z_sweep cases
 // step: index of the step this stage belongs to (hybrid decisions are made at
 // stage 0 of every hybrid_interval-th step, and by every single RHS)
 int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, const unsigned long long *lcur,
@@ -284,7 +293,7 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
             HIPCHK(c->L->sweep(d, c->p.scheme[d], ns, mode, c->db, c->dsweep[d], c->nsweep[d], K, stage, bin,
                                bun, bout, c->ddt, lcur, lnext, c->dcount, s));
     }
-    if (ns) {
+    if (ns && !(mode == 1 && visc_in_z_sweep(c))) {
         Timed t(c, s, &c->ev[5]);
         HIPCHK(c->L->vdiv(mode, c->db, c->dtiles, c->ntiles, K, stage, bin, bun, bout, c->ddt, lcur, lnext,
                           c->dcount, s));
@@ -433,9 +442,20 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         for (int k = 0; k < B.n[2]; k++)
             for (int l0 = 0; l0 < B.n[0]; l0 += 16)
                 for (int a0 = 0; a0 < B.n[1]; a0 += 15) sw[1].push_back(make_int4(b, a0, l0, k));
-        for (int j = 0; j < B.n[1]; j++)
+        if (ns && p->scheme[2] != NUC3D_CRWENO5) {
+            // the z sweep also takes the viscous divergence, whose fv_eta
+            // stencil spans j-3..j+3: runs of 8 rows in j at one k segment are
+            // neighbours in the list (they run together and share one XCD's
+            // L2), and so are the k segments of one run (their k halos overlap)
             for (int l0 = 0; l0 < B.n[0]; l0 += 16)
-                for (int a0 = 0; a0 < B.n[2]; a0 += 15) sw[2].push_back(make_int4(b, a0, l0, j));
+                for (int j0 = 0; j0 < B.n[1]; j0 += 8)
+                    for (int a0 = 0; a0 < B.n[2]; a0 += 15)
+                        for (int j = j0; j < std::min(j0 + 8, B.n[1]); j++) sw[2].push_back(make_int4(b, a0, l0, j));
+        } else {
+            for (int j = 0; j < B.n[1]; j++)
+                for (int l0 = 0; l0 < B.n[0]; l0 += 16)
+                    for (int a0 = 0; a0 < B.n[2]; a0 += 15) sw[2].push_back(make_int4(b, a0, l0, j));
+        }
     }
     // whole lines of the CRWENO5 directions and their Thomas scratch
     size_t scratch = 0;
@@ -476,7 +496,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         }
     }
     // work items of the kernel that finishes stage 2, and their blocks
-    if (ns) {
+    if (ns && p->scheme[2] == NUC3D_CRWENO5) {
         for (const auto &t : tiles) c->work_block.push_back(t.b);
     } else if (p->scheme[2] == NUC3D_CRWENO5) {
         const long long L = NUC3D_CRW_LINES;
@@ -1062,8 +1082,8 @@ std::vector<nuc3d_tile> column_tiles(const nuc3d_case *c, const std::vector<int>
     return tiles;
 }
 
-// Replace the case's column tiles (and, for Navier-Stokes, the residual work
-// items that follow them; the residual history restarts).
+// Replace the case's column tiles (and, where k_visc_div finishes the stages,
+// the residual work items that follow them; the residual history restarts).
 int set_column_tiles(nuc3d_case *c, const std::vector<int> &kseg)
 {
     std::vector<int> all(c->nb);
@@ -1073,7 +1093,7 @@ int set_column_tiles(nuc3d_case *c, const std::vector<int> &kseg)
     if (hipMalloc(&dt, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess) return fail(NUC3D_ERR_OOM, "tile table");
     HIPCHK(hipMemcpy(dt, tiles.data(), sizeof(nuc3d_tile) * tiles.size(), hipMemcpyHostToDevice));
     const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
-    if (ns) {
+    if (ns && !visc_in_z_sweep(c)) {  // the residual partials come from k_visc_div
         double *dr = nullptr;
         if (hipMalloc(&dr, sizeof(double) * 2 * tiles.size()) != hipSuccess) {
             (void)hipFree(dt);

@@ -386,7 +386,9 @@ __device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const n
 // Cell (i,j,k) receives the flux difference dF of direction DIR:
 //   DIR 0: rhs = dF;  DIR 1, 2: rhs += dF (euler3d.cpp:383-422).  For Euler
 //   the z sweep then writes the RHS (MODE 0) or applies the RK stage; for
-//   Navier-Stokes k_visc_div finishes the cell.
+//   Navier-Stokes k_visc_div finishes the cell of a single RHS evaluation and
+//   of the CRWENO5 z lines (the stages of the other z sweeps finish their
+//   cells in finish_cell_visc_z).
 template <int DIR, bool NS, int MODE>
 __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
                                             const nuc3d_kparams &K, const PhysParams &P, int i, int j, int k,
@@ -401,6 +403,78 @@ __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const
 #pragma unroll
         for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
         return;
+    }
+    rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
+}
+
+// Navier-Stokes stages: the z sweep also finishes its cells with the viscous
+// divergence and the RK stage.  Cell (i,j,k) of the sweep's tile (16 lines
+// i = l0..l0+15, nc <= 15 cells k = a0..a0+nc-1, j fixed) takes
+//   r = rhs + dH/dzeta;  r[1+v] = ((r[1+v] - dFv/dxi) - dGv/deta) - dHv/dzeta
+// (NaiverStokes3d.cpp:401-513, the order of the separate divergence pass), then
+// the TVD-RK3 stage.  fv_xi of i-3..i0+18 and fv_zeta of k-3..a0+nc+2 are
+// staged in LDS (the split-flux area, free once the faces are computed);
+// fv_eta is read along j from memory.
+constexpr int VZ_NL = 16, VZ_NC = 15;
+constexpr int VZ_XW = VZ_NL + 2 * NUC3D_HALO, VZ_ZW = VZ_NC + 2 * NUC3D_HALO;
+constexpr int VZ_LDS = 4 * VZ_NC * VZ_XW + 4 * VZ_ZW * VZ_NL;  // doubles
+
+constexpr int VZ_PER = (VZ_LDS + 255) / 256;  // staged values per thread of a 256-thread sweep
+
+// Loads of the staged values into registers, issued at the start of the sweep
+// so that they are in flight while the faces are computed: thread t takes
+// elements t, t + 256, ... of V, where V[(v * NC + kk) * XW + ii] = fv_xi of
+// cell (l0 - 3 + ii, j, a0 + kk) and V[4 NC XW + (v * ZW + kz) * NL + ii] =
+// fv_zeta of cell (l0 + ii, j, a0 - 3 + kz).
+__device__ __forceinline__ void visc_prefetch_z(const nuc3d_block_dev &B, int a0, int l0, int j, int nc, int nlines,
+                                                double *pv)
+{
+    constexpr int H = NUC3D_HALO, NX = 4 * VZ_NC * VZ_XW;
+#pragma unroll
+    for (int r = 0; r < VZ_PER; r++) {
+        const int e = threadIdx.x + 256 * r;
+        pv[r] = 0.0;
+        if (e < NX) {
+            const int ii = e % VZ_XW, kk = (e / VZ_XW) % VZ_NC, v = e / (VZ_XW * VZ_NC);
+            if (kk < nc && ii < nlines + 2 * H) pv[r] = B.fv[v][cidx(B, l0 - H + ii, j, a0 + kk)];
+        } else if (e < VZ_LDS) {
+            const int ez = e - NX;
+            const int ii = ez % VZ_NL, kz = (ez / VZ_NL) % VZ_ZW, v = ez / (VZ_NL * VZ_ZW);
+            if (ii < nlines && kz < nc + 2 * H) pv[r] = B.fv[8 + v][cidx(B, l0 + ii, j, a0 - H + kz)];
+        }
+    }
+}
+
+// The prefetched values into the LDS area V (once the split fluxes are dead).
+__device__ __forceinline__ void visc_store_z(double *V, const double *pv)
+{
+#pragma unroll
+    for (int r = 0; r < VZ_PER; r++) {
+        const int e = threadIdx.x + 256 * r;
+        if (e < VZ_LDS) V[e] = pv[r];
+    }
+}
+
+// Cell (i,j,k) = line `line`, cell kk of the tile: flux difference dF of the z
+// sweep, viscous divergence from the staged V, RK stage.
+__device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
+                                                   const nuc3d_kparams &K, const PhysParams &P, const double *V,
+                                                   int line, int kk, int i, int j, int k, const double *dF, int stage,
+                                                   int bin, int bun, int bout, double dt, double *lmax,
+                                                   unsigned long long &nfix, double &r2)
+{
+    constexpr int H = NUC3D_HALO;
+    const double *Vx = V, *Vz = V + 4 * VZ_NC * VZ_XW;
+    const long long x = cidx(B, i, j, k);
+    double r[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) r[c] = B.rhs[c][x] + dF[c];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const double gx = line_diff_lds(B, K, &Vx[(v * VZ_NC + kk) * VZ_XW + line + H], 1, 0, i);
+        const double gy = line_diff(B, K, B.fv[4 + v] + x, 1, j);
+        const double gz = line_diff_lds(B, K, &Vz[(v * VZ_ZW + kk + H) * VZ_NL + line], VZ_NL, 2, k);
+        r[1 + v] = ((r[1 + v] - gx) - gy) - gz;
     }
     rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
 }
@@ -445,6 +519,9 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     const int nc = min(NC, B.n[DIR] - a0);
     const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
     double *const *qin = B.q[bin];
+    constexpr bool VISC = NS && DIR == 2 && MODE == 1;  // Navier-Stokes stage: viscous divergence here
+    double pv[VISC ? VZ_PER : 1];
+    if constexpr (VISC) visc_prefetch_z(B, a0, l0, cc, nc, nlines, pv);
 
     // (line, slot) -> LDS offset: slot-fastest for DIR 0, line-fastest otherwise
     auto sidx = [](int line, int slot) { return DIR == 0 ? line * NSL + slot : slot * NL + line; };
@@ -496,19 +573,27 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     unsigned long long nfix = 0;
     double r2 = 0.0;  // residual partial (stage 2)
     const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
+    static_assert(!VISC || 10 * NSL * NL >= VZ_LDS, "viscous staging fits the split-flux area");
+    if constexpr (VISC) {
+        visc_store_z(&S[0][0], pv);
+        __syncthreads();
+    }
     if (line < nlines && f >= 1 && f <= nc) {
         int i, j, k;
         cell(line, a0 + f - 1, i, j, k);
         double dF[5];
 #pragma unroll
         for (int c = 0; c < 5; c++) dF[c] = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
-        finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
+        if constexpr (VISC)
+            finish_cell_visc_z(blocks, B, K, P, &S[0][0], line, f - 1, i, j, k, dF, stage, bin, bun, bout, dt, lmax,
+                               nfix, r2);
+        else
+            finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
     }
     if (DIR == 2 && MODE == 1) {
         if (nfix) atomicAdd(counters, nfix);
         if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * t.x);
-        // for Navier-Stokes k_visc_div finishes the stage and holds the partials
-        if (!NS && stage == 2 && K.resid) block_sum_store(r2, K.resid + witem);
+        if (stage == 2 && K.resid) block_sum_store(r2, K.resid + witem);
     }
 }
 
@@ -584,6 +669,9 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     const int nc = min(NC, B.n[DIR] - a0);
     const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
     double *const *qin = B.q[bin];
+    constexpr bool VISC = NS && DIR == 2 && MODE == 1;  // Navier-Stokes stage: viscous divergence here
+    double pv[VISC ? VZ_PER : 1];
+    if constexpr (VISC) visc_prefetch_z(B, a0, l0, cc, nc, nlines, pv);
 
     auto sidx = [](int line, int slot) { return DIR == 0 ? line * NSL + slot : slot * NL + line; };
     auto hidx = [](int line, int f) { return DIR == 0 ? line * NF + f : f * NL + line; };
@@ -751,18 +839,27 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     unsigned long long nfix = 0;
     double r2 = 0.0;
     const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
+    static_assert(!VISC || 10 * NSL * NL >= VZ_LDS, "viscous staging fits the split-flux area");
+    if constexpr (VISC) {
+        visc_store_z(&S[0][0], pv);
+        __syncthreads();
+    }
     if (line < nlines && f >= 1 && f <= nc) {
         int i, j, k;
         cell(line, a0 + f - 1, i, j, k);
         double dF[5];
 #pragma unroll
         for (int c = 0; c < 5; c++) dF[c] = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
-        finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
+        if constexpr (VISC)
+            finish_cell_visc_z(blocks, B, K, P, &S[0][0], line, f - 1, i, j, k, dF, stage, bin, bun, bout, dt, lmax,
+                               nfix, r2);
+        else
+            finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
     }
     if (DIR == 2 && MODE == 1) {
         if (nfix) atomicAdd(counters, nfix);
         if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * t.x);
-        if (!NS && stage == 2 && K.resid) block_sum_store(r2, K.resid + witem);
+        if (stage == 2 && K.resid) block_sum_store(r2, K.resid + witem);
     }
 }
 
