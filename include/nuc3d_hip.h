@@ -195,7 +195,9 @@ int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt);
  * default.  For Navier-Stokes cases this takes the distinct block shapes in
  * turn and, for kseg = 8, 16, 32, 64, 128 (up to twice the block's depth) on
  * every block of that shape, the other blocks keeping their current choice,
- * times the two viscous passes of one RHS over the whole case on the current state;
+ * times the column-tile passes of one stage (the viscous fluxes, and the viscous
+ * divergence where CRWENO5 z lines leave it to its own pass) over the whole
+ * case on the current state;
  * it keeps the fastest per shape.  kseg_out (nblocks, may be NULL) receives the choices.  Results do
  * not change, except the rounding of the diagnostics' partial sums; the
  * residual history restarts (the next nuc3d_get_residual after one step is 0).
@@ -223,10 +225,12 @@ int nuc3d_get_fixups(nuc3d_case *c, long long *count);
 /* Kernel timing (no counterpart in the reference, which times whole steps
  * with MPI_Wtime in block::solve, block.cpp:576-584).  While enabled, every
  * kernel launch is bracketed by HIP events on its stream, in NUC3D_NTIMING
- * classes: 0 x sweep, 1 y sweep, 2 z sweep (with the RK update for Euler),
- * and for Navier-Stokes 4 viscous fluxes (u,v,w,T computed in the same
- * pass), 5 viscous divergence with the RK update; class 3 (a separate
- * primitives pass) is no longer launched by the stage pipeline.  nuc3d_timing_read returns the summed
+ * classes: 0 x sweep, 1 y sweep, 2 z sweep (with the RK update), and for
+ * Navier-Stokes 4 viscous fluxes (u,v,w,T computed in the same pass), 5
+ * viscous divergence (single RHS evaluations, and the stages of cases with
+ * CRWENO5 z lines; the other stages take it in the z sweep, class 2);
+ * class 3 (a separate primitives pass) is no longer launched.
+ * nuc3d_timing_read returns the summed
  * milliseconds and launch counts per class since the last read (arrays of
  * NUC3D_NTIMING) and resets them. */
 #define NUC3D_NTIMING 6

@@ -1164,11 +1164,13 @@ int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out)
                 }
                 hipError_t e = hipMemcpy(dt, tiles.data(), sizeof(nuc3d_tile) * tiles.size(), hipMemcpyHostToDevice);
                 const int nt = (int)tiles.size();
-                // the column-tile passes of one RHS over the block: viscous
-                // fluxes, viscous divergence into the RHS (no state change)
+                // the column-tile passes of one stage over the block: viscous
+                // fluxes, and where the z sweep does not take it the viscous
+                // divergence (into the RHS: no state change)
+                const bool div_pass = !visc_in_z_sweep(c);
                 auto pass = [&]() {
                     hipError_t r = c->L->vflux(c->db, dt, nt, c->K, c->cur, s);
-                    if (r == hipSuccess)
+                    if (r == hipSuccess && div_pass)
                         r = c->L->vdiv(0, c->db, dt, nt, c->K, 0, c->cur, 0, 0, c->ddt, c->dlam, c->dlam, c->dcount, s);
                     return r;
                 };
