@@ -124,7 +124,7 @@ struct nuc3d_case {
     std::vector<void *> chunks;        // one device allocation per block
     std::vector<long long> padded;     // padded cells per block
     nuc3d_block_dev *db = nullptr;     // device descriptors
-    nuc3d_tile *dtiles = nullptr;      // 16x16 column tiles with k segments (pointwise kernels)
+    nuc3d_tile *dtiles = nullptr;      // column tiles (NUC3D_TILE_X x NUC3D_TILE_Y) with k segments (pointwise kernels)
     int ntiles = 0;
     std::vector<int> kseg;             // k-segment length of the column tiles per block (nuc3d_autotune)
     int kmax = NUC3D_KSEG;             // the longest
@@ -425,8 +425,8 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         }
         c->kseg.push_back(KSEG);
         for (int k0 = 0; k0 < B.n[2]; k0 += KSEG)
-            for (int j0 = 0; j0 < B.n[1]; j0 += NUC3D_TILE)
-                for (int i0 = 0; i0 < B.n[0]; i0 += NUC3D_TILE) {
+            for (int j0 = 0; j0 < B.n[1]; j0 += NUC3D_TILE_Y)
+                for (int i0 = 0; i0 < B.n[0]; i0 += NUC3D_TILE_X) {
                     nuc3d_tile t = {b, i0, j0, k0, std::min(KSEG, B.n[2] - k0), 0, 0, 0};
                     tiles.push_back(t);
                 }
@@ -1073,8 +1073,8 @@ std::vector<nuc3d_tile> column_tiles(const nuc3d_case *c, const std::vector<int>
         const nuc3d_block_dev &B = c->hb[b];
         const int ks = kseg[b];
         for (int k0 = 0; k0 < B.n[2]; k0 += ks)
-            for (int j0 = 0; j0 < B.n[1]; j0 += NUC3D_TILE)
-                for (int i0 = 0; i0 < B.n[0]; i0 += NUC3D_TILE) {
+            for (int j0 = 0; j0 < B.n[1]; j0 += NUC3D_TILE_Y)
+                for (int i0 = 0; i0 < B.n[0]; i0 += NUC3D_TILE_X) {
                     nuc3d_tile t = {b, i0, j0, k0, std::min(ks, B.n[2] - k0), 0, 0, 0};
                     tiles.push_back(t);
                 }

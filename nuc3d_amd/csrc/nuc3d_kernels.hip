@@ -1077,7 +1077,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 
 // ---------------------------------------------------------------------------
 // Navier-Stokes viscous terms (NaiverStokes3d.cpp:51-513) in two passes
-// over 16x16 column tiles marching in k.
+// over 16x16 column tiles (NUC3D_TILE_X x NUC3D_TILE_Y) marching in k.
 //
 // k_visc_flux: u, v, w, T from Q/J and J as each plane arrives (con2prim,
 //              physicsModel.cpp:313-353, T = gamma Ma^2 p / rho,
@@ -1142,7 +1142,7 @@ __device__ __forceinline__ void prim4(const nuc3d_block_dev &B, double *const *q
 __global__ __launch_bounds__(256) void KNAME(k_prim)(const nuc3d_block_dev *__restrict__ blocks,
                                                      const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int bin)
 {
-    constexpr int T = NUC3D_TILE;
+    constexpr int T = NUC3D_TILE_X;
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     if ((int)blockIdx.y >= tl.kn) return;
     const nuc3d_block_dev &B = blocks[tl.b];
@@ -1165,20 +1165,21 @@ __global__ __launch_bounds__(256) void KNAME(k_prim)(const nuc3d_block_dev *__re
     }
 }
 
-// Halo cell h (0..191) of a 16x16 column tile: h < 96 are the strips of 3
-// cells left and right of each row (x halo), h >= 96 the strips above and
-// below each column (y halo); no corners (the derivatives are not mixed).
-// Tile-local (li, lj); false for rows/columns beyond a partial tile.
+// Halo cell h of a TX x TY column tile (0 <= h < 6 (TX + TY)): h < 6 TY are
+// the strips of 3 cells left and right of each row (x halo), then the rows of
+// 3 cells below and above the tile (y halo, consecutive h along i); no corners
+// (the derivatives are not mixed).  Tile-local (li, lj); false for rows or
+// columns beyond a partial tile.
 __device__ __forceinline__ bool halo_cell(int h, int tw, int th, int &li, int &lj)
 {
-    constexpr int H = NUC3D_HALO;
-    if (h < 16 * 2 * H) {
+    constexpr int H = NUC3D_HALO, TX = NUC3D_TILE_X, TY = NUC3D_TILE_Y;
+    if (h < TY * 2 * H) {
         const int r = h / (2 * H), s = h % (2 * H);
         if (r >= th) return false;
         lj = r;
         li = s < H ? s - H : tw + (s - H);
     } else {
-        const int c = (h - 16 * 2 * H) / (2 * H), s = (h - 16 * 2 * H) % (2 * H);
+        const int c = (h - TY * 2 * H) % TX, s = (h - TY * 2 * H) / TX;
         if (c >= tw) return false;
         li = c;
         lj = s < H ? s - H : th + (s - H);
@@ -1232,18 +1233,18 @@ __device__ __forceinline__ void visc_flux_cell(const nuc3d_kparams &K, const Phy
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_FLUX_WAVES))) void KNAME(k_visc_flux)(
     const nuc3d_block_dev *__restrict__ blocks, const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int bin)
 {
-    constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
-    __shared__ double Wp[2][4][L][L];  // u,v,w,T of plane k with the xy halo, double-buffered
+    constexpr int TX = NUC3D_TILE_X, TY = NUC3D_TILE_Y, H = NUC3D_HALO, LX = TX + 2 * H, LY = TY + 2 * H;
+    __shared__ double Wp[2][4][LY][LX];  // u,v,w,T of plane k with the xy halo, double-buffered
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K, B);
     double *const *qb = B.q[bin];
-    const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
-    const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
+    const int ti = threadIdx.x & (TX - 1), tj = threadIdx.x / TX;
+    const int tw = min(TX, B.n[0] - tl.i0), th = min(TY, B.n[1] - tl.j0);
     const bool incell = ti < tw && tj < th;
     const int i = tl.i0 + ti, j = tl.j0 + tj;
     int hi = 0, hj = 0;
-    const bool hv = threadIdx.x < 2 * (2 * H * T) && halo_cell(threadIdx.x, tw, th, hi, hj);
+    const bool hv = threadIdx.x < 2 * H * (TX + TY) && halo_cell(threadIdx.x, tw, th, hi, hj);
     const long long xc = cidx(B, i, j, 0), xh = cidx(B, tl.i0 + hi, tl.j0 + hj, 0), sz = B.sz;
     const int kb = tl.k0, ke = tl.k0 + tl.kn;
 
@@ -1297,7 +1298,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
 #pragma unroll
             for (int a = 0; a < 2 * H + 1; a++) col[a] = zr[a][v];
             dxi[v] = line_diff_lds(B, K, &Wp[buf][v][tj + H][ti + H], 1, 0, i);
-            deta[v] = line_diff_lds(B, K, &Wp[buf][v][tj + H][ti + H], L, 1, j);
+            deta[v] = line_diff_lds(B, K, &Wp[buf][v][tj + H][ti + H], LX, 1, j);
             dzeta[v] = line_diff_lds(B, K, &col[H], 1, 2, k);
             w4[v] = zr[H][v];
         }
@@ -1324,19 +1325,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
     int bin, int bun, int bout, double *dtbuf, const unsigned long long *lam_cur, unsigned long long *lam_next,
     unsigned long long *counters)
 {
-    constexpr int T = NUC3D_TILE, H = NUC3D_HALO, L = T + 2 * H;
-    __shared__ double Vx[2][4][T][L];  // fv_xi (components 1..4) of plane k, x halo, double-buffered
-    __shared__ double Vy[2][4][L][T];  // fv_eta, y halo
+    constexpr int TX = NUC3D_TILE_X, TY = NUC3D_TILE_Y, H = NUC3D_HALO, LX = TX + 2 * H, LY = TY + 2 * H;
+    __shared__ double Vx[2][4][TY][LX];  // fv_xi (components 1..4) of plane k, x halo, double-buffered
+    __shared__ double Vy[2][4][LY][TX];  // fv_eta, y halo
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K, B);
-    const int ti = threadIdx.x & (T - 1), tj = threadIdx.x / T;
-    const int tw = min(T, B.n[0] - tl.i0), th = min(T, B.n[1] - tl.j0);
+    const int ti = threadIdx.x & (TX - 1), tj = threadIdx.x / TX;
+    const int tw = min(TX, B.n[0] - tl.i0), th = min(TY, B.n[1] - tl.j0);
     const bool incell = ti < tw && tj < th;
     const int i = tl.i0 + ti, j = tl.j0 + tj;
     int hi = 0, hj = 0;
-    const bool hv = threadIdx.x < 2 * (2 * H * T) && halo_cell(threadIdx.x, tw, th, hi, hj);
-    const bool hx = threadIdx.x < 2 * H * T;  // x halo (fv_xi) or y halo (fv_eta)
+    const bool hv = threadIdx.x < 2 * H * (TX + TY) && halo_cell(threadIdx.x, tw, th, hi, hj);
+    const bool hx = threadIdx.x < 2 * H * TY;  // x halo (fv_xi) or y halo (fv_eta)
     const long long xc = cidx(B, i, j, 0), xh = cidx(B, tl.i0 + hi, tl.j0 + hj, 0), sz = B.sz;
     const int kb = tl.k0, ke = tl.k0 + tl.kn;
     const double dt = MODE == 1 ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
@@ -1410,7 +1411,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
 #pragma unroll
             for (int a = 0; a < 2 * H + 1; a++) col[a] = zr[a][v];
             const double gx = line_diff_lds(B, K, &Vx[buf][v][tj][ti + H], 1, 0, i);
-            const double gy = line_diff_lds(B, K, &Vy[buf][v][tj + H][ti], T, 1, j);
+            const double gy = line_diff_lds(B, K, &Vy[buf][v][tj + H][ti], TX, 1, j);
             const double gz = line_diff_lds(B, K, &col[H], 1, 2, k);
             r[1 + v] = ((r[1 + v] - gx) - gy) - gz;
         }
@@ -1441,7 +1442,7 @@ __global__ __launch_bounds__(256) void KNAME(k_diag)(const nuc3d_block_dev *__re
                                                      const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int bin,
                                                      int bf, double *__restrict__ fields, double *__restrict__ partial)
 {
-    constexpr int T = NUC3D_TILE;
+    constexpr int T = NUC3D_TILE_X;
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T, k = tl.k0 + blockIdx.y;
@@ -1502,7 +1503,7 @@ __global__ __launch_bounds__(256) void KNAME(k_eigen)(const nuc3d_block_dev *__r
                                                       const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int buf,
                                                       unsigned long long *lam)
 {
-    constexpr int T = NUC3D_TILE;
+    constexpr int T = NUC3D_TILE_X;
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K, B);
@@ -1534,7 +1535,7 @@ __global__ __launch_bounds__(256) void KNAME(k_fixup)(const nuc3d_block_dev *__r
                                                       const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int buf,
                                                       unsigned long long *counters)
 {
-    constexpr int T = NUC3D_TILE;
+    constexpr int T = NUC3D_TILE_X;
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
     const PhysParams P = phys_of(K, B);
