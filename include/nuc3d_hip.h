@@ -294,6 +294,16 @@ int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold);
  * ((n_d+1) x n_{d+1} x n_{d+2}, face index fastest): 0 central, 1 WENO5-JS by
  * the network, 2 WENO5-JS by the fallback, 3 physical-face closure. */
 int nuc3d_download_hybrid_flags(nuc3d_case *c, int b, int d, unsigned char *flags);
+/* Uniform-mesh path (no counterpart in the reference, whose blocks always
+ * hold J and the nine metric fields).  When enabled (the default), a block
+ * whose J and metric terms are each one value, bit for bit, over its interior
+ * and the ghost slabs of its neighbour faces (a uniform Cartesian block) has
+ * the stage kernels take them from its descriptor instead of loading the
+ * fields; results are identical, the HBM traffic per cell lower.  The check
+ * runs whenever the case's metrics, faces or state were replaced.
+ * nuc3d_get_uniform_mesh reports whether block b takes that path. */
+int nuc3d_set_uniform_mesh(nuc3d_case *c, int enable);
+int nuc3d_get_uniform_mesh(nuc3d_case *c, int b, int *uniform);
 /* Steps between hybrid decisions: 1 (default, stage 0 of every step) or 2
  * (stage 0 of every other step; the stages between reuse them). */
 int nuc3d_set_sensor_interval(nuc3d_case *c, int steps);

@@ -53,6 +53,7 @@ extern "C" hipError_t nuc3d_launch_sensor_adam(float *, float *, float *, const 
                                                hipStream_t);
 extern "C" hipError_t nuc3d_launch_mesh_metrics(nuc3d_block_dev, const double *, const double *, const double *,
                                                 double *, int *, hipStream_t);
+extern "C" hipError_t nuc3d_launch_uniform_check(nuc3d_block_dev, int *, hipStream_t);
 
 namespace {
 
@@ -157,6 +158,9 @@ struct nuc3d_case {
     nuc3d_sensor_dev *dsensor = nullptr;
     std::vector<void *> dflags;
     int hybrid_interval = 1;           // steps between hybrid decisions (1 or 2)
+    // uniform blocks take J and the metric terms from their descriptor
+    bool uniform_mesh = true;
+    int *duniform = nullptr;           // [nb] uniform-mesh check flags
     // kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -246,9 +250,27 @@ struct Timed {
 int refresh(nuc3d_case *c, hipStream_t s)
 {
     if (!c->dirty) return NUC3D_OK;
+    for (auto &B : c->hb) B.uniform = 0;
     HIPCHK(hipMemcpy(c->db, c->hb.data(), sizeof(nuc3d_block_dev) * c->nb, hipMemcpyHostToDevice));
     // ghosts of J and metrics, fix-up and ghosts of the state, eigenvalues
     HIPCHK(c->L->ghost(c->db, c->nb, 1, c->cur, s));
+    // blocks whose J and metric terms are one value each (uniform Cartesian
+    // meshes): the stage kernels take them from the descriptor
+    if (c->uniform_mesh) {
+        HIPCHK(hipMemsetAsync(c->duniform, 0, sizeof(int) * c->nb, s));
+        for (int b = 0; b < c->nb; b++) HIPCHK(nuc3d_launch_uniform_check(c->hb[b], c->duniform + b, s));
+        std::vector<int> bad(c->nb);
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipMemcpy(bad.data(), c->duniform, sizeof(int) * c->nb, hipMemcpyDeviceToHost));
+        for (int b = 0; b < c->nb; b++) {
+            nuc3d_block_dev &B = c->hb[b];
+            if (bad[b]) continue;
+            HIPCHK(hipMemcpy(&B.jac0, B.jac, sizeof(double), hipMemcpyDeviceToHost));
+            for (int m = 0; m < 9; m++) HIPCHK(hipMemcpy(&B.met0[m], B.met[m], sizeof(double), hipMemcpyDeviceToHost));
+            B.uniform = 1;
+        }
+        HIPCHK(hipMemcpy(c->db, c->hb.data(), sizeof(nuc3d_block_dev) * c->nb, hipMemcpyHostToDevice));
+    }
     HIPCHK(c->L->fixup(c->db, c->dtiles, c->ntiles, c->K, c->cur, c->dcount, s));
     HIPCHK(c->L->ghost(c->db, c->nb, 0, c->cur, s));
     HIPCHK(hipMemsetAsync(c->dlam + 3 * c->nb * c->lampar, 0, sizeof(unsigned long long) * 3 * c->nb, s));
@@ -512,7 +534,8 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         nuc3d_case_destroy(c);
         return fail(NUC3D_ERR_OOM, "device allocation of residual partials failed");
     }
-    if (hipMalloc(&c->db, sizeof(nuc3d_block_dev) * nblocks) != hipSuccess ||
+    if (hipMalloc(&c->duniform, sizeof(int) * nblocks) != hipSuccess ||
+        hipMalloc(&c->db, sizeof(nuc3d_block_dev) * nblocks) != hipSuccess ||
         hipMalloc(&c->dtiles, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess ||
         hipMalloc(&c->ddt, sizeof(double) * NUC3D_MAX_BLOCKS) != hipSuccess ||
         hipMalloc(&c->dlam, sizeof(unsigned long long) * 6 * nblocks) != hipSuccess ||
@@ -546,6 +569,7 @@ int nuc3d_case_destroy(nuc3d_case *c)
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (void *m : c->chunks) (void)hipFree(m);
     (void)hipFree(c->db);
+    (void)hipFree(c->duniform);
     (void)hipFree(c->dtiles);
     for (int d = 0; d < 3; d++) (void)hipFree(c->dsweep[d]);
     for (int d = 0; d < 3; d++) (void)hipFree(c->dlines[d]);
@@ -1505,6 +1529,25 @@ int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold)
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(c->dsensor, &h, sizeof h, hipMemcpyHostToDevice));
     c->K.sensor = c->dsensor;
+    return NUC3D_OK;
+}
+
+int nuc3d_set_uniform_mesh(nuc3d_case *c, int enable)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "switch the uniform-mesh path between full steps");
+    c->uniform_mesh = enable != 0;
+    c->dirty = true;
+    return NUC3D_OK;
+}
+
+int nuc3d_get_uniform_mesh(nuc3d_case *c, int b, int *uniform)
+{
+    if (!c || !uniform) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    if (b < 0 || b >= c->nb) return fail(NUC3D_ERR_BAD_ARG, "block index out of range");
+    int rc = refresh(c, nullptr);
+    if (rc) return rc;
+    *uniform = c->hb[b].uniform;
     return NUC3D_OK;
 }
 

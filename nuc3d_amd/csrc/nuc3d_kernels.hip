@@ -67,6 +67,18 @@ __device__ __forceinline__ long long dstride(const nuc3d_block_dev &B, int d)
     return d == 0 ? 1LL : (d == 1 ? B.sy : B.sz);
 }
 
+// J and metric term e of cell x (uniform blocks: the block's constants, the
+// same values as the fields hold)
+__device__ __forceinline__ double jac_at(const nuc3d_block_dev &B, long long x)
+{
+    return B.uniform ? B.jac0 : B.jac[x];
+}
+
+__device__ __forceinline__ double met_at(const nuc3d_block_dev &B, int e, long long x)
+{
+    return B.uniform ? B.met0[e] : B.met[e][x];
+}
+
 // split fluxes of cell x of buffer qb in direction d; returns the local
 // eigenvalue.  prim (may be null) receives rho, u, v, w, p.
 __device__ __forceinline__ double cell_split(const nuc3d_block_dev &B, double *const *qb, long long x, int d,
@@ -75,7 +87,7 @@ __device__ __forceinline__ double cell_split(const nuc3d_block_dev &B, double *c
     double Q[5];
 #pragma unroll
     for (int c = 0; c < 5; c++) Q[c] = qb[c][x];
-    double jac = B.jac[x];
+    double jac = jac_at(B, x);
     double rho, u, v, w, p, cs;
     con2prim(Q, jac, P, rho, u, v, w, p, cs);
     if (prim) {
@@ -85,8 +97,8 @@ __device__ __forceinline__ double cell_split(const nuc3d_block_dev &B, double *c
         prim[3] = w;
         prim[4] = p;
     }
-    return split_cell(Q, jac, B.met[3 * d][x], B.met[3 * d + 1][x], B.met[3 * d + 2][x], u, v, w, p, cs, P.riemann,
-                      fp, fn);
+    return split_cell(Q, jac, met_at(B, 3 * d, x), met_at(B, 3 * d + 1, x), met_at(B, 3 * d + 2, x), u, v, w, p, cs,
+                      P.riemann, fp, fn);
 }
 
 // BC ghost split flux of physical face f from its boundary cell xb
@@ -97,7 +109,7 @@ __device__ __forceinline__ void bc_ghost(const nuc3d_block_dev &B, double *const
 {
     const int d = f / 2, id = B.fid[f];
     double prim[5];
-    const double jac = B.jac[xb];
+    const double jac = jac_at(B, xb);
     if (id == BC_INLET) {
 #pragma unroll
         for (int c = 0; c < 5; c++) prim[c] = B.fval[f][c];
@@ -114,9 +126,9 @@ __device__ __forceinline__ void bc_ghost(const nuc3d_block_dev &B, double *const
         prim[2] = 0.0;
         prim[3] = 0.0;
     } else if (id == BC_SYMM) {
-        const double U = B.met[0][xb] * prim[1] + B.met[1][xb] * prim[2] + B.met[2][xb] * prim[3];
-        const double V = B.met[3][xb] * prim[1] + B.met[4][xb] * prim[2] + B.met[5][xb] * prim[3];
-        const double W = B.met[6][xb] * prim[1] + B.met[7][xb] * prim[2] + B.met[8][xb] * prim[3];
+        const double U = met_at(B, 0, xb) * prim[1] + met_at(B, 1, xb) * prim[2] + met_at(B, 2, xb) * prim[3];
+        const double V = met_at(B, 3, xb) * prim[1] + met_at(B, 4, xb) * prim[2] + met_at(B, 5, xb) * prim[3];
+        const double W = met_at(B, 6, xb) * prim[1] + met_at(B, 7, xb) * prim[2] + met_at(B, 8, xb) * prim[3];
 #pragma unroll
         for (int m = 0; m < 3; m++) {
             const double xx = B.dxyz[3 * m][xb], xe = B.dxyz[3 * m + 1][xb], xz = B.dxyz[3 * m + 2][xb];
@@ -128,7 +140,7 @@ __device__ __forceinline__ void bc_ghost(const nuc3d_block_dev &B, double *const
                 prim[1 + m] = xx * U + xe * V - xz * W;
         }
     }
-    split_point(prim, jac, B.met[3 * d][xb], B.met[3 * d + 1][xb], B.met[3 * d + 2][xb], P, fl, fr);
+    split_point(prim, jac, met_at(B, 3 * d, xb), met_at(B, 3 * d + 1, xb), met_at(B, 3 * d + 2, xb), P, fl, fr);
 }
 
 // Split fluxes of cell (i,j,k) in direction d, where the cell may lie in the
@@ -359,7 +371,7 @@ __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, co
         con2prim(qn, jac, P, rho, u, v, w, pr, cs);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const double mx = B.met[3 * d][x], my = B.met[3 * d + 1][x], mz = B.met[3 * d + 2][x];
+            const double mx = met_at(B, 3 * d, x), my = met_at(B, 3 * d + 1, x), mz = met_at(B, 3 * d + 2, x);
             const double theta = sqrt(mx * mx + my * my + mz * mz);
             const double U0 = (mx * u + my * v + mz * w);
             const double lam = fabs(U0) + cs * theta;
@@ -380,7 +392,7 @@ __device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const n
         u5[c] = B.q[bin][c][x];
         un5[c] = stage == 0 ? 0.0 : B.q[bun][c][x];
     }
-    rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, B.jac[x], stage, bout, dt, lmax, nfix, r2);
+    rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, jac_at(B, x), stage, bout, dt, lmax, nfix, r2);
 }
 
 // Cell (i,j,k) receives the flux difference dF of direction DIR:
@@ -1124,7 +1136,7 @@ __device__ __forceinline__ void load_q6(const nuc3d_block_dev &B, double *const 
 {
 #pragma unroll
     for (int c = 0; c < 5; c++) q6[c] = qb[c][x];
-    q6[5] = B.jac[x];
+    q6[5] = jac_at(B, x);
 }
 
 __device__ __forceinline__ void prim4(const nuc3d_block_dev &B, double *const *qb, long long x, const PhysParams &P,
@@ -1284,9 +1296,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
         if (k + 1 >= kb && k + 1 < ke) {
             if (hv) load_q6(B, qb, xh + (k + 1) * sz, ph);
             if (incell) {
-                pm[0] = B.jac[xc + (k + 1) * sz];
+                pm[0] = jac_at(B, xc + (k + 1) * sz);
 #pragma unroll
-                for (int e = 0; e < 9; e++) pm[1 + e] = B.met[e][xc + (k + 1) * sz];
+                for (int e = 0; e < 9; e++) pm[1 + e] = met_at(B, e, xc + (k + 1) * sz);
             }
         }
         __syncthreads();
@@ -1454,16 +1466,16 @@ __global__ __launch_bounds__(256) void KNAME(k_diag)(const nuc3d_block_dev *__re
         for (int v = 0; v < 3; v++) {
             const double *f = B.prim[v] + x;
             const double a = line_diff(B, K, f, 0, i), bb = line_diff(B, K, f, 1, j), cc = line_diff(B, K, f, 2, k);
-            g[v][0] = a * B.met[0][x] + bb * B.met[3][x] + cc * B.met[6][x];
-            g[v][1] = a * B.met[1][x] + bb * B.met[4][x] + cc * B.met[7][x];
-            g[v][2] = a * B.met[2][x] + bb * B.met[5][x] + cc * B.met[8][x];
+            g[v][0] = a * met_at(B, 0, x) + bb * met_at(B, 3, x) + cc * met_at(B, 6, x);
+            g[v][1] = a * met_at(B, 1, x) + bb * met_at(B, 4, x) + cc * met_at(B, 7, x);
+            g[v][2] = a * met_at(B, 2, x) + bb * met_at(B, 5, x) + cc * met_at(B, 8, x);
         }
         const double ux = g[0][0], uy = g[0][1], uz = g[0][2];
         const double vx = g[1][0], vy = g[1][1], vz = g[1][2];
         const double wx = g[2][0], wy = g[2][1], wz = g[2][2];
         const double w0 = wy - vz, w1 = uz - wx, w2 = vx - uy;
         const double q0 = ux * vy + ux * wz + vy * wz - uy * vx - uz * wx - vz * wy;
-        const double rho = B.q[bin][0][x] * B.jac[x];  // physicsModel.cpp:313-353
+        const double rho = B.q[bin][0][x] * jac_at(B, x);  // physicsModel.cpp:313-353
         const double u = B.prim[0][x], v = B.prim[1][x], w = B.prim[2][x];
         ens = 0.5 * (w0 * w0 + w1 * w1 + w2 * w2);
         kin = 0.5 * rho * (u * u + v * v + w * w);
@@ -1516,10 +1528,10 @@ __global__ __launch_bounds__(256) void KNAME(k_eigen)(const nuc3d_block_dev *__r
 #pragma unroll
             for (int c = 0; c < 5; c++) Q[c] = B.q[buf][c][x];
             double rho, u, v, w, p, cs;
-            con2prim(Q, B.jac[x], P, rho, u, v, w, p, cs);
+            con2prim(Q, jac_at(B, x), P, rho, u, v, w, p, cs);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const double mx = B.met[3 * d][x], my = B.met[3 * d + 1][x], mz = B.met[3 * d + 2][x];
+                const double mx = met_at(B, 3 * d, x), my = met_at(B, 3 * d + 1, x), mz = met_at(B, 3 * d + 2, x);
                 const double theta = sqrt(mx * mx + my * my + mz * mz);
                 const double U0 = (mx * u + my * v + mz * w);
                 const double l = fabs(U0) + cs * theta;
@@ -1547,7 +1559,7 @@ __global__ __launch_bounds__(256) void KNAME(k_fixup)(const nuc3d_block_dev *__r
         double Q[5];
 #pragma unroll
         for (int c = 0; c < 5; c++) Q[c] = B.q[buf][c][x];
-        if (fixup_energy(Q, B.jac[x], P)) {
+        if (fixup_energy(Q, jac_at(B, x), P)) {
             B.q[buf][4][x] = Q[4];
             nfix++;
         }

@@ -263,6 +263,47 @@ extern "C" hipError_t nuc3d_launch_op_rk3(int stage, double dt, long long n, con
     return hipGetLastError();
 }
 
+// Uniform-mesh check of a block (no counterpart in the reference): bad[0]
+// becomes 1 if J or a metric term of a cell that the stage kernels read (the
+// interior, and the ghost slabs of the faces joined to a neighbour) differs,
+// bit for bit, from cell (0,0,0)'s.  Edge and corner ghosts and the ghost
+// slabs of physical faces are not read by them.
+__global__ __launch_bounds__(256) void k_uniform_check(nuc3d_block_dev B, int *bad)
+{
+    const int G = B.G;
+    const long long px = B.n[0] + 2LL * G, py = B.n[1] + 2LL * G, pz = B.n[2] + 2LL * G;
+    const long long total = px * py * pz;
+    const unsigned long long j0 = __double_as_longlong(B.jac[0]);
+    int mism = 0;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+        const int c3[3] = {(int)(e % px) - G, (int)((e / px) % py) - G, (int)(e / (px * py)) - G};
+        int out = 0, face = 0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (c3[d] < 0) {
+                out++;
+                face = 2 * d;
+            } else if (c3[d] >= B.n[d]) {
+                out++;
+                face = 2 * d + 1;
+            }
+        }
+        if (out > 1 || (out == 1 && B.ftype[face] < 0)) continue;
+        const long long x = c3[0] + B.sy * c3[1] + B.sz * c3[2];
+        if ((unsigned long long)__double_as_longlong(B.jac[x]) != j0) mism = 1;
+#pragma unroll
+        for (int m = 0; m < 9; m++)
+            if (__double_as_longlong(B.met[m][x]) != __double_as_longlong(B.met[m][0])) mism = 1;
+    }
+    if (mism) atomicOr(bad, 1);
+}
+
+extern "C" hipError_t nuc3d_launch_uniform_check(nuc3d_block_dev B, int *bad, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_uniform_check, dim3(1024), dim3(256), 0, st, B, bad);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t nuc3d_launch_mesh_metrics(nuc3d_block_dev B, const double *xn, const double *yn,
                                                 const double *zn, double *centre, int *bad, hipStream_t st)
 {

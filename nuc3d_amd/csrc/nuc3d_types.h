@@ -40,6 +40,13 @@ struct nuc3d_block_dev {
     int group;
     int pad_g;
     double mach, reynolds;
+    // uniform mesh: J and every metric term equal jac0 and met0 in the
+    // interior and in the ghost slabs of the neighbour faces (uniform
+    // Cartesian blocks); the stage kernels then take them from here instead
+    // of loading the fields (the fields stay valid for every other user)
+    int uniform;
+    int pad_u;
+    double jac0, met0[9];
 };
 
 // Learned sensor of the hybrid reconstruction: 30 features -> 32 -> 32 -> 1.

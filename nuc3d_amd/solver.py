@@ -136,6 +136,8 @@ def lib():
         "nuc3d_download_hybrid_flags": [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte)],
         "nuc3d_get_hybrid_counts": [vp, llp],
         "nuc3d_set_sensor_interval": [vp, ctypes.c_int],
+        "nuc3d_set_uniform_mesh": [vp, ctypes.c_int],
+        "nuc3d_get_uniform_mesh": [vp, ctypes.c_int, ip],
         "nuc3d_op_interpolate": [ctypes.c_int] * 8 + [dp, dp, dp, vp],
         "nuc3d_op_differentiate": [ctypes.c_int] * 6 + [dp, dp, dp, vp],
         "nuc3d_op_rk3": [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_longlong,
@@ -351,6 +353,17 @@ class Case:
         assert p.size == SENSOR_NPARAM
         _check(lib().nuc3d_set_sensor(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), float(threshold)),
                "set_sensor")
+
+    def set_uniform_mesh(self, enable):
+        """Let blocks of one J and one value per metric term take them from
+        their descriptor (default on; results are identical)."""
+        _check(lib().nuc3d_set_uniform_mesh(self.h, 1 if enable else 0), "set_uniform_mesh")
+
+    def uniform_mesh(self, b):
+        """Whether block b takes the uniform-mesh path."""
+        v = ctypes.c_int(0)
+        _check(lib().nuc3d_get_uniform_mesh(self.h, int(b), ctypes.byref(v)), "get_uniform_mesh")
+        return bool(v.value)
 
     def set_sensor_interval(self, steps):
         """Steps between hybrid decisions (1 or 2)."""
