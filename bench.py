@@ -37,14 +37,19 @@ METRIC = "cell-updates/s per RK3 step FP64: plain WENO5 vs learned hybrid at equ
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per cell-update of one fused TVD-RK3 step: stage 0
 # reads Q (40) and J + nine metric terms (80) and writes Q1 (40); stages 1
-# and 2 also read Q^n (40): 160 + 200 + 200 = 560.
+# and 2 also read Q^n (40): 160 + 200 + 200 = 560.  Blocks on the uniform-mesh
+# path (J and the metric terms one value each, taken from the block's
+# descriptor) skip the 80 B of metrics per stage: 320.
 ALG_BYTES_PER_STEP = 560.0
+ALG_BYTES_PER_STEP_UNIFORM = 320.0
 
 
-def kernel_alg_bytes(cls, ns):
+def kernel_alg_bytes(cls, ns, uniform=False):
     """Compulsory HBM bytes per cell of one launch of a kernel class, averaged
     over the three RK stages (every input field read once, every output
-    written once; ghost layers and halo re-reads not counted):
+    written once; ghost layers and halo re-reads not counted); with
+    `uniform` the J and metric bytes are left out (the uniform-mesh path
+    does not load them):
       x sweep:      Q 40 + J 8 + xi metrics 24 + RHS write 40                 = 112
       y sweep:      Q 40 + J 8 + eta metrics 24 + RHS read and write 80        = 152
       z sweep:      Q 40 + J 8 + zeta metrics 24 + RHS read 40, Q^n 40 at
@@ -57,16 +62,19 @@ def kernel_alg_bytes(cls, ns):
       viscous div:  fv 96 + RHS 40 + Q 40 + J 8 read, Q_out 40 write,
                     Q^n 40 at stages 1 and 2, nine metrics 72 at stage 2      = 274.7
                     (single RHS evaluations, and CRWENO5 z lines)
+    uniform: x sweep 80, y sweep 120, z sweep 146.7 (Navier-Stokes 242.7),
+    viscous flux 136, viscous div 242.7.
     """
+    m = 0.0 if uniform else 1.0
     if cls == "x_sweep":
-        return 112.0
+        return 80.0 + m * 32.0
     if cls == "y_sweep":
-        return 152.0
+        return 120.0 + m * 32.0
     if cls == "z_sweep":
-        return 112.0 + (0 + 40 + 40) / 3.0 + 40.0 + 48.0 / 3.0 + (96.0 if ns else 0.0)
+        return 80.0 + m * 32.0 + (0 + 40 + 40) / 3.0 + 40.0 + m * 48.0 / 3.0 + (96.0 if ns else 0.0)
     if cls == "viscous_flux":
-        return 216.0
-    return 96.0 + 40.0 + 40.0 + 8.0 + 40.0 + (0 + 40 + 40) / 3.0 + 72.0 / 3.0
+        return 136.0 + m * 80.0
+    return 96.0 + 40.0 + 40.0 + m * 8.0 + 40.0 + (0 + 40 + 40) / 3.0 + m * 72.0 / 3.0
 
 
 def log(msg):
@@ -203,6 +211,7 @@ def bench_multiblock(args):
 
     def make(params, bl, state=None):
         c = solver.Case(params, bl)
+        c.set_uniform_mesh(not args.no_uniform_mesh)
         if members:
             for i, m in enumerate(members):
                 c.set_group(i, m.group, m.mach, m.reynolds)
@@ -273,6 +282,7 @@ def bench_shock(args):
     res = {}
     for name, sch in (("weno5z", solver.WENO5Z), ("crweno5", solver.CRWENO5)):
         c = solver.Case(solver.Params(model=solver.EULER3D, mach=1.0, scheme=(sch,) * 3, exact=exact), blocks)
+        c.set_uniform_mesh(not args.no_uniform_mesh)
         v, ms, tm = time_method(c, args.steps, args.warmup, cells)
         c.close()
         res[name] = {"value": v, "ms_per_step": ms,
@@ -312,6 +322,8 @@ def main():
                          "cells); both plain WENO5-JS and the learned hybrid; shu_osher: CRWENO5 against "
                          "WENO5-Z on the 1024x128x128 Shu-Osher problem")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-uniform-mesh", action="store_true",
+                    help="load J and the metric fields even where they are one value each")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -336,6 +348,7 @@ def main():
     blocks, phys = cases.isotropic_turbulence(n=n, seed=0)
     common = dict(model=solver.NAVIERSTOKES3D, mach=phys.mach, reynolds=phys.reynolds, exact=0 if args.fast else 1)
     plain = solver.Case(solver.Params(**common), blocks)
+    plain.set_uniform_mesh(not args.no_uniform_mesh)
     kseg = plain.autotune()
     log("bench: autotuned k segments of the column tiles: %s" % kseg)
     if args.develop:
@@ -351,10 +364,12 @@ def main():
     for k, (ms, cnt) in tm.items():
         if cnt:
             log("bench: plain  %-19s %8.3f ms avg over %d launches" % (k, ms / cnt, cnt))
+    uniform = plain.uniform_mesh(0)
+    alg_step = ALG_BYTES_PER_STEP_UNIFORM if uniform else ALG_BYTES_PER_STEP
     dom = max(tm, key=lambda k: tm[k][0])
     dom_ms, dom_n = tm[dom]
     avg_ms = dom_ms / max(dom_n, 1)
-    achieved = kernel_alg_bytes(dom, ns) * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    achieved = kernel_alg_bytes(dom, ns, uniform) * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     variant = "fast" if args.fast else "exact"
     traffic = read_traffic(dom, n, variant)
     log("bench: plain %.3f ms/step, %.4g cell-updates/s" % (ms_plain, v_plain))
@@ -363,6 +378,7 @@ def main():
     if not args.no_hybrid:
         params, thr = train_sensor.load()
         hcase = solver.Case(solver.Params(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), **common), start)
+        hcase.set_uniform_mesh(not args.no_uniform_mesh)
         hcase.set_sensor(params, thr)
         hcase.set_sensor_interval(args.sensor_interval)
         hcase.autotune()
@@ -440,8 +456,10 @@ def main():
             "traffic": traffic,
             "kernel": KERNEL_OF_CLASS[dom],
             "avg_launch_ms": avg_ms,
-            "alg_bytes_per_cell": kernel_alg_bytes(dom, ns),
-            "step_level_gbs": ALG_BYTES_PER_STEP * v_plain / 1e9,
+            "alg_bytes_per_cell": kernel_alg_bytes(dom, ns, uniform),
+            "uniform_mesh": uniform,
+            "step_alg_bytes_per_cell_update": alg_step,
+            "step_level_gbs": alg_step * v_plain / 1e9,
         },
         "cpu_baseline": cpu,
     }
