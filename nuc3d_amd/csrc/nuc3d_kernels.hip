@@ -520,8 +520,15 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     // 3-cell halo (21 slots per line) for WENO5; the central scheme reaches 4 cells (23 slots)
     constexpr int H = SCHEME == SCH_CENTRAL8 ? 4 : NUC3D_HALO;
     constexpr int NL = 16, NF = 16, NC = NF - 1, NSL = NC + 2 * H, NST = 2 * H;
+    constexpr bool VISC = NS && DIR == 2 && MODE == 1;  // Navier-Stokes stage: viscous divergence here
     __shared__ double S[10][NSL * NL];
-    __shared__ double HF[5][NF * NL];
+    // face fluxes: once every face is computed the split fluxes are dead and
+    // their area takes them (27 KB of LDS per workgroup instead of 37, five
+    // workgroups per CU); the viscous z sweep needs that area for fv
+    __shared__ double HFv[VISC ? 5 : 1][VISC ? NF * NL : 1];
+    static_assert(10 * NSL >= 5 * NF, "the face fluxes fit the split-flux area");
+    double(*HF)[NF * NL] = VISC ? reinterpret_cast<double(*)[NF * NL]>(&HFv[0][0])
+                                : reinterpret_cast<double(*)[NF * NL]>(&S[0][0]);
 
     const int witem = xcd_tile(blockIdx.x, gridDim.x);
     const int4 t = tiles[witem];
@@ -531,7 +538,6 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     const int nc = min(NC, B.n[DIR] - a0);
     const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
     double *const *qin = B.q[bin];
-    constexpr bool VISC = NS && DIR == 2 && MODE == 1;  // Navier-Stokes stage: viscous divergence here
     double pv[VISC ? VZ_PER : 1];
     if constexpr (VISC) visc_prefetch_z(B, a0, l0, cc, nc, nlines, pv);
 
@@ -565,7 +571,9 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
 
     const int line = DIR == 0 ? threadIdx.x / NF : threadIdx.x % NL;
     const int f = DIR == 0 ? threadIdx.x % NF : threadIdx.x / NL;
-    if (line < nlines && f <= nc) {
+    const bool has_face = line < nlines && f <= nc;
+    double hv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (has_face) {
         const int m = a0 + f;
 #pragma unroll
         for (int c = 0; c < 5; c++) {
@@ -576,9 +584,13 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
                 sp[q] = S[c][sidx(line, f + q)];
                 sn[q] = S[5 + c][sidx(line, f + q)];
             }
-            HF[c][hidx(line, f)] = face_value<SCHEME>(B, K, DIR, m, sp + (H - 3), sn + (H - 3));
+            hv[c] = face_value<SCHEME>(B, K, DIR, m, sp + (H - 3), sn + (H - 3));
         }
     }
+    if constexpr (!VISC) __syncthreads();  // every face has read the split fluxes
+    if (has_face)
+#pragma unroll
+        for (int c = 0; c < 5; c++) HF[c][hidx(line, f)] = hv[c];
     __syncthreads();
 
     double lmax[3] = {0.0, 0.0, 0.0};
