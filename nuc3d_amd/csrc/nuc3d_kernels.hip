@@ -673,8 +673,20 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
             double hf[5][NF * NL];
         } u;
     };
-    __shared__ __align__(16) std::conditional_t<DECIDE, LdsDecide, LdsReuse> U;
-    auto &HF = U.u.hf;
+    struct LdsNone {
+        union {
+            double hf[1][1];
+        } u;
+    };
+    // Where decisions are reused, the face fluxes take the split-flux area
+    // once every face is computed (five workgroups per CU), except in the
+    // viscous z sweep, which stages the viscous fluxes there.
+    constexpr bool VISC = NS && DIR == 2 && MODE == 1;  // Navier-Stokes stage: viscous divergence here
+    constexpr bool HF_IN_S = !DECIDE && !VISC;
+    static_assert(10 * NSL >= 5 * NF, "the face fluxes fit the split-flux area");
+    __shared__ __align__(16) std::conditional_t<DECIDE, LdsDecide, std::conditional_t<HF_IN_S, LdsNone, LdsReuse>> U;
+    double(*HF)[NF * NL] = HF_IN_S ? reinterpret_cast<double(*)[NF * NL]>(&S[0][0])
+                                   : reinterpret_cast<double(*)[NF * NL]>(&U.u.hf[0][0]);
     float(*PR)[NSL * NL] = nullptr;
     float *ZL = nullptr;
     if constexpr (DECIDE) {
@@ -693,7 +705,6 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     const int nc = min(NC, B.n[DIR] - a0);
     const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
     double *const *qin = B.q[bin];
-    constexpr bool VISC = NS && DIR == 2 && MODE == 1;  // Navier-Stokes stage: viscous divergence here
     double pv[VISC ? VZ_PER : 1];
     if constexpr (VISC) visc_prefetch_z(B, a0, l0, cc, nc, nlines, pv);
 
@@ -820,6 +831,10 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     // slots f..f+7 = cells m-4..m+3; the six-cell stencil starts at m-3.
     // Closure faces (next to physical faces only) and central faces are
     // separate branches, so waves without closure faces skip that code.
+    // Where the face fluxes take the split-flux area, every value is kept in
+    // registers until no thread reads the split fluxes any more.
+    double hv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const bool own_face = active && (closure || !weno);
     if (active && closure) {
 #pragma unroll
         for (int c = 0; c < 5; c++) {
@@ -829,7 +844,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
                 sp[q] = S[c][sidx(line, f + q)];
                 sn[q] = S[5 + c][sidx(line, f + q)];
             }
-            HF[c][hidx(line, f)] = face_value<SCH_WENO5JS>(B, K, DIR, m, sp + 1, sn + 1);
+            hv[c] = face_value<SCH_WENO5JS>(B, K, DIR, m, sp + 1, sn + 1);
         }
     } else if (decided && !weno) {
 #pragma unroll
@@ -840,22 +855,47 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
                 sp[q] = S[c][sidx(line, f + q)];
                 sn[q] = S[5 + c][sidx(line, f + q)];
             }
-            HF[c][hidx(line, f)] = central8(sp + 1) + central8(sn + 1);
+            hv[c] = central8(sp + 1) + central8(sn + 1);
         }
+    }
+    if constexpr (!HF_IN_S) {
+        if (own_face)
+#pragma unroll
+            for (int c = 0; c < 5; c++) HF[c][hidx(line, f)] = hv[c];
     }
     __syncthreads();
     // items component-major, so that neighbouring lanes read one component's
-    // LDS array at neighbouring faces (no bank conflicts between components)
-    for (int e = threadIdx.x; e < 5 * total; e += 256) {
-        const int c = e / total, tf = LST[e - c * total];
-        const int ln = DIR == 0 ? tf / NF : tf % NL, ff = DIR == 0 ? tf % NF : tf / NL;
-        double sp[6], sn[6];
+    // LDS array at neighbouring faces (no bank conflicts between components);
+    // at most five per thread (5 x 256 faces)
+    double wv5[5];
+    int wo5[5];
 #pragma unroll
-        for (int q = 0; q < 6; q++) {
-            sp[q] = S[c][sidx(ln, ff + 1 + q)];
-            sn[q] = S[5 + c][sidx(ln, ff + 1 + q)];
+    for (int r = 0; r < 5; r++) {
+        const int e = threadIdx.x + 256 * r;
+        wo5[r] = -1;
+        wv5[r] = 0.0;
+        if (e < 5 * total) {
+            const int c = e / total, tf = LST[e - c * total];
+            const int ln = DIR == 0 ? tf / NF : tf % NL, ff = DIR == 0 ? tf % NF : tf / NL;
+            double sp[6], sn[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                sp[q] = S[c][sidx(ln, ff + 1 + q)];
+                sn[q] = S[5 + c][sidx(ln, ff + 1 + q)];
+            }
+            wv5[r] = face_interior<SCH_WENO5JS>(sp, true, false) + face_interior<SCH_WENO5JS>(sn, false, false);
+            wo5[r] = c * (NF * NL) + hidx(ln, ff);
+            if constexpr (!HF_IN_S) (&HF[0][0])[wo5[r]] = wv5[r];
         }
-        HF[c][hidx(ln, ff)] = face_interior<SCH_WENO5JS>(sp, true, false) + face_interior<SCH_WENO5JS>(sn, false, false);
+    }
+    if constexpr (HF_IN_S) {
+        __syncthreads();  // no thread reads the split fluxes any more
+        if (own_face)
+#pragma unroll
+            for (int c = 0; c < 5; c++) HF[c][hidx(line, f)] = hv[c];
+#pragma unroll
+        for (int r = 0; r < 5; r++)
+            if (wo5[r] >= 0) (&HF[0][0])[wo5[r]] = wv5[r];
     }
     __syncthreads();
 
