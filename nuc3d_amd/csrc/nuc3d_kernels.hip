@@ -658,8 +658,9 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     __shared__ double S[10][NSL * NL];
     // Where decisions are made, one area has three lives: the FP32 roundings
     // of rho, u, v, w, p of the staged cells; then the bf16 features of the
-    // 256 faces; then the face fluxes.  Where they are reused, the area holds
-    // the face fluxes only, and the kernel fits four workgroups per CU.
+    // 256 faces; then the face fluxes.  Where they are reused, the face fluxes
+    // go to the split-flux area (below) or, in the viscous z sweep, to an area
+    // of their own.
     struct LdsDecide {
         union {
             float pr[5][NSL * NL];
