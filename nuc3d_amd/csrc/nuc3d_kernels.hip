@@ -424,35 +424,30 @@ __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const
 // i = l0..l0+15, nc <= 15 cells k = a0..a0+nc-1, j fixed) takes
 //   r = rhs + dH/dzeta;  r[1+v] = ((r[1+v] - dFv/dxi) - dGv/deta) - dHv/dzeta
 // (NaiverStokes3d.cpp:401-513, the order of the separate divergence pass), then
-// the TVD-RK3 stage.  fv_xi of i-3..i0+18 and fv_zeta of k-3..a0+nc+2 are
-// staged in LDS (the split-flux area, free once the faces are computed);
-// fv_eta is read along j from memory.
+// the TVD-RK3 stage.  fv_xi of i-3..i0+18 is staged in LDS (in the split-flux
+// area, free once the faces are computed, beside the face fluxes); fv_eta and
+// fv_zeta are read along j and k from memory.
 constexpr int VZ_NL = 16, VZ_NC = 15;
-constexpr int VZ_XW = VZ_NL + 2 * NUC3D_HALO, VZ_ZW = VZ_NC + 2 * NUC3D_HALO;
-constexpr int VZ_LDS = 4 * VZ_NC * VZ_XW + 4 * VZ_ZW * VZ_NL;  // doubles
+constexpr int VZ_XW = VZ_NL + 2 * NUC3D_HALO;
+constexpr int VZ_LDS = 4 * VZ_NC * VZ_XW;  // doubles
 
 constexpr int VZ_PER = (VZ_LDS + 255) / 256;  // staged values per thread of a 256-thread sweep
 
 // Loads of the staged values into registers, issued at the start of the sweep
 // so that they are in flight while the faces are computed: thread t takes
 // elements t, t + 256, ... of V, where V[(v * NC + kk) * XW + ii] = fv_xi of
-// cell (l0 - 3 + ii, j, a0 + kk) and V[4 NC XW + (v * ZW + kz) * NL + ii] =
-// fv_zeta of cell (l0 + ii, j, a0 - 3 + kz).
+// cell (l0 - 3 + ii, j, a0 + kk).
 __device__ __forceinline__ void visc_prefetch_z(const nuc3d_block_dev &B, int a0, int l0, int j, int nc, int nlines,
                                                 double *pv)
 {
-    constexpr int H = NUC3D_HALO, NX = 4 * VZ_NC * VZ_XW;
+    constexpr int H = NUC3D_HALO;
 #pragma unroll
     for (int r = 0; r < VZ_PER; r++) {
         const int e = threadIdx.x + 256 * r;
         pv[r] = 0.0;
-        if (e < NX) {
+        if (e < VZ_LDS) {
             const int ii = e % VZ_XW, kk = (e / VZ_XW) % VZ_NC, v = e / (VZ_XW * VZ_NC);
             if (kk < nc && ii < nlines + 2 * H) pv[r] = B.fv[v][cidx(B, l0 - H + ii, j, a0 + kk)];
-        } else if (e < VZ_LDS) {
-            const int ez = e - NX;
-            const int ii = ez % VZ_NL, kz = (ez / VZ_NL) % VZ_ZW, v = ez / (VZ_NL * VZ_ZW);
-            if (ii < nlines && kz < nc + 2 * H) pv[r] = B.fv[8 + v][cidx(B, l0 + ii, j, a0 - H + kz)];
         }
     }
 }
@@ -476,7 +471,7 @@ __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks
                                                    unsigned long long &nfix, double &r2)
 {
     constexpr int H = NUC3D_HALO;
-    const double *Vx = V, *Vz = V + 4 * VZ_NC * VZ_XW;
+    const double *Vx = V;
     const long long x = cidx(B, i, j, k);
     double r[5];
 #pragma unroll
@@ -485,7 +480,7 @@ __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks
     for (int v = 0; v < 4; v++) {
         const double gx = line_diff_lds(B, K, &Vx[(v * VZ_NC + kk) * VZ_XW + line + H], 1, 0, i);
         const double gy = line_diff(B, K, B.fv[4 + v] + x, 1, j);
-        const double gz = line_diff_lds(B, K, &Vz[(v * VZ_ZW + kk + H) * VZ_NL + line], VZ_NL, 2, k);
+        const double gz = line_diff(B, K, B.fv[8 + v] + x, 2, k);
         r[1 + v] = ((r[1 + v] - gx) - gy) - gz;
     }
     rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
@@ -524,11 +519,9 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     __shared__ double S[10][NSL * NL];
     // face fluxes: once every face is computed the split fluxes are dead and
     // their area takes them (27 KB of LDS per workgroup instead of 37, five
-    // workgroups per CU); the viscous z sweep needs that area for fv
-    __shared__ double HFv[VISC ? 5 : 1][VISC ? NF * NL : 1];
-    static_assert(10 * NSL >= 5 * NF, "the face fluxes fit the split-flux area");
-    double(*HF)[NF * NL] = VISC ? reinterpret_cast<double(*)[NF * NL]>(&HFv[0][0])
-                                : reinterpret_cast<double(*)[NF * NL]>(&S[0][0]);
+    // workgroups per CU), after the staged fv_xi in the viscous z sweep
+    static_assert(10 * NSL * NL >= (VISC ? VZ_LDS : 0) + 5 * NF * NL, "the face fluxes fit the split-flux area");
+    double(*HF)[NF * NL] = reinterpret_cast<double(*)[NF * NL]>(&S[0][0] + (VISC ? VZ_LDS : 0));
 
     const int witem = xcd_tile(blockIdx.x, gridDim.x);
     const int4 t = tiles[witem];
@@ -587,21 +580,17 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
             hv[c] = face_value<SCHEME>(B, K, DIR, m, sp + (H - 3), sn + (H - 3));
         }
     }
-    if constexpr (!VISC) __syncthreads();  // every face has read the split fluxes
+    __syncthreads();  // every face has read the split fluxes
     if (has_face)
 #pragma unroll
         for (int c = 0; c < 5; c++) HF[c][hidx(line, f)] = hv[c];
+    if constexpr (VISC) visc_store_z(&S[0][0], pv);
     __syncthreads();
 
     double lmax[3] = {0.0, 0.0, 0.0};
     unsigned long long nfix = 0;
     double r2 = 0.0;  // residual partial (stage 2)
     const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
-    static_assert(!VISC || 10 * NSL * NL >= VZ_LDS, "viscous staging fits the split-flux area");
-    if constexpr (VISC) {
-        visc_store_z(&S[0][0], pv);
-        __syncthreads();
-    }
     if (line < nlines && f >= 1 && f <= nc) {
         int i, j, k;
         cell(line, a0 + f - 1, i, j, k);
@@ -659,8 +648,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     // Where decisions are made, one area has three lives: the FP32 roundings
     // of rho, u, v, w, p of the staged cells; then the bf16 features of the
     // 256 faces; then the face fluxes.  Where they are reused, the face fluxes
-    // go to the split-flux area (below) or, in the viscous z sweep, to an area
-    // of their own.
+    // go to the split-flux area (below).
     struct LdsDecide {
         union {
             float pr[5][NSL * NL];
@@ -669,24 +657,19 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
         } u;
         float zl[256];
     };
-    struct LdsReuse {
-        union {
-            double hf[5][NF * NL];
-        } u;
-    };
     struct LdsNone {
         union {
             double hf[1][1];
         } u;
     };
     // Where decisions are reused, the face fluxes take the split-flux area
-    // once every face is computed (five workgroups per CU), except in the
-    // viscous z sweep, which stages the viscous fluxes there.
+    // once every face is computed (five workgroups per CU), after the staged
+    // fv_xi in the viscous z sweep.
     constexpr bool VISC = NS && DIR == 2 && MODE == 1;  // Navier-Stokes stage: viscous divergence here
-    constexpr bool HF_IN_S = !DECIDE && !VISC;
-    static_assert(10 * NSL >= 5 * NF, "the face fluxes fit the split-flux area");
-    __shared__ __align__(16) std::conditional_t<DECIDE, LdsDecide, std::conditional_t<HF_IN_S, LdsNone, LdsReuse>> U;
-    double(*HF)[NF * NL] = HF_IN_S ? reinterpret_cast<double(*)[NF * NL]>(&S[0][0])
+    constexpr bool HF_IN_S = !DECIDE;
+    static_assert(10 * NSL * NL >= (VISC ? VZ_LDS : 0) + 5 * NF * NL, "the face fluxes fit the split-flux area");
+    __shared__ __align__(16) std::conditional_t<DECIDE, LdsDecide, LdsNone> U;
+    double(*HF)[NF * NL] = HF_IN_S ? reinterpret_cast<double(*)[NF * NL]>(&S[0][0] + (VISC ? VZ_LDS : 0))
                                    : reinterpret_cast<double(*)[NF * NL]>(&U.u.hf[0][0]);
     float(*PR)[NSL * NL] = nullptr;
     float *ZL = nullptr;
@@ -897,6 +880,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
 #pragma unroll
         for (int r = 0; r < 5; r++)
             if (wo5[r] >= 0) (&HF[0][0])[wo5[r]] = wv5[r];
+        if constexpr (VISC) visc_store_z(&S[0][0], pv);
     }
     __syncthreads();
 
@@ -904,8 +888,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     unsigned long long nfix = 0;
     double r2 = 0.0;
     const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
-    static_assert(!VISC || 10 * NSL * NL >= VZ_LDS, "viscous staging fits the split-flux area");
-    if constexpr (VISC) {
+    if constexpr (VISC && DECIDE) {  // the face fluxes are in the decision area: the split-flux area is free
         visc_store_z(&S[0][0], pv);
         __syncthreads();
     }
