@@ -927,7 +927,15 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
 // physical closures.  Line list: groups of NUC3D_CRW_LINES lines of one
 // block (x = block, y, z = transverse coordinates j,k / i,k / i,j), padded
 // with x = -1.
-constexpr int CRW_R = 32;
+#ifndef NUC3D_CRW_R
+#define NUC3D_CRW_R 32
+#endif
+#ifndef NUC3D_CRW_BATCH
+#define NUC3D_CRW_BATCH 4
+#endif
+constexpr int CRW_R = NUC3D_CRW_R;       // rows per staged chunk
+constexpr int CRW_B = NUC3D_CRW_BATCH;   // rows whose coefficients a lane computes together
+constexpr int CRW_HQ = 9;                // staging slot of the high end cells (after the w + 5 <= 9 low ones)
 
 template <int DIR, bool NS, int MODE>
 __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__restrict__ blocks,
@@ -939,7 +947,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 {
     constexpr int LW = NUC3D_CRW_LINES, R = CRW_R, NQ = R + 5;
     static_assert(LW * 10 <= 64, "one lane per system");
-    static_assert(NQ >= 16 + 4 + 5, "end cells of both line ends fit one chunk");
+    static_assert(NQ >= CRW_HQ + 9, "end cells of both line ends fit one chunk");
     __shared__ union {
         double sf[10][LW][NQ + 1];  // split fluxes F+ (0..4), F- (5..9) of the staged cells
         double xl[64][R + 1];       // face solutions of one chunk, per system lane
@@ -980,15 +988,15 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
     };
 
     // explicit faces 0..w-1 (cells -3..w+1 at q 0..w+4) and n-w+1..n
-    // (cells n-w-2..n+2 at q 16..20+w): halo face variants or physical closures
+    // (cells n-w-2..n+2 at q CRW_HQ..CRW_HQ+w+4): halo face variants or physical closures
     stage_cells(-3, w + 5, 0);
-    stage_cells(n - w - 2, w + 5, 16);
+    stage_cells(n - w - 2, w + 5, CRW_HQ);
     __syncthreads();
     for (int e = threadIdx.x; e < LW * 5 * 2 * w; e += 64) {
         const int l = e / (10 * w), c = (e / (2 * w)) % 5, t = e % (2 * w);
         if (lsh[l].x < 0) continue;
         const int m = t < w ? t : n - w + 1 + (t - w);
-        const int q = t < w ? m : 16 + (t - w);  // q of cell m-3
+        const int q = t < w ? m : CRW_HQ + (t - w);  // q of cell m-3
         double sp[6], sn[6];
 #pragma unroll
         for (int o = 0; o < 6; o++) {
@@ -1027,12 +1035,12 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
             const int rin = last ? nr - 1 : nr;
             // the row coefficients do not depend on the recurrence: four rows
             // at a time give the lane four independent chains to interleave
-            for (; rr + 4 <= rin; rr += 4) {
-                double a[4], b[4], cc[4], d[4];
+            for (; rr + CRW_B <= rin; rr += CRW_B) {
+                double a[CRW_B], b[CRW_B], cc[CRW_B], d[CRW_B];
 #pragma unroll
-                for (int u = 0; u < 4; u++) crweno_row(f + rr + u + 2, 1, plus, a[u], b[u], cc[u], d[u]);
+                for (int u = 0; u < CRW_B; u++) crweno_row(f + rr + u + 2, 1, plus, a[u], b[u], cc[u], d[u]);
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < CRW_B; u++) {
                     cc[u] = cc[u] / (b[u] - cprev * a[u]);
                     d[u] = (d[u] - dprev * a[u]) / (b[u] - cprev * a[u]);
                     Cg[(long long)(r0 + rr + u) * 64 + s] = cc[u];
