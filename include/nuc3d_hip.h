@@ -193,7 +193,7 @@ int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt);
  * (viscous fluxes with u,v,w,T, viscous divergence, diagnostics) run over
  * 16 x 16 column tiles that march along k in segments of kseg planes, 32 by
  * default.  For Navier-Stokes cases this takes the distinct block shapes in
- * turn and, for kseg = 8, 16, 32, 64, 128 (up to twice the block's depth) on
+ * turn and, for kseg = 4, 8, 16, 32, 64, 128 (up to twice the block's depth) on
  * every block of that shape, the other blocks keeping their current choice,
  * times the column-tile passes of one stage (the viscous fluxes, and the viscous
  * divergence where CRWENO5 z lines leave it to its own pass) over the whole

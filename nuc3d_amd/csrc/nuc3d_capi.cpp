@@ -1176,8 +1176,8 @@ int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out)
             for (int b2 = 0; b2 < c->nb; b2++) all[b2] = b2;
             float best = 1e30f;
             int bestk = kseg[b];
-            for (int ks : {8, 16, 32, 64, 128}) {
-                if (ks > 2 * B.n[2] && ks != 8) break;
+            for (int ks : {4, 8, 16, 32, 64, 128}) {
+                if (ks > 2 * B.n[2] && ks > 8) break;
                 std::vector<int> trial = kseg;
                 for (int b2 : same) trial[b2] = ks;
                 const std::vector<nuc3d_tile> tiles = column_tiles(c, all, trial);

@@ -106,7 +106,7 @@ def test_autotune_keeps_results(exact):
                 b.set_group(i, m.group, m.mach, m.reynolds)
         ks = a.autotune()
         print("  k segments %s" % ks, flush=True)
-        assert all(k in (8, 16, 32, 64, 128) for k in ks)
+        assert all(k in (4, 8, 16, 32, 64, 128) for k in ks)
         a.steps_graph(2, 0.5)
         b.steps_graph(2, 0.5)
         for i in range(len(blocks)):
