@@ -294,6 +294,13 @@ int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold);
  * ((n_d+1) x n_{d+1} x n_{d+2}, face index fastest): 0 central, 1 WENO5-JS by
  * the network, 2 WENO5-JS by the fallback, 3 physical-face closure. */
 int nuc3d_download_hybrid_flags(nuc3d_case *c, int b, int d, unsigned char *flags);
+/* Overlap (no counterpart in the reference): the viscous-flux pass of a
+ * Navier-Stokes stage reads only the stage's input state and writes only the
+ * viscous fluxes, so it runs on a second stream of the case beside the x and
+ * y sweeps, and the z sweep waits for it (also inside captured step graphs).
+ * On by default; off runs every pass in order on the caller's stream.  Results
+ * are identical either way; with kernel timing on, the passes run in order. */
+int nuc3d_set_overlap(nuc3d_case *c, int enable);
 /* Uniform-mesh path (no counterpart in the reference, whose blocks always
  * hold J and the nine metric fields).  When enabled (the default), a block
  * whose J and metric terms are each one value, bit for bit, over its interior

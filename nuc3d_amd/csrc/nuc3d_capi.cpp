@@ -150,6 +150,12 @@ struct nuc3d_case {
     int nwork = 0;
     std::vector<int> work_block;
     long long steps_done = 0;
+    // the viscous-flux pass of a Navier-Stokes stage runs on a second stream
+    // beside the x and y sweeps (it reads only the stage's input state and
+    // writes only fv; the z sweep waits for it)
+    bool overlap = true;
+    hipStream_t vstream = nullptr;
+    hipEvent_t vfork = nullptr, vjoin = nullptr;
     // captured two-step graphs, one per parity of the eigenvalue buffers
     hipStream_t gstream = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -302,11 +308,20 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
     for (int d = 0; d < 3; d++)
         if (c->p.scheme[d] == NUC3D_HYBRID && !K.sensor)
             return fail(NUC3D_ERR_BAD_ARG, "the hybrid scheme needs a sensor (nuc3d_set_sensor)");
-    if (ns) {  // u, v, w, T are computed inside the viscous-flux pass
+    // u, v, w, T are computed inside the viscous-flux pass; with kernel timing
+    // on, every pass runs alone on s
+    const bool side = ns && c->overlap && !c->timing && c->vstream;
+    if (side) {
+        HIPCHK(hipEventRecord(c->vfork, s));
+        HIPCHK(hipStreamWaitEvent(c->vstream, c->vfork, 0));
+        HIPCHK(c->L->vflux(c->db, c->dtiles, c->ntiles, K, bin, c->vstream));
+        HIPCHK(hipEventRecord(c->vjoin, c->vstream));
+    } else if (ns) {
         Timed t(c, s, &c->ev[4]);
         HIPCHK(c->L->vflux(c->db, c->dtiles, c->ntiles, K, bin, s));
     }
     for (int d = 0; d < 3; d++) {
+        if (d == 2 && side) HIPCHK(hipStreamWaitEvent(s, c->vjoin, 0));
         Timed t(c, s, &c->ev[d]);
         if (c->p.scheme[d] == NUC3D_CRWENO5)
             HIPCHK(c->L->crweno(d, ns, mode, c->db, c->dlines[d], c->nlines[d], c->nmax[d], K, stage, bin, bun,
@@ -545,6 +560,12 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         return fail(NUC3D_ERR_OOM, "device allocation of control arrays failed");
     }
     c->K.hcount = c->dhcount;
+    if (ns && (hipStreamCreateWithFlags(&c->vstream, hipStreamNonBlocking) != hipSuccess ||
+               hipEventCreateWithFlags(&c->vfork, hipEventDisableTiming) != hipSuccess ||
+               hipEventCreateWithFlags(&c->vjoin, hipEventDisableTiming) != hipSuccess)) {
+        nuc3d_case_destroy(c);
+        return fail(NUC3D_ERR_HIP, "stream for the viscous-flux pass");
+    }
     if (hipMemcpy(c->dtiles, tiles.data(), sizeof(nuc3d_tile) * tiles.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->ddt, 0, sizeof(double) * NUC3D_MAX_BLOCKS) != hipSuccess ||
         hipMemset(c->dlam, 0, sizeof(unsigned long long) * 6 * nblocks) != hipSuccess ||
@@ -564,6 +585,9 @@ int nuc3d_case_destroy(nuc3d_case *c)
     for (auto &g : c->gexec)
         if (g) (void)hipGraphExecDestroy(g);
     if (c->gstream) (void)hipStreamDestroy(c->gstream);
+    if (c->vstream) (void)hipStreamDestroy(c->vstream);
+    if (c->vfork) (void)hipEventDestroy(c->vfork);
+    if (c->vjoin) (void)hipEventDestroy(c->vjoin);
     for (auto &l : c->ev)
         for (auto &pr : l) c->ev_pool.push_back(pr.first), c->ev_pool.push_back(pr.second);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -1529,6 +1553,20 @@ int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold)
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(c->dsensor, &h, sizeof h, hipMemcpyHostToDevice));
     c->K.sensor = c->dsensor;
+    return NUC3D_OK;
+}
+
+int nuc3d_set_overlap(nuc3d_case *c, int enable)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "switch the overlap between full steps");
+    c->overlap = enable != 0;
+    for (auto &g : c->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    c->gcfl[0] = c->gcfl[1] = -1.0;
     return NUC3D_OK;
 }
 

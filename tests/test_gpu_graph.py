@@ -60,3 +60,27 @@ def test_graph_steps_shu_osher_boundaries():
         same(eager, graph, 1)
     eager.close()
     graph.close()
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_overlapped_viscous_flux_matches_serial(graph):
+    """The viscous-flux pass on its own stream beside the x and y sweeps (the
+    default) equals the passes run in order, bit for bit, eagerly and inside
+    captured graphs; three uneven blocks with walls (curvilinear flat plate)."""
+    print("case: overlap on/off, flat plate 3 blocks, graph=%s" % graph, flush=True)
+    blocks = cases.flat_plate([12, 16, 20], [8], 16)
+    phys = dict(mach=2.9, reynolds=2.0e4, t_wall=2.5)
+    out = []
+    for overlap in (True, False):
+        c = solver.Case(solver.Params(**phys), blocks)
+        cases.setup_flat_plate(c, blocks)
+        c.set_overlap(overlap)
+        if graph:
+            c.steps_graph(4, 0.5)
+        else:
+            c.steps(3, 0.5)
+        out.append([c.get_q(b) for b in range(len(blocks))] + [c.dt()])
+        c.close()
+    for a, b in zip(out[0][:-1], out[1][:-1]):
+        np.testing.assert_array_equal(a, b)
+    assert out[0][-1] == out[1][-1]
