@@ -212,6 +212,7 @@ def bench_multiblock(args):
     def make(params, bl, state=None):
         c = solver.Case(params, bl)
         c.set_uniform_mesh(not args.no_uniform_mesh)
+        c.set_overlap(not args.no_overlap)
         if members:
             for i, m in enumerate(members):
                 c.set_group(i, m.group, m.mach, m.reynolds)
@@ -283,6 +284,7 @@ def bench_shock(args):
     for name, sch in (("weno5z", solver.WENO5Z), ("crweno5", solver.CRWENO5)):
         c = solver.Case(solver.Params(model=solver.EULER3D, mach=1.0, scheme=(sch,) * 3, exact=exact), blocks)
         c.set_uniform_mesh(not args.no_uniform_mesh)
+        c.set_overlap(not args.no_overlap)
         v, ms, tm = time_method(c, args.steps, args.warmup, cells)
         c.close()
         res[name] = {"value": v, "ms_per_step": ms,
@@ -324,6 +326,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-uniform-mesh", action="store_true",
                     help="load J and the metric fields even where they are one value each")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run the viscous-flux pass in order with the sweeps (for profiles of single kernels)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -349,6 +353,7 @@ def main():
     common = dict(model=solver.NAVIERSTOKES3D, mach=phys.mach, reynolds=phys.reynolds, exact=0 if args.fast else 1)
     plain = solver.Case(solver.Params(**common), blocks)
     plain.set_uniform_mesh(not args.no_uniform_mesh)
+    plain.set_overlap(not args.no_overlap)
     kseg = plain.autotune()
     log("bench: autotuned k segments of the column tiles: %s" % kseg)
     if args.develop:
@@ -379,6 +384,7 @@ def main():
         params, thr = train_sensor.load()
         hcase = solver.Case(solver.Params(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), **common), start)
         hcase.set_uniform_mesh(not args.no_uniform_mesh)
+        hcase.set_overlap(not args.no_overlap)
         hcase.set_sensor(params, thr)
         hcase.set_sensor_interval(args.sensor_interval)
         hcase.autotune()
