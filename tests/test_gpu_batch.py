@@ -116,3 +116,33 @@ def test_autotune_keeps_results(exact):
         assert abs(ea - eb) <= 1e-12 * abs(eb) and abs(ka - kb) <= 1e-12 * abs(kb)
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+def test_batch_crweno5_line_groups(exact):
+    """CRWENO5 lines run in groups of six lines of one block (the last group
+    of a block padded): uneven Euler cases in one batch equal the same cases
+    run alone, bit for bit, and the oracle on one of them."""
+    members = cases.batch_cases(n_cases=4, lo=16, hi=28, seed=9)
+    print("case: CRWENO5 batch of %d: %s, exact=%d" % (
+        len(members), ", ".join("%d^3" % m.block.dims[0] for m in members), exact), flush=True)
+    p = solver.Params(model=solver.EULER3D, scheme=(solver.CRWENO5,) * 3, exact=exact)
+    g = solver.Case(p, [m.block for m in members])
+    for b, m in enumerate(members):
+        g.set_group(b, m.group, m.mach, m.reynolds)
+    g.steps(2, 0.2)
+    for b, m in enumerate(members):
+        blk = cases.BlockSpec(m.block.dims, cases.periodic_faces(0), m.block.jac, m.block.met, m.block.dxyz, m.block.q)
+        alone = solver.Case(solver.Params(model=solver.EULER3D, scheme=(solver.CRWENO5,) * 3, exact=exact,
+                                          mach=m.mach, reynolds=m.reynolds), [blk])
+        alone.steps(2, 0.2)
+        np.testing.assert_array_equal(g.get_q(b), alone.get_q(0))
+        alone.close()
+    m = members[0]
+    blk = O.Block(m.block.dims, cases.periodic_faces(0), m.block.jac, m.block.met, m.block.dxyz, m.block.q)
+    o = O.OracleCase(O.Params(model=O.EULER, mach=m.mach, scheme=(solver.CRWENO5,) * 3), [blk])
+    o.steps(2, 0.2)
+    err = relerr(g.get_q(0), o.get_q(0))
+    print("  block 0 (%d^3) against the oracle: rel err %.3e" % (m.block.dims[0], err), flush=True)
+    assert err <= TOL
+    g.close()
