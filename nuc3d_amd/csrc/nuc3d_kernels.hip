@@ -82,11 +82,15 @@ __device__ __forceinline__ double met_at(const nuc3d_block_dev &B, int e, long l
 // split fluxes of cell x of buffer qb in direction d; returns the local
 // eigenvalue.  prim (may be null) receives rho, u, v, w, p.
 __device__ __forceinline__ double cell_split(const nuc3d_block_dev &B, double *const *qb, long long x, int d,
-                                             const PhysParams &P, double *fp, double *fn, double *prim = nullptr)
+                                             const PhysParams &P, double *fp, double *fn, double *prim = nullptr,
+                                             double *qout = nullptr)
 {
     double Q[5];
 #pragma unroll
     for (int c = 0; c < 5; c++) Q[c] = qb[c][x];
+    if (qout)
+#pragma unroll
+        for (int c = 0; c < 5; c++) qout[c] = Q[c];
     double jac = jac_at(B, x);
     double rho, u, v, w, p, cs;
     con2prim(Q, jac, P, rho, u, v, w, p, cs);
@@ -149,7 +153,8 @@ __device__ __forceinline__ void bc_ghost(const nuc3d_block_dev &B, double *const
 // prim (may be null) receives the cell's rho, u, v, w, p (zero beyond a
 // physical face).
 __device__ __forceinline__ void split_any(const nuc3d_block_dev &B, double *const *qb, int i, int j, int k, int d,
-                                          const PhysParams &P, double *fp, double *fn, double *prim = nullptr)
+                                          const PhysParams &P, double *fp, double *fn, double *prim = nullptr,
+                                          double *qout = nullptr)
 {
     const int c3[3] = {i, j, k};
     const int t = c3[d], n = B.n[d];
@@ -178,7 +183,7 @@ __device__ __forceinline__ void split_any(const nuc3d_block_dev &B, double *cons
         }
         return;
     }
-    cell_split(B, qb, cidx(B, i, j, k), d, P, fp, fn, prim);
+    cell_split(B, qb, cidx(B, i, j, k), d, P, fp, fn, prim, qout);
 }
 
 // Face value of direction d at face m from the six-slot stencils sp (F+) and
@@ -424,66 +429,29 @@ __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const
 // i = l0..l0+15, nc <= 15 cells k = a0..a0+nc-1, j fixed) takes
 //   r = rhs + dH/dzeta;  r[1+v] = ((r[1+v] - dFv/dxi) - dGv/deta) - dHv/dzeta
 // (NaiverStokes3d.cpp:401-513, the order of the separate divergence pass), then
-// the TVD-RK3 stage.  fv_xi of i-3..i0+18 is staged in LDS (in the split-flux
-// area, free once the faces are computed, beside the face fluxes); fv_eta and
-// fv_zeta are read along j and k from memory.
-constexpr int VZ_NL = 16, VZ_NC = 15;
-constexpr int VZ_XW = VZ_NL + 2 * NUC3D_HALO;
-constexpr int VZ_LDS = 4 * VZ_NC * VZ_XW;  // doubles
-
-constexpr int VZ_PER = (VZ_LDS + 255) / 256;  // staged values per thread of a 256-thread sweep
-
-// Loads of the staged values into registers, issued at the start of the sweep
-// so that they are in flight while the faces are computed: thread t takes
-// elements t, t + 256, ... of V, where V[(v * NC + kk) * XW + ii] = fv_xi of
-// cell (l0 - 3 + ii, j, a0 + kk).
-__device__ __forceinline__ void visc_prefetch_z(const nuc3d_block_dev &B, int a0, int l0, int j, int nc, int nlines,
-                                                double *pv)
-{
-    constexpr int H = NUC3D_HALO;
-#pragma unroll
-    for (int r = 0; r < VZ_PER; r++) {
-        const int e = threadIdx.x + 256 * r;
-        pv[r] = 0.0;
-        if (e < VZ_LDS) {
-            const int ii = e % VZ_XW, kk = (e / VZ_XW) % VZ_NC, v = e / (VZ_XW * VZ_NC);
-            if (kk < nc && ii < nlines + 2 * H) pv[r] = B.fv[v][cidx(B, l0 - H + ii, j, a0 + kk)];
-        }
-    }
-}
-
-// The prefetched values into the LDS area V (once the split fluxes are dead).
-__device__ __forceinline__ void visc_store_z(double *V, const double *pv)
-{
-#pragma unroll
-    for (int r = 0; r < VZ_PER; r++) {
-        const int e = threadIdx.x + 256 * r;
-        if (e < VZ_LDS) V[e] = pv[r];
-    }
-}
-
-// Cell (i,j,k) = line `line`, cell kk of the tile: flux difference dF of the z
-// sweep, viscous divergence from the staged V, RK stage.
+// the TVD-RK3 stage.  The viscous fluxes are read along i, j and k from memory
+// (neighbouring threads and tiles share them in L1 and L2); the cell's state
+// u5 comes from the thread's own staging of that cell.
 __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
-                                                   const nuc3d_kparams &K, const PhysParams &P, const double *V,
-                                                   int line, int kk, int i, int j, int k, const double *dF, int stage,
-                                                   int bin, int bun, int bout, double dt, double *lmax,
-                                                   unsigned long long &nfix, double &r2)
+                                                   const nuc3d_kparams &K, const PhysParams &P, const double *u5,
+                                                   int i, int j, int k, const double *dF, int stage, int bun, int bout,
+                                                   double dt, double *lmax, unsigned long long &nfix, double &r2)
 {
-    constexpr int H = NUC3D_HALO;
-    const double *Vx = V;
     const long long x = cidx(B, i, j, k);
     double r[5];
 #pragma unroll
     for (int c = 0; c < 5; c++) r[c] = B.rhs[c][x] + dF[c];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const double gx = line_diff_lds(B, K, &Vx[(v * VZ_NC + kk) * VZ_XW + line + H], 1, 0, i);
+        const double gx = line_diff(B, K, B.fv[v] + x, 0, i);
         const double gy = line_diff(B, K, B.fv[4 + v] + x, 1, j);
         const double gz = line_diff(B, K, B.fv[8 + v] + x, 2, k);
         r[1 + v] = ((r[1 + v] - gx) - gy) - gz;
     }
-    rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
+    double un5[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) un5[c] = stage == 0 ? 0.0 : B.q[bun][c][x];
+    rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, jac_at(B, x), stage, bout, dt, lmax, nfix, r2);
 }
 
 }  // namespace
@@ -519,9 +487,9 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     __shared__ double S[10][NSL * NL];
     // face fluxes: once every face is computed the split fluxes are dead and
     // their area takes them (27 KB of LDS per workgroup instead of 37, five
-    // workgroups per CU), after the staged fv_xi in the viscous z sweep
-    static_assert(10 * NSL * NL >= (VISC ? VZ_LDS : 0) + 5 * NF * NL, "the face fluxes fit the split-flux area");
-    double(*HF)[NF * NL] = reinterpret_cast<double(*)[NF * NL]>(&S[0][0] + (VISC ? VZ_LDS : 0));
+    // workgroups per CU)
+    static_assert(10 * NSL >= 5 * NF, "the face fluxes fit the split-flux area");
+    double(*HF)[NF * NL] = reinterpret_cast<double(*)[NF * NL]>(&S[0][0]);
 
     const int witem = xcd_tile(blockIdx.x, gridDim.x);
     const int4 t = tiles[witem];
@@ -531,8 +499,6 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     const int nc = min(NC, B.n[DIR] - a0);
     const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
     double *const *qin = B.q[bin];
-    double pv[VISC ? VZ_PER : 1];
-    if constexpr (VISC) visc_prefetch_z(B, a0, l0, cc, nc, nlines, pv);
 
     // (line, slot) -> LDS offset: slot-fastest for DIR 0, line-fastest otherwise
     auto sidx = [](int line, int slot) { return DIR == 0 ? line * NSL + slot : slot * NL + line; };
@@ -544,20 +510,33 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     };
 
     const int nslots = nc + 2 * H;
-    for (int e = threadIdx.x; e < NSL * NL; e += 256) {
-        const int line = DIR == 0 ? e / NSL : e % NL;
-        const int sl = DIR == 0 ? e % NSL : e / NL;
-        if (line < nlines && sl < nslots) {
-            int i, j, k;
-            cell(line, a0 - H + sl, i, j, k);
-            double fp[5], fn[5];
-            split_any(B, qin, i, j, k, DIR, P, fp, fn);
-            const int o = sidx(line, sl);
+    auto stage_slot = [&](int line, int sl, double *qout) {
+        int i, j, k;
+        cell(line, a0 - H + sl, i, j, k);
+        double fp[5], fn[5];
+        split_any(B, qin, i, j, k, DIR, P, fp, fn, nullptr, qout);
+        const int o = sidx(line, sl);
 #pragma unroll
-            for (int c = 0; c < 5; c++) {
-                S[c][o] = fp[c];
-                S[5 + c][o] = fn[c];
-            }
+        for (int c = 0; c < 5; c++) {
+            S[c][o] = fp[c];
+            S[5 + c][o] = fn[c];
+        }
+    };
+    double u5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // viscous z sweep: the state of the thread's own cell
+    if constexpr (VISC) {
+        // the thread that finishes cell f-1 of a line stages it (and keeps its
+        // state for the RK stage); then the 2H halo slots of every line
+        const int line = threadIdx.x % NL, f = threadIdx.x / NL;
+        if (f >= 1 && f <= nc && line < nlines) stage_slot(line, H + f - 1, u5);
+        for (int e = threadIdx.x; e < 2 * H * NL; e += 256) {
+            const int ln = e % NL, r = e / NL, sl = r < H ? r : nc + r;
+            if (ln < nlines) stage_slot(ln, sl, nullptr);
+        }
+    } else {
+        for (int e = threadIdx.x; e < NSL * NL; e += 256) {
+            const int line = DIR == 0 ? e / NSL : e % NL;
+            const int sl = DIR == 0 ? e % NSL : e / NL;
+            if (line < nlines && sl < nslots) stage_slot(line, sl, nullptr);
         }
     }
     __syncthreads();
@@ -584,7 +563,6 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     if (has_face)
 #pragma unroll
         for (int c = 0; c < 5; c++) HF[c][hidx(line, f)] = hv[c];
-    if constexpr (VISC) visc_store_z(&S[0][0], pv);
     __syncthreads();
 
     double lmax[3] = {0.0, 0.0, 0.0};
@@ -598,8 +576,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
 #pragma unroll
         for (int c = 0; c < 5; c++) dF[c] = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
         if constexpr (VISC)
-            finish_cell_visc_z(blocks, B, K, P, &S[0][0], line, f - 1, i, j, k, dF, stage, bin, bun, bout, dt, lmax,
-                               nfix, r2);
+            finish_cell_visc_z(blocks, B, K, P, u5, i, j, k, dF, stage, bun, bout, dt, lmax, nfix, r2);
         else
             finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
     }
@@ -667,9 +644,9 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     // fv_xi in the viscous z sweep.
     constexpr bool VISC = NS && DIR == 2 && MODE == 1;  // Navier-Stokes stage: viscous divergence here
     constexpr bool HF_IN_S = !DECIDE;
-    static_assert(10 * NSL * NL >= (VISC ? VZ_LDS : 0) + 5 * NF * NL, "the face fluxes fit the split-flux area");
+    static_assert(10 * NSL >= 5 * NF, "the face fluxes fit the split-flux area");
     __shared__ __align__(16) std::conditional_t<DECIDE, LdsDecide, LdsNone> U;
-    double(*HF)[NF * NL] = HF_IN_S ? reinterpret_cast<double(*)[NF * NL]>(&S[0][0] + (VISC ? VZ_LDS : 0))
+    double(*HF)[NF * NL] = HF_IN_S ? reinterpret_cast<double(*)[NF * NL]>(&S[0][0])
                                    : reinterpret_cast<double(*)[NF * NL]>(&U.u.hf[0][0]);
     float(*PR)[NSL * NL] = nullptr;
     float *ZL = nullptr;
@@ -689,8 +666,6 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     const int nc = min(NC, B.n[DIR] - a0);
     const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
     double *const *qin = B.q[bin];
-    double pv[VISC ? VZ_PER : 1];
-    if constexpr (VISC) visc_prefetch_z(B, a0, l0, cc, nc, nlines, pv);
 
     auto sidx = [](int line, int slot) { return DIR == 0 ? line * NSL + slot : slot * NL + line; };
     auto hidx = [](int line, int f) { return DIR == 0 ? line * NF + f : f * NL + line; };
@@ -701,21 +676,34 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     };
 
     const int nslots = nc + 2 * H;
-    for (int e = threadIdx.x; e < NSL * NL; e += 256) {
-        const int line = DIR == 0 ? e / NSL : e % NL;
-        const int sl = DIR == 0 ? e % NSL : e / NL;
-        if (line < nlines && sl < nslots) {
-            int i, j, k;
-            cell(line, a0 - H + sl, i, j, k);
-            double fp[5], fn[5], pr[5];
-            split_any(B, qin, i, j, k, DIR, P, fp, fn, pr);
-            const int o = sidx(line, sl);
+    auto stage_slot = [&](int line, int sl, double *qout) {
+        int i, j, k;
+        cell(line, a0 - H + sl, i, j, k);
+        double fp[5], fn[5], pr[5];
+        split_any(B, qin, i, j, k, DIR, P, fp, fn, pr, qout);
+        const int o = sidx(line, sl);
 #pragma unroll
-            for (int c = 0; c < 5; c++) {
-                S[c][o] = fp[c];
-                S[5 + c][o] = fn[c];
-                if constexpr (DECIDE) PR[c][o] = (float)pr[c];
-            }
+        for (int c = 0; c < 5; c++) {
+            S[c][o] = fp[c];
+            S[5 + c][o] = fn[c];
+            if constexpr (DECIDE) PR[c][o] = (float)pr[c];
+        }
+    };
+    double u5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // viscous z sweep: the state of the thread's own cell
+    if constexpr (VISC) {
+        // the thread that finishes cell f-1 of a line stages it (and keeps its
+        // state for the RK stage); then the 2H halo slots of every line
+        const int line = threadIdx.x % NL, f = threadIdx.x / NL;
+        if (f >= 1 && f <= nc && line < nlines) stage_slot(line, H + f - 1, u5);
+        for (int e = threadIdx.x; e < 2 * H * NL; e += 256) {
+            const int ln = e % NL, r = e / NL, sl = r < H ? r : nc + r;
+            if (ln < nlines) stage_slot(ln, sl, nullptr);
+        }
+    } else {
+        for (int e = threadIdx.x; e < NSL * NL; e += 256) {
+            const int line = DIR == 0 ? e / NSL : e % NL;
+            const int sl = DIR == 0 ? e % NSL : e / NL;
+            if (line < nlines && sl < nslots) stage_slot(line, sl, nullptr);
         }
     }
     __syncthreads();
@@ -880,7 +868,6 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
 #pragma unroll
         for (int r = 0; r < 5; r++)
             if (wo5[r] >= 0) (&HF[0][0])[wo5[r]] = wv5[r];
-        if constexpr (VISC) visc_store_z(&S[0][0], pv);
     }
     __syncthreads();
 
@@ -888,10 +875,6 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     unsigned long long nfix = 0;
     double r2 = 0.0;
     const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
-    if constexpr (VISC && DECIDE) {  // the face fluxes are in the decision area: the split-flux area is free
-        visc_store_z(&S[0][0], pv);
-        __syncthreads();
-    }
     if (line < nlines && f >= 1 && f <= nc) {
         int i, j, k;
         cell(line, a0 + f - 1, i, j, k);
@@ -899,8 +882,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
 #pragma unroll
         for (int c = 0; c < 5; c++) dF[c] = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
         if constexpr (VISC)
-            finish_cell_visc_z(blocks, B, K, P, &S[0][0], line, f - 1, i, j, k, dF, stage, bin, bun, bout, dt, lmax,
-                               nfix, r2);
+            finish_cell_visc_z(blocks, B, K, P, u5, i, j, k, dF, stage, bun, bout, dt, lmax, nfix, r2);
         else
             finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
     }
