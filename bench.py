@@ -2,7 +2,7 @@
 """Throughput of the per-block RHS + TVD-RK3 path on one MI355X: plain WENO5-JS
 against the learned hybrid (central-8 / WENO5-JS per face, chosen by the
 BF16-MFMA sensor) on the same workload, with the hybrid's L2 density deviation
-from plain WENO5-JS as the accuracy judge.
+from plain WENO5-JS as the accuracy measure.
 
 Metric: cell-updates/s per RK3 step in FP64 (one cell-update = one cell
 advanced one full TVD-RK3 step: three RHS evaluations and three stage

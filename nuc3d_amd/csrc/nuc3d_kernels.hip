@@ -486,8 +486,8 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     constexpr bool VISC = NS && DIR == 2 && MODE == 1;  // Navier-Stokes stage: viscous divergence here
     __shared__ double S[10][NSL * NL];
     // face fluxes: once every face is computed the split fluxes are dead and
-    // their area takes them (27 KB of LDS per workgroup instead of 37, five
-    // workgroups per CU)
+    // their area takes them (27 KB of LDS per workgroup instead of 37: six
+    // workgroups fit a CU's LDS instead of four)
     static_assert(10 * NSL >= 5 * NF, "the face fluxes fit the split-flux area");
     double(*HF)[NF * NL] = reinterpret_cast<double(*)[NF * NL]>(&S[0][0]);
 

@@ -116,8 +116,8 @@ __device__ __forceinline__ float stencil_mean(const Get &getf, int v)
 // the sound speed of the means: U for the velocities, rho c U for the
 // pressure and rho U / c for the density (the acoustic relations
 // dp ~ rho c du, drho ~ rho du / c, so that a tolerance tau S is the same
-// disturbance in every variable, and low-Mach flows are not judged on the
-// scale of the mean pressure).  False when mean rho or mean p is not
+// disturbance in every variable, and low-Mach flows are not measured on
+// the scale of the mean pressure).  False when mean rho or mean p is not
 // positive (the stencil is invalid; the face takes WENO5-JS).
 template <class Get>
 __device__ __forceinline__ bool feature_scales(const Get &getf, float gamma, float *S)
