@@ -34,18 +34,40 @@ static const double C_CD2 = 0.5;
 static const double W_SS = 1.0e-6; /* weno5js.cpp:7, weno5z.cpp:6, crweno5.cpp:13 */
 static const double W_P = 2.0;     /* weno5js.cpp:8 (runtime member p) */
 
+/* ------------------------------------------------------------- squares
+ * The reference squares with pow(): with a literal exponent 2 / 2.0
+ * (smoothness indicators, closures, AUSM polynomials, residual) and with the
+ * runtime member p = 2 (the WENO weight denominators, weno5js.cpp:97-99,
+ * weno5z.cpp:96-98, crweno5.cpp:113-115).  Which calls reach libm depends on
+ * how the reference is built, so the restatement has three conventions:
+ *   ORC_SQ_LIBM  (0, default): every square from libm pow() -- the
+ *                makefile's -O0 build (src/makefile:34-35);
+ *   ORC_SQ_O2    (1): literal-exponent squares as x*x, runtime-p squares
+ *                from libm pow() -- the same sources built with -O2, where
+ *                the compiler folds pow(x, 2.0) into x*x;
+ *   ORC_SQ_EXACT (2): every square as x*x, the correctly rounded square.
+ * glibc's pow(x, 2.0) differs from the correctly rounded x*x by one ulp for
+ * about 0.1% of arguments. */
+static int g_sqmode = ORC_SQ_LIBM;
+
+void orc_set_square_mode(int mode) { g_sqmode = mode; }
+int orc_get_square_mode(void) { return g_sqmode; }
+
+static double sql(double x) { return g_sqmode == ORC_SQ_LIBM ? pow(x, 2.0) : x * x; }
+static double sqp(double x) { return g_sqmode == ORC_SQ_EXACT ? x * x : pow(x, W_P); }
+
 /* ------------------------------------------------------------- WENO pieces */
 
 /* Smoothness indicators of weno5js.cpp:75-82 on f[0..4] (already mirrored for
  * the minus side).  zq=1 reproduces the +inner beta2 of weno5z.cpp:80. */
 static void smooth5(const double *f, int zq, double *is0, double *is1, double *is2)
 {
-    *is0 = C_G0 * pow((f[0] - 2.0 * f[1] + f[2]), 2) + C_G1 * pow((f[0] - 4.0 * f[1] + 3.0 * f[2]), 2);
-    *is1 = C_G0 * pow((f[1] - 2.0 * f[2] + f[3]), 2) + C_G1 * pow((f[1] - f[3]), 2);
+    *is0 = C_G0 * sql((f[0] - 2.0 * f[1] + f[2])) + C_G1 * sql((f[0] - 4.0 * f[1] + 3.0 * f[2]));
+    *is1 = C_G0 * sql((f[1] - 2.0 * f[2] + f[3])) + C_G1 * sql((f[1] - f[3]));
     if (zq)
-        *is2 = C_G0 * pow((f[2] - 2.0 * f[3] + f[3]), 2.0) + C_G1 * pow((3.0 * f[2] - 4.0 * f[3] + f[4]), 2.0);
+        *is2 = C_G0 * sql((f[2] - 2.0 * f[3] + f[3])) + C_G1 * sql((3.0 * f[2] - 4.0 * f[3] + f[4]));
     else
-        *is2 = C_G0 * pow((f[2] - 2.0 * f[3] + f[4]), 2) + C_G1 * pow((3.0 * f[2] - 4.0 * f[3] + f[4]), 2);
+        *is2 = C_G0 * sql((f[2] - 2.0 * f[3] + f[4])) + C_G1 * sql((3.0 * f[2] - 4.0 * f[3] + f[4]));
 }
 
 static void cand5(const double *f, double *q0, double *q1, double *q2)
@@ -61,9 +83,9 @@ static double weno5js_face(const double *f)
     double is0, is1, is2, q0, q1, q2;
     smooth5(f, 0, &is0, &is1, &is2);
     cand5(f, &q0, &q1, &q2);
-    double a0 = C_W5C[0] / pow((W_SS + is0), W_P);
-    double a1 = C_W5C[1] / pow((W_SS + is1), W_P);
-    double a2 = C_W5C[2] / pow((W_SS + is2), W_P);
+    double a0 = C_W5C[0] / sqp((W_SS + is0));
+    double a1 = C_W5C[1] / sqp((W_SS + is1));
+    double a2 = C_W5C[2] / sqp((W_SS + is2));
     double as = a0 + a1 + a2;
     double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
     return o0 * q0 + o1 * q1 + o2 * q2;
@@ -76,9 +98,9 @@ static double weno5z_face(const double *f, int zq)
     smooth5(f, zq, &is0, &is1, &is2);
     cand5(f, &q0, &q1, &q2);
     double tau = fabs(is2 - is0);
-    double a0 = C_W5C[0] * (1.0 + pow(tau / (is0 + W_SS), W_P));
-    double a1 = C_W5C[1] * (1.0 + pow(tau / (is1 + W_SS), W_P));
-    double a2 = C_W5C[2] * (1.0 + pow(tau / (is2 + W_SS), W_P));
+    double a0 = C_W5C[0] * (1.0 + sqp(tau / (is0 + W_SS)));
+    double a1 = C_W5C[1] * (1.0 + sqp(tau / (is1 + W_SS)));
+    double a2 = C_W5C[2] * (1.0 + sqp(tau / (is2 + W_SS)));
     double as = a0 + a1 + a2;
     double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
     return o0 * q0 + o1 * q1 + o2 * q2;
@@ -109,20 +131,33 @@ static double central8_face(const double *c)
 /* ReconstructionboundaryScheme stencils: reconstructionBoundaryScheme.cpp:322-343 */
 static double third_order(double f0, double f1, double f2)
 {
-    double is0 = pow((f0 - f1), 2);
-    double is1 = pow((f1 - f2), 2);
+    double is0 = sql((f0 - f1));
+    double is1 = sql((f1 - f2));
     double q20 = -0.5 * f0 + 1.5 * f1;
     double q21 = 0.5 * f1 + 0.5 * f2;
-    double aa0 = 1.0 / 3.0 / pow(1.0e-6 + is0, 2.0);
-    double aa1 = 2.0 / 3.0 / pow(1.0e-6 + is1, 2.0);
+    double aa0 = 1.0 / 3.0 / sql(1.0e-6 + is0);
+    double aa1 = 2.0 / 3.0 / sql(1.0e-6 + is1);
     double w0 = aa0 / (aa0 + aa1);
     double w1 = aa1 / (aa0 + aa1);
     return w0 * q20 + w1 * q21;
 }
 
-/* fifthOrderP/N (reconstructionBoundaryScheme.cpp:345-388,425-474) are the
- * WENO5-JS formula with the same constants. */
-static double fifth_order(const double *f) { return weno5js_face(f); }
+/* fifthOrderP/N (reconstructionBoundaryScheme.cpp:345-388,425-474): the
+ * WENO5-JS formula with the same constants, its weight denominators squared
+ * with a literal power of 2.0 (weno5js.cpp:97-99 raise them to the runtime
+ * member p instead). */
+static double fifth_order(const double *f)
+{
+    double is0, is1, is2, q0, q1, q2;
+    smooth5(f, 0, &is0, &is1, &is2);
+    cand5(f, &q0, &q1, &q2);
+    double a0 = C_W5C[0] / sql((1.0e-6 + is0));
+    double a1 = C_W5C[1] / sql((1.0e-6 + is1));
+    double a2 = C_W5C[2] / sql((1.0e-6 + is2));
+    double as = a0 + a1 + a2;
+    double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
+    return o0 * q0 + o1 * q1 + o2 * q2;
+}
 
 /* Exterior (physical-face) closure, reconstructionBoundaryScheme.cpp:52-312.
  * Writes faces 0..w-1 (left) or n-w+1..n (right). */
@@ -194,9 +229,9 @@ static void crweno5_inner(int upwind, int n, int w, int W, const double *ext, do
         double is0, is1, is2;
         stencil5(ext, W, i + 1, upwind, f);
         smooth5(f, 0, &is0, &is1, &is2);
-        double a0 = C_CRC[0] / pow((W_SS + is0), W_P);
-        double a1 = C_CRC[1] / pow((W_SS + is1), W_P);
-        double a2 = C_CRC[2] / pow((W_SS + is2), W_P);
+        double a0 = C_CRC[0] / sqp((W_SS + is0));
+        double a1 = C_CRC[1] / sqp((W_SS + is1));
+        double a2 = C_CRC[2] / sqp((W_SS + is2));
         double as = a0 + a1 + a2;
         double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
         if (upwind > 0) { /* crweno5.cpp:155-160 */
@@ -342,18 +377,18 @@ int orc_diff_line(int scheme, int n, int w, const double *ext, int typeL, int ty
 /* ------------------------------------------------------------- physics */
 
 /* AUSM Mach/pressure splittings: physicsModel.cpp:803-850 (AUSM), :891-938 (AUSM+) */
-static double machL(double m) { return fabs(m) < 1.0 ? 0.25 * pow((m + 1.0), 2.0) + 0.125 * pow((m * m - 1), 2.0) : 0.50 * (m + fabs(m)); }
-static double machR(double m) { return fabs(m) < 1.0 ? -0.25 * pow((m - 1.0), 2.0) - 0.125 * pow((m * m - 1), 2.0) : 0.50 * (m - fabs(m)); }
+static double machL(double m) { return fabs(m) < 1.0 ? 0.25 * sql((m + 1.0)) + 0.125 * sql((m * m - 1)) : 0.50 * (m + fabs(m)); }
+static double machR(double m) { return fabs(m) < 1.0 ? -0.25 * sql((m - 1.0)) - 0.125 * sql((m * m - 1)) : 0.50 * (m - fabs(m)); }
 static double presL(double m, double p) { return fabs(m) < 1.0 ? p * (m + 1.0) / 2.0 : 0.50 * p * (m + fabs(m)) / m; }
 static double presR(double m, double p) { return fabs(m) < 1.0 ? -p * (m - 1.0) / 2.0 : 0.5 * p * (m - fabs(m)) / m; }
 static double presLp(double m, double p)
 {
-    return fabs(m) < 1.0 ? p * (0.25 * pow((m + 1.0), 2.0) * (2.0 - m) + 0.1875 * m * pow((m * m - 1.0), 2.0))
+    return fabs(m) < 1.0 ? p * (0.25 * sql((m + 1.0)) * (2.0 - m) + 0.1875 * m * sql((m * m - 1.0)))
                          : 0.50 * p * (m + fabs(m)) / m;
 }
 static double presRp(double m, double p)
 {
-    return fabs(m) < 1.0 ? p * (0.25 * pow(m - 1.0, 2.0) * (2.0 + m) - 0.1875 * m * pow((m * m - 1.0), 2.0))
+    return fabs(m) < 1.0 ? p * (0.25 * sql(m - 1.0) * (2.0 + m) - 0.1875 * m * sql((m * m - 1.0)))
                          : 0.5 * p * (m - fabs(m)) / m;
 }
 
@@ -1084,7 +1119,7 @@ int orc_stage(orc_case *C, int stage, double cfl)
             double res_this = 0.0;
             long x;
             for (c = 0; c < 5; c++)
-                for (x = 0; x < B->N; x++) res_this += pow((B->q[c][x] - B->qn[c][x]), 2);
+                for (x = 0; x < B->N; x++) res_this += sql((B->q[c][x] - B->qn[c][x]));
             B->res_local = fabs(sqrt(res_this / (B->N * 5)) - B->res_last) / (C->dt_global * C->dt_global);
             B->res_last = sqrt(res_this / (B->N * 5));
             if (B->res_local > C->res_global) C->res_global = B->res_local; /* MPI_MAX */

@@ -42,6 +42,13 @@ typedef struct {
 typedef struct orc_case orc_case;
 
 /* Case with nblocks blocks; dims = nblocks*3 cell counts. Returns 0 or <0. */
+/* How the reference's pow(x, 2) squares are restated (nuc3d_oracle.c): 0 libm
+ * pow (the makefile's -O0 build), 1 literal-exponent squares as x*x (the -O2
+ * build), 2 every square as x*x.  Process-wide; default 0. */
+enum { ORC_SQ_LIBM = 0, ORC_SQ_O2 = 1, ORC_SQ_EXACT = 2 };
+void orc_set_square_mode(int mode);
+int orc_get_square_mode(void);
+
 int orc_case_create(const orc_params *p, int nblocks, const int *dims, orc_case **out);
 void orc_case_destroy(orc_case *c);
 

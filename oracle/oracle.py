@@ -33,6 +33,10 @@ CD6, CD8 = 0, 1
 LF, AUSM, AUSMP = 0, 1, 2
 EULER, NS = 0, 1
 BC_INLET, BC_OUTLET, BC_WALL, BC_SYMM = 0, 1, 2, 3
+# how the reference's pow(x, 2) squares are restated: libm pow (the makefile's
+# -O0 build), literal-exponent squares folded to x*x (the same sources at -O2),
+# or every square as the correctly rounded x*x (nuc3d_oracle.c)
+SQ_LIBM, SQ_O2, SQ_EXACT = 0, 1, 2
 
 SCHEME_NAMES = {WENO5JS: "weno5js", WENO5Z: "weno5z", CRWENO5: "crweno5"}
 DIFF_NAMES = {CD6: "cd6", CD8: "cd8"}
@@ -129,6 +133,13 @@ def _lib():
     lib.orc_get_residual.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_get_residual.restype = ctypes.c_double
     lib.orc_set_hybrid_mask.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    lib.orc_set_square_mode.argtypes = [ctypes.c_int]
+    lib.orc_set_square_mode.restype = None
+    lib.orc_get_square_mode.argtypes = []
+    # NUC3D_ORACLE_SQ selects the square convention for a whole run (default libm)
+    env = os.environ.get("NUC3D_ORACLE_SQ")
+    if env:
+        lib.orc_set_square_mode(int(env))
     return lib
 
 
@@ -140,6 +151,24 @@ def lib():
     if _LIB is None:
         _LIB = _lib()
     return _LIB
+
+
+class square_mode:
+    """Context manager: evaluate the oracle with another square convention
+    (SQ_LIBM, SQ_O2 or SQ_EXACT), restoring the previous one on exit."""
+
+    def __init__(self, mode: int):
+        self.mode = mode
+        self.prev = None
+
+    def __enter__(self):
+        self.prev = lib().orc_get_square_mode()
+        lib().orc_set_square_mode(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_square_mode(self.prev)
+        return False
 
 
 def _dp(a: np.ndarray):

@@ -8,7 +8,7 @@ import pytest
 
 from nuc3d_amd import cases, solver
 from oracle import oracle as O
-from tests.parity import TOL, relerr
+from tests.parity import check
 
 pytestmark = pytest.mark.gpu
 
@@ -30,9 +30,9 @@ def test_flat_plate_small_against_oracle(exact):
     g.steps(2, 0.5)
     o.steps(2, 0.5)
     for b in range(len(blocks)):
-        err = relerr(g.get_q(b), o.get_q(b))
-        print("  block %d rel err %.3e" % (b, err), flush=True)
-        assert err <= TOL
+        err = check(g.get_q(b), o.get_q(b), case="flatplate_small_steps2", exact=exact)
+        print("  block %d parity ratio %.3f" % (b, err), flush=True)
+        assert err <= 1.0
     assert np.isfinite(g.get_q(0)).all()
     g.close()
 

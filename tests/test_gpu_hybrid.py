@@ -16,7 +16,7 @@ import pytest
 from nuc3d_amd import cases, solver
 from oracle import oracle as O
 from oracle import sensor as S
-from tests.parity import TOL, relerr
+from tests.parity import check
 
 pytestmark = pytest.mark.gpu
 
@@ -105,7 +105,9 @@ def test_forward_matches_oracle():
     zo, _ = S.forward(x, params)
     err = np.abs(z - zo).max() / max(1.0, np.abs(zo).max())
     print("  max |dz| / max|z| = %.3e" % err, flush=True)
-    assert err <= 1e-2
+    # the same bf16 operands with FP32 accumulation in another order: measured
+    # 2e-7 of max|z|
+    assert err <= 2e-6
     s.close()
 
 
@@ -212,9 +214,9 @@ def test_hybrid_rhs_parity_periodic(model, exact):
     for d in range(3):
         o.set_hybrid_mask(0, d, flags[d])
     o.rhs()
-    err = relerr(g.get_rhs(0), o.get_rhs(0))
-    print("  rhs rel err %.3e" % err, flush=True)
-    assert err <= TOL
+    err = check(g.get_rhs(0), o.get_rhs(0))
+    print("  rhs parity ratio %.3f" % err, flush=True)
+    assert err <= 1.0
     cen, net, fb = g.hybrid_counts()
     assert cen == sum(int((f == 0).sum()) for f in flags) and net == sum(int((f == 1).sum()) for f in flags)
     g.close()
@@ -253,11 +255,11 @@ def test_hybrid_steps_walls_two_blocks(exact, interval):
                 o.set_hybrid_mask(b, d, f)
             o.stage(st, 0.5)
             for b in range(2):
-                err = relerr(g.get_q(b), o.get_q(b))
-                assert err <= TOL, (step, st, b, err)
+                err = check(g.get_q(b), o.get_q(b))
+                assert err <= 1.0, (step, st, b, err)
     f0 = g.hybrid_flags(0, 0)
     assert (f0[..., :4] == 3).all() and (f0[..., 4:] != 3).all()   # left wall closure of block 0
-    print("  2 steps within %.1e" % TOL, flush=True)
+    print("  2 steps within the parity bar", flush=True)
     g.close()
 
 

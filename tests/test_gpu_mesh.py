@@ -8,7 +8,7 @@ import pytest
 from nuc3d_amd import cases, solver
 from oracle import oracle as O
 from tests.golden import all_fixtures
-from tests.parity import TOL, relerr
+from tests.parity import check
 
 pytestmark = pytest.mark.gpu
 
@@ -53,9 +53,9 @@ def test_mesh_fixture_from_reference_block_initial():
     np.testing.assert_array_equal(dxyz, blk.dxyz)
     g.set_q(0, blk.q)   # the reference's Taylor-Green initial state (block::initialQ)
     g.steps(fx.meta["nsteps"], fx.meta["cfl"])
-    err = relerr(g.get_q(0), fx.out(0, "q"))
-    print("  q rel err %.3e" % err, flush=True)
-    assert err <= TOL
+    err = check(g.get_q(0), fx.out(0, "q"))
+    print("  q parity ratio %.3f" % err, flush=True)
+    assert err <= 1.0
     g.close()
 
 
@@ -102,7 +102,9 @@ def test_restart_file_read_by_the_reference(tmp_path):
     np.testing.assert_array_equal(g2.get_q(0), want)
     g.close()
     g2.close()
-    if os.path.exists(O.REF_BIN_METRICS):
-        O.write_mesh_case(d, O.Params(model=O.NS), [nodes], [cases.periodic_faces()], start_step=2)
-        ref = O.run_ref_metrics(d, 1)[0]
-        np.testing.assert_array_equal(ref["q"], want)
+    # the reference's reader (built here by __graft_entry__.build(); it travels
+    # with the tree)
+    assert os.path.exists(O.REF_BIN_METRICS), "oracle/_ref/ref_metrics missing: run __graft_entry__.build()"
+    O.write_mesh_case(d, O.Params(model=O.NS), [nodes], [cases.periodic_faces()], start_step=2)
+    ref = O.run_ref_metrics(d, 1)[0]
+    np.testing.assert_array_equal(ref["q"], want)

@@ -2,15 +2,15 @@
 same inputs.  The oracle agrees bit-for-bit with the reference compiled from
 its own sources (tests/test_oracle_ref_live.py, tests/golden/).
 
-Bar: <= 1e-12 relative max-norm per conserved variable (density, momentum
-vector, total energy), as defined in tests/parity.py.
+Bar: <= 1e-12 relative max-norm per conserved component, as defined in
+tests/parity.py.
 """
 import numpy as np
 import pytest
 
 from nuc3d_amd import cases, solver
 from oracle import oracle as O
-from tests.parity import TOL, relerr
+from tests.parity import check, relerr_components
 
 pytestmark = pytest.mark.gpu
 
@@ -36,9 +36,9 @@ def test_rhs_tgv(exact, model, scheme):
     blocks = cases.taylor_green(dims=(20, 18, 22), noise=1e-3, seed=3)
     sp, op = both_params(model=model, scheme=(scheme,) * 3, exact=exact)
     g, o = run_rhs(sp, op, blocks)
-    err = relerr(g.get_rhs(0), o.get_rhs(0))
-    print("  rel err %.3e" % err, flush=True)
-    assert err <= TOL
+    err = check(g.get_rhs(0), o.get_rhs(0))
+    print("  parity ratio %.3f" % err, flush=True)
+    assert err <= 1.0
     np.testing.assert_allclose(g.max_eigen(0), o.max_eigen(0), rtol=1e-14)
     g.close()
 
@@ -52,9 +52,9 @@ def test_steps_tgv_ns(exact):
     o = O.OracleCase(op, blocks)
     g.steps(3, 0.5)
     dts = o.steps(3, 0.5)
-    err = relerr(g.get_q(0), o.get_q(0))
-    print("  rel err %.3e dt %.17g %.17g" % (err, g.dt(), dts[-1]), flush=True)
-    assert err <= TOL
+    err = check(g.get_q(0), o.get_q(0))
+    print("  parity ratio %.3f dt %.17g %.17g" % (err, g.dt(), dts[-1]), flush=True)
+    assert err <= 1.0
     assert abs(g.dt() - dts[-1]) <= 1e-14 * dts[-1]
     g.close()
 
@@ -70,9 +70,9 @@ def test_multiblock_weno5z(exact):
     g.steps(2, 0.5)
     o.steps(2, 0.5)
     for b in range(3):
-        err = relerr(g.get_q(b), o.get_q(b))
-        print("  block %d rel err %.3e" % (b, err), flush=True)
-        assert err <= TOL
+        err = check(g.get_q(b), o.get_q(b))
+        print("  block %d parity ratio %.3f" % (b, err), flush=True)
+        assert err <= 1.0
     g.close()
 
 
@@ -87,9 +87,9 @@ def test_shu_osher_inflow_outflow(scheme, riemann, exact):
     o = O.OracleCase(op, blocks)
     g.steps(3, 0.4)
     o.steps(3, 0.4)
-    err = relerr(g.get_q(0), o.get_q(0))
-    print("  rel err %.3e" % err, flush=True)
-    assert err <= TOL
+    err = check(g.get_q(0), o.get_q(0))
+    print("  parity ratio %.3f" % err, flush=True)
+    assert err <= 1.0
     g.close()
 
 
@@ -104,17 +104,17 @@ def test_physical_faces_ns(faces, exact):
     b.faces = faces
     sp, op = both_params(t_wall=1.2, exact=exact)
     g, o = run_rhs(sp, op, [b])
-    err = relerr(g.get_rhs(0), o.get_rhs(0))
-    print("  rhs rel err %.3e" % err, flush=True)
-    assert err <= TOL
+    err = check(g.get_rhs(0), o.get_rhs(0))
+    print("  rhs parity ratio %.3f" % err, flush=True)
+    assert err <= 1.0
     g.close()
     g = solver.Case(sp, [b])
     o = O.OracleCase(op, [b])
     g.steps(2, 0.5)
     o.steps(2, 0.5)
-    err = relerr(g.get_q(0), o.get_q(0))
-    print("  steps rel err %.3e" % err, flush=True)
-    assert err <= TOL
+    err = check(g.get_q(0), o.get_q(0))
+    print("  steps parity ratio %.3f" % err, flush=True)
+    assert err <= 1.0
     g.close()
 
 
@@ -144,9 +144,9 @@ def test_crweno5_rhs_tgv(exact, model):
     blocks = cases.taylor_green(dims=(20, 18, 22), noise=1e-3, seed=31)
     sp, op = both_params(model=model, scheme=(solver.CRWENO5,) * 3, exact=exact)
     g, o = run_rhs(sp, op, blocks)
-    err = relerr(g.get_rhs(0), o.get_rhs(0))
-    print("  rel err %.3e" % err, flush=True)
-    assert err <= TOL
+    err = check(g.get_rhs(0), o.get_rhs(0))
+    print("  parity ratio %.3f" % err, flush=True)
+    assert err <= 1.0
     g.close()
 
 
@@ -161,9 +161,15 @@ def test_crweno5_long_lines_shu_osher(exact):
     o = O.OracleCase(op, blocks)
     g.steps(2, 0.5)
     o.steps(2, 0.5)
-    err = relerr(g.get_q(0), o.get_q(0))
-    print("  rel err %.3e" % err, flush=True)
-    assert err <= TOL
+    # the reference is undefined on these lines: the components' spread is the
+    # restatement's between libm squares (the -O0 build) and rounded squares
+    with O.square_mode(O.SQ_EXACT):
+        o2 = O.OracleCase(op, blocks)
+        o2.steps(2, 0.5)
+    spread = relerr_components(o2.get_q(0), o.get_q(0))
+    err = check(g.get_q(0), o.get_q(0), exact=exact, spread=spread)
+    print("  parity ratio %.3f (square spread %s)" % (err, " ".join("%.1e" % x for x in spread)), flush=True)
+    assert err <= 1.0
     g.close()
 
 
@@ -176,9 +182,9 @@ def test_mixed_schemes_per_direction(exact):
     o = O.OracleCase(op, blocks)
     g.steps(2, 0.5)
     o.steps(2, 0.5)
-    err = relerr(g.get_q(0), o.get_q(0))
-    print("  rel err %.3e" % err, flush=True)
-    assert err <= TOL
+    err = check(g.get_q(0), o.get_q(0))
+    print("  parity ratio %.3f" % err, flush=True)
+    assert err <= 1.0
     g.close()
 
 
@@ -195,9 +201,9 @@ def test_curvilinear_multiblock(scheme, exact):
     g.steps(2, 0.5)
     o.steps(2, 0.5)
     for b in range(2):
-        err = relerr(g.get_q(b), o.get_q(b))
-        print("  block %d rel err %.3e" % (b, err), flush=True)
-        assert err <= TOL
+        err = check(g.get_q(b), o.get_q(b))
+        print("  block %d parity ratio %.3f" % (b, err), flush=True)
+        assert err <= 1.0
     g.close()
 
 
@@ -218,9 +224,9 @@ def test_central8_two_blocks(walls, exact):
     g.steps(2, 0.5)
     o.steps(2, 0.5)
     for b in range(2):
-        err = relerr(g.get_q(b), o.get_q(b))
-        print("  block %d rel err %.3e" % (b, err), flush=True)
-        assert err <= TOL
+        err = check(g.get_q(b), o.get_q(b))
+        print("  block %d parity ratio %.3f" % (b, err), flush=True)
+        assert err <= 1.0
     g.close()
 
 
@@ -233,7 +239,7 @@ def test_central8_x_weno5js_yz(exact):
     sp, op = both_params(model=solver.EULER3D, scheme=(solver.CENTRAL8, solver.WENO5JS, solver.WENO5JS),
                          width=(4, 3, 3), exact=exact)
     g, o = run_rhs(sp, op, blocks)
-    err = relerr(g.get_rhs(0), o.get_rhs(0))
-    print("  rhs rel err %.3e" % err, flush=True)
-    assert err <= TOL
+    err = check(g.get_rhs(0), o.get_rhs(0))
+    print("  rhs parity ratio %.3f" % err, flush=True)
+    assert err <= 1.0
     g.close()

@@ -340,3 +340,24 @@ def test_diagnostics_taylor_green_vorticity_and_energy():
     u = np.sin(X) * np.cos(Y) * np.cos(Z)
     v = -np.cos(X) * np.sin(Y) * np.cos(Z)
     assert abs(kin - (0.5 * (u * u + v * v)).sum()) <= 1e-12 * kin
+
+
+def test_square_conventions():
+    """The three square conventions of the restatement: identical where every
+    square is exact (integer data: libm pow(x, 2) is then x*x), and the libm
+    one differs from the correctly rounded squares on general data, as glibc's
+    pow(x, 2.0) is not correctly rounded for every argument."""
+    ext_int = np.array([3.0, -1.0, 4.0, 1.0, -5.0, 9.0, 2.0, -6.0, 5.0, 3.0, 5.0, -8.0, 9.0, 7.0], dtype=np.float64)
+    rng = np.random.default_rng(5)
+    ext_rnd = rng.standard_normal(4000)
+    outs = {}
+    for mode in (O.SQ_LIBM, O.SQ_O2, O.SQ_EXACT):
+        with O.square_mode(mode):
+            outs[mode] = (O.recon_line(O.WENO5JS, 1, ext_int, 3), O.recon_line(O.WENO5Z, -1, ext_rnd, 3))
+    for mode in (O.SQ_O2, O.SQ_EXACT):
+        np.testing.assert_array_equal(outs[mode][0], outs[O.SQ_LIBM][0])
+    # general data: the conventions differ in the last bits of some faces
+    d = np.abs(outs[O.SQ_EXACT][1] - outs[O.SQ_LIBM][1])
+    assert 0 < np.count_nonzero(d) < d.size
+    assert d.max() <= 1e-14 * np.abs(outs[O.SQ_LIBM][1]).max()
+    assert O.lib().orc_get_square_mode() == O.SQ_LIBM

@@ -63,12 +63,13 @@ def test_live_energy_fixup_path():
 
 @pytest.mark.skipif(not O.ref_available(optimised=True), reason="-O2 reference build absent")
 @pytest.mark.parametrize("name", ["shuosher_euler_ausmp_steps2", "shuosher_euler_crweno5_steps3"])
-def test_reference_builds_agree_under_parity_measure(name):
+def test_reference_build_spread_matches_record(name):
     """The reference's -O0 and -O2 builds of the same sources differ by more
-    than 1e-12 in the near-zero v/w momentum of the quasi-1D Shu-Osher flow,
-    but agree to < 1e-12 with momentum measured as a vector."""
+    than 1e-12 in the near-zero v/w momentum of the quasi-1D Shu-Osher flow;
+    the committed record (tests/golden/ref_build_spread.json, which the GPU
+    parity tests bound those components by) is that measurement."""
     from tests.golden import all_fixtures
-    from tests.parity import relerr, relerr_components
+    from tests.parity import build_spread, relerr_components
 
     fx = [f for f in all_fixtures() if f.name == name][0]
     pk = fx.params()
@@ -80,8 +81,9 @@ def test_reference_builds_agree_under_parity_measure(name):
         d = tempfile.mkdtemp(prefix="nuc3d_ref_")
         O.write_ref_case(d, p, fx.blocks())
         outs.append(O.run_ref(d, 1, "steps", fx.meta["nsteps"], fx.meta["cfl"], optimised=opt)[0]["q"])
-    assert max(relerr_components(outs[1], outs[0])) > 1e-12
-    assert relerr(outs[1], outs[0]) < 1e-13
+    live = relerr_components(outs[1], outs[0])
+    assert max(live) > 1e-12
+    np.testing.assert_allclose(live, build_spread(name), rtol=1e-12, atol=0)
 
 
 def test_live_curvilinear_multiblock_two_ranks():
