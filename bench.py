@@ -81,48 +81,85 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n, nsteps=3, ranks=None, timeout=240):
-    """Reference (oracle/_ref, -O2) on a bounded sample of the workload:
-    a 256 x 128 x 32 periodic slab of the same turbulence field, split into
-    `ranks` MPI blocks along x, 3 RK3 steps, mean of steps 2..3."""
+def host_cores():
+    """(physical cores of the host from lscpu, CPUs this process may run on)."""
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=20).stdout
+        cores = {tuple(l.split(",")) for l in out.splitlines() if l and not l.startswith("#")}
+        phys = len(cores) or None
+    except Exception:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return phys, avail
+
+
+def cpu_baseline(n, warm=10, timed=2, timeout=600):
+    """The reference (oracle/_ref: its own sources built twice, the makefile's
+    -O0 and -O2 -march=x86-64-v3) on a bounded sample of the workload: a
+    periodic slab of the same developed-free turbulence field split along x
+    into one block per MPI rank, one rank per core this process may use (at
+    most the host's physical cores).  Timed as block::solve times a run
+    (block.cpp:576-584, MPI_Wtime around each RK3 step): `warm` warm-up
+    steps, then the mean wall time of the next `timed` steps.  `value` is the
+    -O2 build's cell-updates/s; the -O0 build's is beside it."""
     import numpy as np
     from nuc3d_amd import cases
     from oracle import oracle as O
 
-    ranks = ranks or min(16, os.cpu_count() or 1)
+    phys_cores, avail = host_cores()
+    ranks = min(avail, phys_cores or avail)
     blocks, phys = cases.isotropic_turbulence(n=n, seed=0)
     spec = blocks[0]
-    nx, ny, nz = 256, 128, 32
-    sub = cases.BlockSpec((nx, ny, nz), cases.periodic_faces(), spec.jac[:nz, :ny, :nx].copy(),
-                          spec.met[:, :nz, :ny, :nx].copy(), spec.dxyz[:, :nz, :ny, :nx].copy(),
-                          spec.q[:, :nz, :ny, :nx].copy())
-    while nx // ranks < 8:
-        ranks //= 2
-    parts = cases.split_block_x(sub, ranks)
     p = O.Params(model=O.NS, mach=phys.mach, reynolds=phys.reynolds)
-    cells = nx * ny * nz
-    if O.ref_available(optimised=True):
+
+    def slab(nx, ny, nz):
+        sub = cases.BlockSpec((nx, ny, nz), cases.periodic_faces(), spec.jac[:nz, :ny, :nx].copy(),
+                              spec.met[:, :nz, :ny, :nx].copy(), spec.dxyz[:, :nz, :ny, :nx].copy(),
+                              spec.q[:, :nz, :ny, :nx].copy())
+        r = ranks
+        while nx // r < 8:
+            r //= 2
+        return cases.split_block_x(sub, r), r
+
+    if not (O.ref_available(optimised=True) and O.ref_available()):
+        sub = cases.BlockSpec((64, 64, 32), cases.periodic_faces(), spec.jac[:32, :64, :64].copy(),
+                              spec.met[:, :32, :64, :64].copy(), spec.dxyz[:, :32, :64, :64].copy(),
+                              spec.q[:, :32, :64, :64].copy())
+        oc = O.OracleCase(p, [sub])
+        t0 = time.perf_counter()
+        oc.steps(1, 0.5)
+        t = time.perf_counter() - t0
+        return {"value": 64 * 64 * 32 / t, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                "sample": "C restatement (oracle/), 1 core, 64x64x32 periodic slab of the 256^3 field, 1 RK3 step "
+                          "(the reference build oracle/_ref is absent)"}
+    res = {}
+    # the -O0 build is about 4x slower: half the slab keeps both runs near 10-20 s
+    for name, opt, dims in (("O2", True, (256, 128, 32)), ("O0", False, (256, 64, 32))):
+        parts, r = slab(*dims)
         d = tempfile.mkdtemp(prefix="nuc3d_cpu_")
-        try:
-            O.write_ref_case(d, p, parts)
-            out = O.run_ref(d, ranks, "steps", nsteps, 0.5, optimised=True, timeout=timeout)
-            walls = out[0]["walls"]
-            t = float(np.mean(walls[1:]))
-            return {"value": cells / t, "unit": "cell-updates/s", "cores": ranks, "kind": "reference",
-                    "sample": "reference built from its sources at -O2 (oracle/_ref), %d MPI ranks, "
-                              "%dx%dx%d periodic slab of the 256^3 turbulence field in %d x-blocks, "
-                              "mean wall time of RK3 steps 2..%d" % (ranks, nx, ny, nz, ranks, nsteps)}
-        except Exception as e:  # fall back to the oracle below
-            log("cpu baseline: reference run failed (%s); timing the oracle" % e)
-    sub = cases.BlockSpec((64, 64, 32), cases.periodic_faces(), spec.jac[:32, :64, :64].copy(),
-                          spec.met[:, :32, :64, :64].copy(), spec.dxyz[:, :32, :64, :64].copy(),
-                          spec.q[:, :32, :64, :64].copy())
-    oc = O.OracleCase(p, [sub])
-    t0 = time.perf_counter()
-    oc.steps(1, 0.5)
-    t = time.perf_counter() - t0
-    return {"value": 64 * 64 * 32 / t, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "C restatement (oracle/), 1 core, 64x64x32 periodic slab of the 256^3 field, 1 RK3 step"}
+        O.write_ref_case(d, p, parts)
+        t0 = time.perf_counter()
+        out = O.run_ref(d, r, "steps", warm + timed, 0.5, optimised=opt, timeout=timeout)
+        walls = np.asarray(out[0]["walls"], dtype=np.float64)
+        t = float(np.mean(walls[warm:warm + timed]))
+        cells = dims[0] * dims[1] * dims[2]
+        res[name] = {"value": cells / t, "ms_per_step": 1e3 * t, "ranks": r, "slab": "%dx%dx%d" % dims,
+                     "run_s": time.perf_counter() - t0}
+        log("cpu baseline: -%s build, %d ranks, %s slab: %.4g cell-updates/s" % (name, r, res[name]["slab"],
+                                                                                   res[name]["value"]))
+    return {"value": res["O2"]["value"], "unit": "cell-updates/s", "cores": res["O2"]["ranks"], "kind": "reference",
+            "host_physical_cores": phys_cores, "cpus_available": avail,
+            "O0_makefile": res["O0"], "O2": res["O2"],
+            "sample": "the reference built from its own sources (oracle/_ref): -O2 -march=x86-64-v3 (value) and the "
+                      "makefile's -O0 (O0_makefile), %d MPI ranks (one per available core; the host has %s physical "
+                      "cores, %d available to this job), periodic slab of the 256^3 turbulence field (%s for -O2, "
+                      "%s for -O0) in x-blocks; MPI_Wtime per RK3 step as block.cpp:576-584, %d warm-up steps, "
+                      "mean of steps %d..%d" % (res["O2"]["ranks"], phys_cores, avail, res["O2"]["slab"],
+                                               res["O0"]["slab"], warm, warm + 1, warm + timed)}
 
 
 # timing class -> kernel (rocprofv3 names: k_sweep_<variant><DIR,...>, ...)
@@ -315,9 +352,9 @@ def main():
     ap.add_argument("--fast", action="store_true",
                     help="contracted WENO arithmetic (within 1e-12 of the reference on well-conditioned fields only)")
     ap.add_argument("--no-hybrid", action="store_true")
-    ap.add_argument("--sensor-interval", type=int, default=2, choices=(1, 2),
-                    help="steps between the hybrid's decisions (2: stage 0 of every other step; the 256^3 "
-                         "turbulence keeps the same L2 deviation as with 1)")
+    ap.add_argument("--sensor-interval", type=int, default=1, choices=(1, 2),
+                    help="multi-block workloads: steps between the hybrid network's decisions (the turbulence "
+                         "workload measures both)")
     ap.add_argument("--workload", choices=("turbulence", "batch48", "flatplate", "shu_osher"), default="turbulence",
                     help="batch48: 48 independent TGV and blast cases of seeded sizes 32^3..160^3 in one case "
                          "(block groups); flatplate: the 16-block supersonic flat plate over a bump (60.5 M "
@@ -374,7 +411,12 @@ def main():
     dom = max(tm, key=lambda k: tm[k][0])
     dom_ms, dom_n = tm[dom]
     avg_ms = dom_ms / max(dom_n, 1)
-    achieved = kernel_alg_bytes(dom, ns, uniform) * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    # the algorithmic bytes of one stage-kernel launch: a third of the step's
+    # compulsory bytes per cell-update (320 B on the uniform-mesh path, 560 B
+    # with the metric fields) for every cell of the case
+    alg_launch = alg_step / 3.0 * cells
+    achieved = alg_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    model_gbs = kernel_alg_bytes(dom, ns, uniform) * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     variant = "fast" if args.fast else "exact"
     traffic = read_traffic(dom, n, variant)
     log("bench: plain %.3f ms/step, %.4g cell-updates/s" % (ms_plain, v_plain))
@@ -382,44 +424,54 @@ def main():
     hybrid = None
     if not args.no_hybrid:
         params, thr = train_sensor.load()
-        hcase = solver.Case(solver.Params(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), **common), start)
-        hcase.set_uniform_mesh(not args.no_uniform_mesh)
-        hcase.set_overlap(not args.no_overlap)
-        hcase.set_sensor(params, thr)
-        hcase.set_sensor_interval(args.sensor_interval)
-        hcase.autotune()
-        log("bench: learned hybrid, %d timed steps" % args.steps)
-        v_h, ms_h, tmh = time_method(hcase, args.steps, args.warmup, cells)
-        for k, (ms, cnt) in tmh.items():
-            if cnt:
-                log("bench: hybrid %-19s %8.3f ms avg over %d launches" % (k, ms / cnt, cnt))
-        cen, net, fb = hcase.hybrid_counts()
-        tot = max(cen + net + fb, 1)
-        # both cases have now run the same steps from the developed state
-        jac = spec.jac
-        rho_p = plain.get_q(0)[0] * jac
-        rho_h = hcase.get_q(0)[0] * jac
-        l2 = float(np.sqrt(np.mean((rho_h - rho_p) ** 2)))
+        rho_p = plain.get_q(0)[0] * spec.jac
         rms = float(np.sqrt(np.mean(rho_p ** 2)))
         rms_f = float(np.sqrt(np.mean((rho_p - rho_p.mean()) ** 2)))
-        hybrid = {
-            "value": v_h, "ms_per_step": ms_h, "speedup_vs_plain": v_h / v_plain,
-            "central_fraction": cen / tot, "weno_by_network_fraction": net / tot, "fallback_fraction": fb / tot,
-            "logit_threshold": thr,
-            "l2_density_deviation": l2, "l2_over_density_rms": l2 / rms, "l2_over_fluctuation_rms": l2 / rms_f,
-            "max_density_deviation": float(np.abs(rho_h - rho_p).max()),
-            "kernel_ms_per_step": {k: v[0] / 2 for k, v in tmh.items()},
-            "decision_interval_steps": args.sensor_interval,
-        }
-        log("bench: hybrid %.3f ms/step, %.4g cell-updates/s (x%.3f), central faces %.3f, "
-            "L2 density deviation %.3e of rms (%.3e of the fluctuation rms)" % (
-                ms_h, v_h, v_h / v_plain, cen / tot, l2 / rms, l2 / rms_f))
-        hcase.close()
+        cadences = {}
+        # the network decides at stage 0 of every step (interval 1) or of every
+        # other step (interval 2); the positivity fallback is evaluated at every
+        # stage either way
+        for interval in (1, 2):
+            hcase = solver.Case(solver.Params(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), **common), start)
+            hcase.set_uniform_mesh(not args.no_uniform_mesh)
+            hcase.set_overlap(not args.no_overlap)
+            hcase.set_sensor(params, thr)
+            hcase.set_sensor_interval(interval)
+            hcase.autotune()
+            log("bench: learned hybrid, decisions every %d step(s), %d timed steps" % (interval, args.steps))
+            v_h, ms_h, tmh = time_method(hcase, args.steps, args.warmup, cells)
+            for k, (ms, cnt) in tmh.items():
+                if cnt:
+                    log("bench: hybrid %-19s %8.3f ms avg over %d launches" % (k, ms / cnt, cnt))
+            cen, net, fb = hcase.hybrid_counts()
+            tot = max(cen + net + fb, 1)
+            # both cases have now run the same steps from the developed state
+            rho_h = hcase.get_q(0)[0] * spec.jac
+            l2 = float(np.sqrt(np.mean((rho_h - rho_p) ** 2)))
+            cadences[interval] = {
+                "value": v_h, "ms_per_step": ms_h, "speedup_vs_plain": v_h / v_plain,
+                "central_fraction": cen / tot, "weno_by_network_fraction": net / tot, "fallback_fraction": fb / tot,
+                "l2_density_deviation": l2, "l2_over_density_rms": l2 / rms, "l2_over_fluctuation_rms": l2 / rms_f,
+                "max_density_deviation": float(np.abs(rho_h - rho_p).max()),
+                "kernel_ms_per_step": {k: v[0] / 2 for k, v in tmh.items()},
+            }
+            log("bench: hybrid (interval %d) %.3f ms/step, %.4g cell-updates/s (x%.3f), central faces %.3f, "
+                "L2 density deviation %.3e of rms, %.3e of the fluctuation rms" % (
+                    interval, ms_h, v_h, v_h / v_plain, cen / tot, l2 / rms, l2 / rms_f))
+            hcase.close()
+        hybrid = {"logit_threshold": thr, "decisions_every_step": cadences[1],
+                  "decisions_every_other_step": cadences[2],
+                  "gate": "L2 density deviation from plain WENO5-JS <= 1e-3 of the rms of the density "
+                          "fluctuation (the stricter reading; the deviation over the rms of the density itself "
+                          "is reported beside it)"}
     plain.close()
 
     value, method, ms_step = v_plain, "plain", ms_plain
-    if hybrid and hybrid["l2_over_density_rms"] <= 1e-3 and hybrid["value"] > v_plain:
-        value, method, ms_step = hybrid["value"], "hybrid", hybrid["ms_per_step"]
+    if hybrid:
+        for interval, key in ((1, "decisions_every_step"), (2, "decisions_every_other_step")):
+            h = hybrid[key]
+            if h["l2_over_fluctuation_rms"] <= 1e-3 and h["value"] > value:
+                value, method, ms_step = h["value"], "hybrid (decision interval %d)" % interval, h["ms_per_step"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -460,12 +512,17 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_over_alg": traffic / alg_launch if traffic else None,
             "kernel": KERNEL_OF_CLASS[dom],
             "avg_launch_ms": avg_ms,
-            "alg_bytes_per_cell": kernel_alg_bytes(dom, ns, uniform),
+            "alg_bytes_per_launch": alg_launch,
+            "alg_bytes_basis": "%g B per cell-update / 3 stages x %d cells per stage-kernel launch" % (alg_step, cells),
             "uniform_mesh": uniform,
             "step_alg_bytes_per_cell_update": alg_step,
             "step_level_gbs": alg_step * v_plain / 1e9,
+            "step_frac": alg_step * v_plain / 1e9 / HBM_PEAK_GBS,
+            "kernel_model_bytes_per_cell": kernel_alg_bytes(dom, ns, uniform),
+            "kernel_model_gbs": model_gbs,
         },
         "cpu_baseline": cpu,
     }

@@ -36,7 +36,10 @@ enum {
     NUC3D_ERR_UNSUPPORTED = -4, /* configuration outside what this build implements */
     NUC3D_ERR_NO_DEVICE = -5,   /* no GPU visible */
     NUC3D_ERR_ORDER = -6,       /* RK stages called out of order */
-    NUC3D_ERR_IO = -7           /* a restart file could not be written or read */
+    NUC3D_ERR_IO = -7,          /* a restart file could not be written or read */
+    NUC3D_ERR_NEG_DENSITY = -8, /* a state with non-positive (or NaN) density: see nuc3d_get_positivity */
+    NUC3D_ERR_NEG_PRESSURE = -9 /* a state with negative (or NaN) pressure after the e<0 fix-up
+                                   (physicsModel.cpp:1009-1016 exits there) */
 };
 
 /* Schemes and models, named as in inp/Method.in and inp/PhysModel.in
@@ -204,8 +207,23 @@ int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt);
  * Runs between full steps. */
 int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out);
 
-/* Wait for the case's work on its streams. */
+/* Wait for the case's work on its streams.  Like nuc3d_get_dt,
+ * nuc3d_get_dt_group and nuc3d_run, it returns NUC3D_ERR_NEG_DENSITY or
+ * NUC3D_ERR_NEG_PRESSURE once a state failed the positivity check (below);
+ * the status holds until a new state is uploaded or loaded. */
 int nuc3d_sync(nuc3d_case *c);
+
+/* Positivity of the states advanced so far.  Every state written by an RK
+ * stage (and every uploaded state, at the first call that uses it) takes the
+ * e<0 fix-up of con2prim (physicsModel.cpp:322-332) and then the check that
+ * the reference's con2prim makes next: pressure (E - rho|u|^2/2)(gamma - 1)
+ * not negative (EoSIdealGasgetPressure, physicsModel.cpp:1009-1016, where the
+ * reference prints "negative pressure!!!" and exits).  Density must also be
+ * positive; a NaN fails both.  counts2 receives the cells that failed
+ * [density, pressure] since the state was last uploaded or loaded.  The
+ * stages go on computing (nothing exits); the synchronising calls above
+ * report the failure as a status. */
+int nuc3d_get_positivity(nuc3d_case *c, long long *counts2);
 
 /* dt of the current step (PDEData3d::getDt) and per-block max eigenvalues
  * of the last stage-0 state, or of the state of the last nuc3d_rhs call
@@ -288,11 +306,17 @@ int nuc3d_sensor_train(nuc3d_sensor *s, int steps, int batch, float lr, float po
 /* The network the case's hybrid directions use (NUC3D_SENSOR_NPARAM FP32
  * parameters, rounded to BF16 as in training) and the logit threshold. */
 int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold);
-/* The hybrid decisions are made by nuc3d_rhs and by stage 0 of every step
- * (from the stage-0 state); stages 1 and 2 of the step reuse them.  The last
- * decisions of block b and hybrid direction d, in the layout of the face array
- * ((n_d+1) x n_{d+1} x n_{d+2}, face index fastest): 0 central, 1 WENO5-JS by
- * the network, 2 WENO5-JS by the fallback, 3 physical-face closure. */
+/* The network's decisions are made by nuc3d_rhs and by stage 0 of a step
+ * (from the stage-0 state); the other stages reuse them (see
+ * nuc3d_set_sensor_interval).  The positivity fallback is evaluated again at
+ * every stage on that stage's state: a reused central face whose stencil
+ * means or central face values of rho or p are not positive takes WENO5-JS
+ * (and is recorded as 2 from then on).  A single RHS evaluation keeps its
+ * decisions apart: it never changes the decisions of the step in progress.
+ * The decisions of the last evaluation (nuc3d_rhs, or a stage) of block b
+ * and hybrid direction d, in the layout of the face array ((n_d+1) x n_{d+1}
+ * x n_{d+2}, face index fastest): 0 central, 1 WENO5-JS by the network, 2
+ * WENO5-JS by the fallback, 3 physical-face closure. */
 int nuc3d_download_hybrid_flags(nuc3d_case *c, int b, int d, unsigned char *flags);
 /* Overlap (no counterpart in the reference): the viscous-flux pass of a
  * Navier-Stokes stage reads only the stage's input state and writes only the
@@ -311,8 +335,10 @@ int nuc3d_set_overlap(nuc3d_case *c, int enable);
  * nuc3d_get_uniform_mesh reports whether block b takes that path. */
 int nuc3d_set_uniform_mesh(nuc3d_case *c, int enable);
 int nuc3d_get_uniform_mesh(nuc3d_case *c, int b, int *uniform);
-/* Steps between hybrid decisions: 1 (default, stage 0 of every step) or 2
- * (stage 0 of every other step; the stages between reuse them). */
+/* Steps between the network's decisions: 1 (default, stage 0 of every step)
+ * or 2 (stage 0 of every other step; the stages between reuse them).  Either
+ * way the first stage 0 after a new state (upload, load, faces, metrics), a
+ * new sensor or a new interval decides afresh. */
 int nuc3d_set_sensor_interval(nuc3d_case *c, int steps);
 /* Faces decided so far (counted where decisions are made): central, WENO5-JS
  * by the network, WENO5-JS by the fallback. */

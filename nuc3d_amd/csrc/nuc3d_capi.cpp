@@ -137,7 +137,7 @@ struct nuc3d_case {
     double *dscratch = nullptr;                     // Thomas c', d' of the CRWENO5 lines
     double *ddt = nullptr;             // dt of the current step
     unsigned long long *dlam = nullptr;   // [2][nb][3] eigenvalue maxima (bit patterns)
-    unsigned long long *dcount = nullptr; // [4] fix-up counter
+    unsigned long long *dcount = nullptr; // [4] fix-ups, non-positive density, negative pressure
     unsigned long long *dhcount = nullptr; // hybrid decision counts, NUC3D_HCOUNT_SLOTS x 16
     int cur = 0;                       // state buffer holding Q_work
     int next_stage = 0;
@@ -164,6 +164,10 @@ struct nuc3d_case {
     nuc3d_sensor_dev *dsensor = nullptr;
     std::vector<void *> dflags;
     int hybrid_interval = 1;           // steps between hybrid decisions (1 or 2)
+    // the next stage 0 decides whatever the interval says: the state, the
+    // faces or the sensor changed since the stored decisions were made
+    bool hforce = true;
+    bool flags_rhs = false;            // the last evaluation was a single RHS (its decisions: second half)
     // uniform blocks take J and the metric terms from their descriptor
     bool uniform_mesh = true;
     int *duniform = nullptr;           // [nb] uniform-mesh check flags
@@ -175,6 +179,29 @@ struct nuc3d_case {
 };
 
 namespace {
+
+// After a synchronisation: the positivity status of the states advanced
+// since the last upload (nuc3d_get_positivity).
+int positivity_status(nuc3d_case *c)
+{
+    unsigned long long v[3] = {0, 0, 0};
+    HIPCHK(hipMemcpy(v, c->dcount, sizeof v, hipMemcpyDeviceToHost));
+    if (v[1])
+        return fail(NUC3D_ERR_NEG_DENSITY,
+                    std::to_string(v[1]) + " cells with non-positive or NaN density after an RK stage");
+    if (v[2])
+        return fail(NUC3D_ERR_NEG_PRESSURE, std::to_string(v[2]) +
+                                                " cells with negative or NaN pressure after the e<0 fix-up "
+                                                "(physicsModel.cpp:1009-1016)");
+    return NUC3D_OK;
+}
+
+// a new state: its positivity is checked afresh
+int clear_positivity(nuc3d_case *c)
+{
+    HIPCHK(hipMemset(c->dcount + 1, 0, 2 * sizeof(unsigned long long)));
+    return NUC3D_OK;
+}
 
 int check_block(nuc3d_case *c, int b)
 {
@@ -256,6 +283,7 @@ struct Timed {
 int refresh(nuc3d_case *c, hipStream_t s)
 {
     if (!c->dirty) return NUC3D_OK;
+    c->hforce = true;
     for (auto &B : c->hb) B.uniform = 0;
     HIPCHK(hipMemcpy(c->db, c->hb.data(), sizeof(nuc3d_block_dev) * c->nb, hipMemcpyHostToDevice));
     // ghosts of J and metrics, fix-up and ghosts of the state, eigenvalues
@@ -304,7 +332,8 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
     const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
     nuc3d_kparams K = c->K;
     K.resid = (mode == 1 && stage == 2) ? resid : nullptr;
-    K.hdecide = (mode == 0 || (stage == 0 && step % c->hybrid_interval == 0)) ? 1 : 0;
+    K.hdecide = (mode == 0 || (stage == 0 && (step % c->hybrid_interval == 0 || c->hforce))) ? 1 : 0;
+    K.hrhs = mode == 0 ? 1 : 0;
     for (int d = 0; d < 3; d++)
         if (c->p.scheme[d] == NUC3D_HYBRID && !K.sensor)
             return fail(NUC3D_ERR_BAD_ARG, "the hybrid scheme needs a sensor (nuc3d_set_sensor)");
@@ -450,7 +479,8 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         // hybrid decisions per face: made at stage 0, reused by stages 1 and 2
         for (int d = 0; d < 3; d++) {
             if (p->scheme[d] != NUC3D_HYBRID) continue;
-            const size_t nf = (size_t)(B.n[d] + 1) * B.n[(d + 1) % 3] * B.n[(d + 2) % 3];
+            // the step's decisions, then those of the last single RHS evaluation
+            const size_t nf = 2 * (size_t)(B.n[d] + 1) * B.n[(d + 1) % 3] * B.n[(d + 2) % 3];
             void *fm = nullptr;
             if (hipMalloc(&fm, nf) != hipSuccess || hipMemset(fm, 0xff, nf) != hipSuccess) {
                 (void)hipFree(fm);
@@ -751,6 +781,7 @@ int nuc3d_upload_q(nuc3d_case *c, int b, const double *q5)
     const long long N = (long long)B.n[0] * B.n[1] * B.n[2];
     for (int q = 0; q < 5; q++)
         if ((rc = put_field(c, b, B.q[c->cur][q], q5 + q * N))) return rc;
+    if ((rc = clear_positivity(c))) return rc;
     c->dirty = true;
     return NUC3D_OK;
 }
@@ -793,7 +824,9 @@ int nuc3d_rhs(nuc3d_case *c, void *stream)
         HIPCHK(hipMemsetAsync(lam, 0, sizeof(unsigned long long) * 3 * c->nb, s));
         HIPCHK(c->L->eigen(c->db, c->dtiles, c->ntiles, c->K, c->cur, lam, s));
     }
-    return launch_rhs(c, 0, 0, c->cur, 0, 0, c->dlam, lam, nullptr, s);
+    if ((rc = launch_rhs(c, 0, 0, c->cur, 0, 0, c->dlam, lam, nullptr, s))) return rc;
+    c->flags_rhs = true;
+    return NUC3D_OK;
 }
 
 int nuc3d_stage(nuc3d_case *c, int stage, double cfl, void *stream)
@@ -814,6 +847,8 @@ int nuc3d_stage(nuc3d_case *c, int stage, double cfl, void *stream)
     if ((rc = launch_rhs(c, 1, stage, bin, bun, bout, lcur, lnext, c->dresid + (long long)c->lampar * c->nwork, s,
                          c->steps_done)))
         return rc;
+    if (stage == 0) c->hforce = false;
+    c->flags_rhs = false;
     c->cur = bout;
     c->next_stage = (stage + 1) % 3;
     if (stage == 2) {
@@ -846,6 +881,12 @@ int nuc3d_step_graph(nuc3d_case *c, int nsteps, double cfl, void *stream)
     int rc = refresh(c, s);
     if (rc) return rc;
     int n = 0;
+    // the captured steps decide on the interval's pattern only: a decision
+    // forced by a new state or sensor is made by an eager step
+    if (c->hforce && nsteps > 0) {
+        if ((rc = nuc3d_steps(c, 1, cfl, stream))) return rc;
+        n = 1;
+    }
     for (; n + 2 <= nsteps; n += 2) {
         const int par = c->lampar;
         if (!c->gexec[par] || c->gcfl[par] != cfl) {
@@ -972,6 +1013,17 @@ int nuc3d_sync(nuc3d_case *c)
 {
     if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
     HIPCHK(hipDeviceSynchronize());
+    return positivity_status(c);
+}
+
+int nuc3d_get_positivity(nuc3d_case *c, long long *counts2)
+{
+    if (!c || !counts2) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    HIPCHK(hipDeviceSynchronize());
+    unsigned long long v[3] = {0, 0, 0};
+    HIPCHK(hipMemcpy(v, c->dcount, sizeof v, hipMemcpyDeviceToHost));
+    counts2[0] = (long long)v[1];
+    counts2[1] = (long long)v[2];
     return NUC3D_OK;
 }
 
@@ -980,7 +1032,7 @@ int nuc3d_get_dt(nuc3d_case *c, double *dt)
     if (!c || !dt) return fail(NUC3D_ERR_BAD_ARG, "null argument");
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(dt, c->ddt, sizeof(double), hipMemcpyDeviceToHost));
-    return NUC3D_OK;
+    return positivity_status(c);
 }
 
 int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt)
@@ -989,7 +1041,7 @@ int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt)
     if (group < 0 || group >= c->K.ngroups) return fail(NUC3D_ERR_BAD_ARG, "group out of range");
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(dt, c->ddt + group, sizeof(double), hipMemcpyDeviceToHost));
-    return NUC3D_OK;
+    return positivity_status(c);
 }
 
 int nuc3d_set_block_group(nuc3d_case *c, int b, int group, double mach, double reynolds)
@@ -1553,6 +1605,7 @@ int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold)
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(c->dsensor, &h, sizeof h, hipMemcpyHostToDevice));
     c->K.sensor = c->dsensor;
+    c->hforce = true;
     return NUC3D_OK;
 }
 
@@ -1595,6 +1648,7 @@ int nuc3d_set_sensor_interval(nuc3d_case *c, int steps)
     if (steps != 1 && steps != 2) return fail(NUC3D_ERR_BAD_ARG, "the decision interval is 1 or 2 steps");
     if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "set the interval between full steps");
     c->hybrid_interval = steps;
+    c->hforce = true;
     for (auto &g : c->gexec)
         if (g) {
             (void)hipGraphExecDestroy(g);
@@ -1613,7 +1667,7 @@ int nuc3d_download_hybrid_flags(nuc3d_case *c, int b, int d, unsigned char *flag
     if (!B.hflag[d]) return fail(NUC3D_ERR_BAD_ARG, "direction d does not use the hybrid scheme");
     const size_t nf = (size_t)(B.n[d] + 1) * B.n[(d + 1) % 3] * B.n[(d + 2) % 3];
     HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemcpy(flags, B.hflag[d], nf, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(flags, B.hflag[d] + (c->flags_rhs ? nf : 0), nf, hipMemcpyDeviceToHost));
     return NUC3D_OK;
 }
 

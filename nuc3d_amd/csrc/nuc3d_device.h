@@ -234,23 +234,46 @@ __device__ __forceinline__ void con2prim(const double *Q, double jac, const Phys
     c = sqrt(fabs(P.gamma * p / rho));
 }
 
-// e<0 fix-up of physicsModel.cpp:322-332 applied to a freshly written state.
-// Returns true if Q[4] was replaced.
-__device__ __forceinline__ bool fixup_energy(double *Q, double jac, const PhysParams &P)
+// Per-thread counts of a stage: e<0 fix-ups, and states that stay unusable
+// after the fix-up (density not positive, pressure negative; NaN counts as
+// both).  flush() adds them to counters[0..2] with one atomic each.
+struct StageCounts {
+    unsigned long long fix = 0;
+    unsigned negd = 0, negp = 0;
+    __device__ __forceinline__ void flush(unsigned long long *counters) const
+    {
+        if (fix) atomicAdd(counters, fix);
+        if (negd) atomicAdd(counters + 1, (unsigned long long)negd);
+        if (negp) atomicAdd(counters + 2, (unsigned long long)negp);
+    }
+};
+
+// e<0 fix-up of physicsModel.cpp:322-332 applied to a freshly written state,
+// then the positivity check the reference makes next: con2prim's pressure
+// (E - rho|u|^2/2)(gamma - 1) must not be negative (physicsModel.cpp:1009-1016,
+// where it exits).  Counts the fix-up and a non-positive density or negative
+// pressure (either one NaN) in cnt.  Returns true if Q[4] was replaced.
+__device__ __forceinline__ bool fixup_energy(double *Q, double jac, const PhysParams &P, StageCounts &cnt)
 {
     double rho = Q[0] * jac;
     double u = Q[1] / rho * jac;
     double v = Q[2] / rho * jac;
     double w = Q[3] / rho * jac;
     double E = Q[4] * jac;
-    double e0 = E - 0.5 * rho * (u * u + v * v + w * w);
+    const double ke = 0.5 * rho * (u * u + v * v + w * w);
+    double e0 = E - ke;
+    bool fixed = false;
     if (e0 < 0.0) {
         double g = P.gamma, m = P.mach;
         E = (pow(rho, g) / ((g * m * m) * (g - 1.0)) + 0.5 * rho * (u * u + v * v + w * w));
         Q[4] = E / jac;
-        return true;
+        e0 = E - ke;
+        cnt.fix++;
+        fixed = true;
     }
-    return false;
+    if (!(rho > 0.0)) cnt.negd++;
+    if (!(e0 >= 0.0)) cnt.negp++;
+    return fixed;
 }
 
 __device__ __forceinline__ double machL(double m) { return fabs(m) < 1.0 ? 0.25 * sq(m + 1.0) + 0.125 * sq(m * m - 1) : 0.50 * (m + fabs(m)); }

@@ -348,7 +348,7 @@ __device__ __forceinline__ double stage_dt(const nuc3d_block_dev *blocks, const 
 __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
                                                const PhysParams &P, int i, int j, int k, long long x, const double *r,
                                                const double *u5, const double *un5, double jac, int stage, int bout,
-                                               double dt, double *lmax, unsigned long long &nfix, double &r2)
+                                               double dt, double *lmax, StageCounts &nfix, double &r2)
 {
     double qn[5];
 #pragma unroll
@@ -367,7 +367,7 @@ __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, co
     if (stage == 2)
 #pragma unroll
         for (int c = 0; c < 5; c++) r2 += (qn[c] - un5[c]) * (qn[c] - un5[c]);
-    if (fixup_energy(qn, jac, P)) nfix++;
+    fixup_energy(qn, jac, P, nfix);
 #pragma unroll
     for (int c = 0; c < 5; c++) B.q[bout][c][x] = qn[c];
     push_q(blocks, B, bout, i, j, k, qn);
@@ -389,7 +389,7 @@ __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, co
 __device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
                                           const PhysParams &P, int i, int j, int k, long long x, const double *r,
                                           int stage, int bin, int bun, int bout, double dt, double *lmax,
-                                          unsigned long long &nfix, double &r2)
+                                          StageCounts &nfix, double &r2)
 {
     double u5[5], un5[5];
 #pragma unroll
@@ -410,7 +410,7 @@ template <int DIR, bool NS, int MODE>
 __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
                                             const nuc3d_kparams &K, const PhysParams &P, int i, int j, int k,
                                             const double *dF, int stage, int bin, int bun, int bout, double dt,
-                                            double *lmax, unsigned long long &nfix, double &r2)
+                                            double *lmax, StageCounts &nfix, double &r2)
 {
     const long long x = cidx(B, i, j, k);
     double r[5];
@@ -435,7 +435,7 @@ __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const
 __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
                                                    const nuc3d_kparams &K, const PhysParams &P, const double *u5,
                                                    int i, int j, int k, const double *dF, int stage, int bun, int bout,
-                                                   double dt, double *lmax, unsigned long long &nfix, double &r2)
+                                                   double dt, double *lmax, StageCounts &nfix, double &r2)
 {
     const long long x = cidx(B, i, j, k);
     double r[5];
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     __syncthreads();
 
     double lmax[3] = {0.0, 0.0, 0.0};
-    unsigned long long nfix = 0;
+    StageCounts nfix;
     double r2 = 0.0;  // residual partial (stage 2)
     const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
     if (line < nlines && f >= 1 && f <= nc) {
@@ -581,7 +581,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
             finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
     }
     if (DIR == 2 && MODE == 1) {
-        if (nfix) atomicAdd(counters, nfix);
+        nfix.flush(counters);
         if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * t.x);
         if (stage == 2 && K.resid) block_sum_store(r2, K.resid + witem);
     }
@@ -654,7 +654,20 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
         PR = U.u.pr;
         ZL = U.zl;
     }
-    __shared__ unsigned short LST[256];
+    // Where decisions are reused, the FP32 roundings of rho and p of the
+    // staged cells, for the positivity check every stage repeats; the list of
+    // WENO5-JS faces takes the same area once that check is made.
+    struct LdsReuse {
+        union {
+            float prp[2][NSL * NL];
+            unsigned short lst[256];
+        } u;
+    };
+    struct LdsList {
+        unsigned short lst[256];
+    };
+    __shared__ std::conditional_t<DECIDE, LdsList, LdsReuse> LR;
+    unsigned short *LST = reinterpret_cast<unsigned short *>(&LR);
     __shared__ int WCNT[4];
     __shared__ int WSTAT[4][3];
 
@@ -688,6 +701,10 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
             S[5 + c][o] = fn[c];
             if constexpr (DECIDE) PR[c][o] = (float)pr[c];
         }
+        if constexpr (!DECIDE) {
+            reinterpret_cast<LdsReuse &>(LR).u.prp[0][o] = (float)pr[0];
+            reinterpret_cast<LdsReuse &>(LR).u.prp[1][o] = (float)pr[4];
+        }
     };
     double u5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // viscous z sweep: the state of the thread's own cell
     if constexpr (VISC) {
@@ -716,7 +733,8 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     const bool closure = (B.ftype[2 * DIR] == -1 && m < w) || (B.ftype[2 * DIR + 1] == -1 && m > n - w);
     const bool decided = active && !closure;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // the face's index in the decision array of direction DIR
+    // the face's index in the decision array of direction DIR (a single RHS
+    // evaluation keeps its decisions in the second half)
     long long fidx = 0;
     {
         int i, j, k;
@@ -724,6 +742,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
         if (DIR == 0) fidx = m + (long long)(B.n[0] + 1) * (j + (long long)B.n[1] * k);
         else if (DIR == 1) fidx = m + (long long)(B.n[1] + 1) * (k + (long long)B.n[2] * i);
         else fidx = m + (long long)(B.n[2] + 1) * (i + (long long)B.n[0] * j);
+        if (K.hrhs) fidx += (long long)(B.n[DIR] + 1) * B.n[(DIR + 1) % 3] * B.n[(DIR + 2) % 3];
     }
     bool posok = false, weno = false;
     if constexpr (DECIDE) {
@@ -769,10 +788,30 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
         __syncthreads();  // the features are dead: the area takes the face fluxes
         weno = decided && (!posok || ZL[threadIdx.x] >= K.sensor->thr);
     } else {
-        // stages 1 and 2 of a step take the decisions of its stage 0
-        const unsigned char fl = decided ? B.hflag[DIR][fidx] : (unsigned char)0;
+        // stages 1 and 2 of a step (and every stage of a step that reuses the
+        // previous step's decisions) take the stored network decisions, but
+        // the positivity fallback is evaluated again on this stage's state: a
+        // central face whose stencil means or central face values of rho or p
+        // are no longer positive takes WENO5-JS and is recorded as a fallback
+        unsigned char fl = decided ? B.hflag[DIR][fidx] : (unsigned char)0;
+        if (decided) {
+            const float(*PP)[NSL * NL] = reinterpret_cast<LdsReuse &>(LR).u.prp;
+            auto getf = [&](int v, int j) { return PP[v == 0 ? 0 : 1][sidx(line, f + 1 + j)]; };
+            auto c8 = [&](int v) {
+                return central8f(PP[v][sidx(line, f)], PP[v][sidx(line, f + 1)], PP[v][sidx(line, f + 2)],
+                                 PP[v][sidx(line, f + 3)], PP[v][sidx(line, f + 4)], PP[v][sidx(line, f + 5)],
+                                 PP[v][sidx(line, f + 6)], PP[v][sidx(line, f + 7)]);
+            };
+            const bool pos = stencil_mean(getf, 0) > 0.0f && stencil_mean(getf, 4) > 0.0f && c8(0) > 0.0f &&
+                             c8(1) > 0.0f;
+            if (fl == 0 && !pos) {
+                fl = 2;
+                B.hflag[DIR][fidx] = 2;
+            }
+        }
         weno = decided && (fl == 1 || fl == 2);
         posok = fl != 2;
+        __syncthreads();  // rho and p are dead: the area takes the list of WENO5-JS faces
     }
     const unsigned long long bal = __ballot(weno);
     // a tile owns faces 1..nc (and face 0 of the first tile): the face shared
@@ -872,7 +911,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     __syncthreads();
 
     double lmax[3] = {0.0, 0.0, 0.0};
-    unsigned long long nfix = 0;
+    StageCounts nfix;
     double r2 = 0.0;
     const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
     if (line < nlines && f >= 1 && f <= nc) {
@@ -887,7 +926,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
             finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
     }
     if (DIR == 2 && MODE == 1) {
-        if (nfix) atomicAdd(counters, nfix);
+        nfix.flush(counters);
         if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * t.x);
         if (stage == 2 && K.resid) block_sum_store(r2, K.resid + witem);
     }
@@ -1055,7 +1094,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
     }
 
     double lmax[3] = {0.0, 0.0, 0.0};
-    unsigned long long nfix = 0;
+    StageCounts nfix;
     double r2 = 0.0;  // residual partial (stage 2)
     const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
     // back substitution (out[n-w] = d'[M-1]; out[w+r] = d'[r] - c'[r] out[w+r+1])
@@ -1100,7 +1139,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
         }
     }
     if (DIR == 2 && MODE == 1) {
-        if (nfix) atomicAdd(counters, nfix);
+        nfix.flush(counters);
         if (stage == 2) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -1380,7 +1419,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
     const int kb = tl.k0, ke = tl.k0 + tl.kn;
     const double dt = MODE == 1 ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
     double lmax[3] = {0.0, 0.0, 0.0};
-    unsigned long long nfix = 0;
+    StageCounts nfix;
     double r2 = 0.0;  // residual partial (stage 2)
 
     double zr[2 * H + 1][4];  // fv_zeta (components 1..4) of the own column, planes p-6..p
@@ -1461,7 +1500,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
         }
     }
     if (MODE == 1) {
-        if (nfix) atomicAdd(counters, nfix);
+        nfix.flush(counters);
         if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * tl.b);
         if (stage == 2 && K.resid) block_sum_store(r2, K.resid + xcd_tile(blockIdx.x, gridDim.x));
     }
@@ -1579,18 +1618,15 @@ __global__ __launch_bounds__(256) void KNAME(k_fixup)(const nuc3d_block_dev *__r
     const PhysParams P = phys_of(K, B);
     const int i = tl.i0 + (threadIdx.x & (T - 1)), j = tl.j0 + threadIdx.x / T;
     if (i >= B.n[0] || j >= B.n[1]) return;
-    unsigned long long nfix = 0;
+    StageCounts nfix;
     for (int k = tl.k0; k < tl.k0 + tl.kn; k++) {
         const long long x = cidx(B, i, j, k);
         double Q[5];
 #pragma unroll
         for (int c = 0; c < 5; c++) Q[c] = B.q[buf][c][x];
-        if (fixup_energy(Q, jac_at(B, x), P)) {
-            B.q[buf][4][x] = Q[4];
-            nfix++;
-        }
+        if (fixup_energy(Q, jac_at(B, x), P, nfix)) B.q[buf][4][x] = Q[4];
     }
-    if (nfix) atomicAdd(counters, nfix);
+    nfix.flush(counters);
 }
 
 // Pull-style ghost fill of the face slabs of every non-physical face:

@@ -117,6 +117,7 @@ def lib():
         "nuc3d_get_dt_group": [vp, ctypes.c_int, dp],
         "nuc3d_autotune": [vp, vp, ip],
         "nuc3d_get_fixups": [vp, ctypes.POINTER(ctypes.c_longlong)],
+        "nuc3d_get_positivity": [vp, llp],
         "nuc3d_timing": [vp, ctypes.c_int],
         "nuc3d_timing_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
         "nuc3d_sensor_create": [ctypes.c_longlong, ctypes.POINTER(vp)],
@@ -340,6 +341,13 @@ class Case:
         v = ctypes.c_longlong()
         _check(lib().nuc3d_get_fixups(self.h, ctypes.byref(v)), "get_fixups")
         return v.value
+
+    def positivity(self):
+        """Cells that failed the positivity check since the state was set:
+        (non-positive density, negative pressure after the e<0 fix-up)."""
+        v = (ctypes.c_longlong * 2)()
+        _check(lib().nuc3d_get_positivity(self.h, v), "get_positivity")
+        return int(v[0]), int(v[1])
 
     def timing(self, enable=True):
         _check(lib().nuc3d_timing(self.h, 1 if enable else 0), "timing")

@@ -533,6 +533,7 @@ struct orc_case {
     double dt_global;
     double res_global;
     long fixups;
+    long negd, negp; /* con2prim inputs with density <= 0 / NaN pressure */
     double *work; /* line scratch */
     double *ext, *lout;
 };
@@ -727,6 +728,11 @@ int orc_get_prim(const orc_case *C, int b, double *w8)
 double orc_get_dt_local(const orc_case *C, int b) { return C->B[b].dt_local; }
 double orc_get_dt_global(const orc_case *C) { return C->dt_global; }
 long orc_get_fixups(const orc_case *C) { return C->fixups; }
+void orc_get_positivity(const orc_case *C, long *counts2)
+{
+    counts2[0] = C->negd;
+    counts2[1] = C->negp;
+}
 int orc_get_max_eigen(const orc_case *C, int b, double *e3)
 {
     for (int d = 0; d < 3; d++) e3[d] = C->B[b].maxeig[d];
@@ -752,6 +758,10 @@ static int con2prim(orc_case *C, oblock *B)
         }
         double tmp = (E - 0.5 * rho * (u * u + v * v + w * w)) * (g - 1.0);
         if (tmp < 0.0) return -10; /* physicsModel.cpp:1009-1016 exits */
+        /* states the reference carries on with but the GPU path reports
+         * (nuc3d_get_positivity): density not positive, pressure NaN */
+        if (!(rho > 0.0)) C->negd++;
+        if (!(tmp >= 0.0)) C->negp++;
         double p = tmp;
         B->W[0][i] = rho; B->W[1][i] = u; B->W[2][i] = v; B->W[3][i] = w; B->W[4][i] = p;
         B->W0[0][i] = g * mach * mach * p / rho;

@@ -67,6 +67,9 @@ double orc_get_dt_local(const orc_case *c, int b);
 double orc_get_dt_global(const orc_case *c);
 int orc_get_max_eigen(const orc_case *c, int b, double *eig3);
 long orc_get_fixups(const orc_case *c);                    /* con2prim e<0 fix-ups */
+/* con2prim inputs with density <= 0 (or NaN) and with NaN pressure after the
+ * fix-up; a negative pressure makes orc_stage return -10 (the reference exits) */
+void orc_get_positivity(const orc_case *c, long *counts2);
 
 /* Decisions of the hybrid reconstruction (ORC_HYBRID, no counterpart in the
  * reference) for direction d of block b, in the layout of the reconstructed

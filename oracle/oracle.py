@@ -133,6 +133,8 @@ def _lib():
     lib.orc_get_residual.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_get_residual.restype = ctypes.c_double
     lib.orc_set_hybrid_mask.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    lib.orc_get_positivity.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+    lib.orc_get_positivity.restype = None
     lib.orc_set_square_mode.argtypes = [ctypes.c_int]
     lib.orc_set_square_mode.restype = None
     lib.orc_get_square_mode.argtypes = []
@@ -262,6 +264,12 @@ class OracleCase:
 
     def fixups(self):
         return lib().orc_get_fixups(self.h)
+
+    def positivity(self):
+        """con2prim inputs so far with density <= 0 (or NaN) and with NaN pressure."""
+        v = (ctypes.c_long * 2)()
+        lib().orc_get_positivity(self.h, v)
+        return int(v[0]), int(v[1])
 
     def rhs(self):
         self._check(lib().orc_rhs(self.h), "rhs")

@@ -361,3 +361,22 @@ def test_square_conventions():
     assert 0 < np.count_nonzero(d) < d.size
     assert d.max() <= 1e-14 * np.abs(outs[O.SQ_LIBM][1]).max()
     assert O.lib().orc_get_square_mode() == O.SQ_LIBM
+
+
+@pytest.mark.parametrize("kind", ["density", "pressure"])
+def test_positivity_counts(kind):
+    """The restatement counts the con2prim inputs the GPU path reports as a
+    status: density <= 0, and NaN pressure after the e<0 fix-up."""
+    from nuc3d_amd import cases
+    b = cases.taylor_green(dims=(12, 10, 8), noise=1e-3, seed=41)[0]
+    if kind == "density":
+        b.q[0, 4, 5, 6] = -b.q[0, 4, 5, 6]
+    else:
+        b.q[4, 4, 5, 6] = np.nan
+    o = O.OracleCase(O.Params(), [b])
+    assert o.positivity() == (0, 0)
+    o.rhs()
+    assert o.positivity() == ((1, 0) if kind == "density" else (0, 1))
+    clean = O.OracleCase(O.Params(), cases.taylor_green(dims=(12, 10, 8), noise=1e-3, seed=41))
+    clean.steps(2, 0.5)
+    assert clean.positivity() == (0, 0)
