@@ -82,9 +82,21 @@ def test_bench_json_line():
     assert rf["achieved"] > 0 and rf["frac"] == pytest.approx(rf["achieved"] / rf["peak"])
     assert rf["traffic"] is None  # no committed counter summary for 32^3
     assert d["cpu_baseline"] is None
-    # value is the faster method's; the hybrid counts only within the accuracy bar
-    # against plain WENO5-JS after the same steps
-    h, p = d["hybrid"], d["plain"]
-    assert 0.0 <= h["l2_over_density_rms"] <= 1e-3
-    assert d["value"] == pytest.approx(max(h["value"], p["value"]))
-    assert d["config"]["value_method"] == ("hybrid" if h["value"] > p["value"] else "plain")
+    # the roofline fraction follows the algorithmic bytes of a stage launch:
+    # a third of the step's bytes per cell-update for every cell
+    cells = d["config"]["cells"]
+    assert rf["alg_bytes_per_launch"] == pytest.approx(rf["step_alg_bytes_per_cell_update"] / 3.0 * cells)
+    assert rf["achieved"] == pytest.approx(rf["alg_bytes_per_launch"] / (rf["avg_launch_ms"] * 1e-3) / 1e9)
+    assert rf["step_frac"] == pytest.approx(rf["step_alg_bytes_per_cell_update"] * d["plain"]["value"] / 1e9
+                                            / rf["peak"])
+    # value is the fastest method's among those within the accuracy bar (the
+    # density fluctuation's rms) against plain WENO5-JS after the same steps
+    p = d["plain"]
+    best = p["value"]
+    for key in ("decisions_every_step", "decisions_every_other_step"):
+        h = d["hybrid"][key]
+        assert h["l2_over_fluctuation_rms"] >= h["l2_over_density_rms"] >= 0.0
+        if h["l2_over_fluctuation_rms"] <= 1e-3:
+            best = max(best, h["value"])
+    assert d["value"] == pytest.approx(best)
+    assert d["config"]["value_method"].startswith("plain" if best == p["value"] else "hybrid")

@@ -31,8 +31,16 @@ def test_positivity_status(kind, code, exact):
     b = injected(kind)
     g = solver.Case(solver.Params(exact=exact), [b])
     assert g.positivity() == (0, 0)
-    g.stage(0, 0.5)
+    # the uploaded state alone (its fix-up and check run when the case is
+    # refreshed, which the uniform-mesh query does): one cell
+    g.uniform_mesh(0)
     with pytest.raises(solver.Nuc3dError, match=r"\(%d\)" % code):
+        g.sync()
+    assert g.positivity() == ((1, 0) if kind == "density" else (0, 1))
+    g.stage(0, 0.5)
+    # a stage spreads a NaN into its neighbours' density as well: density
+    # failures are reported first
+    with pytest.raises(solver.Nuc3dError, match=r"\((%d|%d)\)" % (code, NEG_DENSITY)):
         g.sync()
     counts = g.positivity()
     print("  GPU counts (density, pressure) after one stage:", counts, flush=True)
@@ -45,7 +53,7 @@ def test_positivity_status(kind, code, exact):
     # a healthy state clears the status (between full steps)
     g.stage(1, 0.5)
     g.stage(2, 0.5)
-    with pytest.raises(solver.Nuc3dError, match=r"\(%d\)" % code):
+    with pytest.raises(solver.Nuc3dError, match=r"\((%d|%d)\)" % (code, NEG_DENSITY)):
         g.dt()
     g.set_q(0, cases.taylor_green(dims=(16, 12, 10), noise=1e-3, seed=41)[0].q)
     g.steps(1, 0.5)
