@@ -348,6 +348,19 @@ int nuc3d_get_hybrid_counts(nuc3d_case *c, long long *counts3);
  * reference: a capability query for callers). */
 int nuc3d_has_crweno5(void);
 
+/* Self-test of the arithmetic (no counterpart in the reference).  The exact
+ * kernels divide by a shortened form of the compiler's IEEE FP64 division
+ * (v_rcp_f64 and two Newton steps, quotient with one FMA correction, without
+ * the scaling and special-value steps), used only where every operand is in
+ * the range in which both give the same correctly rounded quotient.  This
+ * compares the two on n hashed operand pairs of that range (x3 quotients:
+ * one reciprocal serves three numerators) on the current device and returns
+ * the number of quotients whose bits differ (0 expected). */
+int nuc3d_selftest_division(long long n, unsigned long long seed, long long *mismatches);
+/* ieee = 1: the case's kernels use the compiler's IEEE division everywhere
+ * instead (results are the same bit for bit; for tests). */
+int nuc3d_set_division(nuc3d_case *c, int ieee);
+
 /* Description of the calling thread's last error. */
 const char *nuc3d_last_error(void);
 

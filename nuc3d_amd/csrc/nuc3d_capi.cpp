@@ -54,6 +54,8 @@ extern "C" hipError_t nuc3d_launch_sensor_adam(float *, float *, float *, const 
 extern "C" hipError_t nuc3d_launch_mesh_metrics(nuc3d_block_dev, const double *, const double *, const double *,
                                                 double *, int *, hipStream_t);
 extern "C" hipError_t nuc3d_launch_uniform_check(nuc3d_block_dev, int *, hipStream_t);
+extern "C" hipError_t nuc3d_launch_selftest_division(long long, unsigned long long, unsigned long long *,
+                                                     hipStream_t);
 
 namespace {
 
@@ -375,6 +377,22 @@ const char *nuc3d_last_error(void) { return g_err.c_str(); }
 
 int nuc3d_has_crweno5(void) { return 1; }
 
+int nuc3d_selftest_division(long long n, unsigned long long seed, long long *mismatches)
+{
+    if (n < 0 || !mismatches) return fail(NUC3D_ERR_BAD_ARG, "bad argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(NUC3D_ERR_NO_DEVICE, "no HIP device");
+    OpBuffers T;
+    unsigned long long *d = reinterpret_cast<unsigned long long *>(T.get(1));
+    if (!d) return fail(NUC3D_ERR_OOM, "device allocation failed");
+    HIPCHK(hipMemset(d, 0, sizeof(unsigned long long)));
+    HIPCHK(nuc3d_launch_selftest_division(n, seed, d, nullptr));
+    unsigned long long v = 0;
+    HIPCHK(hipMemcpy(&v, d, sizeof v, hipMemcpyDeviceToHost));
+    *mismatches = (long long)v;
+    return NUC3D_OK;
+}
+
 int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d_case **out)
 {
     if (!p || !dims || !out) return fail(NUC3D_ERR_BAD_ARG, "null argument");
@@ -421,6 +439,8 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     K.nblocks = nblocks;
     K.sensor = nullptr;
     K.hcount = nullptr;
+    K.ieee_div = 0;
+    K.hrhs = 0;
     K.ngroups = 1;
     const bool ns = p->model == NUC3D_NAVIERSTOKES3D;
     // Q x 3, RHS, J, metrics, dxyz, u v w T (viscous terms and diagnostics),
@@ -1606,6 +1626,19 @@ int nuc3d_set_sensor(nuc3d_case *c, const float *params, float threshold)
     HIPCHK(hipMemcpy(c->dsensor, &h, sizeof h, hipMemcpyHostToDevice));
     c->K.sensor = c->dsensor;
     c->hforce = true;
+    return NUC3D_OK;
+}
+
+int nuc3d_set_division(nuc3d_case *c, int ieee)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    c->K.ieee_div = ieee != 0;
+    for (auto &g : c->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    c->gcfl[0] = c->gcfl[1] = -1.0;
     return NUC3D_OK;
 }
 

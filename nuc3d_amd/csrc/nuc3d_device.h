@@ -43,6 +43,52 @@ enum { BC_INLET = 0, BC_OUTLET = 1, BC_WALL = 2, BC_SYMM = 3 };
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
+// ---- FP64 division in two halves ----
+// The compiler's IEEE division a/b on gfx950 is v_div_scale (of b and of a),
+// v_rcp_f64 and two Newton steps for the reciprocal, the quotient a*y with
+// one FMA residual correction (v_div_fmas), and v_div_fixup for special
+// values.  Where a and b are finite and in the range in which v_div_scale
+// leaves them unscaled and v_div_fixup passes the quotient through (b
+// normal, 1/b normal, a = +0 or |a| >= 2^-969, a/b normal and not near
+// overflow), the same sequence without those steps gives the same correctly
+// rounded quotient -- and the reciprocal half can be shared by divisions with
+// one denominator.  recip_nr is that half, div_nr the rest.  Callers use them
+// only on operands they have shown to be in range (the sweeps check the
+// magnitude of every split flux they stage); nuc3d_selftest_division compares
+// the pair with a/b on the device.
+__device__ __forceinline__ double recip_nr(double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+__device__ __forceinline__ double div_nr(double a, double b, double y)
+{
+    const double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+}
+// a / b, by the reduced sequence (RD) or the compiler's IEEE division
+template <bool RD>
+__device__ __forceinline__ double ddiv(double a, double b)
+{
+    if constexpr (RD) return div_nr(a, b, recip_nr(b));
+    else return a / b;
+}
+// Split-flux values for which every division of the WENO5 weights is in the
+// range of the reduced sequence: 0, or 2^-100 <= |v| <= 2^60.  Then every
+// smoothness indicator is 0 or in [2^-306, 2^127], every weight denominator
+// (eps + beta)^2 in [1e-12, 2^254], every WENO5-Z ratio tau / (beta + eps)
+// +0 or in [2^-485, 2^147], every unnormalised weight in [2^-256, 2^295] and
+// every normalised one in [2^-300, 1].
+__device__ __forceinline__ bool rd_range(double v)
+{
+    const double a = fabs(v);
+    return v == 0.0 || (a >= 0x1p-100 && a <= 0x1p60);
+}
+
 // ---- smoothness indicators on an oriented stencil f0..f4 ----
 // zq: the +inner beta2 of weno5z.cpp:80 ((f2-2f3+f3)^2 in place of (f2-2f3+f4)^2)
 #if NUC3D_EXACT
@@ -79,7 +125,9 @@ __device__ __forceinline__ void cand5(double f0, double f1, double f2, double f3
     q2 = A20 * f2 + A21 * f3 + A22 * f4;
 }
 
-// WENO5-JS value at the face of an oriented 5-point stencil.
+// WENO5-JS value at the face of an oriented 5-point stencil.  RD: the
+// divisions by the reduced sequence (same quotients; operands in range).
+template <bool RD = false>
 __device__ __forceinline__ double weno5js(double f0, double f1, double f2, double f3, double f4)
 {
     NUC3D_WENO_FP
@@ -87,22 +135,33 @@ __device__ __forceinline__ double weno5js(double f0, double f1, double f2, doubl
     smooth5(f0, f1, f2, f3, f4, false, b0, b1, b2);
     cand5(f0, f1, f2, f3, f4, q0, q1, q2);
 #if NUC3D_EXACT
-    double a0 = C0 / sq(EPS + b0);
-    double a1 = C1 / sq(EPS + b1);
-    double a2 = C2 / sq(EPS + b2);
+    double a0 = ddiv<RD>(C0, sq(EPS + b0));
+    double a1 = ddiv<RD>(C1, sq(EPS + b1));
+    double a2 = ddiv<RD>(C2, sq(EPS + b2));
     double as = a0 + a1 + a2;
-    double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
+    double o0, o1, o2;
+    if constexpr (RD) {
+        const double y = recip_nr(as);   // one reciprocal for the three weights
+        o0 = div_nr(a0, as, y);
+        o1 = div_nr(a1, as, y);
+        o2 = div_nr(a2, as, y);
+    } else {
+        o0 = a0 / as;
+        o1 = a1 / as;
+        o2 = a2 / as;
+    }
     return o0 * q0 + o1 * q1 + o2 * q2;
 #else
     // alpha_k = c_k/d_k with d_k=(eps+beta_k)^2; multiplying numerator and
     // denominator of sum(alpha q)/sum(alpha) by d0 d1 d2 leaves one division.
     double d0 = sq(EPS + b0), d1 = sq(EPS + b1), d2 = sq(EPS + b2);
     double w0 = C0 * (d1 * d2), w1 = C1 * (d0 * d2), w2 = C2 * (d0 * d1);
-    return (w0 * q0 + w1 * q1 + w2 * q2) / (w0 + w1 + w2);
+    return ddiv<RD>(w0 * q0 + w1 * q1 + w2 * q2, w0 + w1 + w2);
 #endif
 }
 
 // WENO5-Z value (weno5z.cpp:96-109).
+template <bool RD = false>
 __device__ __forceinline__ double weno5z(double f0, double f1, double f2, double f3, double f4, bool zq)
 {
     NUC3D_WENO_FP
@@ -111,16 +170,26 @@ __device__ __forceinline__ double weno5z(double f0, double f1, double f2, double
     cand5(f0, f1, f2, f3, f4, q0, q1, q2);
     double tau = fabs(b2 - b0);
 #if NUC3D_EXACT
-    double a0 = C0 * (1.0 + sq(tau / (b0 + EPS)));
-    double a1 = C1 * (1.0 + sq(tau / (b1 + EPS)));
-    double a2 = C2 * (1.0 + sq(tau / (b2 + EPS)));
+    double a0 = C0 * (1.0 + sq(ddiv<RD>(tau, b0 + EPS)));
+    double a1 = C1 * (1.0 + sq(ddiv<RD>(tau, b1 + EPS)));
+    double a2 = C2 * (1.0 + sq(ddiv<RD>(tau, b2 + EPS)));
     double as = a0 + a1 + a2;
-    double o0 = a0 / as, o1 = a1 / as, o2 = a2 / as;
+    double o0, o1, o2;
+    if constexpr (RD) {
+        const double y = recip_nr(as);
+        o0 = div_nr(a0, as, y);
+        o1 = div_nr(a1, as, y);
+        o2 = div_nr(a2, as, y);
+    } else {
+        o0 = a0 / as;
+        o1 = a1 / as;
+        o2 = a2 / as;
+    }
     return o0 * q0 + o1 * q1 + o2 * q2;
 #else
     double d0 = sq(b0 + EPS), d1 = sq(b1 + EPS), d2 = sq(b2 + EPS), t2 = tau * tau;
     double w0 = C0 * (d0 + t2) * (d1 * d2), w1 = C1 * (d1 + t2) * (d0 * d2), w2 = C2 * (d2 + t2) * (d0 * d1);
-    return (w0 * q0 + w1 * q1 + w2 * q2) / (w0 + w1 + w2);
+    return ddiv<RD>(w0 * q0 + w1 * q1 + w2 * q2, w0 + w1 + w2);
 #endif
 }
 
@@ -152,16 +221,16 @@ __device__ __forceinline__ double central8(const double *s)
 // Interior face value for the + (upwind>0) or - stencil.  s[0..5] holds the
 // split flux at cells m-3..m+2 of face m (between cells m-1 and m); the
 // central scheme also reads s[-1] and s[6] (cells m-4 and m+3).
-template <int SCHEME>
+template <int SCHEME, bool RD = false>
 __device__ __forceinline__ double face_interior(const double *s, bool plus, bool zq)
 {
     if (SCHEME == SCH_CENTRAL8) return central8(s);
     if (plus) {
-        if (SCHEME == SCH_WENO5Z) return weno5z(s[0], s[1], s[2], s[3], s[4], zq);
-        return weno5js(s[0], s[1], s[2], s[3], s[4]);
+        if (SCHEME == SCH_WENO5Z) return weno5z<RD>(s[0], s[1], s[2], s[3], s[4], zq);
+        return weno5js<RD>(s[0], s[1], s[2], s[3], s[4]);
     }
-    if (SCHEME == SCH_WENO5Z) return weno5z(s[5], s[4], s[3], s[2], s[1], false);
-    return weno5js(s[5], s[4], s[3], s[2], s[1]);
+    if (SCHEME == SCH_WENO5Z) return weno5z<RD>(s[5], s[4], s[3], s[2], s[1], false);
+    return weno5js<RD>(s[5], s[4], s[3], s[2], s[1]);
 }
 
 // One inner row of the CRWENO5 system (crweno5.cpp:130-160 for +,

@@ -188,7 +188,7 @@ __device__ __forceinline__ void split_any(const nuc3d_block_dev &B, double *cons
 
 // Face value of direction d at face m from the six-slot stencils sp (F+) and
 // sn (F-), with the physical closures and the WENO5-Z quirk of weno5z.cpp:80.
-template <int SCHEME>
+template <int SCHEME, bool RD = false>
 __device__ __forceinline__ double face_value(const nuc3d_block_dev &B, const nuc3d_kparams &K, int d, int m,
                                              const double *sp, const double *sn)
 {
@@ -196,7 +196,7 @@ __device__ __forceinline__ double face_value(const nuc3d_block_dev &B, const nuc
     if (B.ftype[2 * d] == -1 && m < w) return face_exterior(sp, true, m) + face_exterior(sn, false, m);
     if (B.ftype[2 * d + 1] == -1 && m > n - w) return face_exterior(sp, true, n - m) + face_exterior(sn, false, n - m);
     const bool zq = SCHEME == SCH_WENO5Z && K.zquirk && m >= w && m <= n - w;
-    return face_interior<SCHEME>(sp, true, zq) + face_interior<SCHEME>(sn, false, false);
+    return face_interior<SCHEME, RD>(sp, true, zq) + face_interior<SCHEME, RD>(sn, false, false);
 }
 
 // Derivative of field f along d at cell t of a line (cd6 / exterior closure),
@@ -510,6 +510,9 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     };
 
     const int nslots = nc + 2 * H;
+    // whether every staged split flux is in the range of the reduced division
+    // (then the workgroup's WENO weights use it; same quotients either way)
+    bool rd_ok = true;
     auto stage_slot = [&](int line, int sl, double *qout) {
         int i, j, k;
         cell(line, a0 - H + sl, i, j, k);
@@ -520,6 +523,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
         for (int c = 0; c < 5; c++) {
             S[c][o] = fp[c];
             S[5 + c][o] = fn[c];
+            rd_ok = rd_ok && rd_range(fp[c]) && rd_range(fn[c]);
         }
     };
     double u5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // viscous z sweep: the state of the thread's own cell
@@ -539,7 +543,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
             if (line < nlines && sl < nslots) stage_slot(line, sl, nullptr);
         }
     }
-    __syncthreads();
+    const bool rd = __syncthreads_and(rd_ok) != 0 && !K.ieee_div;
 
     const int line = DIR == 0 ? threadIdx.x / NF : threadIdx.x % NL;
     const int f = DIR == 0 ? threadIdx.x % NF : threadIdx.x / NL;
@@ -556,7 +560,8 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
                 sp[q] = S[c][sidx(line, f + q)];
                 sn[q] = S[5 + c][sidx(line, f + q)];
             }
-            hv[c] = face_value<SCHEME>(B, K, DIR, m, sp + (H - 3), sn + (H - 3));
+            hv[c] = rd ? face_value<SCHEME, true>(B, K, DIR, m, sp + (H - 3), sn + (H - 3))
+                       : face_value<SCHEME, false>(B, K, DIR, m, sp + (H - 3), sn + (H - 3));
         }
     }
     __syncthreads();  // every face has read the split fluxes

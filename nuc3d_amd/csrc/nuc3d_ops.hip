@@ -311,3 +311,69 @@ extern "C" hipError_t nuc3d_launch_mesh_metrics(nuc3d_block_dev B, const double 
     hipLaunchKernelGGL(k_mesh_metrics, dim3(grid_of(N, 256)), dim3(256), 0, st, B, xn, yn, zn, centre, bad);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Self-test of the reduced FP64 division (recip_nr / div_nr of
+// nuc3d_device.h) against the compiler's IEEE a / b on this device: n
+// operand pairs from a hash of (seed, index), b = +-m * 2^eb and a = +0 (one
+// in 16) or +-m * 2^ea with eb, ea in [-600, 600] and |ea - eb| < 700, the
+// mantissa m random or an edge pattern (1, 2 - ulp, 1 + ulp, 1.5); each pair
+// also divides a - ulp and 2a - ulp by the same reciprocal (the shared form).
+// Counts quotients whose bits differ.
+namespace {
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
+__device__ __forceinline__ double make_operand(unsigned long long h, int e)
+{
+    unsigned long long mant;
+    switch ((h >> 60) & 7) {
+    case 0: mant = 0; break;                       // 1.0
+    case 1: mant = 0xfffffffffffffULL; break;      // 2 - ulp
+    case 2: mant = 1; break;                       // 1 + ulp
+    case 3: mant = 0x8000000000000ULL; break;      // 1.5
+    default: mant = h & 0xfffffffffffffULL; break;
+    }
+    const unsigned long long sign = (h >> 59) & 1;
+    return __longlong_as_double((long long)((sign << 63) | ((unsigned long long)(e + 1023) << 52) | mant));
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_selftest_division(long long n, unsigned long long seed,
+                                                            unsigned long long *mismatch)
+{
+    unsigned long long bad = 0;
+    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
+        const unsigned long long h1 = mix64(seed ^ (unsigned long long)(2 * t + 1) * 0x9e3779b97f4a7c15ULL);
+        const unsigned long long h2 = mix64(h1 + 0x632be59bd9b4e019ULL);
+        const int eb = (int)(h1 % 1201ULL) - 600;
+        int ea = (int)(h2 % 1201ULL) - 600;
+        if (ea - eb >= 700) ea = eb + 699;
+        if (eb - ea >= 700) ea = eb - 699;
+        const double b = make_operand(h1, eb);
+        const double a = ((h2 >> 40) & 15) == 0 ? 0.0 : make_operand(h2, ea);
+        const double a2 = a == 0.0 ? 0.0 : __longlong_as_double(__double_as_longlong(a) - 1);
+        const double a3 = a2 + a2;
+        const double y = nuc3d_dev::recip_nr(b);
+        const double q[3] = {nuc3d_dev::div_nr(a, b, y), nuc3d_dev::div_nr(a2, b, y), nuc3d_dev::div_nr(a3, b, y)};
+        const double r[3] = {a / b, a2 / b, a3 / b};
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (__double_as_longlong(q[k]) != __double_as_longlong(r[k])) bad++;
+    }
+    if (bad) atomicAdd(mismatch, bad);
+}
+
+extern "C" hipError_t nuc3d_launch_selftest_division(long long n, unsigned long long seed,
+                                                     unsigned long long *mismatch, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_selftest_division, dim3(4096), dim3(256), 0, st, n, seed, mismatch);
+    return hipGetLastError();
+}

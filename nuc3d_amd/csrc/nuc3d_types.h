@@ -92,6 +92,7 @@ struct nuc3d_kparams {
     int hdecide;    // hybrid sweeps of this launch make decisions (else reuse the stored ones)
     int hrhs;       // a single RHS evaluation: its decisions go to the second half of each decision
                     // array, so that the step's own decisions (first half) stay untouched
+    int ieee_div;   // 1: the compiler's IEEE division everywhere (no reduced division)
     double *resid;  // stage 2: per-work-item partial sums of (Q - Q_n)^2, or null
     const nuc3d_sensor_dev *sensor;  // hybrid reconstruction: the network, or null
     // hybrid decision counts (central, WENO5-JS by the network, by the

@@ -118,6 +118,8 @@ def lib():
         "nuc3d_autotune": [vp, vp, ip],
         "nuc3d_get_fixups": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "nuc3d_get_positivity": [vp, llp],
+        "nuc3d_set_division": [vp, ctypes.c_int],
+        "nuc3d_selftest_division": [ctypes.c_longlong, ctypes.c_ulonglong, llp],
         "nuc3d_timing": [vp, ctypes.c_int],
         "nuc3d_timing_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
         "nuc3d_sensor_create": [ctypes.c_longlong, ctypes.POINTER(vp)],
@@ -155,6 +157,14 @@ def lib():
     L.nuc3d_last_error.restype = ctypes.c_char_p
     _LIB = L
     return L
+
+
+def selftest_division(n, seed=0):
+    """Quotients of the reduced FP64 division that differ from a / b over n
+    hashed operand pairs (x3) on the device."""
+    v = ctypes.c_longlong()
+    _check(lib().nuc3d_selftest_division(int(n), int(seed), ctypes.byref(v)), "selftest_division")
+    return v.value
 
 
 def crweno5_available():
@@ -341,6 +351,10 @@ class Case:
         v = ctypes.c_longlong()
         _check(lib().nuc3d_get_fixups(self.h, ctypes.byref(v)), "get_fixups")
         return v.value
+
+    def set_division(self, ieee):
+        """ieee: the compiler's IEEE division everywhere (same results)."""
+        _check(lib().nuc3d_set_division(self.h, 1 if ieee else 0), "set_division")
 
     def positivity(self):
         """Cells that failed the positivity check since the state was set:
