@@ -155,6 +155,18 @@ int nuc3d_step_graph(nuc3d_case *c, int nsteps, double cfl, void *stream);
  * added in a fixed order, so they are reproducible; their rounding differs
  * from the reference's cell-by-cell sum by a few ulps per cell added. */
 int nuc3d_diagnostics(nuc3d_case *c, double *enstrophy, double *kinetic, int b, double *fields5, void *stream);
+/* nuc3d_diagnostics without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The enstrophy and
+ * kinetic-energy sums only (no fields), equal bit for bit to those of
+ * nuc3d_diagnostics; the per-workgroup partials and the sums go to a fixed
+ * device array of the library (16 MiB per GPU, held until the process ends)
+ * instead of buffers allocated per call.  Limit: 2^20 workgroups, one per
+ * 256 cells per column-tile plane (about 268 M cells at the default column
+ * tiles); larger cases get NUC3D_ERR_UNSUPPORTED.  The case's device must be
+ * current.  Calls on one GPU share the array: calls that overlap on one GPU
+ * (other threads, other streams) race and must be serialised by the caller. */
+int nuc3d_diagnostics_noalloc(nuc3d_case *c, double *enstrophy, double *kinetic, void *stream);
 
 /* The host time loop of singleBlock::solvePDE (singleBlock.cpp:51-71) with
  * the IOController cadence (IOcontroller.cpp:181-212):

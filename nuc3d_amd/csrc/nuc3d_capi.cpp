@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,7 @@ extern "C" hipError_t nuc3d_launch_sensor_adam(float *, float *, float *, const 
 extern "C" hipError_t nuc3d_launch_mesh_metrics(nuc3d_block_dev, const double *, const double *, const double *,
                                                 double *, int *, hipStream_t);
 extern "C" hipError_t nuc3d_launch_uniform_check(nuc3d_block_dev, int *, hipStream_t);
+extern "C" hipError_t nuc3d_launch_sum_pairs(const double *, long long, double *, hipStream_t);
 extern "C" hipError_t nuc3d_launch_selftest_division(long long, unsigned long long, unsigned long long *,
                                                      hipStream_t);
 
@@ -982,6 +984,72 @@ int nuc3d_diagnostics(nuc3d_case *c, double *enstrophy, double *kinetic, int b, 
     }
     *enstrophy = e;
     *kinetic = k;
+    return NUC3D_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Scratch of the entry point that allocates nothing (nuc3d_diagnostics_noalloc):
+// a file-scope device array of NUC3D_NOALLOC_WORK partial pairs plus the two
+// sums.  Every GPU the library runs on holds its own copy until the process
+// ends; its device address is looked up once per device (hipGetSymbolAddress
+// with that device current) and only that address is handed to kernels and
+// copies -- the array's name in host code is not a device address.
+#define NUC3D_NOALLOC_WORK (1 << 20)
+static __device__ __align__(256) double scratch[2 * NUC3D_NOALLOC_WORK + 2];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's copy of `scratch` (looked up on the first call there)
+int scratch_of_device(int dev, double **out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fail(NUC3D_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hipfail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return NUC3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nuc3d_diagnostics_noalloc(nuc3d_case *c, double *enstrophy, double *kinetic, void *stream)
+{
+    if (!c || !enstrophy || !kinetic) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != c->device) return fail(NUC3D_ERR_BAD_ARG, "the case's device is not the current device");
+    const long long nwg = (long long)c->ntiles * c->kmax;
+    if (nwg > NUC3D_NOALLOC_WORK)
+        return fail(NUC3D_ERR_UNSUPPORTED, "case too large for the fixed scratch of nuc3d_diagnostics_noalloc (" +
+                                               std::to_string(nwg) + " work items > " +
+                                               std::to_string(NUC3D_NOALLOC_WORK) + "); use nuc3d_diagnostics");
+    double *sp = nullptr;
+    int rc = scratch_of_device(dev, &sp);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((rc = refresh(c, s))) return rc;
+    // every partial pair is written by its workgroup before the sum reads it
+    HIPCHK(c->L->prim(c->db, c->dtiles, c->ntiles, c->kmax, c->K, c->cur, s));
+    HIPCHK(c->L->diag(c->db, c->dtiles, c->ntiles, c->kmax, c->K, c->cur, -1, nullptr, sp, s));
+    double *sums = sp + 2 * (long long)NUC3D_NOALLOC_WORK;
+    HIPCHK(nuc3d_launch_sum_pairs(sp, nwg, sums, s));
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, sums, sizeof h, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *enstrophy = h[0];
+    *kinetic = h[1];
     return NUC3D_OK;
 }
 

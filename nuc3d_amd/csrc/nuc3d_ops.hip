@@ -377,3 +377,25 @@ extern "C" hipError_t nuc3d_launch_selftest_division(long long n, unsigned long 
     hipLaunchKernelGGL(k_selftest_division, dim3(4096), dim3(256), 0, st, n, seed, mismatch);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Sums of the diagnostics' per-workgroup partials (pairs: enstrophy, kinetic
+// energy) in index order by one thread: the order, and so the rounding, of
+// the host loop of nuc3d_diagnostics.  out2 receives the two sums.
+__global__ void k_sum_pairs_ordered(const double *__restrict__ part, long long n, double *__restrict__ out2)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double e = 0.0, k = 0.0;
+    for (long long w = 0; w < n; w++) {
+        e += part[2 * w];
+        k += part[2 * w + 1];
+    }
+    out2[0] = e;
+    out2[1] = k;
+}
+
+extern "C" hipError_t nuc3d_launch_sum_pairs(const double *part, long long n, double *out2, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sum_pairs_ordered, dim3(1), dim3(64), 0, st, part, n, out2);
+    return hipGetLastError();
+}

@@ -119,6 +119,7 @@ def lib():
         "nuc3d_get_fixups": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "nuc3d_get_positivity": [vp, llp],
         "nuc3d_set_division": [vp, ctypes.c_int],
+        "nuc3d_diagnostics_noalloc": [vp, dp, dp, vp],
         "nuc3d_selftest_division": [ctypes.c_longlong, ctypes.c_ulonglong, llp],
         "nuc3d_timing": [vp, ctypes.c_int],
         "nuc3d_timing_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
@@ -283,6 +284,14 @@ class Case:
         _check(lib().nuc3d_diagnostics(self.h, ctypes.byref(e), ctypes.byref(k), b, _dp(f), self.stream),
                "diagnostics")
         return e.value, k.value, f
+
+    def diagnostics_noalloc(self):
+        """(enstrophy, kinetic) sums through the entry point that allocates
+        nothing (fixed device scratch of the library)."""
+        e, k = ctypes.c_double(), ctypes.c_double()
+        _check(lib().nuc3d_diagnostics_noalloc(self.h, ctypes.byref(e), ctypes.byref(k), self.stream),
+               "diagnostics_noalloc")
+        return e.value, k.value
 
     def save_q(self, b, path, time):
         """Restart file of block b in the reference's format (block.cpp:754-778)."""
