@@ -124,6 +124,21 @@ int nuc3d_download_rhs(nuc3d_case *c, int b, double *rhs5);
 int nuc3d_save_q(nuc3d_case *c, int b, const char *path, double time);
 int nuc3d_load_q(nuc3d_case *c, int b, const char *path, double *time);
 
+/* Tecplot ASCII files of the reference (block::outputQ_tecplot,
+ * block.cpp:692-752, named flowData/NUC3d_step_<step>_id_<rank>.dat; and
+ * block::outputGEO_tecplot, block.cpp:857-922, flowData/GEO__id_<rank>.dat):
+ * "title=NUC3d", the variables x, y, z, Val_0..Val_{nv-1}, one zone of
+ * (nx+1)(ny+1)(nz+1) nodes in BLOCK packing with the values cell-centred,
+ * every value on its own line as %.12g (the reference's setprecision(12)).
+ * xn, yn, zn are block b's node coordinates ((nx+1)(ny+1)(nz+1) each,
+ * i-fastest: the mesh file's, block.cpp:42-98).  nuc3d_write_tecplot writes
+ * the flow (nv = 8: rho, u, v, w, p, T, e, c of physicsModel::getPrim on the
+ * current state, between full steps); nuc3d_write_tecplot_geo the nine
+ * metric terms xi_x .. zeta_z (nv = 9). */
+int nuc3d_write_tecplot(nuc3d_case *c, int b, const char *path, const double *xn, const double *yn, const double *zn);
+int nuc3d_write_tecplot_geo(nuc3d_case *c, int b, const char *path, const double *xn, const double *yn,
+                            const double *zn);
+
 /* One right-hand-side evaluation of every block on the current state
  * (EulerData3D::solve / NaiverStokesData3d::solve, including the halo
  * exchange and boundary conditions). */

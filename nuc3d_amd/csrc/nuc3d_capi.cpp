@@ -416,8 +416,12 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
             return fail(NUC3D_ERR_UNSUPPORTED, wneed == 3 ? "ghost width must be 3 for WENO5 schemes"
                                                           : "ghost width must be 4 for the central and hybrid schemes");
         if (wneed > G) G = wneed;   // block.cpp:506-507: the buffer is the widest scheme's
-        if (p->diff[d] != NUC3D_CD6)
-            return fail(NUC3D_ERR_UNSUPPORTED, "cd8 needs ghost width 4 (centraldifference8th.cpp:34,49)");
+        if (p->diff[d] != NUC3D_CD6 && p->diff[d] != NUC3D_CD8) return fail(NUC3D_ERR_BAD_ARG, "diff scheme");
+        // cd8 reads four cells each side: a width-4 direction (the central and
+        // hybrid schemes); the reference reads out of bounds with 3
+        if (p->diff[d] == NUC3D_CD8 && p->width[d] != 4)
+            return fail(NUC3D_ERR_UNSUPPORTED, "cd8 needs ghost width 4 (centraldifference8th.cpp:34,49): use it "
+                                               "with the central or hybrid reconstruction");
     }
     if (p->model != NUC3D_EULER3D && p->model != NUC3D_NAVIERSTOKES3D) return fail(NUC3D_ERR_BAD_ARG, "model");
     if (p->riemann < 0 || p->riemann > 2) return fail(NUC3D_ERR_BAD_ARG, "riemann");
@@ -437,6 +441,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     K.cfl = 0.5;
     K.riemann = p->riemann;
     for (int d = 0; d < 3; d++) K.width[d] = p->width[d];
+    for (int d = 0; d < 3; d++) K.diff[d] = p->diff[d] == NUC3D_CD8 ? 1 : 0;
     K.zquirk = p->zquirk;
     K.nblocks = nblocks;
     K.sensor = nullptr;
@@ -749,6 +754,101 @@ int nuc3d_download_metrics(nuc3d_case *c, int b, double *jac, double *met, doubl
         if (dxyz && (rc = get_field(c, b, dxyz + q * N, B.dxyz[q]))) return rc;
     }
     return NUC3D_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Tecplot ASCII file of block::outputQ_tecplot / outputGEO_tecplot
+// (block.cpp:692-752, 857-922): the header, the node coordinates x, y, z
+// ((nx+1)(ny+1)(nz+1) each) and nv cell-centred fields (nx ny nz each), one
+// value per line as block::writeField prints them (std::setprecision(12),
+// block.cpp:102-117, i.e. %.12g), i fastest.
+int write_tecplot_file(const char *path, const nuc3d_block_dev &B, const double *xn, const double *yn,
+                       const double *zn, const std::vector<const double *> &fields)
+{
+    FILE *fp = std::fopen(path, "w");
+    if (!fp) return fail(NUC3D_ERR_IO, std::string("cannot open ") + path + " for writing");
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2], nv = (int)fields.size();
+    std::fprintf(fp, "title=NUC3d\nvariables=x,y,z");
+    for (int i = 0; i < nv; i++) std::fprintf(fp, ",Val_%d", i);
+    std::fprintf(fp, "\n Zone I = %d, J= %d, K=%d\n DATAPACKING=BLOCK, VARLOCATION=([%d-%d]=CELLCENTERED)\n", nx + 1,
+                 ny + 1, nz + 1, 4, 3 + nv);
+    const size_t NN = (size_t)(nx + 1) * (ny + 1) * (nz + 1), N = (size_t)nx * ny * nz;
+    for (const double *a : {xn, yn, zn})
+        for (size_t e = 0; e < NN; e++) std::fprintf(fp, "%.12g\n", a[e]);
+    for (const double *a : fields)
+        for (size_t e = 0; e < N; e++) std::fprintf(fp, "%.12g\n", a[e]);
+    const bool ok = !std::ferror(fp);
+    if (std::fclose(fp) != 0 || !ok) return fail(NUC3D_ERR_IO, std::string("writing ") + path + " failed");
+    return NUC3D_OK;
+}
+
+// physicsModel::getPrim on the host (con2prim with the e<0 fix-up and
+// EoSIdealGasFWD, physicsModel.cpp:313-353, 981-1054), in the reference's
+// operation order: rho, u, v, w, p, then T, e, c.
+void host_prim(const double *q5, const double *jac, size_t N, double gamma, double mach, std::vector<double> &w8)
+{
+#pragma clang fp contract(off)
+    w8.assign(8 * N, 0.0);
+    for (size_t x = 0; x < N; x++) {
+        const double J = jac[x];
+        const double rho = q5[x] * J;
+        const double u = q5[N + x] / rho * J;
+        const double v = q5[2 * N + x] / rho * J;
+        const double w = q5[3 * N + x] / rho * J;
+        double E = q5[4 * N + x] * J;
+        const double e0 = E - 0.5 * rho * (u * u + v * v + w * w);
+        if (e0 < 0.0) E = (std::pow(rho, gamma) / ((gamma * mach * mach) * (gamma - 1.0)) + 0.5 * rho * (u * u + v * v + w * w));
+        const double p = (E - 0.5 * rho * (u * u + v * v + w * w)) * (gamma - 1.0);
+        w8[x] = rho;
+        w8[N + x] = u;
+        w8[2 * N + x] = v;
+        w8[3 * N + x] = w;
+        w8[4 * N + x] = p;
+        w8[5 * N + x] = gamma * mach * mach * p / rho;
+        w8[6 * N + x] = p / rho / (gamma - 1.0);
+        w8[7 * N + x] = std::sqrt(std::fabs(gamma * p / rho));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int nuc3d_write_tecplot(nuc3d_case *c, int b, const char *path, const double *xn, const double *yn, const double *zn)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!path || !xn || !yn || !zn) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "flow files are written between full steps");
+    const nuc3d_block_dev &B = c->hb[b];
+    const size_t N = (size_t)B.n[0] * B.n[1] * B.n[2];
+    std::vector<double> q(5 * N), jac(N), w8;
+    if ((rc = nuc3d_download_q(c, b, q.data()))) return rc;
+    if ((rc = get_field(c, b, jac.data(), B.jac))) return rc;
+    host_prim(q.data(), jac.data(), N, c->p.gamma, B.mach, w8);
+    std::vector<const double *> f;
+    for (int v = 0; v < 8; v++) f.push_back(w8.data() + v * N);
+    return write_tecplot_file(path, B, xn, yn, zn, f);
+}
+
+int nuc3d_write_tecplot_geo(nuc3d_case *c, int b, const char *path, const double *xn, const double *yn,
+                            const double *zn)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!path || !xn || !yn || !zn) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    const nuc3d_block_dev &B = c->hb[b];
+    const size_t N = (size_t)B.n[0] * B.n[1] * B.n[2];
+    std::vector<double> met(9 * N);
+    HIPCHK(hipDeviceSynchronize());
+    for (int m = 0; m < 9; m++)
+        if ((rc = get_field(c, b, met.data() + m * N, B.met[m]))) return rc;
+    std::vector<const double *> f;
+    for (int m = 0; m < 9; m++) f.push_back(met.data() + m * N);
+    return write_tecplot_file(path, B, xn, yn, zn, f);
 }
 
 int nuc3d_save_q(nuc3d_case *c, int b, const char *path, double time)

@@ -199,8 +199,10 @@ __device__ __forceinline__ double face_value(const nuc3d_block_dev &B, const nuc
     return face_interior<SCHEME, RD>(sp, true, zq) + face_interior<SCHEME, RD>(sn, false, false);
 }
 
-// Derivative of field f along d at cell t of a line (cd6 / exterior closure),
-// NaiverStokes3d.cpp:122-168 with fieldOperator.cpp:505-542.
+// Derivative of field f along d at cell t of a line (cd6 or cd8 of the
+// direction, or the exterior closure), NaiverStokes3d.cpp:122-168 with
+// fieldOperator.cpp:505-542; cd8 (centraldifference8th.cpp:49-52) reads four
+// cells each side, which the width-4 cases hold.
 __device__ __forceinline__ double line_diff(const nuc3d_block_dev &B, const nuc3d_kparams &K, const double *f,
                                             int d, int t)
 {
@@ -208,7 +210,7 @@ __device__ __forceinline__ double line_diff(const nuc3d_block_dev &B, const nuc3
     const int n = B.n[d], w = K.width[d];
     if (B.ftype[2 * d] == -1 && t < w) return diff_exterior(f, st, t, true);
     if (B.ftype[2 * d + 1] == -1 && t >= n - w) return diff_exterior(f, st, n - 1 - t, false);
-    return cd6(f, st);
+    return K.diff[d] ? cd8(f, st) : cd6(f, st);
 }
 
 // Same derivative on a line held in LDS (element stride st along the line).
@@ -218,7 +220,7 @@ __device__ __forceinline__ double line_diff_lds(const nuc3d_block_dev &B, const 
     const int n = B.n[d], w = K.width[d];
     if (B.ftype[2 * d] == -1 && t < w) return diff_exterior(f, st, t, true);
     if (B.ftype[2 * d + 1] == -1 && t >= n - w) return diff_exterior(f, st, n - 1 - t, false);
-    return cd6(f, st);
+    return K.diff[d] ? cd8(f, st) : cd6(f, st);
 }
 
 // Ghost cell of the neighbour that receives cell (i,j,k) through face f: the
@@ -1247,14 +1249,16 @@ __global__ __launch_bounds__(256) void KNAME(k_prim)(const nuc3d_block_dev *__re
     }
 }
 
-// Halo cell h of a TX x TY column tile (0 <= h < 6 (TX + TY)): h < 6 TY are
-// the strips of 3 cells left and right of each row (x halo), then the rows of
-// 3 cells below and above the tile (y halo, consecutive h along i); no corners
-// (the derivatives are not mixed).  Tile-local (li, lj); false for rows or
-// columns beyond a partial tile.
+// Halo cell h of a TX x TY column tile (0 <= h < 2H (TX + TY), H = 3 for
+// cd6, 4 for cd8): h < 2H TY are the strips of H cells left and right of each
+// row (x halo), then the rows of H cells below and above the tile (y halo,
+// consecutive h along i); no corners (the derivatives are not mixed).
+// Tile-local (li, lj); false for rows or columns beyond a partial tile.
+template <int H>
 __device__ __forceinline__ bool halo_cell(int h, int tw, int th, int &li, int &lj)
 {
-    constexpr int H = NUC3D_HALO, TX = NUC3D_TILE_X, TY = NUC3D_TILE_Y;
+    constexpr int TX = NUC3D_TILE_X, TY = NUC3D_TILE_Y;
+    static_assert(2 * H * (TX + TY) <= 256, "one halo cell per thread");
     if (h < TY * 2 * H) {
         const int r = h / (2 * H), s = h % (2 * H);
         if (r >= th) return false;
@@ -1312,10 +1316,12 @@ __device__ __forceinline__ void visc_flux_cell(const nuc3d_kparams &K, const Phy
     }
 }
 
+// HD: the derivatives' half-width (3 for cd6, 4 where a direction uses cd8)
+template <int HD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_FLUX_WAVES))) void KNAME(k_visc_flux)(
     const nuc3d_block_dev *__restrict__ blocks, const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int bin)
 {
-    constexpr int TX = NUC3D_TILE_X, TY = NUC3D_TILE_Y, H = NUC3D_HALO, LX = TX + 2 * H, LY = TY + 2 * H;
+    constexpr int TX = NUC3D_TILE_X, TY = NUC3D_TILE_Y, H = HD, LX = TX + 2 * H, LY = TY + 2 * H;
     __shared__ double Wp[2][4][LY][LX];  // u,v,w,T of plane k with the xy halo, double-buffered
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
@@ -1326,7 +1332,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
     const bool incell = ti < tw && tj < th;
     const int i = tl.i0 + ti, j = tl.j0 + tj;
     int hi = 0, hj = 0;
-    const bool hv = threadIdx.x < 2 * H * (TX + TY) && halo_cell(threadIdx.x, tw, th, hi, hj);
+    const bool hv = threadIdx.x < 2 * H * (TX + TY) && halo_cell<H>(threadIdx.x, tw, th, hi, hj);
     const long long xc = cidx(B, i, j, 0), xh = cidx(B, tl.i0 + hi, tl.j0 + hj, 0), sz = B.sz;
     const int kb = tl.k0, ke = tl.k0 + tl.kn;
 
@@ -1401,13 +1407,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
     }
 }
 
-template <int MODE>
+template <int MODE, int HD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_DIV_WAVES))) void KNAME(k_visc_div)(
     const nuc3d_block_dev *__restrict__ blocks, const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int stage,
     int bin, int bun, int bout, double *dtbuf, const unsigned long long *lam_cur, unsigned long long *lam_next,
     unsigned long long *counters)
 {
-    constexpr int TX = NUC3D_TILE_X, TY = NUC3D_TILE_Y, H = NUC3D_HALO, LX = TX + 2 * H, LY = TY + 2 * H;
+    constexpr int TX = NUC3D_TILE_X, TY = NUC3D_TILE_Y, H = HD, LX = TX + 2 * H, LY = TY + 2 * H;
     __shared__ double Vx[2][4][TY][LX];  // fv_xi (components 1..4) of plane k, x halo, double-buffered
     __shared__ double Vy[2][4][LY][TX];  // fv_eta, y halo
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
@@ -1418,7 +1424,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
     const bool incell = ti < tw && tj < th;
     const int i = tl.i0 + ti, j = tl.j0 + tj;
     int hi = 0, hj = 0;
-    const bool hv = threadIdx.x < 2 * H * (TX + TY) && halo_cell(threadIdx.x, tw, th, hi, hj);
+    const bool hv = threadIdx.x < 2 * H * (TX + TY) && halo_cell<H>(threadIdx.x, tw, th, hi, hj);
     const bool hx = threadIdx.x < 2 * H * TY;  // x halo (fv_xi) or y halo (fv_eta)
     const long long xc = cidx(B, i, j, 0), xh = cidx(B, tl.i0 + hi, tl.j0 + hj, 0), sz = B.sz;
     const int kb = tl.k0, ke = tl.k0 + tl.kn;
@@ -1773,7 +1779,10 @@ extern "C" hipError_t KNAME(nuc3d_launch_prim)(const nuc3d_block_dev *blocks, co
 extern "C" hipError_t KNAME(nuc3d_launch_visc_flux)(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles,
                                                     int ntiles, nuc3d_kparams K, int bin, hipStream_t st)
 {
-    hipLaunchKernelGGL(KNAME(k_visc_flux), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+    if (K.diff[0] || K.diff[1] || K.diff[2])
+        hipLaunchKernelGGL(KNAME(k_visc_flux)<4>, dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+    else
+        hipLaunchKernelGGL(KNAME(k_visc_flux)<3>, dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
     return hipGetLastError();
 }
 
@@ -1783,11 +1792,18 @@ extern "C" hipError_t KNAME(nuc3d_launch_visc_div)(int mode, const nuc3d_block_d
                                                    unsigned long long *lam_next, unsigned long long *counters,
                                                    hipStream_t st)
 {
-    if (mode)
-        hipLaunchKernelGGL(KNAME(k_visc_div)<1>, dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, stage, bin, bun,
+    const bool wide = K.diff[0] || K.diff[1] || K.diff[2];
+    if (mode && wide)
+        hipLaunchKernelGGL((KNAME(k_visc_div)<1, 4>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, stage, bin, bun,
+                           bout, dtbuf, lam_cur, lam_next, counters);
+    else if (mode)
+        hipLaunchKernelGGL((KNAME(k_visc_div)<1, 3>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, stage, bin, bun,
+                           bout, dtbuf, lam_cur, lam_next, counters);
+    else if (wide)
+        hipLaunchKernelGGL((KNAME(k_visc_div)<0, 4>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, stage, bin, bun,
                            bout, dtbuf, lam_cur, lam_next, counters);
     else
-        hipLaunchKernelGGL(KNAME(k_visc_div)<0>, dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, stage, bin, bun,
+        hipLaunchKernelGGL((KNAME(k_visc_div)<0, 3>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, stage, bin, bun,
                            bout, dtbuf, lam_cur, lam_next, counters);
     return hipGetLastError();
 }

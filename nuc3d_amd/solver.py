@@ -120,6 +120,8 @@ def lib():
         "nuc3d_get_positivity": [vp, llp],
         "nuc3d_set_division": [vp, ctypes.c_int],
         "nuc3d_diagnostics_noalloc": [vp, dp, dp, vp],
+        "nuc3d_write_tecplot": [vp, ctypes.c_int, ctypes.c_char_p, dp, dp, dp],
+        "nuc3d_write_tecplot_geo": [vp, ctypes.c_int, ctypes.c_char_p, dp, dp, dp],
         "nuc3d_selftest_division": [ctypes.c_longlong, ctypes.c_ulonglong, llp],
         "nuc3d_timing": [vp, ctypes.c_int],
         "nuc3d_timing_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
@@ -292,6 +294,13 @@ class Case:
         _check(lib().nuc3d_diagnostics_noalloc(self.h, ctypes.byref(e), ctypes.byref(k), self.stream),
                "diagnostics_noalloc")
         return e.value, k.value
+
+    def write_tecplot(self, b, path, xn, yn, zn, geo=False):
+        """Tecplot file of block b (block::outputQ_tecplot, or with geo the
+        metric file of block::outputGEO_tecplot) over the node coordinates."""
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (xn, yn, zn)]
+        fn = lib().nuc3d_write_tecplot_geo if geo else lib().nuc3d_write_tecplot
+        _check(fn(self.h, int(b), os.fsencode(path), _dp(a[0]), _dp(a[1]), _dp(a[2])), "write_tecplot")
 
     def save_q(self, b, path, time):
         """Restart file of block b in the reference's format (block.cpp:754-778)."""

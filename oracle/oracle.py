@@ -425,7 +425,8 @@ def read_ref_output(path: str, rank: int):
                 max_eigen=np.array([e0, e1, e2]), dts=dts, walls=walls, res=res)
 
 
-def write_mesh_case(path: str, params: Params, nodes, faces, benchmark: int = 2, start_step: int = 0):
+def write_mesh_case(path: str, params: Params, nodes, faces, benchmark: int = 2, start_step: int = 0,
+                    tecplot: bool = False):
     """A case the reference builds from mesh files: mesh/mesh_<rank>.x (node
     counts, then x, y, z one value per line, block.cpp:42-70), the inp/ files
     of write_ref_case, inp/IOController.in with `Benchmark` (the initial
@@ -433,6 +434,7 @@ def write_mesh_case(path: str, params: Params, nodes, faces, benchmark: int = 2,
     flowData/ directory its geometry output goes to.  With start_step > 0 the
     reference reads its initial state from
     flowData/NUC3d_step_<start_step>_ID_<rank>.bin instead (block.cpp:92-95).
+    `tecplot` switches on the Tecplot flow output of block::Output.
     nodes: per block (xn, yn, zn) of shape (nz+1, ny+1, nx+1)."""
     dummy = []
     for (xn, yn, zn), fc in zip(nodes, faces):
@@ -445,8 +447,8 @@ def write_mesh_case(path: str, params: Params, nodes, faces, benchmark: int = 2,
     os.makedirs(os.path.join(path, "flowData"), exist_ok=True)
     with open(os.path.join(path, "inp", "IOController.in"), "w") as f:
         f.write("startStep %d\nendStep %d\nsaveStep 1000\npostStep 1000\nBenchmark %d\n"
-                "cfl 0.5\ntl 0.0\ndt 0.01\nBinary no\nTecplot no\nPostProc no\n"
-                % (start_step, start_step + 1, benchmark))
+                "cfl 0.5\ntl 0.0\ndt 0.01\nBinary no\nTecplot %s\nPostProc no\n"
+                % (start_step, start_step + 1, benchmark, "yes" if tecplot else "no"))
     with open(os.path.join(path, "inp", "PostProc.in"), "w") as f:  # postproc.cpp:29-72: no statistics
         f.write("AverageType 0\nAverageStartStep 0\nSimutaneousData 0\nAveragedSaveType 0\n"
                 "SimutaneousSaveType 0\n")
@@ -459,14 +461,17 @@ def write_mesh_case(path: str, params: Params, nodes, faces, benchmark: int = 2,
                 f.write("\n")
 
 
-def run_ref_metrics(path: str, nranks: int, timeout: float = 600.0):
+def run_ref_metrics(path: str, nranks: int, timeout: float = 600.0, output: bool = False):
     """Run the reference's block::initial (oracle/_ref/ref_metrics) in `path`;
-    per rank: dict(jac, met, dxyz, centre, q)."""
+    per rank: dict(jac, met, dxyz, centre, q).  block::initial also writes
+    flowData/GEO__id_<rank>.dat; with `output`, block::Output then writes the
+    Tecplot flow file flowData/NUC3d_step_<step>_id_<rank>.dat."""
+    args = [REF_BIN_METRICS] + (["output"] if output else [])
     if nranks == 1:
-        cmd = [REF_BIN_METRICS]
+        cmd = args
     else:
         mpiexec = shutil.which("mpiexec", path="/opt/conda/bin") or "/opt/conda/bin/mpiexec"
-        cmd = [mpiexec, "-n", str(nranks), REF_BIN_METRICS]
+        cmd = [mpiexec, "-n", str(nranks)] + args
     subprocess.run(cmd, cwd=path, check=True, timeout=timeout, stdout=subprocess.PIPE, stderr=subprocess.PIPE)
     out = []
     for r in range(nranks):

@@ -8,11 +8,16 @@
 //
 // usage, in a case directory holding mesh/mesh_<rank>.x, inp/PhysModel.in,
 // inp/Method.in, inp/IOController.in, bc_topo/BC_Topo_<rank>.in and flowData/:
-//   ref_metrics        -> metrics_<rank>.bin
+//   ref_metrics        -> metrics_<rank>.bin (and block::initial's own
+//                         flowData/GEO__id_<rank>.dat, block.cpp:87)
+//   ref_metrics output -> the same, then block::Output (block.cpp:620-654):
+//                         flowData/NUC3d_step_<step>_id_<rank>.dat (Tecplot)
+//                         and the binary restart file
 // metrics_<rank>.bin: int32 nx,ny,nz, then float64 J[N], met[9N] (xi_x..zeta_z),
 //                     dxyz[9N] (x_xi,x_eta,x_zeta,y_xi..z_zeta), centre[3N], Q[5N]
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include "mpi.h"
 #include "IOcontroller.h"
@@ -64,6 +69,7 @@ int main(int argc, char **argv)
         MetricBlock blk;
         blk.initial(op, phys, mpi, bc, io);
         blk.write(mpi.getMyId());
+        if (argc > 1 && std::string(argv[1]) == "output") blk.Output(op, phys, mpi, bc, io);
     }
     return 0;
 }

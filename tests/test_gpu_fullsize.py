@@ -93,3 +93,63 @@ def test_shu_osher_512_crweno5_and_weno5z(exact):
     d, m = diff.max() / out["weno5z"].max(), diff.mean() / out["weno5z"].mean()
     print("  |rho_crweno5 - rho_weno5z|: max %.3e of max rho, mean %.3e of mean rho" % (d, m), flush=True)
     assert d < 0.25 and m < 5e-3
+
+
+@pytest.mark.parametrize("method", ["plain", "hybrid"])
+def test_batch48_fullsize(method):
+    """The 48 independent cases of seeded sizes 32^3..160^3 (57.0 M cells; 24
+    Taylor-Green vortices, 24 spherical blasts, Navier-Stokes, one block group
+    each), 20 graph-launched steps at CFL 0.2: every case stays finite, passes
+    the positivity check and conserves mass, momentum and energy per case to
+    round-off (every case is a periodic block)."""
+    from nuc3d_amd import train_sensor
+    members = cases.batch_cases(n_cases=48, lo=32, hi=160, seed=0)
+    blocks = [m.block for m in members]
+    print("case: batch of 48, %d cells, %s, 20 steps" % (sum(int(np.prod(b.dims)) for b in blocks), method),
+          flush=True)
+    kw = dict(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4)) if method == "hybrid" else {}
+    g = solver.Case(solver.Params(**kw), blocks)
+    for i, m in enumerate(members):
+        g.set_group(i, m.group, m.mach, m.reynolds)
+    if method == "hybrid":
+        params, thr = train_sensor.load()
+        g.set_sensor(params, thr)
+    q0 = [g.get_q(b) for b in range(len(blocks))]
+    g.steps_graph(20, 0.2)
+    g.sync()                                   # raises on a positivity failure
+    assert g.positivity() == (0, 0)
+    print("  fix-ups %d" % g.fixups(), flush=True)
+    worst = 0.0
+    for b in range(len(blocks)):
+        q1 = g.get_q(b)
+        assert np.isfinite(q1).all()
+        for c in range(5):
+            s0, s1 = q0[b][c].astype(np.longdouble).sum(), q1[c].astype(np.longdouble).sum()
+            # momentum on the scale of its magnitude (the blasts start at rest)
+            scale = np.abs(q0[b][c]).sum() if c in (0, 4) else max(np.abs(q0[b][1:4]).sum(), np.abs(q1[1:4]).sum())
+            worst = max(worst, float(abs(s1 - s0)) / scale)
+    print("  worst per-case conservation drift %.3e" % worst, flush=True)
+    assert worst <= 1e-13
+    g.close()
+
+
+def test_flatplate_fullsize():
+    """The 16-block supersonic flat plate over a bump (60.5 M cells,
+    curvilinear tanh-stretched mesh with its metrics computed on the GPU,
+    isothermal wall, inflow/outflow, z periodic), 20 graph-launched steps at
+    CFL 0.5: finite, with positive density and pressure everywhere."""
+    blocks = cases.flat_plate_blocks()
+    print("case: flat plate, 16 blocks, %d cells, 20 steps" % sum(int(np.prod(b.dims)) for b in blocks), flush=True)
+    g = solver.Case(solver.Params(mach=2.9, reynolds=2.0e4, t_wall=2.5), blocks)
+    cases.setup_flat_plate(g, blocks)
+    g.steps_graph(20, 0.5)
+    g.sync()
+    assert g.positivity() == (0, 0)
+    print("  fix-ups %d" % g.fixups(), flush=True)
+    for b in range(len(blocks)):
+        q = g.get_q(b)
+        assert np.isfinite(q).all()
+        jac = g.get_metrics(b)[0]
+        rho = q[0] * jac
+        assert rho.min() > 0.0
+    g.close()

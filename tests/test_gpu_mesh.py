@@ -105,6 +105,22 @@ def test_restart_file_read_by_the_reference(tmp_path):
     # the reference's reader (built here by __graft_entry__.build(); it travels
     # with the tree)
     assert os.path.exists(O.REF_BIN_METRICS), "oracle/_ref/ref_metrics missing: run __graft_entry__.build()"
-    O.write_mesh_case(d, O.Params(model=O.NS), [nodes], [cases.periodic_faces()], start_step=2)
-    ref = O.run_ref_metrics(d, 1)[0]
+    O.write_mesh_case(d, O.Params(model=O.NS), [nodes], [cases.periodic_faces()], start_step=2, tecplot=True)
+    ref = O.run_ref_metrics(d, 1, output=True)[0]
     np.testing.assert_array_equal(ref["q"], want)
+    # Tecplot files: the reference's block::initial wrote the geometry file and
+    # block::Output the flow file of this state; the GPU case writes the same
+    # files byte for byte (block.cpp:692-752, 857-922)
+    mine = os.path.join(d, "mine")
+    os.makedirs(mine, exist_ok=True)
+    g2 = solver.Case(solver.Params(), [case_for(dims)])
+    g2.upload_mesh(0, *nodes)
+    g2.load_q(0, path)
+    g2.write_tecplot(0, os.path.join(mine, "flow.dat"), *nodes)
+    g2.write_tecplot(0, os.path.join(mine, "geo.dat"), *nodes, geo=True)
+    g2.close()
+    for ours, theirs in (("flow.dat", "NUC3d_step_2_id_0.dat"), ("geo.dat", "GEO__id_0.dat")):
+        a = open(os.path.join(mine, ours)).read()
+        b = open(os.path.join(d, "flowData", theirs)).read()
+        print("  %s: %d lines, reference %s: %d lines" % (ours, a.count("\n"), theirs, b.count("\n")), flush=True)
+        assert a == b

@@ -243,3 +243,58 @@ def test_central8_x_weno5js_yz(exact):
     print("  rhs parity ratio %.3f" % err, flush=True)
     assert err <= 1.0
     g.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+@pytest.mark.parametrize("walls", [False, True])
+def test_cd8_viscous_terms_two_blocks(walls, exact):
+    """centraldifference8th at the stage level (centraldifference8th.cpp:22-192,
+    routed by fieldOperator.cpp:508-520): the eighth-order viscous derivatives
+    in the gradients and the divergence, four ghost layers, the reference's
+    exterior closures at walls; with the central scheme (width 4), NS, 2
+    blocks, 2 steps against the oracle."""
+    print("case: central8 + cd8 NS 2 blocks walls=%s exact=%d" % (walls, exact), flush=True)
+    spec = cases.taylor_green(dims=(24, 14, 12), noise=1e-3, seed=36)[0]
+    if walls:
+        spec.faces = [(-1, 2, None), (-1, 2, None), (-1, 3, None), (-1, 3, None), (0, 5, None), (0, 4, None)]
+    blocks = cases.split_block_x(spec, 2)
+    sp, op = both_params(scheme=(solver.CENTRAL8,) * 3, diff=(solver.CD8,) * 3, width=(4, 4, 4), t_wall=1.2,
+                         exact=exact)
+    g = solver.Case(sp, blocks)
+    o = O.OracleCase(op, blocks)
+    g.steps(2, 0.5)
+    o.steps(2, 0.5)
+    for b in range(2):
+        err = check(g.get_q(b), o.get_q(b))
+        print("  block %d parity ratio %.3f" % (b, err), flush=True)
+        assert err <= 1.0
+    g.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+def test_cd8_in_x_only(exact):
+    """cd8 in x (central8, width 4), cd6 in y and z (WENO5-JS, width 3): one
+    RHS evaluation and 2 steps against the oracle."""
+    print("case: x central8 + cd8, y z weno5js + cd6, NS exact=%d" % exact, flush=True)
+    blocks = cases.taylor_green(dims=(22, 16, 12), noise=1e-3, seed=37)
+    sp, op = both_params(scheme=(solver.CENTRAL8, solver.WENO5JS, solver.WENO5JS),
+                         diff=(solver.CD8, solver.CD6, solver.CD6), width=(4, 3, 3), exact=exact)
+    g, o = run_rhs(sp, op, blocks)
+    err = check(g.get_rhs(0), o.get_rhs(0))
+    print("  rhs parity ratio %.3f" % err, flush=True)
+    assert err <= 1.0
+    g.close()
+    g = solver.Case(sp, blocks)
+    o = O.OracleCase(op, blocks)
+    g.steps(2, 0.5)
+    o.steps(2, 0.5)
+    err = check(g.get_q(0), o.get_q(0))
+    print("  steps parity ratio %.3f" % err, flush=True)
+    assert err <= 1.0
+    g.close()
+
+
+def test_cd8_needs_width_4():
+    blocks = cases.taylor_green(dims=(16, 12, 10), noise=1e-3, seed=38)
+    with pytest.raises(solver.Nuc3dError, match=r"\(-4\).*cd8 needs ghost width 4"):
+        solver.Case(solver.Params(diff=(solver.CD8,) * 3), blocks)
