@@ -233,6 +233,19 @@ int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt);
  * residual history restarts (the next nuc3d_get_residual after one step is 0).
  * Runs between full steps. */
 int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out);
+/* The autotuner also picks, per block shape and direction, the tile shape of
+ * the WENO5-JS / WENO5-Z direction sweeps: 16 faces x 16 lines per workgroup
+ * (the default) or 32 faces x 8 lines (fewer halo cells staged per face, half
+ * as many lines side by side), timing each shape's candidates on the
+ * direction's own launches in the context of the whole case (the x and y
+ * sweeps as in an RHS evaluation, the z sweep as in stage 0; the last
+ * nuc3d_rhs result is overwritten).  Results do not change.  The choice of
+ * block b (faces per tile line for x, y, z) is read with
+ * nuc3d_get_sweep_tiles and can be set with nuc3d_set_sweep_tiles (32 only for
+ * WENO5-JS and WENO5-Z directions; between full steps; the residual history
+ * restarts). */
+int nuc3d_get_sweep_tiles(nuc3d_case *c, int b, int *nf3);
+int nuc3d_set_sweep_tiles(nuc3d_case *c, int b, const int *nf3);
 
 /* Wait for the case's work on its streams.  Like nuc3d_get_dt,
  * nuc3d_get_dt_group and nuc3d_run, it returns NUC3D_ERR_NEG_DENSITY or

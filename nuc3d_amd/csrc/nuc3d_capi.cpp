@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <array>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -19,8 +20,8 @@
 #include "nuc3d_types.h"
 
 #define NUC3D_LAUNCHERS(SFX)                                                                                         \
-    extern "C" hipError_t nuc3d_launch_sweep_##SFX(int, int, int, int, const nuc3d_block_dev *, const int4 *, int,   \
-                                                   nuc3d_kparams, int, int, int, int, double *,                      \
+    extern "C" hipError_t nuc3d_launch_sweep_##SFX(int, int, int, int, int, const nuc3d_block_dev *, const int4 *,   \
+                                                   int, nuc3d_kparams, int, int, int, int, double *,                 \
                                                    const unsigned long long *, unsigned long long *,                 \
                                                    unsigned long long *, hipStream_t);                               \
     extern "C" hipError_t nuc3d_launch_diag_##SFX(const nuc3d_block_dev *, const nuc3d_tile *, int, int,            \
@@ -133,8 +134,12 @@ struct nuc3d_case {
     int ntiles = 0;
     std::vector<int> kseg;             // k-segment length of the column tiles per block (nuc3d_autotune)
     int kmax = NUC3D_KSEG;             // the longest
-    int4 *dsweep[3] = {nullptr, nullptr, nullptr};  // direction-sweep tiles
-    int nsweep[3] = {0, 0, 0};
+    // direction-sweep tiles per direction, in two groups: 16 faces x 16
+    // lines, and 32 faces x 8 lines (the tile shape of each block and
+    // direction, swnf, chosen by nuc3d_autotune)
+    int4 *dsweep[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    int nsweep[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    std::vector<std::array<int, 3>> swnf;
     int4 *dlines[3] = {nullptr, nullptr, nullptr};  // whole lines (CRWENO5 directions)
     int nlines[3] = {0, 0, 0};
     int nmax[3] = {0, 0, 0};
@@ -324,6 +329,74 @@ bool visc_in_z_sweep(const nuc3d_case *c)
     return c->p.model == NUC3D_NAVIERSTOKES3D && c->p.scheme[2] != NUC3D_CRWENO5;
 }
 
+// Direction-sweep tile lists from the tile shape of every block and
+// direction (c->swnf: 16 faces x 16 lines, or 32 faces x 8 lines), one list
+// per shape, tiles of NF-1 cells along the sweep axis; where the z sweep
+// finishes stage 2 its work items are the residual's (whose history restarts).
+int build_sweeps(nuc3d_case *c)
+{
+    const bool zvisc = c->p.model == NUC3D_NAVIERSTOKES3D && c->p.scheme[2] != NUC3D_CRWENO5;
+    std::vector<int4> sw[3][2];
+    for (int b = 0; b < c->nb; b++) {
+        const nuc3d_block_dev &B = c->hb[b];
+        for (int d = 0; d < 3; d++) {
+            const int nf = c->swnf[b][d], g = nf == 32 ? 1 : 0, nl = 256 / nf, ncell = nf - 1;
+            std::vector<int4> &L = sw[d][g];
+            if (d == 0) {
+                for (int k = 0; k < B.n[2]; k++)
+                    for (int l0 = 0; l0 < B.n[1]; l0 += nl)
+                        for (int a0 = 0; a0 < B.n[0]; a0 += ncell) L.push_back(make_int4(b, a0, l0, k));
+            } else if (d == 1) {
+                for (int k = 0; k < B.n[2]; k++)
+                    for (int l0 = 0; l0 < B.n[0]; l0 += nl)
+                        for (int a0 = 0; a0 < B.n[1]; a0 += ncell) L.push_back(make_int4(b, a0, l0, k));
+            } else if (zvisc) {
+                // the z sweep also takes the viscous divergence, whose fv_eta
+                // stencil spans j-3..j+3: runs of 8 rows in j at one k segment are
+                // neighbours in the list (they run together and share one XCD's
+                // L2), and so are the k segments of one run (their k halos overlap)
+                for (int l0 = 0; l0 < B.n[0]; l0 += nl)
+                    for (int j0 = 0; j0 < B.n[1]; j0 += 8)
+                        for (int a0 = 0; a0 < B.n[2]; a0 += ncell)
+                            for (int j = j0; j < std::min(j0 + 8, B.n[1]); j++) L.push_back(make_int4(b, a0, l0, j));
+            } else {
+                for (int j = 0; j < B.n[1]; j++)
+                    for (int l0 = 0; l0 < B.n[0]; l0 += nl)
+                        for (int a0 = 0; a0 < B.n[2]; a0 += ncell) L.push_back(make_int4(b, a0, l0, j));
+            }
+        }
+    }
+    for (int d = 0; d < 3; d++)
+        for (int g = 0; g < 2; g++) {
+            (void)hipFree(c->dsweep[d][g]);
+            c->dsweep[d][g] = nullptr;
+            c->nsweep[d][g] = (int)sw[d][g].size();
+            if (sw[d][g].empty()) continue;
+            if (hipMalloc(&c->dsweep[d][g], sizeof(int4) * sw[d][g].size()) != hipSuccess)
+                return fail(NUC3D_ERR_OOM, "device allocation of sweep tiles failed");
+            HIPCHK(hipMemcpy(c->dsweep[d][g], sw[d][g].data(), sizeof(int4) * sw[d][g].size(), hipMemcpyHostToDevice));
+        }
+    if (c->p.scheme[2] != NUC3D_CRWENO5) {
+        c->work_block.clear();
+        for (int g = 0; g < 2; g++)
+            for (const auto &t : sw[2][g]) c->work_block.push_back(t.x);
+        c->nwork = (int)c->work_block.size();
+        double *dr = nullptr;
+        if (hipMalloc(&dr, sizeof(double) * 2 * c->nwork) != hipSuccess)
+            return fail(NUC3D_ERR_OOM, "device allocation of residual partials failed");
+        HIPCHK(hipMemset(dr, 0, sizeof(double) * 2 * c->nwork));
+        (void)hipFree(c->dresid);
+        c->dresid = dr;
+    }
+    for (auto &gx : c->gexec)
+        if (gx) {
+            (void)hipGraphExecDestroy(gx);
+            gx = nullptr;
+        }
+    c->gcfl[0] = c->gcfl[1] = -1.0;
+    return NUC3D_OK;
+}
+
 // viscous fluxes (Navier-Stokes), then the three sweeps; the z sweep
 // completes the RHS (mode 0) or applies RK stage `stage` (mode 1); for
 // Navier-Stokes the viscous divergence follows in its own pass, except in the
@@ -356,12 +429,22 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
     for (int d = 0; d < 3; d++) {
         if (d == 2 && side) HIPCHK(hipStreamWaitEvent(s, c->vjoin, 0));
         Timed t(c, s, &c->ev[d]);
-        if (c->p.scheme[d] == NUC3D_CRWENO5)
+        if (c->p.scheme[d] == NUC3D_CRWENO5) {
             HIPCHK(c->L->crweno(d, ns, mode, c->db, c->dlines[d], c->nlines[d], c->nmax[d], K, stage, bin, bun,
                                 bout, c->ddt, lcur, lnext, c->dcount, c->dscratch, s));
-        else
-            HIPCHK(c->L->sweep(d, c->p.scheme[d], ns, mode, c->db, c->dsweep[d], c->nsweep[d], K, stage, bin,
-                               bun, bout, c->ddt, lcur, lnext, c->dcount, s));
+            continue;
+        }
+        // one launch per tile shape; the residual partials of the z sweep's
+        // work items follow one another in that order
+        long long off = 0;
+        for (int g = 0; g < 2; g++) {
+            if (!c->nsweep[d][g]) continue;
+            nuc3d_kparams Kg = K;
+            if (Kg.resid) Kg.resid += off;
+            HIPCHK(c->L->sweep(d, c->p.scheme[d], ns, mode, g ? 32 : 16, c->db, c->dsweep[d][g], c->nsweep[d][g], Kg,
+                               stage, bin, bun, bout, c->ddt, lcur, lnext, c->dcount, s));
+            off += c->nsweep[d][g];
+        }
     }
     if (ns && !(mode == 1 && visc_in_z_sweep(c))) {
         Timed t(c, s, &c->ev[5]);
@@ -526,31 +609,6 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
                 }
     }
     c->ntiles = (int)tiles.size();
-    // direction sweeps: 15 cells along the sweep axis x 16 lines
-    std::vector<int4> sw[3];
-    for (int b = 0; b < nblocks; b++) {
-        const nuc3d_block_dev &B = c->hb[b];
-        for (int k = 0; k < B.n[2]; k++)
-            for (int l0 = 0; l0 < B.n[1]; l0 += 16)
-                for (int a0 = 0; a0 < B.n[0]; a0 += 15) sw[0].push_back(make_int4(b, a0, l0, k));
-        for (int k = 0; k < B.n[2]; k++)
-            for (int l0 = 0; l0 < B.n[0]; l0 += 16)
-                for (int a0 = 0; a0 < B.n[1]; a0 += 15) sw[1].push_back(make_int4(b, a0, l0, k));
-        if (ns && p->scheme[2] != NUC3D_CRWENO5) {
-            // the z sweep also takes the viscous divergence, whose fv_eta
-            // stencil spans j-3..j+3: runs of 8 rows in j at one k segment are
-            // neighbours in the list (they run together and share one XCD's
-            // L2), and so are the k segments of one run (their k halos overlap)
-            for (int l0 = 0; l0 < B.n[0]; l0 += 16)
-                for (int j0 = 0; j0 < B.n[1]; j0 += 8)
-                    for (int a0 = 0; a0 < B.n[2]; a0 += 15)
-                        for (int j = j0; j < std::min(j0 + 8, B.n[1]); j++) sw[2].push_back(make_int4(b, a0, l0, j));
-        } else {
-            for (int j = 0; j < B.n[1]; j++)
-                for (int l0 = 0; l0 < B.n[0]; l0 += 16)
-                    for (int a0 = 0; a0 < B.n[2]; a0 += 15) sw[2].push_back(make_int4(b, a0, l0, j));
-        }
-    }
     // whole lines of the CRWENO5 directions and their Thomas scratch
     size_t scratch = 0;
     for (int d = 0; d < 3; d++) {
@@ -581,30 +639,29 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         nuc3d_case_destroy(c);
         return fail(NUC3D_ERR_OOM, "device allocation of CRWENO5 scratch failed");
     }
-    for (int d = 0; d < 3; d++) {
-        c->nsweep[d] = (int)sw[d].size();
-        if (hipMalloc(&c->dsweep[d], sizeof(int4) * sw[d].size()) != hipSuccess ||
-            hipMemcpy(c->dsweep[d], sw[d].data(), sizeof(int4) * sw[d].size(), hipMemcpyHostToDevice) != hipSuccess) {
-            nuc3d_case_destroy(c);
-            return fail(NUC3D_ERR_OOM, "device allocation of sweep tiles failed");
-        }
-    }
-    // work items of the kernel that finishes stage 2, and their blocks
-    if (ns && p->scheme[2] == NUC3D_CRWENO5) {
-        for (const auto &t : tiles) c->work_block.push_back(t.b);
-    } else if (p->scheme[2] == NUC3D_CRWENO5) {
-        const long long L = NUC3D_CRW_LINES;
-        for (int b = 0; b < nblocks; b++)
-            for (long long g = 0; g < ((long long)c->hb[b].n[0] * c->hb[b].n[1] + L - 1) / L; g++)
-                c->work_block.push_back(b);
-    } else {
-        for (const auto &t : sw[2]) c->work_block.push_back(t.x);
-    }
-    c->nwork = (int)c->work_block.size();
-    if (hipMalloc(&c->dresid, sizeof(double) * 2 * c->nwork) != hipSuccess ||
-        hipMemset(c->dresid, 0, sizeof(double) * 2 * c->nwork) != hipSuccess) {
+    // direction sweeps: 16 faces x 16 lines for every block until the autotuner chooses
+    c->swnf.assign(nblocks, {16, 16, 16});
+    if (int rc = build_sweeps(c)) {
         nuc3d_case_destroy(c);
-        return fail(NUC3D_ERR_OOM, "device allocation of residual partials failed");
+        return rc;
+    }
+    // work items of the kernel that finishes stage 2, and their blocks (the
+    // z sweep's are set by build_sweeps)
+    if (p->scheme[2] == NUC3D_CRWENO5) {
+        if (ns) {
+            for (const auto &t : tiles) c->work_block.push_back(t.b);
+        } else {
+            const long long L = NUC3D_CRW_LINES;
+            for (int b = 0; b < nblocks; b++)
+                for (long long g = 0; g < ((long long)c->hb[b].n[0] * c->hb[b].n[1] + L - 1) / L; g++)
+                    c->work_block.push_back(b);
+        }
+        c->nwork = (int)c->work_block.size();
+        if (hipMalloc(&c->dresid, sizeof(double) * 2 * c->nwork) != hipSuccess ||
+            hipMemset(c->dresid, 0, sizeof(double) * 2 * c->nwork) != hipSuccess) {
+            nuc3d_case_destroy(c);
+            return fail(NUC3D_ERR_OOM, "device allocation of residual partials failed");
+        }
     }
     if (hipMalloc(&c->duniform, sizeof(int) * nblocks) != hipSuccess ||
         hipMalloc(&c->db, sizeof(nuc3d_block_dev) * nblocks) != hipSuccess ||
@@ -652,7 +709,8 @@ int nuc3d_case_destroy(nuc3d_case *c)
     (void)hipFree(c->db);
     (void)hipFree(c->duniform);
     (void)hipFree(c->dtiles);
-    for (int d = 0; d < 3; d++) (void)hipFree(c->dsweep[d]);
+    for (int d = 0; d < 3; d++)
+        for (int g = 0; g < 2; g++) (void)hipFree(c->dsweep[d][g]);
     for (int d = 0; d < 3; d++) (void)hipFree(c->dlines[d]);
     (void)hipFree(c->dscratch);
     (void)hipFree(c->ddt);
@@ -1408,6 +1466,92 @@ int set_column_tiles(nuc3d_case *c, const std::vector<int> &kseg)
     return NUC3D_OK;
 }
 
+// whether direction d may take 32-face tiles
+bool wide_tiles_ok(const nuc3d_case *c, int d)
+{
+    if (c->p.scheme[d] != NUC3D_WENO5JS && c->p.scheme[d] != NUC3D_WENO5Z) return false;
+    const bool cd8 = c->K.diff[0] || c->K.diff[1] || c->K.diff[2];
+    return !(d == 2 && cd8 && c->p.model == NUC3D_NAVIERSTOKES3D);
+}
+
+// Tile shape of the WENO5-JS / WENO5-Z direction sweeps per block shape and
+// direction: 16 faces x 16 lines or 32 faces x 8 lines (fewer halo cells
+// staged per face, half-width rows in the transverse directions).  Each
+// shape's candidates are timed in the context of the whole case (the other
+// blocks keep their choice) on the direction's own launches: the x and y
+// sweeps as in an RHS evaluation, the z sweep as in stage 0 (into the stage's
+// output buffer, which the next stage 0 writes again; the fix-up and
+// positivity counts are restored).  The last nuc3d_rhs result is overwritten.
+int tune_sweeps(nuc3d_case *c, hipStream_t s)
+{
+    const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
+    bool any = false;
+    for (int d = 0; d < 3; d++) any = any || c->p.scheme[d] == NUC3D_WENO5JS || c->p.scheme[d] == NUC3D_WENO5Z;
+    if (!any) return NUC3D_OK;
+    unsigned long long saved[4];
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpy(saved, c->dcount, sizeof saved, hipMemcpyDeviceToHost));
+    if (ns) HIPCHK(c->L->vflux(c->db, c->dtiles, c->ntiles, c->K, c->cur, s));  // the z sweep reads fv
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    unsigned long long *lcur = c->dlam + 3 * c->nb * c->lampar, *lnext = c->dlam + 3 * c->nb * (1 - c->lampar);
+    int rc = NUC3D_OK;
+    auto run = [&](int d) {
+        const int mode = d == 2 ? 1 : 0;
+        hipError_t e = hipSuccess;
+        for (int g = 0; g < 2 && e == hipSuccess; g++)
+            if (c->nsweep[d][g])
+                e = c->L->sweep(d, c->p.scheme[d], ns, mode, g ? 32 : 16, c->db, c->dsweep[d][g], c->nsweep[d][g],
+                                c->K, 0, c->cur, 0, 1, c->ddt, lcur, lnext, c->dcount, s);
+        return e;
+    };
+    for (int d = 0; d < 3 && rc == 0; d++) {
+        if (!wide_tiles_ok(c, d)) continue;
+        std::vector<int> done(c->nb, 0);
+        for (int b = 0; b < c->nb && rc == 0; b++) {
+            if (done[b]) continue;
+            std::vector<int> same;
+            for (int b2 = b; b2 < c->nb; b2++)
+                if (c->hb[b2].n[0] == c->hb[b].n[0] && c->hb[b2].n[1] == c->hb[b].n[1] &&
+                    c->hb[b2].n[2] == c->hb[b].n[2])
+                    same.push_back(b2);
+            float best = 1e30f;
+            int bestnf = c->swnf[b][d];
+            for (int nf : {16, 32}) {
+                for (int b2 : same) c->swnf[b2][d] = nf;
+                if ((rc = build_sweeps(c))) break;
+                hipError_t e = run(d);
+                if (e == hipSuccess) e = hipEventRecord(e0, s);
+                for (int r = 0; r < 3 && e == hipSuccess; r++) e = run(d);
+                if (e == hipSuccess) e = hipEventRecord(e1, s);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e != hipSuccess) {
+                    rc = hipfail(e, "sweep tile tuning");
+                    break;
+                }
+                if (ms < best) {
+                    best = ms;
+                    bestnf = nf;
+                }
+            }
+            for (int b2 : same) {
+                c->swnf[b2][d] = bestnf;
+                done[b2] = 1;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if ((rc = build_sweeps(c))) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpy(c->dcount, saved, sizeof saved, hipMemcpyHostToDevice));
+    return NUC3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1489,8 +1633,34 @@ int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out)
         if (rc) return rc;
         if ((rc = set_column_tiles(c, kseg))) return rc;
     }
+    if ((rc = tune_sweeps(c, s))) return rc;
     if (kseg_out)
         for (int b = 0; b < c->nb; b++) kseg_out[b] = c->kseg[b];
+    return NUC3D_OK;
+}
+
+int nuc3d_set_sweep_tiles(nuc3d_case *c, int b, const int *nf3)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!nf3) return fail(NUC3D_ERR_BAD_ARG, "null argument");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "tile shapes change between full steps");
+    for (int d = 0; d < 3; d++) {
+        if (nf3[d] != 16 && nf3[d] != 32) return fail(NUC3D_ERR_BAD_ARG, "faces per tile line must be 16 or 32");
+        if (nf3[d] == 32 && !wide_tiles_ok(c, d))
+            return fail(NUC3D_ERR_UNSUPPORTED, "32-face tiles are built for the WENO5-JS and WENO5-Z sweeps only "
+                                               "(and not for the viscous z sweep of a case with cd8 directions)");
+    }
+    for (int d = 0; d < 3; d++) c->swnf[b][d] = nf3[d];
+    return build_sweeps(c);
+}
+
+int nuc3d_get_sweep_tiles(nuc3d_case *c, int b, int *nf3)
+{
+    int rc = check_block(c, b);
+    if (rc) return rc;
+    if (!nf3) return fail(NUC3D_ERR_BAD_ARG, "null output");
+    for (int d = 0; d < 3; d++) nf3[d] = c->swnf[b][d];
     return NUC3D_OK;
 }
 

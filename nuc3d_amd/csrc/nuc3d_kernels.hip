@@ -203,6 +203,9 @@ __device__ __forceinline__ double face_value(const nuc3d_block_dev &B, const nuc
 // direction, or the exterior closure), NaiverStokes3d.cpp:122-168 with
 // fieldOperator.cpp:505-542; cd8 (centraldifference8th.cpp:49-52) reads four
 // cells each side, which the width-4 cases hold.
+// WIDE: the kernel is built for cases with cd8 directions (kernels of cd6-only
+// cases leave the cd8 code out, which keeps their registers down).
+template <bool WIDE = true>
 __device__ __forceinline__ double line_diff(const nuc3d_block_dev &B, const nuc3d_kparams &K, const double *f,
                                             int d, int t)
 {
@@ -210,17 +213,22 @@ __device__ __forceinline__ double line_diff(const nuc3d_block_dev &B, const nuc3
     const int n = B.n[d], w = K.width[d];
     if (B.ftype[2 * d] == -1 && t < w) return diff_exterior(f, st, t, true);
     if (B.ftype[2 * d + 1] == -1 && t >= n - w) return diff_exterior(f, st, n - 1 - t, false);
-    return K.diff[d] ? cd8(f, st) : cd6(f, st);
+    if constexpr (WIDE)
+        if (K.diff[d]) return cd8(f, st);
+    return cd6(f, st);
 }
 
 // Same derivative on a line held in LDS (element stride st along the line).
+template <bool WIDE = true>
 __device__ __forceinline__ double line_diff_lds(const nuc3d_block_dev &B, const nuc3d_kparams &K, const double *f,
                                                 int st, int d, int t)
 {
     const int n = B.n[d], w = K.width[d];
     if (B.ftype[2 * d] == -1 && t < w) return diff_exterior(f, st, t, true);
     if (B.ftype[2 * d + 1] == -1 && t >= n - w) return diff_exterior(f, st, n - 1 - t, false);
-    return K.diff[d] ? cd8(f, st) : cd6(f, st);
+    if constexpr (WIDE)
+        if (K.diff[d]) return cd8(f, st);
+    return cd6(f, st);
 }
 
 // Ghost cell of the neighbour that receives cell (i,j,k) through face f: the
@@ -434,6 +442,7 @@ __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const
 // the TVD-RK3 stage.  The viscous fluxes are read along i, j and k from memory
 // (neighbouring threads and tiles share them in L1 and L2); the cell's state
 // u5 comes from the thread's own staging of that cell.
+template <bool WIDE>
 __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
                                                    const nuc3d_kparams &K, const PhysParams &P, const double *u5,
                                                    int i, int j, int k, const double *dF, int stage, int bun, int bout,
@@ -445,9 +454,9 @@ __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks
     for (int c = 0; c < 5; c++) r[c] = B.rhs[c][x] + dF[c];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const double gx = line_diff(B, K, B.fv[v] + x, 0, i);
-        const double gy = line_diff(B, K, B.fv[4 + v] + x, 1, j);
-        const double gz = line_diff(B, K, B.fv[8 + v] + x, 2, k);
+        const double gx = line_diff<WIDE>(B, K, B.fv[v] + x, 0, i);
+        const double gy = line_diff<WIDE>(B, K, B.fv[4 + v] + x, 1, j);
+        const double gz = line_diff<WIDE>(B, K, B.fv[8 + v] + x, 2, k);
         r[1 + v] = ((r[1 + v] - gx) - gy) - gz;
     }
     double un5[5];
@@ -475,7 +484,10 @@ __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks
 // Lines run along j at fixed k (DIR 0) and along i otherwise, so that global
 // loads and stores of a wavefront touch contiguous i.
 // tile: x = block, y = first cell a0 along DIR, z = first line l0, w = third index.
-template <int DIR, int SCHEME, bool NS, int MODE>
+// NF faces per line and 256 / NF lines per workgroup: 16 x 16 (the default)
+// or 32 faces x 8 lines (fewer halo cells staged per face; the autotuner picks
+// per block shape and direction).
+template <int DIR, int SCHEME, bool NS, int MODE, int NF = 16, bool WIDE = false>
 __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__restrict__ blocks,
                                                       const int4 *__restrict__ tiles, nuc3d_kparams K, int stage,
                                                       int bin, int bun, int bout, double *dtbuf,
@@ -484,7 +496,8 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
 {
     // 3-cell halo (21 slots per line) for WENO5; the central scheme reaches 4 cells (23 slots)
     constexpr int H = SCHEME == SCH_CENTRAL8 ? 4 : NUC3D_HALO;
-    constexpr int NL = 16, NF = 16, NC = NF - 1, NSL = NC + 2 * H, NST = 2 * H;
+    constexpr int NL = 256 / NF, NC = NF - 1, NSL = NC + 2 * H, NST = 2 * H;
+    static_assert(NL * NF == 256, "one face per thread");
     constexpr bool VISC = NS && DIR == 2 && MODE == 1;  // Navier-Stokes stage: viscous divergence here
     __shared__ double S[10][NSL * NL];
     // face fluxes: once every face is computed the split fluxes are dead and
@@ -583,7 +596,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
 #pragma unroll
         for (int c = 0; c < 5; c++) dF[c] = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
         if constexpr (VISC)
-            finish_cell_visc_z(blocks, B, K, P, u5, i, j, k, dF, stage, bun, bout, dt, lmax, nfix, r2);
+            finish_cell_visc_z<WIDE>(blocks, B, K, P, u5, i, j, k, dF, stage, bun, bout, dt, lmax, nfix, r2);
         else
             finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
     }
@@ -617,7 +630,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
 // network, WENO5-JS by the fallback; one atomic per workgroup and count,
 // spread over slots) and kept per face in B.hflag[DIR]: 0 central, 1 network,
 // 2 fallback, 3 physical-face closure.
-template <int DIR, bool NS, int MODE, bool DECIDE>
+template <int DIR, bool NS, int MODE, bool DECIDE, bool WIDE = false>
 __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_dev *__restrict__ blocks,
                                                              const int4 *__restrict__ tiles, nuc3d_kparams K,
                                                              int stage, int bin, int bun, int bout, double *dtbuf,
@@ -928,7 +941,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
 #pragma unroll
         for (int c = 0; c < 5; c++) dF[c] = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
         if constexpr (VISC)
-            finish_cell_visc_z(blocks, B, K, P, u5, i, j, k, dF, stage, bun, bout, dt, lmax, nfix, r2);
+            finish_cell_visc_z<WIDE>(blocks, B, K, P, u5, i, j, k, dF, stage, bun, bout, dt, lmax, nfix, r2);
         else
             finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
     }
@@ -1385,9 +1398,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
             double col[2 * H + 1];
 #pragma unroll
             for (int a = 0; a < 2 * H + 1; a++) col[a] = zr[a][v];
-            dxi[v] = line_diff_lds(B, K, &Wp[buf][v][tj + H][ti + H], 1, 0, i);
-            deta[v] = line_diff_lds(B, K, &Wp[buf][v][tj + H][ti + H], LX, 1, j);
-            dzeta[v] = line_diff_lds(B, K, &col[H], 1, 2, k);
+            dxi[v] = line_diff_lds<HD == 4>(B, K, &Wp[buf][v][tj + H][ti + H], 1, 0, i);
+            deta[v] = line_diff_lds<HD == 4>(B, K, &Wp[buf][v][tj + H][ti + H], LX, 1, j);
+            dzeta[v] = line_diff_lds<HD == 4>(B, K, &col[H], 1, 2, k);
             w4[v] = zr[H][v];
         }
         double out[12];
@@ -1498,9 +1511,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
             double col[2 * H + 1];
 #pragma unroll
             for (int a = 0; a < 2 * H + 1; a++) col[a] = zr[a][v];
-            const double gx = line_diff_lds(B, K, &Vx[buf][v][tj][ti + H], 1, 0, i);
-            const double gy = line_diff_lds(B, K, &Vy[buf][v][tj + H][ti], TX, 1, j);
-            const double gz = line_diff_lds(B, K, &col[H], 1, 2, k);
+            const double gx = line_diff_lds<HD == 4>(B, K, &Vx[buf][v][tj][ti + H], 1, 0, i);
+            const double gy = line_diff_lds<HD == 4>(B, K, &Vy[buf][v][tj + H][ti], TX, 1, j);
+            const double gz = line_diff_lds<HD == 4>(B, K, &col[H], 1, 2, k);
             r[1 + v] = ((r[1 + v] - gx) - gy) - gz;
         }
         if (MODE == 0) {
@@ -1678,8 +1691,36 @@ __global__ __launch_bounds__(256) void KNAME(k_ghost_fill)(const nuc3d_block_dev
 template <int DIR, int SCHEME, bool NS, int MODE>
 static void launch_sweep(const nuc3d_block_dev *blocks, const int4 *tiles, int ntiles, nuc3d_kparams K, int stage,
                          int bin, int bun, int bout, double *dtbuf, const unsigned long long *lam_cur,
-                         unsigned long long *lam_next, unsigned long long *counters, hipStream_t st, bool decide)
+                         unsigned long long *lam_next, unsigned long long *counters, hipStream_t st, bool decide,
+                         int nf)
 {
+    // the fused viscous z sweep of a case with cd8 directions takes the cd8
+    // divergence (the kernels of cd6-only cases are built without it)
+    if constexpr (NS && DIR == 2 && MODE == 1) {
+        if (K.diff[0] || K.diff[1] || K.diff[2]) {
+            if constexpr (SCHEME == SCH_HYBRID) {
+                if (decide)
+                    hipLaunchKernelGGL((KNAME(k_sweep_hybrid)<DIR, NS, MODE, true, true>), dim3(ntiles), dim3(256),
+                                       0, st, blocks, tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next,
+                                       counters);
+                else
+                    hipLaunchKernelGGL((KNAME(k_sweep_hybrid)<DIR, NS, MODE, false, true>), dim3(ntiles), dim3(256),
+                                       0, st, blocks, tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next,
+                                       counters);
+            } else {
+                hipLaunchKernelGGL((KNAME(k_sweep)<DIR, SCHEME, NS, MODE, 16, true>), dim3(ntiles), dim3(256), 0, st,
+                                   blocks, tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
+            }
+            return;
+        }
+    }
+    if constexpr (SCHEME == SCH_WENO5JS || SCHEME == SCH_WENO5Z) {
+        if (nf == 32) {
+            hipLaunchKernelGGL((KNAME(k_sweep)<DIR, SCHEME, NS, MODE, 32>), dim3(ntiles), dim3(256), 0, st, blocks,
+                               tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
+            return;
+        }
+    }
     if constexpr (SCHEME == SCH_HYBRID) {
         if (decide)
             hipLaunchKernelGGL((KNAME(k_sweep_hybrid)<DIR, NS, MODE, true>), dim3(ntiles), dim3(256), 0, st, blocks,
@@ -1693,7 +1734,7 @@ static void launch_sweep(const nuc3d_block_dev *blocks, const int4 *tiles, int n
 }
 
 template <int SCHEME, bool NS>
-static void sweep_dispatch(int dir, int mode, const nuc3d_block_dev *blocks, const int4 *tiles, int ntiles,
+static void sweep_dispatch(int dir, int mode, int nf, const nuc3d_block_dev *blocks, const int4 *tiles, int ntiles,
                            nuc3d_kparams K, int stage, int bin, int bun, int bout, double *dtbuf,
                            const unsigned long long *lam_cur, unsigned long long *lam_next,
                            unsigned long long *counters, hipStream_t st)
@@ -1702,7 +1743,7 @@ static void sweep_dispatch(int dir, int mode, const nuc3d_block_dev *blocks, con
     // stage 0 of a step at the case's decision interval), reused otherwise
     const bool decide = K.hdecide != 0;
     (void)mode;
-#define NUC3D_A blocks, tiles, ntiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, st, decide
+#define NUC3D_A blocks, tiles, ntiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, st, decide, nf
     if (dir == 0)
         launch_sweep<0, SCHEME, NS, 0>(NUC3D_A);
     else if (dir == 1)
@@ -1715,13 +1756,15 @@ static void sweep_dispatch(int dir, int mode, const nuc3d_block_dev *blocks, con
 }
 
 // dir 0/1 accumulate the RHS; dir 2 completes it (mode 0) or applies the RK stage (mode 1)
-extern "C" hipError_t KNAME(nuc3d_launch_sweep)(int dir, int scheme, int ns, int mode, const nuc3d_block_dev *blocks,
-                                                const int4 *tiles, int ntiles, nuc3d_kparams K, int stage, int bin,
-                                                int bun, int bout, double *dtbuf,
+// nf: faces per line of the tiles (16, or 32 for WENO5-JS / WENO5-Z)
+extern "C" hipError_t KNAME(nuc3d_launch_sweep)(int dir, int scheme, int ns, int mode, int nf,
+                                                const nuc3d_block_dev *blocks, const int4 *tiles, int ntiles,
+                                                nuc3d_kparams K, int stage, int bin, int bun, int bout, double *dtbuf,
                                                 const unsigned long long *lam_cur, unsigned long long *lam_next,
                                                 unsigned long long *counters, hipStream_t st)
 {
-#define NUC3D_A dir, mode, blocks, tiles, ntiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, st
+    if (nf != 16 && !(nf == 32 && (scheme == SCH_WENO5JS || scheme == SCH_WENO5Z))) return hipErrorInvalidValue;
+#define NUC3D_A dir, mode, nf, blocks, tiles, ntiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, st
     if (scheme == SCH_WENO5JS) {
         if (ns) sweep_dispatch<SCH_WENO5JS, true>(NUC3D_A);
         else sweep_dispatch<SCH_WENO5JS, false>(NUC3D_A);

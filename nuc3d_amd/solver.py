@@ -121,6 +121,8 @@ def lib():
         "nuc3d_set_division": [vp, ctypes.c_int],
         "nuc3d_diagnostics_noalloc": [vp, dp, dp, vp],
         "nuc3d_write_tecplot": [vp, ctypes.c_int, ctypes.c_char_p, dp, dp, dp],
+        "nuc3d_get_sweep_tiles": [vp, ctypes.c_int, ip],
+        "nuc3d_set_sweep_tiles": [vp, ctypes.c_int, ip],
         "nuc3d_write_tecplot_geo": [vp, ctypes.c_int, ctypes.c_char_p, dp, dp, dp],
         "nuc3d_selftest_division": [ctypes.c_longlong, ctypes.c_ulonglong, llp],
         "nuc3d_timing": [vp, ctypes.c_int],
@@ -369,6 +371,16 @@ class Case:
         v = ctypes.c_longlong()
         _check(lib().nuc3d_get_fixups(self.h, ctypes.byref(v)), "get_fixups")
         return v.value
+
+    def sweep_tiles(self, b):
+        """Faces per tile line of block b's x, y, z sweeps (16 or 32)."""
+        v = (ctypes.c_int * 3)()
+        _check(lib().nuc3d_get_sweep_tiles(self.h, int(b), v), "get_sweep_tiles")
+        return tuple(int(x) for x in v)
+
+    def set_sweep_tiles(self, b, nf3):
+        v = (ctypes.c_int * 3)(*[int(x) for x in nf3])
+        _check(lib().nuc3d_set_sweep_tiles(self.h, int(b), v), "set_sweep_tiles")
 
     def set_division(self, ieee):
         """ieee: the compiler's IEEE division everywhere (same results)."""

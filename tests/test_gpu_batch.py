@@ -105,8 +105,9 @@ def test_autotune_keeps_results(exact):
                 a.set_group(i, m.group, m.mach, m.reynolds)
                 b.set_group(i, m.group, m.mach, m.reynolds)
         ks = a.autotune()
-        print("  k segments %s" % ks, flush=True)
+        print("  k segments %s, sweep tiles %s" % (ks, [a.sweep_tiles(i) for i in range(len(blocks))]), flush=True)
         assert all(k in (4, 8, 16, 32, 64, 128) for k in ks)
+        assert all(nf in (16, 32) for i in range(len(blocks)) for nf in a.sweep_tiles(i))
         a.steps_graph(2, 0.5)
         b.steps_graph(2, 0.5)
         for i in range(len(blocks)):
@@ -146,3 +147,36 @@ def test_batch_crweno5_line_groups(exact):
     print("  block 0 (%d^3) against the oracle: parity ratio %.3f" % (m.block.dims[0], err), flush=True)
     assert err <= 1.0
     g.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+@pytest.mark.parametrize("scheme", [solver.WENO5JS, solver.WENO5Z])
+def test_sweep_tile_shapes_keep_results(scheme, exact):
+    """32-face x 8-line sweep tiles give the same steps as 16 x 16 bit for bit,
+    in every direction or mixed per block (uneven blocks of a batch, Navier-
+    Stokes with the viscous z sweep, and an Euler Shu-Osher block with
+    inflow/outflow faces)."""
+    members = cases.batch_cases(n_cases=3, lo=16, hi=40, seed=9)
+    blocks = [m.block for m in members]
+    for label, bl, params, cfl, groups in (
+            ("batch NS", blocks, solver.Params(exact=exact, scheme=(scheme,) * 3), 0.2, members),
+            ("shu-osher Euler", cases.shu_osher(70, 10, 12, seed=3),
+             solver.Params(model=solver.EULER3D, mach=1.0, exact=exact, scheme=(scheme,) * 3), 0.4, None)):
+        print("case: sweep tiles, %s, scheme %d, exact=%d" % (label, scheme, exact), flush=True)
+        runs = []
+        for shapes in ([(16, 16, 16)] * len(bl), [(32, 32, 32)] * len(bl),
+                       [((32, 16, 32), (16, 32, 16), (32, 32, 16))[i % 3] for i in range(len(bl))]):
+            g = solver.Case(params, bl)
+            if groups:
+                for i, m in enumerate(groups):
+                    g.set_group(i, m.group, m.mach, m.reynolds)
+            for i, sh in enumerate(shapes):
+                g.set_sweep_tiles(i, sh)
+                assert g.sweep_tiles(i) == tuple(sh)
+            g.steps_graph(2, cfl)
+            g.steps(1, cfl)
+            runs.append([g.get_q(i) for i in range(len(bl))])
+            g.close()
+        for other in runs[1:]:
+            for x, y in zip(runs[0], other):
+                np.testing.assert_array_equal(x, y)
