@@ -314,6 +314,17 @@ int refresh(nuc3d_case *c, hipStream_t s)
         }
         HIPCHK(hipMemcpy(c->db, c->hb.data(), sizeof(nuc3d_block_dev) * c->nb, hipMemcpyHostToDevice));
     }
+    int alluni = 1;
+    for (const auto &B : c->hb) alluni = alluni && B.uniform;
+    if (alluni != c->K.alluni) {  // the captured steps hold the launch parameters by value
+        c->K.alluni = alluni;
+        for (auto &g : c->gexec)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+        c->gcfl[0] = c->gcfl[1] = -1.0;
+    }
     HIPCHK(c->L->fixup(c->db, c->dtiles, c->ntiles, c->K, c->cur, c->dcount, s));
     HIPCHK(c->L->ghost(c->db, c->nb, 0, c->cur, s));
     HIPCHK(hipMemsetAsync(c->dlam + 3 * c->nb * c->lampar, 0, sizeof(unsigned long long) * 3 * c->nb, s));

@@ -1202,6 +1202,9 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 #ifndef NUC3D_VISC_FLUX_WAVES
 #define NUC3D_VISC_FLUX_WAVES 2
 #endif
+#ifndef NUC3D_VISC_FLUX_WAVES_UNI
+#define NUC3D_VISC_FLUX_WAVES_UNI 2
+#endif
 #ifndef NUC3D_VISC_DIV_WAVES
 #define NUC3D_VISC_DIV_WAVES 2
 #endif
@@ -1330,8 +1333,10 @@ __device__ __forceinline__ void visc_flux_cell(const nuc3d_kparams &K, const Phy
 }
 
 // HD: the derivatives' half-width (3 for cd6, 4 where a direction uses cd8)
-template <int HD>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_FLUX_WAVES))) void KNAME(k_visc_flux)(
+// UNI: every block is on the uniform-mesh path, so J and the metric terms
+// are the descriptor's constants (no per-cell loads or prefetch registers)
+template <int HD, bool UNI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UNI ? NUC3D_VISC_FLUX_WAVES_UNI : NUC3D_VISC_FLUX_WAVES))) void KNAME(k_visc_flux)(
     const nuc3d_block_dev *__restrict__ blocks, const nuc3d_tile *__restrict__ tiles, nuc3d_kparams K, int bin)
 {
     constexpr int TX = NUC3D_TILE_X, TY = NUC3D_TILE_Y, H = HD, LX = TX + 2 * H, LY = TY + 2 * H;
@@ -1356,10 +1361,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
         for (int v = 0; v < 4; v++) zr[a][v] = 0.0;
     double pc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, ph[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // Q/J and J of
                                                                                         // plane p / halo plane k
-    double pm[10];                                                       // prefetched J + metrics of cell k
+    double pm[UNI ? 1 : 10];                                             // prefetched J + metrics of cell k
 #pragma unroll
-    for (int e = 0; e < 10; e++) pm[e] = 0.0;
-    if (incell) load_q6(B, qb, xc + (kb - H) * sz, pc);
+    for (int e = 0; e < (UNI ? 1 : 10); e++) pm[e] = 0.0;
+    // Q/J of cell x and J: J is the block's constant on the uniform-mesh path
+    auto load_state = [&](long long x, double *q6) {
+        if constexpr (UNI) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) q6[c] = qb[c][x];
+            q6[5] = B.jac0;
+        } else {
+            load_q6(B, qb, x, q6);
+        }
+    };
+    if (incell) load_state(xc + (kb - H) * sz, pc);
 
     for (int p = kb - H; p < ke + H; p++) {
         const int k = p - H, buf = p & 1;
@@ -1369,8 +1384,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
             for (int v = 0; v < 4; v++) zr[a][v] = zr[a + 1][v];
         prim4_of(pc, P, zr[2 * H]);
         double m[10];
+        if constexpr (UNI) {
+            m[0] = B.jac0;
 #pragma unroll
-        for (int e = 0; e < 10; e++) m[e] = pm[e];
+            for (int e = 0; e < 9; e++) m[1 + e] = B.met0[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 10; e++) m[e] = pm[e];
+        }
         if (k >= kb) {
             double wh[4];
             prim4_of(ph, P, wh);
@@ -1381,13 +1402,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
             }
         }
         // loads for the next plane, in flight during this plane's work
-        if (incell && p + 1 < ke + H) load_q6(B, qb, xc + (p + 1) * sz, pc);
+        if (incell && p + 1 < ke + H) load_state(xc + (p + 1) * sz, pc);
         if (k + 1 >= kb && k + 1 < ke) {
-            if (hv) load_q6(B, qb, xh + (k + 1) * sz, ph);
-            if (incell) {
-                pm[0] = jac_at(B, xc + (k + 1) * sz);
+            if (hv) load_state(xh + (k + 1) * sz, ph);
+            if constexpr (!UNI) {
+                if (incell) {
+                    pm[0] = jac_at(B, xc + (k + 1) * sz);
 #pragma unroll
-                for (int e = 0; e < 9; e++) pm[1 + e] = met_at(B, e, xc + (k + 1) * sz);
+                    for (int e = 0; e < 9; e++) pm[1 + e] = met_at(B, e, xc + (k + 1) * sz);
+                }
             }
         }
         __syncthreads();
@@ -1822,10 +1845,15 @@ extern "C" hipError_t KNAME(nuc3d_launch_prim)(const nuc3d_block_dev *blocks, co
 extern "C" hipError_t KNAME(nuc3d_launch_visc_flux)(const nuc3d_block_dev *blocks, const nuc3d_tile *tiles,
                                                     int ntiles, nuc3d_kparams K, int bin, hipStream_t st)
 {
-    if (K.diff[0] || K.diff[1] || K.diff[2])
-        hipLaunchKernelGGL(KNAME(k_visc_flux)<4>, dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+    const bool wide = K.diff[0] || K.diff[1] || K.diff[2];
+    if (wide && K.alluni)
+        hipLaunchKernelGGL((KNAME(k_visc_flux)<4, true>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+    else if (wide)
+        hipLaunchKernelGGL((KNAME(k_visc_flux)<4, false>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+    else if (K.alluni)
+        hipLaunchKernelGGL((KNAME(k_visc_flux)<3, true>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
     else
-        hipLaunchKernelGGL(KNAME(k_visc_flux)<3>, dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
+        hipLaunchKernelGGL((KNAME(k_visc_flux)<3, false>), dim3(ntiles), dim3(256), 0, st, blocks, tiles, K, bin);
     return hipGetLastError();
 }
 

@@ -94,6 +94,7 @@ struct nuc3d_kparams {
                     // array, so that the step's own decisions (first half) stay untouched
     int ieee_div;   // 1: the compiler's IEEE division everywhere (no reduced division)
     int diff[3];    // viscous derivative per direction: 0 cd6, 1 cd8 (needs width 4)
+    int alluni;     // every block takes the uniform-mesh path (J and metrics from the descriptors)
     double *resid;  // stage 2: per-work-item partial sums of (Q - Q_n)^2, or null
     const nuc3d_sensor_dev *sensor;  // hybrid reconstruction: the network, or null
     // hybrid decision counts (central, WENO5-JS by the network, by the
