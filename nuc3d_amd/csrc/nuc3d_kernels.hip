@@ -709,6 +709,9 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     };
 
     const int nslots = nc + 2 * H;
+    // whether every staged split flux is in the range of the reduced division
+    // (the WENO5-JS faces of the workgroup then use it; same quotients)
+    bool rd_ok = true;
     auto stage_slot = [&](int line, int sl, double *qout) {
         int i, j, k;
         cell(line, a0 - H + sl, i, j, k);
@@ -720,6 +723,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
             S[c][o] = fp[c];
             S[5 + c][o] = fn[c];
             if constexpr (DECIDE) PR[c][o] = (float)pr[c];
+            rd_ok = rd_ok && rd_range(fp[c]) && rd_range(fn[c]);
         }
         if constexpr (!DECIDE) {
             reinterpret_cast<LdsReuse &>(LR).u.prp[0][o] = (float)pr[0];
@@ -743,7 +747,15 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
             if (line < nlines && sl < nslots) stage_slot(line, sl, nullptr);
         }
     }
+    // the workgroup-wide AND of rd_ok through WCNT (free until the decisions
+    // are counted, after further barriers): no LDS of its own, which keeps
+    // the reusing sweeps at five workgroups per CU
+    {
+        const unsigned long long all = __ballot(rd_ok);
+        if ((threadIdx.x & 63) == 0) WCNT[threadIdx.x >> 6] = all == ~0ull ? 1 : 0;
+    }
     __syncthreads();
+    const bool rd = WCNT[0] && WCNT[1] && WCNT[2] && WCNT[3] && !K.ieee_div;
 
     const int line = DIR == 0 ? threadIdx.x / NF : threadIdx.x % NL;
     const int f = DIR == 0 ? threadIdx.x % NF : threadIdx.x / NL;
@@ -814,7 +826,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
         // central face whose stencil means or central face values of rho or p
         // are no longer positive takes WENO5-JS and is recorded as a fallback
         unsigned char fl = decided ? B.hflag[DIR][fidx] : (unsigned char)0;
-        if (decided) {
+        if (decided && fl == 0) {  // only central faces can fall back
             const float(*PP)[NSL * NL] = reinterpret_cast<LdsReuse &>(LR).u.prp;
             auto getf = [&](int v, int j) { return PP[v == 0 ? 0 : 1][sidx(line, f + 1 + j)]; };
             auto c8 = [&](int v) {
@@ -875,7 +887,8 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
                 sp[q] = S[c][sidx(line, f + q)];
                 sn[q] = S[5 + c][sidx(line, f + q)];
             }
-            hv[c] = face_value<SCH_WENO5JS>(B, K, DIR, m, sp + 1, sn + 1);
+            hv[c] = rd ? face_value<SCH_WENO5JS, true>(B, K, DIR, m, sp + 1, sn + 1)
+                       : face_value<SCH_WENO5JS, false>(B, K, DIR, m, sp + 1, sn + 1);
         }
     } else if (decided && !weno) {
 #pragma unroll
@@ -914,7 +927,10 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
                 sp[q] = S[c][sidx(ln, ff + 1 + q)];
                 sn[q] = S[5 + c][sidx(ln, ff + 1 + q)];
             }
-            wv5[r] = face_interior<SCH_WENO5JS>(sp, true, false) + face_interior<SCH_WENO5JS>(sn, false, false);
+            wv5[r] = rd ? face_interior<SCH_WENO5JS, true>(sp, true, false) +
+                              face_interior<SCH_WENO5JS, true>(sn, false, false)
+                        : face_interior<SCH_WENO5JS, false>(sp, true, false) +
+                              face_interior<SCH_WENO5JS, false>(sn, false, false);
             wo5[r] = c * (NF * NL) + hidx(ln, ff);
             if constexpr (!HF_IN_S) (&HF[0][0])[wo5[r]] = wv5[r];
         }

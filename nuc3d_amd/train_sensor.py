@@ -5,6 +5,9 @@
 1. Solver snapshots with the plain schemes (every run on the GPU):
      Taylor-Green vortex 64^3, Navier-Stokes, WENO5-JS, 200 steps;
      Shu-Osher 512x64x64, Euler, WENO5-Z, inflow/outflow, 400 steps;
+     supersonic flat plate over a bump, 8 blocks of uneven sizes (0.72 M
+     cells, curvilinear tanh-stretched mesh, isothermal wall, inflow/outflow),
+     Navier-Stokes, WENO5-JS, 300 steps;
      decaying isotropic turbulence n^3 (256 by default), Navier-Stokes,
      WENO5-JS, from the seeded spectrum (seed 1: the benchmark uses seed 0).
    Every few steps the faces of all three directions are sampled on the GPU
@@ -52,21 +55,27 @@ def init_params(seed):
 
 
 def snapshot_runs(n_turb):
-    """(name, params, blocks, snapshot steps, sample stride) of the runs."""
-    runs = [("tgv64", solver.Params(), cases.taylor_green(64), (0, 40, 80, 120, 160, 200), 4)]
+    """(name, params, blocks, snapshot steps, sample stride, setup) of the runs;
+    setup(case, blocks) sets metrics and state where the blocks do not carry
+    them (the flat plate: metrics from the mesh nodes on the GPU)."""
+    runs = [("tgv64", solver.Params(), cases.taylor_green(64), (0, 40, 80, 120, 160, 200), 4, None)]
     runs.append(("shu_osher", solver.Params(model=solver.EULER3D, mach=1.0, scheme=(solver.WENO5Z,) * 3),
-                 cases.shu_osher(512, 64, 64, seed=1), (0, 100, 200, 300, 400), 8))
+                 cases.shu_osher(512, 64, 64, seed=1), (0, 100, 200, 300, 400), 8, None))
+    runs.append(("flatplate", solver.Params(mach=2.9, reynolds=2.0e4, t_wall=2.5),
+                 cases.flat_plate([44, 36, 40, 36], [32, 40], 64), (0, 100, 200, 300), 2, cases.setup_flat_plate))
     blocks, phys = cases.isotropic_turbulence(n=n_turb, seed=1)
     stride = max(1, int(round(40 * (n_turb / 256.0) ** 3)))
     runs.append(("turbulence%d" % n_turb, solver.Params(mach=phys.mach, reynolds=phys.reynolds), blocks,
-                 (0, 100, 200, 300), stride))
+                 (0, 100, 200, 300), stride, None))
     return runs
 
 
 def collect(train, held, n_turb, tau=TAU):
     record = []
-    for name, params, blocks, snaps, stride in snapshot_runs(n_turb):
+    for name, params, blocks, snaps, stride, setup in snapshot_runs(n_turb):
         case = solver.Case(params, blocks)
+        if setup:
+            setup(case, blocks)
         step = 0
         for s in snaps:
             case.steps_graph(s - step, 0.5)
@@ -113,8 +122,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
 
     t0 = time.time()
-    train = solver.Sensor(14_000_000)
-    held = solver.Sensor(4_000_000)
+    train = solver.Sensor(18_000_000)
+    held = solver.Sensor(5_000_000)
     record = collect(train, held, args.n_turb)
     n, npos = train.count()
     log("train_sensor: %d training stencils, %d labelled WENO5 (%.1f%%)" % (n, npos, 100.0 * npos / max(n, 1)))

@@ -20,6 +20,10 @@ def test_selftest_division():
 def _run(params, blocks, ieee, nsteps, cfl):
     g = solver.Case(params, blocks)
     g.set_division(ieee)
+    if solver.HYBRID in params.scheme:
+        from nuc3d_amd import train_sensor
+        sp, thr = train_sensor.load()
+        g.set_sensor(sp, thr - 2.0)   # more WENO5-JS faces than at the trained threshold
     if nsteps:
         g.steps(nsteps, cfl)
         out = [g.get_q(b) for b in range(len(blocks))]
@@ -40,6 +44,9 @@ CASES = {
                                                 riemann=solver.AUSMP),
                                            lambda: cases.shu_osher(48, 10, 8, seed=2), 3, 0.4),
     "walls_multiblock_ns_steps2": (dict(t_wall=1.2), lambda: _walls(), 2, 0.5),
+    "hybrid_ns_steps3": (dict(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4)),
+                         lambda: cases.taylor_green(dims=(20, 18, 16), noise=5e-2, seed=9), 3, 0.5),
+    "hybrid_walls_ns_rhs": (dict(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), t_wall=1.2), lambda: _walls(), 0, 0.5),
     # split fluxes of ~1e-45 are below the reduced division's range: every
     # workgroup takes the IEEE division, and so must give the same results
     "tiny_state_ns_rhs": (dict(), lambda: _tiny(), 0, 0.5),
