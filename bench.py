@@ -352,6 +352,8 @@ def main():
     ap.add_argument("--fast", action="store_true",
                     help="contracted WENO arithmetic (within 1e-12 of the reference on well-conditioned fields only)")
     ap.add_argument("--no-hybrid", action="store_true")
+    ap.add_argument("--no-other-variant", action="store_true",
+                    help="skip timing the other kernel variant (fast beside exact, or exact beside fast)")
     ap.add_argument("--sensor-interval", type=int, default=1, choices=(1, 2),
                     help="multi-block workloads: steps between the hybrid network's decisions (the turbulence "
                          "workload measures both)")
@@ -420,6 +422,22 @@ def main():
     variant = "fast" if args.fast else "exact"
     traffic = read_traffic(dom, n, variant)
     log("bench: plain %.3f ms/step, %.4g cell-updates/s" % (ms_plain, v_plain))
+    sweep_tiles = plain.sweep_tiles(0)
+
+    # the other kernel variant on the same developed state, for the record
+    other = None
+    if not args.no_other_variant:
+        oc = solver.Case(solver.Params(**dict(common, exact=1 if args.fast else 0)), start)
+        oc.set_uniform_mesh(not args.no_uniform_mesh)
+        oc.set_overlap(not args.no_overlap)
+        oc.autotune()
+        v_o, ms_o, _ = time_method(oc, args.steps, args.warmup, cells)
+        oc.close()
+        other = {"variant": "exact" if args.fast else "fast", "value": v_o, "ms_per_step": ms_o,
+                 "note": "the fast variant contracts the WENO arithmetic into FMA with one division per weight "
+                         "set; its parity bar on ill-conditioned components is 8x the reference's own "
+                         "-O0/-O2 spread (the exact variant's is 1x)"}
+        log("bench: plain, %s variant: %.3f ms/step, %.4g cell-updates/s" % (other["variant"], ms_o, v_o))
 
     hybrid = None
     if not args.no_hybrid:
@@ -503,7 +521,9 @@ def main():
             "value_method": method,
             "column_tile_k_segment": kseg[0],
         },
-        "plain": {"value": v_plain, "ms_per_step": ms_plain, "kernel_ms_per_step": {k: v[0] / 2 for k, v in tm.items()}},
+        "plain": {"value": v_plain, "ms_per_step": ms_plain, "kernel_ms_per_step": {k: v[0] / 2 for k, v in tm.items()},
+                  "sweep_faces_per_tile_line": sweep_tiles},
+        "plain_other_variant": other,
         "hybrid": hybrid,
         "roofline": {
             "bound": "hbm",

@@ -100,3 +100,10 @@ def test_bench_json_line():
             best = max(best, h["value"])
     assert d["value"] == pytest.approx(best)
     assert d["config"]["value_method"].startswith("plain" if best == p["value"] else "hybrid")
+    # the other kernel variant is timed for the record only, and the sweep
+    # tile widths are the autotuner's choice for each direction
+    o = d["plain_other_variant"]
+    assert o["variant"] == ("fast" if d["config"]["variant"] == "exact" else "exact")
+    assert o["value"] > 0 and o["ms_per_step"] > 0
+    assert len(p["sweep_faces_per_tile_line"]) == 3
+    assert all(nf in (16, 32) for nf in p["sweep_faces_per_tile_line"])
