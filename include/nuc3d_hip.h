@@ -393,9 +393,10 @@ int nuc3d_has_crweno5(void);
  * (v_rcp_f64 and two Newton steps, quotient with one FMA correction, without
  * the scaling and special-value steps), used only where every operand is in
  * the range in which both give the same correctly rounded quotient.  This
- * compares the two on n hashed operand pairs of that range (x3 quotients:
- * one reciprocal serves three numerators) on the current device and returns
- * the number of quotients whose bits differ (0 expected). */
+ * compares the two on n hashed operand pairs of that range (x4 quotients:
+ * one reciprocal serves three numerators and a signed zero, whose quotient
+ * keeps the IEEE sign) on the current device and returns the number of
+ * quotients whose bits differ (0 expected). */
 int nuc3d_selftest_division(long long n, unsigned long long seed, long long *mismatches);
 /* ieee = 1: the case's kernels use the compiler's IEEE division everywhere
  * instead (results are the same bit for bit; for tests). */

@@ -70,6 +70,14 @@ __device__ __forceinline__ double div_nr(double a, double b, double y)
     const double r = __builtin_fma(-b, q, a);
     return __builtin_fma(r, y, q);
 }
+// a / b for a = +-0 as well: the IEEE quotient of a zero is a zero with the
+// sign of a times that of b, which a * y has (the reduced sequence alone
+// returns +0 for a = -0)
+__device__ __forceinline__ double div_nrz(double a, double b, double y)
+{
+    const double q = div_nr(a, b, y);
+    return a == 0.0 ? a * y : q;
+}
 // a / b, by the reduced sequence (RD) or the compiler's IEEE division
 template <bool RD>
 __device__ __forceinline__ double ddiv(double a, double b)
@@ -82,7 +90,9 @@ __device__ __forceinline__ double ddiv(double a, double b)
 // smoothness indicator is 0 or in [2^-306, 2^127], every weight denominator
 // (eps + beta)^2 in [1e-12, 2^254], every WENO5-Z ratio tau / (beta + eps)
 // +0 or in [2^-485, 2^147], every unnormalised weight in [2^-256, 2^295] and
-// every normalised one in [2^-300, 1].
+// every normalised one in [2^-300, 1].  The fast variant's one-division form
+// divides sum(w q) (+-0, or of magnitude between 2^-500 and 2^830) by sum(w)
+// (in [2^-84, 2^770]; WENO5-Z included).
 __device__ __forceinline__ bool rd_range(double v)
 {
     const double a = fabs(v);
@@ -156,7 +166,9 @@ __device__ __forceinline__ double weno5js(double f0, double f1, double f2, doubl
     // denominator of sum(alpha q)/sum(alpha) by d0 d1 d2 leaves one division.
     double d0 = sq(EPS + b0), d1 = sq(EPS + b1), d2 = sq(EPS + b2);
     double w0 = C0 * (d1 * d2), w1 = C1 * (d0 * d2), w2 = C2 * (d0 * d1);
-    return ddiv<RD>(w0 * q0 + w1 * q1 + w2 * q2, w0 + w1 + w2);
+    const double num = w0 * q0 + w1 * q1 + w2 * q2, den = w0 + w1 + w2;
+    if constexpr (RD) return div_nrz(num, den, recip_nr(den));
+    else return num / den;
 #endif
 }
 
@@ -189,7 +201,9 @@ __device__ __forceinline__ double weno5z(double f0, double f1, double f2, double
 #else
     double d0 = sq(b0 + EPS), d1 = sq(b1 + EPS), d2 = sq(b2 + EPS), t2 = tau * tau;
     double w0 = C0 * (d0 + t2) * (d1 * d2), w1 = C1 * (d1 + t2) * (d0 * d2), w2 = C2 * (d2 + t2) * (d0 * d1);
-    return ddiv<RD>(w0 * q0 + w1 * q1 + w2 * q2, w0 + w1 + w2);
+    const double num = w0 * q0 + w1 * q1 + w2 * q2, den = w0 + w1 + w2;
+    if constexpr (RD) return div_nrz(num, den, recip_nr(den));
+    else return num / den;
 #endif
 }
 

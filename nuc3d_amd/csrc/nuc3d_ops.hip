@@ -362,10 +362,13 @@ __global__ __launch_bounds__(256) void k_selftest_division(long long n, unsigned
         const double a2 = a == 0.0 ? 0.0 : __longlong_as_double(__double_as_longlong(a) - 1);
         const double a3 = a2 + a2;
         const double y = nuc3d_dev::recip_nr(b);
-        const double q[3] = {nuc3d_dev::div_nr(a, b, y), nuc3d_dev::div_nr(a2, b, y), nuc3d_dev::div_nr(a3, b, y)};
-        const double r[3] = {a / b, a2 / b, a3 / b};
+        // and a signed zero over b by div_nrz
+        const double z = ((h2 >> 44) & 1) ? -0.0 : 0.0;
+        const double q[4] = {nuc3d_dev::div_nr(a, b, y), nuc3d_dev::div_nr(a2, b, y), nuc3d_dev::div_nr(a3, b, y),
+                             nuc3d_dev::div_nrz(z, b, y)};
+        const double r[4] = {a / b, a2 / b, a3 / b, z / b};
 #pragma unroll
-        for (int k = 0; k < 3; k++)
+        for (int k = 0; k < 4; k++)
             if (__double_as_longlong(q[k]) != __double_as_longlong(r[k])) bad++;
     }
     if (bad) atomicAdd(mismatch, bad);
