@@ -17,7 +17,9 @@
 #define NUC3D_TILE_Y (256 / NUC3D_TILE_X)
 #define NUC3D_HALO 3    // stencil half-width of WENO5 / cd6
 #define NUC3D_KSEG 32   // planes per k segment of a column tile
+#ifndef NUC3D_CRW_LINES
 #define NUC3D_CRW_LINES 6  // CRWENO5 lines per wavefront (ten tridiagonal systems each)
+#endif
 
 struct nuc3d_block_dev {
     int n[3];            // interior cells nx, ny, nz
