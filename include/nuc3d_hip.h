@@ -213,8 +213,11 @@ int nuc3d_run(nuc3d_case *c, const nuc3d_run_config *cfg, nuc3d_run_status *out,
  * each group is one case, with dt = cfl * min over its own blocks
  * (PDEData3d.cpp:74-80 per case), and its own Mach and Reynolds numbers
  * (mach, reynolds <= 0 keep the case's).  Blocks of different groups must not
- * be connected.  All groups advance together in the same launches.  Set
- * between full steps; nuc3d_run needs a single group. */
+ * be connected: the first call that launches work after faces or groups
+ * changed returns NUC3D_ERR_BAD_ARG if they are (so connected blocks can be
+ * moved to another group one after the other).  All groups advance together
+ * in the same launches.  Set between full steps; nuc3d_run needs a single
+ * group. */
 int nuc3d_set_block_group(nuc3d_case *c, int b, int group, double mach, double reynolds);
 /* dt of group `group` in the current step (nuc3d_get_dt gives group 0's). */
 int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt);

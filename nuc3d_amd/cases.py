@@ -241,7 +241,7 @@ def isotropic_turbulence(n=256, seed=0, mt=0.6, k0=4.0, re_lambda=100.0, phys: O
     # project onto divergence-free fields: u_hat - k (k.u_hat)/|k|^2
     kd = (KX * uh[0] + KY * uh[1] + KZ * uh[2]) / np.where(K2 > 0, K2, 1.0)
     uh = [uh[0] - KX * kd, uh[1] - KY * kd, uh[2] - KZ * kd]
-    vel = [np.fft.irfftn(a, s=(n, n, n)) for a in uh]
+    vel = [np.fft.irfftn(a, s=(n, n, n), axes=(0, 1, 2)) for a in uh]
     g = phys.gamma
     # c = sqrt(gamma p / rho) = 1/Ma for p = 1/(gamma Ma^2), rho = 1
     c = 1.0 / phys.mach
@@ -250,7 +250,7 @@ def isotropic_turbulence(n=256, seed=0, mt=0.6, k0=4.0, re_lambda=100.0, phys: O
     vel = [a * scale for a in vel]
     u, v, w = vel
     # Taylor microscale from <u^2>/<(du/dx)^2> and Re = Re_lambda / (u' lambda)
-    dudx = np.fft.irfftn(1j * KX * np.fft.rfftn(u), s=(n, n, n))
+    dudx = np.fft.irfftn(1j * KX * np.fft.rfftn(u), s=(n, n, n), axes=(0, 1, 2))
     uprime = np.sqrt(np.mean(u * u))
     lam = np.sqrt(np.mean(u * u) / np.mean(dudx * dudx))
     phys.reynolds = re_lambda / (uprime * lam)

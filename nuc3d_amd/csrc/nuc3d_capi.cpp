@@ -292,6 +292,15 @@ struct Timed {
 int refresh(nuc3d_case *c, hipStream_t s)
 {
     if (!c->dirty) return NUC3D_OK;
+    // a case (group) is the reference's whole MPI job: its blocks may only be
+    // connected to blocks of the same group (checked here, once the faces and
+    // groups are all set, so that connected blocks can change group together)
+    for (int b = 0; b < c->nb; b++)
+        for (int f = 0; f < 6; f++) {
+            const int t = c->hb[b].ftype[f];
+            if (t >= 0 && c->hb[t].group != c->hb[b].group)
+                return fail(NUC3D_ERR_BAD_ARG, "connected faces join blocks of different groups");
+        }
     c->hforce = true;
     for (auto &B : c->hb) B.uniform = 0;
     HIPCHK(hipMemcpy(c->db, c->hb.data(), sizeof(nuc3d_block_dev) * c->nb, hipMemcpyHostToDevice));
@@ -753,7 +762,6 @@ int nuc3d_set_faces(nuc3d_case *c, int b, const int *type6, const int *id6, cons
             const int d = f / 2;
             if (N.n[(d + 1) % 3] != B.n[(d + 1) % 3] || N.n[(d + 2) % 3] != B.n[(d + 2) % 3])
                 return fail(NUC3D_ERR_BAD_ARG, "connected faces differ in tangential size");
-            if (N.group != B.group) return fail(NUC3D_ERR_BAD_ARG, "connected faces join blocks of different groups");
         }
         B.ftype[f] = t;
         B.fid[f] = id;
@@ -1308,9 +1316,6 @@ int nuc3d_set_block_group(nuc3d_case *c, int b, int group, double mach, double r
     if (group < 0 || group >= c->nb) return fail(NUC3D_ERR_BAD_ARG, "group must be in [0, nblocks)");
     if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "groups change between full steps");
     nuc3d_block_dev &B = c->hb[b];
-    for (int f = 0; f < 6; f++)
-        if (B.ftype[f] >= 0 && B.ftype[f] != b && c->hb[B.ftype[f]].group != group)
-            return fail(NUC3D_ERR_BAD_ARG, "a connected face joins a block of another group");
     B.group = group;
     B.mach = mach > 0.0 ? mach : c->p.mach;
     B.reynolds = reynolds > 0.0 ? reynolds : c->p.reynolds;

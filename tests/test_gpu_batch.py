@@ -80,12 +80,26 @@ def test_batch_hybrid():
 
 
 def test_groups_cannot_be_connected():
-    spec = cases.taylor_green(dims=(24, 12, 12), seed=1)[0]
+    """Connected blocks of different groups are refused when work is launched;
+    moving both blocks of a connected case to another group, one after the
+    other, is allowed and changes nothing."""
+    spec = cases.taylor_green(dims=(24, 12, 12), noise=1e-3, seed=1)[0]
     blocks = cases.split_block_x(spec, 2)
     g = solver.Case(solver.Params(), blocks)
+    g.set_group(1, 1)
     with pytest.raises(solver.Nuc3dError, match="group"):
-        g.set_group(1, 1)
+        g.steps(1, 0.5)
     g.close()
+    ref = solver.Case(solver.Params(), blocks)
+    ref.steps(2, 0.5)
+    g = solver.Case(solver.Params(), blocks)
+    g.set_group(0, 1)
+    g.set_group(1, 1)
+    g.steps(2, 0.5)
+    for b in range(2):
+        np.testing.assert_array_equal(g.get_q(b), ref.get_q(b))
+    g.close()
+    ref.close()
 
 
 @pytest.mark.parametrize("exact", [1, 0])
