@@ -351,9 +351,12 @@ def bench_tgv(args):
 def bench_shock(args):
     """The shock workload of the compact scheme: the 3-D Shu-Osher
     shock/entropy-wave problem (1024 x 128 x 128 cells by default; Euler, Mach
-    1, inflow/outflow in x, periodic in y and z, CFL 0.5), CRWENO5 (a
+    1, inflow/outflow in x, periodic in y and z, CFL 0.4), CRWENO5 (a
     tridiagonal system per line, component and upwind side) against WENO5-Z
-    from the same initial state; `value` is CRWENO5's throughput."""
+    from the same initial state; `value` is CRWENO5's throughput.  CFL 0.4:
+    at 0.5 the reference's CRWENO5 (crweno5.cpp with TVD-RK3) loses
+    positivity on this problem within 30 steps and ends in NaN (its own build,
+    512 x 64 x 64 in 8 ranks, 40 steps); at 0.4 it runs the 400 steps."""
     from nuc3d_amd import cases, solver
 
     exact = 0 if args.fast else 1
@@ -365,9 +368,13 @@ def bench_shock(args):
         c = solver.Case(solver.Params(model=solver.EULER3D, mach=1.0, scheme=(sch,) * 3, exact=exact), blocks)
         c.set_uniform_mesh(not args.no_uniform_mesh)
         c.set_overlap(not args.no_overlap)
-        v, ms, tm = time_method(c, args.steps, args.warmup, cells)
+        v, ms, tm = time_method(c, args.steps, args.warmup, cells, cfl=0.4)
+        c.sync()
+        fx, pos = c.fixups(), c.positivity()
         c.close()
-        res[name] = {"value": v, "ms_per_step": ms,
+        if pos != (0, 0):
+            raise RuntimeError("shu_osher %s: positivity failed during the timed steps %s" % (name, pos))
+        res[name] = {"value": v, "ms_per_step": ms, "fixups": fx,
                      "kernel_ms_per_step": {k: t / 2.0 for k, (t, n) in tm.items() if n}}
         log("bench: shu_osher %-8s %.3f ms/step, %.4g cell-updates/s  %s" % (
             name, ms, v, " ".join("%s %.3f" % (k, t / 2.0) for k, (t, n) in tm.items() if n)))
@@ -377,7 +384,7 @@ def bench_shock(args):
         "ms_per_step": res["crweno5"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "3-D Shu-Osher shock/entropy wave %dx%dx%d, Euler, Mach 1, inflow/outflow in x, "
-                               "periodic y and z, CFL 0.5" % (nx, ny, nz), "cells": cells,
+                               "periodic y and z, CFL 0.4" % (nx, ny, nz), "cells": cells,
                    "variant": "fast" if args.fast else "exact"},
         "crweno5": res["crweno5"], "weno5z": res["weno5z"],
         "crweno5_over_weno5z": res["crweno5"]["value"] / res["weno5z"]["value"],
