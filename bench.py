@@ -183,6 +183,35 @@ def read_traffic(cls, n, variant):
     return None
 
 
+def read_valu(kernel, n, variant):
+    """FP64 VALU figures of a kernel from the committed rocprofv3 counter
+    summary (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 and SQ_INSTS_VALU per
+    dispatch over the kernel-trace duration), when present for this workload
+    and variant: its FP64 rate against the 78.6 TFLOP/s vector peak, the
+    fraction of cycles in which a SIMD could issue a VALU instruction that it
+    did, and the step's FP64 instructions per cell-update."""
+    path = os.path.join(ROOT, "profiles", "valu_iso%d_%s.txt" % (n, variant))
+    if not os.path.exists(path):
+        return None
+    out = {"source": os.path.relpath(path, ROOT)}
+    for line in open(path):
+        if line.startswith(kernel + " "):
+            f = line[len(kernel):].split()
+            try:
+                out.update({"kernel": kernel, "kernel_fp64_tflops": float(f[2]), "fp64_peak_tflops": 78.6,
+                            "kernel_fp64_frac": float(f[3].rstrip("%")) / 100.0,
+                            "kernel_valu_issue_frac": float(f[5].rstrip("%")) / 100.0,
+                            "kernel_fp64_lane_ops_per_cell": float(f[7])})
+            except (IndexError, ValueError):
+                return None
+        elif line.startswith("plain step: FP64 instructions executed per cell-update"):
+            try:
+                out["step_fp64_lane_ops_per_cell_update"] = float(line.split("=")[1].split()[0])
+            except (IndexError, ValueError):
+                return None
+    return out if "kernel" in out else None
+
+
 def copy_calibration(nbytes):
     """A plain device-to-device copy of known size, for scaling the
     FETCH_SIZE / WRITE_SIZE counters: reads and writes `nbytes` each."""
@@ -479,6 +508,12 @@ def main():
     traffic = read_traffic(dom, n, variant)
     log("bench: plain %.3f ms/step, %.4g cell-updates/s" % (ms_plain, v_plain))
     sweep_tiles = plain.sweep_tiles(0)
+    # the dominant kernel's FP64 VALU figures, from the committed counters
+    nf_dom = {"x_sweep": sweep_tiles[0], "y_sweep": sweep_tiles[1], "z_sweep": sweep_tiles[2]}.get(dom)
+    dom_kernel = ("k_sweep_%s<%d, 0, true, %d, %d, false>" % (variant, {"x_sweep": 0, "y_sweep": 1, "z_sweep": 2}[dom],
+                                                              1 if dom == "z_sweep" else 0, nf_dom)
+                  if nf_dom else "k_visc_flux_%s<3, true>" % variant)
+    valu = read_valu(dom_kernel, n, variant)
 
     # the other kernel variant on the same developed state, for the record
     other = None
@@ -599,6 +634,7 @@ def main():
             "step_frac": alg_step * v_plain / 1e9 / HBM_PEAK_GBS,
             "kernel_model_bytes_per_cell": kernel_alg_bytes(dom, ns, uniform),
             "kernel_model_gbs": model_gbs,
+            "fp64_valu": valu,
         },
         "cpu_baseline": cpu,
     }

@@ -62,6 +62,24 @@ def test_committed_traffic_matches_its_summary(cls, variant):
     assert bench.read_traffic(cls, 32, variant) is None
 
 
+@pytest.mark.parametrize("variant", ["exact", "fast"])
+def test_committed_valu_summary(variant):
+    """The FP64 VALU figures the bench line carries come from the committed
+    counter summary: every plain stage kernel is listed, its FP64 rate is its
+    FP64 instructions (FMA as two flops) over its duration, and the step's
+    FP64 instructions per cell-update are three stages of the four kernels."""
+    ks = ["k_sweep_%s<0, 0, true, 0, 32, false>", "k_sweep_%s<1, 0, true, 0, 32, false>",
+          "k_sweep_%s<2, 0, true, 1, 16, false>", "k_visc_flux_%s<3, true>"]
+    vs = [bench.read_valu(k % variant, 256, variant) for k in ks]
+    assert all(v is not None for v in vs)
+    for v in vs:
+        assert 0.0 < v["kernel_fp64_frac"] < 1.0 and 0.0 < v["kernel_valu_issue_frac"] <= 1.0
+        assert v["kernel_fp64_tflops"] == pytest.approx(v["kernel_fp64_frac"] * 78.6, rel=0.01)
+    step = 3 * sum(v["kernel_fp64_lane_ops_per_cell"] for v in vs)
+    assert vs[0]["step_fp64_lane_ops_per_cell_update"] == pytest.approx(step, abs=4)
+    assert bench.read_valu(ks[0] % variant, 32, variant) is None
+
+
 @pytest.mark.gpu
 def test_bench_json_line():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--n", "32", "--steps", "2", "--warmup", "1",
