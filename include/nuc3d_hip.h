@@ -396,9 +396,10 @@ int nuc3d_has_crweno5(void);
  * (v_rcp_f64 and two Newton steps, quotient with one FMA correction, without
  * the scaling and special-value steps), used only where every operand is in
  * the range in which both give the same correctly rounded quotient.  This
- * compares the two on n hashed operand pairs of that range (x4 quotients:
+ * compares the two on n hashed operand pairs of that range (x7 quotients:
  * one reciprocal serves three numerators and a signed zero, whose quotient
- * keeps the IEEE sign) on the current device and returns the number of
+ * keeps the IEEE sign, and the sign-keeping correction serves two numerators
+ * and a signed zero over |b|) on the current device and returns the number of
  * quotients whose bits differ (0 expected). */
 int nuc3d_selftest_division(long long n, unsigned long long seed, long long *mismatches);
 /* ieee = 1: the case's kernels use the compiler's IEEE division everywhere

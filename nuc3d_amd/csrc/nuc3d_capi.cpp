@@ -56,6 +56,7 @@ extern "C" hipError_t nuc3d_launch_sensor_adam(float *, float *, float *, const 
 extern "C" hipError_t nuc3d_launch_mesh_metrics(nuc3d_block_dev, const double *, const double *, const double *,
                                                 double *, int *, hipStream_t);
 extern "C" hipError_t nuc3d_launch_uniform_check(nuc3d_block_dev, int *, hipStream_t);
+extern "C" hipError_t nuc3d_launch_block_consts(double, const double *, double *, hipStream_t);
 extern "C" hipError_t nuc3d_launch_sum_pairs(const double *, long long, double *, hipStream_t);
 extern "C" hipError_t nuc3d_launch_selftest_division(long long, unsigned long long, unsigned long long *,
                                                      hipStream_t);
@@ -180,6 +181,8 @@ struct nuc3d_case {
     // uniform blocks take J and the metric terms from their descriptor
     bool uniform_mesh = true;
     int *duniform = nullptr;           // [nb] uniform-mesh check flags
+    double *dconsts = nullptr;         // [5] constants of one uniform block (k_block_consts)
+    std::vector<char> rdok;            // per block: J and metrics allow the split's shared reciprocal
     // kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -302,7 +305,8 @@ int refresh(nuc3d_case *c, hipStream_t s)
                 return fail(NUC3D_ERR_BAD_ARG, "connected faces join blocks of different groups");
         }
     c->hforce = true;
-    for (auto &B : c->hb) B.uniform = 0;
+    c->rdok.assign(c->nb, 0);
+    for (auto &B : c->hb) B.uniform = B.rdsplit = 0;
     HIPCHK(hipMemcpy(c->db, c->hb.data(), sizeof(nuc3d_block_dev) * c->nb, hipMemcpyHostToDevice));
     // ghosts of J and metrics, fix-up and ghosts of the state, eigenvalues
     HIPCHK(c->L->ghost(c->db, c->nb, 1, c->cur, s));
@@ -319,6 +323,15 @@ int refresh(nuc3d_case *c, hipStream_t s)
             if (bad[b]) continue;
             HIPCHK(hipMemcpy(&B.jac0, B.jac, sizeof(double), hipMemcpyDeviceToHost));
             for (int m = 0; m < 9; m++) HIPCHK(hipMemcpy(&B.met0[m], B.met[m], sizeof(double), hipMemcpyDeviceToHost));
+            // |grad xi_d| and the reciprocal of J as the device computes them
+            double k5[5];
+            HIPCHK(nuc3d_launch_block_consts(B.jac0, B.met0, c->dconsts, s));
+            HIPCHK(hipMemcpyAsync(k5, c->dconsts, sizeof k5, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (int d = 0; d < 3; d++) B.theta0[d] = k5[d];
+            B.rjac0 = k5[3];
+            c->rdok[b] = k5[4] != 0.0;
+            B.rdsplit = c->rdok[b] && !c->K.ieee_div;
             B.uniform = 1;
         }
         HIPCHK(hipMemcpy(c->db, c->hb.data(), sizeof(nuc3d_block_dev) * c->nb, hipMemcpyHostToDevice));
@@ -557,6 +570,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     // and for Navier-Stokes the 12 viscous flux components
     const int nfields = 15 + 5 + 1 + 9 + 9 + 4 + (ns ? 12 : 0);
     c->hb.resize(nblocks);
+    c->rdok.assign(nblocks, 0);
     std::vector<nuc3d_tile> tiles;
     for (int b = 0; b < nblocks; b++) {
         nuc3d_block_dev &B = c->hb[b];
@@ -684,6 +698,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         }
     }
     if (hipMalloc(&c->duniform, sizeof(int) * nblocks) != hipSuccess ||
+        hipMalloc(&c->dconsts, sizeof(double) * 8) != hipSuccess ||
         hipMalloc(&c->db, sizeof(nuc3d_block_dev) * nblocks) != hipSuccess ||
         hipMalloc(&c->dtiles, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess ||
         hipMalloc(&c->ddt, sizeof(double) * NUC3D_MAX_BLOCKS) != hipSuccess ||
@@ -728,6 +743,7 @@ int nuc3d_case_destroy(nuc3d_case *c)
     for (void *m : c->chunks) (void)hipFree(m);
     (void)hipFree(c->db);
     (void)hipFree(c->duniform);
+    (void)hipFree(c->dconsts);
     (void)hipFree(c->dtiles);
     for (int d = 0; d < 3; d++)
         for (int g = 0; g < 2; g++) (void)hipFree(c->dsweep[d][g]);
@@ -1987,6 +2003,9 @@ int nuc3d_set_division(nuc3d_case *c, int ieee)
 {
     if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
     c->K.ieee_div = ieee != 0;
+    // the split's divisions by J follow the same switch
+    for (int b = 0; b < c->nb; b++) c->hb[b].rdsplit = c->hb[b].uniform && c->rdok[b] && !c->K.ieee_div;
+    HIPCHK(hipMemcpy(c->db, c->hb.data(), sizeof(nuc3d_block_dev) * c->nb, hipMemcpyHostToDevice));
     for (auto &g : c->gexec)
         if (g) {
             (void)hipGraphExecDestroy(g);

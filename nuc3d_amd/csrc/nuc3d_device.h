@@ -78,6 +78,36 @@ __device__ __forceinline__ double div_nrz(double a, double b, double y)
     const double q = div_nr(a, b, y);
     return a == 0.0 ? a * y : q;
 }
+// a / b for b > 0 and a = +-0 as well, with no extra instruction: the
+// correction of div_nr written as -fma(-r, y, -q), which rounds to the same
+// value for every a != 0 (round to nearest is symmetric) and, for a zero a
+// over a positive b, yields the IEEE zero of a's sign where div_nr's
+// fma(r, y, q) returns +0 for a = -0 (for b < 0 and a = -0 it would return
+// -0, not the IEEE +0: callers divide by positive b only)
+__device__ __forceinline__ double div_nrs(double a, double b, double y)
+{
+    const double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    return -__builtin_fma(-r, y, -q);
+}
+// Operands of divisions that share a reciprocal.  A positive denominator in
+// [2^-200, 2^200] (rd_denominator); numerators zero or of magnitude in
+// [2^-600, 2^600], such as p (rd_operand: 0 or in [2^-400, 2^400]) and its
+// products with terms that are 0 or in [2^-200, 2^200].  The quotient is then
+// zero or normal and far from overflow, and every numerator is far above
+// 2^-969.
+__device__ __forceinline__ bool rd_operand(double v)
+{
+    const double a = fabs(v);
+    return v == 0.0 || (a >= 0x1p-400 && a <= 0x1p400);
+}
+__device__ __forceinline__ bool rd_denominator(double v) { return v >= 0x1p-200 && v <= 0x1p200; }
+// a metric term multiplying p in such a numerator: 0 or of magnitude in [2^-200, 2^200]
+__device__ __forceinline__ bool rd_factor(double v)
+{
+    const double a = fabs(v);
+    return v == 0.0 || (a >= 0x1p-200 && a <= 0x1p200);
+}
 // a / b, by the reduced sequence (RD) or the compiler's IEEE division
 template <bool RD>
 __device__ __forceinline__ double ddiv(double a, double b)
@@ -126,6 +156,20 @@ __device__ __forceinline__ void smooth5(double f0, double f1, double f2, double 
     b2 = G0 * sq(zq ? (f2 - 2.0 * f3 + f3) : (f2 - 2.0 * f3 + f4)) + G1 * sq(3.0 * f2 - 4.0 * f3 + f4);
 }
 
+// The same indicators with the exact multiples 2 f and 4 f folded into an
+// FMA: fma(-2, f1, f0) rounds f0 - 2 f1 once, as f0 - (2.0 * f1) does when
+// 2.0 * f1 is exact, i.e. for every finite product (the reduced-division
+// path, whose staged values are at most 2^60).  Five instructions fewer per
+// reconstruction, same bits.
+__device__ __forceinline__ void smooth5_fma(double f0, double f1, double f2, double f3, double f4, bool zq,
+                                            double &b0, double &b1, double &b2)
+{
+    const double t2 = 3.0 * f2;
+    b0 = G0 * sq(__builtin_fma(-2.0, f1, f0) + f2) + G1 * sq(__builtin_fma(-4.0, f1, f0) + t2);
+    b1 = G0 * sq(__builtin_fma(-2.0, f2, f1) + f3) + G1 * sq(f1 - f3);
+    b2 = G0 * sq(__builtin_fma(-2.0, f3, f2) + (zq ? f3 : f4)) + G1 * sq(__builtin_fma(-4.0, f3, t2) + f4);
+}
+
 __device__ __forceinline__ void cand5(double f0, double f1, double f2, double f3, double f4, double &q0, double &q1,
                                       double &q2)
 {
@@ -142,7 +186,8 @@ __device__ __forceinline__ double weno5js(double f0, double f1, double f2, doubl
 {
     NUC3D_WENO_FP
     double b0, b1, b2, q0, q1, q2;
-    smooth5(f0, f1, f2, f3, f4, false, b0, b1, b2);
+    if (NUC3D_EXACT && RD) smooth5_fma(f0, f1, f2, f3, f4, false, b0, b1, b2);
+    else smooth5(f0, f1, f2, f3, f4, false, b0, b1, b2);
     cand5(f0, f1, f2, f3, f4, q0, q1, q2);
 #if NUC3D_EXACT
     double a0 = ddiv<RD>(C0, sq(EPS + b0));
@@ -178,7 +223,8 @@ __device__ __forceinline__ double weno5z(double f0, double f1, double f2, double
 {
     NUC3D_WENO_FP
     double b0, b1, b2, q0, q1, q2;
-    smooth5(f0, f1, f2, f3, f4, zq, b0, b1, b2);
+    if (NUC3D_EXACT && RD) smooth5_fma(f0, f1, f2, f3, f4, zq, b0, b1, b2);
+    else smooth5(f0, f1, f2, f3, f4, zq, b0, b1, b2);
     cand5(f0, f1, f2, f3, f4, q0, q1, q2);
     double tau = fabs(b2 - b0);
 #if NUC3D_EXACT
@@ -375,22 +421,37 @@ __device__ __forceinline__ double presR(double m, double p, bool plus_variant)
 }
 
 // Split flux of one cell in one direction (RiemannSolver loop body,
-// physicsModel.cpp:557-597 with RiemannLF/AUSM/AUSMp).  Q = conserved/J.
+// physicsModel.cpp:557-597 with RiemannLF/AUSM/AUSMp).  Q = conserved/J,
+// theta = |grad xi| = sqrt(mx^2 + my^2 + mz^2) (a uniform block's constant).
+// yj != 0: the reciprocal half of the division by J (recip_nr(jac)), which the
+// Lax-Friedrichs divisions m p / J and p / J share; the caller has checked
+// their operands (J and the metric terms per block, p per cell).
 // Returns |U| + c|grad xi| (the local eigenvalue of :597).
-__device__ __forceinline__ double split_cell(const double *Q, double jac, double mx, double my, double mz, double u,
-                                             double v, double w, double p, double c, int riemann, double *fp,
-                                             double *fn)
+__device__ __forceinline__ double split_cell(const double *Q, double jac, double mx, double my, double mz,
+                                             double theta, double u, double v, double w, double p, double c,
+                                             int riemann, double *fp, double *fn, double yj = 0.0)
 {
-    double theta = sqrt(mx * mx + my * my + mz * mz);
     double U0 = (mx * u + my * v + mz * w);
     double al = c * theta;
     double lam = fabs(U0) + al;
     if (riemann == RIE_LF) {
+        double px, py, pz, pj;
+        if (yj != 0.0) {
+            px = div_nrs(mx * p, jac, yj);
+            py = div_nrs(my * p, jac, yj);
+            pz = div_nrs(mz * p, jac, yj);
+            pj = div_nrs(p, jac, yj);
+        } else {
+            px = mx * p / jac;
+            py = my * p / jac;
+            pz = mz * p / jac;
+            pj = p / jac;
+        }
         double f0 = U0 * Q[0];
-        double f1 = U0 * Q[1] + mx * p / jac;
-        double f2 = U0 * Q[2] + my * p / jac;
-        double f3 = U0 * Q[3] + mz * p / jac;
-        double f4 = U0 * (Q[4] + p / jac);
+        double f1 = U0 * Q[1] + px;
+        double f2 = U0 * Q[2] + py;
+        double f3 = U0 * Q[3] + pz;
+        double f4 = U0 * (Q[4] + pj);
         fp[0] = 0.5 * (f0 + lam * Q[0]);
         fp[1] = 0.5 * (f1 + lam * Q[1]);
         fp[2] = 0.5 * (f2 + lam * Q[2]);

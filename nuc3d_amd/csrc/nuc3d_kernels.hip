@@ -80,7 +80,9 @@ __device__ __forceinline__ double met_at(const nuc3d_block_dev &B, int e, long l
 }
 
 // split fluxes of cell x of buffer qb in direction d; returns the local
-// eigenvalue.  prim (may be null) receives rho, u, v, w, p.
+// eigenvalue.  prim (may be null) receives rho, u, v, w, p.  A uniform
+// block's |grad xi| is its descriptor's constant, and where J, the metric
+// terms and p allow it the divisions by J share one reciprocal (same bits).
 __device__ __forceinline__ double cell_split(const nuc3d_block_dev &B, double *const *qb, long long x, int d,
                                              const PhysParams &P, double *fp, double *fn, double *prim = nullptr,
                                              double *qout = nullptr)
@@ -101,8 +103,10 @@ __device__ __forceinline__ double cell_split(const nuc3d_block_dev &B, double *c
         prim[3] = w;
         prim[4] = p;
     }
-    return split_cell(Q, jac, met_at(B, 3 * d, x), met_at(B, 3 * d + 1, x), met_at(B, 3 * d + 2, x), u, v, w, p, cs,
-                      P.riemann, fp, fn);
+    const double mx = met_at(B, 3 * d, x), my = met_at(B, 3 * d + 1, x), mz = met_at(B, 3 * d + 2, x);
+    const double theta = B.uniform ? B.theta0[d] : sqrt(mx * mx + my * my + mz * mz);
+    const double yj = (B.uniform && B.rdsplit && rd_operand(p)) ? B.rjac0 : 0.0;
+    return split_cell(Q, jac, mx, my, mz, theta, u, v, w, p, cs, P.riemann, fp, fn, yj);
 }
 
 // BC ghost split flux of physical face f from its boundary cell xb
@@ -387,7 +391,7 @@ __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, co
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const double mx = met_at(B, 3 * d, x), my = met_at(B, 3 * d + 1, x), mz = met_at(B, 3 * d + 2, x);
-            const double theta = sqrt(mx * mx + my * my + mz * mz);
+            const double theta = B.uniform ? B.theta0[d] : sqrt(mx * mx + my * my + mz * mz);
             const double U0 = (mx * u + my * v + mz * w);
             const double lam = fabs(U0) + cs * theta;
             lmax[d] = (lmax[d] < lam) ? lam : lmax[d];

@@ -304,6 +304,39 @@ extern "C" hipError_t nuc3d_launch_uniform_check(nuc3d_block_dev B, int *bad, hi
     return hipGetLastError();
 }
 
+// Constants of a uniform block, computed here so that they are the values the
+// stage kernels would compute per cell: |grad xi_d| = sqrt(mx^2 + my^2 + mz^2)
+// of each direction's metric terms (physicsModel.cpp:557-597), the reciprocal
+// half of a division by J (recip_nr), and whether J and every metric term lie
+// in the range where the split's divisions p/J and m p/J may share that
+// reciprocal (J positive in [2^-200, 2^200], each term 0 or of magnitude there).
+// out: theta[3], recip, flag (1.0 or 0.0).
+struct nuc3d_consts_in {
+    double jac, met[9];
+};
+__global__ void k_block_consts(nuc3d_consts_in in, double *out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    bool ok = nuc3d_dev::rd_denominator(in.jac);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double mx = in.met[3 * d], my = in.met[3 * d + 1], mz = in.met[3 * d + 2];
+        out[d] = sqrt(mx * mx + my * my + mz * mz);
+        ok = ok && nuc3d_dev::rd_factor(mx) && nuc3d_dev::rd_factor(my) && nuc3d_dev::rd_factor(mz);
+    }
+    out[3] = nuc3d_dev::recip_nr(in.jac);
+    out[4] = ok ? 1.0 : 0.0;
+}
+
+extern "C" hipError_t nuc3d_launch_block_consts(double jac0, const double *met0, double *out5, hipStream_t st)
+{
+    nuc3d_consts_in in;
+    in.jac = jac0;
+    for (int m = 0; m < 9; m++) in.met[m] = met0[m];
+    hipLaunchKernelGGL(k_block_consts, dim3(1), dim3(64), 0, st, in, out5);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t nuc3d_launch_mesh_metrics(nuc3d_block_dev B, const double *xn, const double *yn,
                                                 const double *zn, double *centre, int *bad, hipStream_t st)
 {
@@ -318,7 +351,8 @@ extern "C" hipError_t nuc3d_launch_mesh_metrics(nuc3d_block_dev B, const double 
 // operand pairs from a hash of (seed, index), b = +-m * 2^eb and a = +0 (one
 // in 16) or +-m * 2^ea with eb, ea in [-600, 600] and |ea - eb| < 700, the
 // mantissa m random or an edge pattern (1, 2 - ulp, 1 + ulp, 1.5); each pair
-// also divides a - ulp and 2a - ulp by the same reciprocal (the shared form).
+// also divides a - ulp and 2a - ulp by the same reciprocal (the shared form),
+// and a, 2a - ulp and a signed zero over |b| by the sign-keeping form div_nrs.
 // Counts quotients whose bits differ.
 namespace {
 __device__ __forceinline__ unsigned long long mix64(unsigned long long x)
@@ -364,11 +398,16 @@ __global__ __launch_bounds__(256) void k_selftest_division(long long n, unsigned
         const double y = nuc3d_dev::recip_nr(b);
         // and a signed zero over b by div_nrz
         const double z = ((h2 >> 44) & 1) ? -0.0 : 0.0;
-        const double q[4] = {nuc3d_dev::div_nr(a, b, y), nuc3d_dev::div_nr(a2, b, y), nuc3d_dev::div_nr(a3, b, y),
-                             nuc3d_dev::div_nrz(z, b, y)};
-        const double r[4] = {a / b, a2 / b, a3 / b, z / b};
+        // and the same over |b| through div_nrs, whose correction keeps the
+        // IEEE sign of a zero over a positive denominator
+        const double bp = fabs(b), yp = nuc3d_dev::recip_nr(bp);
+        const double q[7] = {nuc3d_dev::div_nr(a, b, y),    nuc3d_dev::div_nr(a2, b, y),
+                             nuc3d_dev::div_nr(a3, b, y),   nuc3d_dev::div_nrz(z, b, y),
+                             nuc3d_dev::div_nrs(a, bp, yp), nuc3d_dev::div_nrs(a3, bp, yp),
+                             nuc3d_dev::div_nrs(z, bp, yp)};
+        const double r[7] = {a / b, a2 / b, a3 / b, z / b, a / bp, a3 / bp, z / bp};
 #pragma unroll
-        for (int k = 0; k < 4; k++)
+        for (int k = 0; k < 7; k++)
             if (__double_as_longlong(q[k]) != __double_as_longlong(r[k])) bad++;
     }
     if (bad) atomicAdd(mismatch, bad);

@@ -49,6 +49,13 @@ struct nuc3d_block_dev {
     int uniform;
     int pad_u;
     double jac0, met0[9];
+    // uniform blocks: |grad xi|, |grad eta|, |grad zeta| of the constant metric
+    // terms and the reciprocal half of a division by jac0 (recip_nr), both
+    // computed on the device (k_block_consts); rdsplit: jac0 and every metric
+    // term lie where the split's divisions by J may share that reciprocal
+    double theta0[3], rjac0;
+    int rdsplit;
+    int pad_r;
 };
 
 // Learned sensor of the hybrid reconstruction: 30 features -> 32 -> 32 -> 1.

@@ -166,7 +166,7 @@ def lib():
 
 def selftest_division(n, seed=0):
     """Quotients of the reduced FP64 division that differ from a / b over n
-    hashed operand pairs (x3) on the device."""
+    hashed operand pairs (x7) on the device."""
     v = ctypes.c_longlong()
     _check(lib().nuc3d_selftest_division(int(n), int(seed), ctypes.byref(v)), "selftest_division")
     return v.value
