@@ -279,6 +279,7 @@ def bench_multiblock(args):
         c = solver.Case(params, bl)
         c.set_uniform_mesh(not args.no_uniform_mesh)
         c.set_overlap(not args.no_overlap)
+        c.set_persistent(args.persistent)
         if members:
             for i, m in enumerate(members):
                 c.set_group(i, m.group, m.mach, m.reynolds)
@@ -338,7 +339,7 @@ def bench_multiblock(args):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": workload + ", %d plain steps developed: plain WENO5-JS vs learned hybrid" % args.develop,
                    "cells": cells, "blocks": len(blocks), "variant": "fast" if args.fast else "exact",
-                   "value_method": method, "column_tile_k_segments": kseg},
+                   "value_method": method, "column_tile_k_segments": kseg, "persistent_sweeps": args.persistent},
         "plain": {"value": v_plain, "ms_per_step": ms_plain},
         "hybrid": {"value": v_h, "ms_per_step": ms_h, "speedup_vs_plain": v_h / v_plain,
                    "decision_interval_steps": args.sensor_interval,
@@ -362,6 +363,7 @@ def bench_tgv(args):
     c = solver.Case(solver.Params(exact=0 if args.fast else 1), blocks)
     c.set_uniform_mesh(not args.no_uniform_mesh)
     c.set_overlap(not args.no_overlap)
+    c.set_persistent(args.persistent)
     kseg = c.autotune()
     v, ms, tm = time_method(c, args.steps, args.warmup, cells)
     c.close()
@@ -397,6 +399,7 @@ def bench_shock(args):
         c = solver.Case(solver.Params(model=solver.EULER3D, mach=1.0, scheme=(sch,) * 3, exact=exact), blocks)
         c.set_uniform_mesh(not args.no_uniform_mesh)
         c.set_overlap(not args.no_overlap)
+        c.set_persistent(args.persistent)
         v, ms, tm = time_method(c, args.steps, args.warmup, cells, cfl=0.4)
         c.sync()
         fx, pos = c.fixups(), c.positivity()
@@ -450,6 +453,8 @@ def main():
                     help="load J and the metric fields even where they are one value each")
     ap.add_argument("--no-overlap", action="store_true",
                     help="run the viscous-flux pass in order with the sweeps (for profiles of single kernels)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="launch every direction sweep as one persistent grid over the (block, tile) work queue")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -478,6 +483,7 @@ def main():
     plain = solver.Case(solver.Params(**common), blocks)
     plain.set_uniform_mesh(not args.no_uniform_mesh)
     plain.set_overlap(not args.no_overlap)
+    plain.set_persistent(args.persistent)
     kseg = plain.autotune()
     log("bench: autotuned k segments of the column tiles: %s" % kseg)
     if args.develop:
@@ -521,6 +527,7 @@ def main():
         oc = solver.Case(solver.Params(**dict(common, exact=1 if args.fast else 0)), start)
         oc.set_uniform_mesh(not args.no_uniform_mesh)
         oc.set_overlap(not args.no_overlap)
+        oc.set_persistent(args.persistent)
         oc.autotune()
         v_o, ms_o, _ = time_method(oc, args.steps, args.warmup, cells)
         oc.close()
@@ -544,6 +551,7 @@ def main():
             hcase = solver.Case(solver.Params(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), **common), start)
             hcase.set_uniform_mesh(not args.no_uniform_mesh)
             hcase.set_overlap(not args.no_overlap)
+            hcase.set_persistent(args.persistent)
             hcase.set_sensor(params, thr)
             hcase.set_sensor_interval(interval)
             hcase.autotune()
@@ -611,6 +619,7 @@ def main():
             "variant": variant,
             "value_method": method,
             "column_tile_k_segment": kseg[0],
+            "persistent_sweeps": args.persistent,
         },
         "plain": {"value": v_plain, "ms_per_step": ms_plain, "kernel_ms_per_step": {k: v[0] / 2 for k, v in tm.items()},
                   "sweep_faces_per_tile_line": sweep_tiles},

@@ -368,6 +368,17 @@ int nuc3d_download_hybrid_flags(nuc3d_case *c, int b, int d, unsigned char *flag
  * On by default; off runs every pass in order on the caller's stream.  Results
  * are identical either way; with kernel timing on, the passes run in order. */
 int nuc3d_set_overlap(nuc3d_case *c, int enable);
+/* Persistent direction sweeps (no counterpart in the reference, whose ranks
+ * loop over their own blocks: block.cpp:576-581).  When enabled, the x and
+ * y sweeps (and the z sweep of Euler cases) of WENO5-JS / WENO5-Z directions
+ * are each one persistent grid of as many workgroups as the GPU holds at
+ * once, whose workgroups take (block, tile) work items of every block of the
+ * case from a device queue until none is left (one queue per XCD over its
+ * contiguous range of the tile list, then from the others); the other
+ * kernels, and every kernel when off (the default), have one workgroup per
+ * tile and the hardware dispatcher assigns them.  Results are identical
+ * either way. */
+int nuc3d_set_persistent(nuc3d_case *c, int enable);
 /* Uniform-mesh path (no counterpart in the reference, whose blocks always
  * hold J and the nine metric fields).  When enabled (the default), a block
  * whose J and metric terms are each one value, bit for bit, over its interior

@@ -164,6 +164,9 @@ struct nuc3d_case {
     // beside the x and y sweeps (it reads only the stage's input state and
     // writes only fv; the z sweep waits for it)
     bool overlap = true;
+    // persistent direction sweeps over the device work queue (nuc3d_set_persistent)
+    bool persistent = false;
+    unsigned *dqueue = nullptr;        // NUC3D_QUEUE_WORDS counters, zero between launches
     hipStream_t vstream = nullptr;
     hipEvent_t vfork = nullptr, vjoin = nullptr;
     // captured two-step graphs, one per parity of the eigenvalue buffers
@@ -474,6 +477,10 @@ int launch_rhs(nuc3d_case *c, int mode, int stage, int bin, int bun, int bout, c
             if (!c->nsweep[d][g]) continue;
             nuc3d_kparams Kg = K;
             if (Kg.resid) Kg.resid += off;
+            if (c->persistent) {
+                Kg.pq = c->dqueue;
+                Kg.pq_n = c->nsweep[d][g];
+            }
             HIPCHK(c->L->sweep(d, c->p.scheme[d], ns, mode, g ? 32 : 16, c->db, c->dsweep[d][g], c->nsweep[d][g], Kg,
                                stage, bin, bun, bout, c->ddt, lcur, lnext, c->dcount, s));
             off += c->nsweep[d][g];
@@ -562,6 +569,8 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     K.nblocks = nblocks;
     K.sensor = nullptr;
     K.hcount = nullptr;
+    K.pq = nullptr;
+    K.pq_n = 0;
     K.ieee_div = 0;
     K.hrhs = 0;
     K.ngroups = 1;
@@ -699,6 +708,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     }
     if (hipMalloc(&c->duniform, sizeof(int) * nblocks) != hipSuccess ||
         hipMalloc(&c->dconsts, sizeof(double) * 8) != hipSuccess ||
+        hipMalloc(&c->dqueue, sizeof(unsigned) * NUC3D_QUEUE_WORDS) != hipSuccess ||
         hipMalloc(&c->db, sizeof(nuc3d_block_dev) * nblocks) != hipSuccess ||
         hipMalloc(&c->dtiles, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess ||
         hipMalloc(&c->ddt, sizeof(double) * NUC3D_MAX_BLOCKS) != hipSuccess ||
@@ -719,6 +729,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         hipMemset(c->ddt, 0, sizeof(double) * NUC3D_MAX_BLOCKS) != hipSuccess ||
         hipMemset(c->dlam, 0, sizeof(unsigned long long) * 6 * nblocks) != hipSuccess ||
         hipMemset(c->dcount, 0, sizeof(unsigned long long) * 4) != hipSuccess ||
+        hipMemset(c->dqueue, 0, sizeof(unsigned) * NUC3D_QUEUE_WORDS) != hipSuccess ||
         hipMemset(c->dhcount, 0, sizeof(unsigned long long) * 16 * NUC3D_HCOUNT_SLOTS) != hipSuccess) {
         nuc3d_case_destroy(c);
         return fail(NUC3D_ERR_HIP, "initialising control arrays failed");
@@ -744,6 +755,7 @@ int nuc3d_case_destroy(nuc3d_case *c)
     (void)hipFree(c->db);
     (void)hipFree(c->duniform);
     (void)hipFree(c->dconsts);
+    (void)hipFree(c->dqueue);
     (void)hipFree(c->dtiles);
     for (int d = 0; d < 3; d++)
         for (int g = 0; g < 2; g++) (void)hipFree(c->dsweep[d][g]);
@@ -2020,6 +2032,20 @@ int nuc3d_set_overlap(nuc3d_case *c, int enable)
     if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
     if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "switch the overlap between full steps");
     c->overlap = enable != 0;
+    for (auto &g : c->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    c->gcfl[0] = c->gcfl[1] = -1.0;
+    return NUC3D_OK;
+}
+
+int nuc3d_set_persistent(nuc3d_case *c, int enable)
+{
+    if (!c) return fail(NUC3D_ERR_BAD_ARG, "null case");
+    if (c->next_stage != 0) return fail(NUC3D_ERR_ORDER, "switch persistent sweeps between full steps");
+    c->persistent = enable != 0;
     for (auto &g : c->gexec)
         if (g) {
             (void)hipGraphExecDestroy(g);

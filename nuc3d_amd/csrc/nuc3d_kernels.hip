@@ -15,6 +15,7 @@
 // on-device form of MPICommunicator.cpp:55-115).  Stage 2 also accumulates
 // the maximum eigenvalues of the new state, from which the next step's dt is
 // taken (euler3d.cpp:429-435, PDEData3d.cpp:74-80).
+#include <algorithm>
 #include <type_traits>
 
 #include "nuc3d_device.h"
@@ -55,6 +56,56 @@ __device__ __forceinline__ int xcd_tile(int w, int nw)
     constexpr int NXCD = 8;
     const int per = nw / NXCD, rem = nw % NXCD, x = w % NXCD, l = w / NXCD;
     return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
+}
+
+// Work items of a direction sweep.  A launch of one workgroup per tile
+// (K.pq null) takes tile xcd_tile(blockIdx.x) once.  A persistent launch
+// (K.pq: eight queue counters and an exit counter, zero at launch; a grid of
+// as many workgroups as the GPU holds at once) takes tiles until none is
+// left: the workgroups of XCD x draw from x's contiguous range of the tile
+// list first (neighbouring tiles share that XCD's L2), then from the other
+// ranges in turn.  Thread 0 keeps the raw counter value of the next draw
+// in flight while the current tile is computed (the atomic's round trip is
+// not waited for per tile); a range found drained is left for the next one.
+// Every thread of the workgroup calls next(); it returns the same tile (or
+// -1) to all of them.  sw: one word of LDS.
+struct WorkQueue {
+    int q = 0, raw = 0;  // thread 0: ranges drained so far, counter value of the pending draw
+    __device__ __forceinline__ int next(const nuc3d_kparams &K, int it, int *sw)
+    {
+        if (!K.pq) return it == 0 ? xcd_tile(blockIdx.x, gridDim.x) : -1;
+        constexpr int NXCD = 8;
+        if (threadIdx.x == 0) {
+            const int n = K.pq_n, per = n / NXCD, rem = n % NXCD, x0 = blockIdx.x % NXCD;
+            if (it == 0) raw = (int)atomicAdd(K.pq + x0, 1u);
+            int w = -1;
+            while (w < 0 && q < NXCD) {
+                const int x = (x0 + q) % NXCD;
+                const int cnt = per + (x < rem ? 1 : 0);
+                const int start = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+                if (raw < cnt) {
+                    w = start + raw;
+                } else if (++q < NXCD) {
+                    raw = (int)atomicAdd(K.pq + (x0 + q) % NXCD, 1u);
+                }
+            }
+            if (w >= 0) raw = (int)atomicAdd(K.pq + (x0 + q) % NXCD, 1u);  // the next draw, in flight
+            sw[0] = w;
+        }
+        __syncthreads();
+        return sw[0];
+    }
+};
+
+// End of a persistent launch: the last workgroup to finish (every other one
+// has drawn its last item) zeroes the queue and exit counters for the next
+// launch on the stream.
+__device__ __forceinline__ void work_done(const nuc3d_kparams &K)
+{
+    if (!K.pq || threadIdx.x != 0) return;
+    __threadfence();
+    if (atomicAdd(K.pq + 8, 1u) == gridDim.x - 1)
+        for (int x = 0; x < 9; x++) K.pq[x] = 0u;
 }
 
 __device__ __forceinline__ long long cidx(const nuc3d_block_dev &B, int i, int j, int k)
@@ -491,7 +542,7 @@ __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks
 // NF faces per line and 256 / NF lines per workgroup: 16 x 16 (the default)
 // or 32 faces x 8 lines (fewer halo cells staged per face; the autotuner picks
 // per block shape and direction).
-template <int DIR, int SCHEME, bool NS, int MODE, int NF = 16, bool WIDE = false>
+template <int DIR, int SCHEME, bool NS, int MODE, int NF = 16, bool WIDE = false, bool PERS = false>
 __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__restrict__ blocks,
                                                       const int4 *__restrict__ tiles, nuc3d_kparams K, int stage,
                                                       int bin, int bun, int bout, double *dtbuf,
@@ -510,104 +561,124 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     static_assert(10 * NSL >= 5 * NF, "the face fluxes fit the split-flux area");
     double(*HF)[NF * NL] = reinterpret_cast<double(*)[NF * NL]>(&S[0][0]);
 
-    const int witem = xcd_tile(blockIdx.x, gridDim.x);
-    const int4 t = tiles[witem];
-    const nuc3d_block_dev &B = blocks[t.x];
-    const PhysParams P = phys_of(K, B);
-    const int a0 = t.y, l0 = t.z, cc = t.w;
-    const int nc = min(NC, B.n[DIR] - a0);
-    const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
-    double *const *qin = B.q[bin];
+    // one tile: witem of the tile list, tid = threadIdx.x
+    auto tile = [&](const int witem, const int tid) __attribute__((always_inline)) {
+        const int4 t = tiles[witem];
+        const nuc3d_block_dev &B = blocks[t.x];
+        const PhysParams P = phys_of(K, B);
+        const int a0 = t.y, l0 = t.z, cc = t.w;
+        const int nc = min(NC, B.n[DIR] - a0);
+        const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
+        double *const *qin = B.q[bin];
 
-    // (line, slot) -> LDS offset: slot-fastest for DIR 0, line-fastest otherwise
-    auto sidx = [](int line, int slot) { return DIR == 0 ? line * NSL + slot : slot * NL + line; };
-    auto hidx = [](int line, int f) { return DIR == 0 ? line * NF + f : f * NL + line; };
-    auto cell = [&](int line, int a, int &i, int &j, int &k) {
-        if (DIR == 0) { i = a; j = l0 + line; k = cc; }
-        else if (DIR == 1) { i = l0 + line; j = a; k = cc; }
-        else { i = l0 + line; j = cc; k = a; }
-    };
+        // (line, slot) -> LDS offset: slot-fastest for DIR 0, line-fastest otherwise
+        auto sidx = [](int line, int slot) { return DIR == 0 ? line * NSL + slot : slot * NL + line; };
+        auto hidx = [](int line, int f) { return DIR == 0 ? line * NF + f : f * NL + line; };
+        auto cell = [&](int line, int a, int &i, int &j, int &k) {
+            if (DIR == 0) { i = a; j = l0 + line; k = cc; }
+            else if (DIR == 1) { i = l0 + line; j = a; k = cc; }
+            else { i = l0 + line; j = cc; k = a; }
+        };
 
-    const int nslots = nc + 2 * H;
-    // whether every staged split flux is in the range of the reduced division
-    // (then the workgroup's WENO weights use it; same quotients either way)
-    bool rd_ok = true;
-    auto stage_slot = [&](int line, int sl, double *qout) {
-        int i, j, k;
-        cell(line, a0 - H + sl, i, j, k);
-        double fp[5], fn[5];
-        split_any(B, qin, i, j, k, DIR, P, fp, fn, nullptr, qout);
-        const int o = sidx(line, sl);
-#pragma unroll
-        for (int c = 0; c < 5; c++) {
-            S[c][o] = fp[c];
-            S[5 + c][o] = fn[c];
-            rd_ok = rd_ok && rd_range(fp[c]) && rd_range(fn[c]);
-        }
-    };
-    double u5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // viscous z sweep: the state of the thread's own cell
-    if constexpr (VISC) {
-        // the thread that finishes cell f-1 of a line stages it (and keeps its
-        // state for the RK stage); then the 2H halo slots of every line
-        const int line = threadIdx.x % NL, f = threadIdx.x / NL;
-        if (f >= 1 && f <= nc && line < nlines) stage_slot(line, H + f - 1, u5);
-        for (int e = threadIdx.x; e < 2 * H * NL; e += 256) {
-            const int ln = e % NL, r = e / NL, sl = r < H ? r : nc + r;
-            if (ln < nlines) stage_slot(ln, sl, nullptr);
-        }
-    } else {
-        for (int e = threadIdx.x; e < NSL * NL; e += 256) {
-            const int line = DIR == 0 ? e / NSL : e % NL;
-            const int sl = DIR == 0 ? e % NSL : e / NL;
-            if (line < nlines && sl < nslots) stage_slot(line, sl, nullptr);
-        }
-    }
-    const bool rd = __syncthreads_and(rd_ok) != 0 && !K.ieee_div;
-
-    const int line = DIR == 0 ? threadIdx.x / NF : threadIdx.x % NL;
-    const int f = DIR == 0 ? threadIdx.x % NF : threadIdx.x / NL;
-    const bool has_face = line < nlines && f <= nc;
-    double hv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (has_face) {
-        const int m = a0 + f;
-#pragma unroll
-        for (int c = 0; c < 5; c++) {
-            // slots f..f+NST-1 = cells m-H..m+H-1; the six-cell WENO stencil starts at cell m-3
-            double sp[NST], sn[NST];
-#pragma unroll
-            for (int q = 0; q < NST; q++) {
-                sp[q] = S[c][sidx(line, f + q)];
-                sn[q] = S[5 + c][sidx(line, f + q)];
+        const int nslots = nc + 2 * H;
+        // whether every staged split flux is in the range of the reduced division
+        // (then the workgroup's WENO weights use it; same quotients either way)
+        bool rd_ok = true;
+        auto stage_slot = [&](int line, int sl, double *qout) {
+            int i, j, k;
+            cell(line, a0 - H + sl, i, j, k);
+            double fp[5], fn[5];
+            split_any(B, qin, i, j, k, DIR, P, fp, fn, nullptr, qout);
+            const int o = sidx(line, sl);
+    #pragma unroll
+            for (int c = 0; c < 5; c++) {
+                S[c][o] = fp[c];
+                S[5 + c][o] = fn[c];
+                rd_ok = rd_ok && rd_range(fp[c]) && rd_range(fn[c]);
             }
-            hv[c] = rd ? face_value<SCHEME, true>(B, K, DIR, m, sp + (H - 3), sn + (H - 3))
-                       : face_value<SCHEME, false>(B, K, DIR, m, sp + (H - 3), sn + (H - 3));
+        };
+        double u5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // viscous z sweep: the state of the thread's own cell
+        if constexpr (VISC) {
+            // the thread that finishes cell f-1 of a line stages it (and keeps its
+            // state for the RK stage); then the 2H halo slots of every line
+            const int line = tid % NL, f = tid / NL;
+            if (f >= 1 && f <= nc && line < nlines) stage_slot(line, H + f - 1, u5);
+            for (int e = tid; e < 2 * H * NL; e += 256) {
+                const int ln = e % NL, r = e / NL, sl = r < H ? r : nc + r;
+                if (ln < nlines) stage_slot(ln, sl, nullptr);
+            }
+        } else {
+            for (int e = tid; e < NSL * NL; e += 256) {
+                const int line = DIR == 0 ? e / NSL : e % NL;
+                const int sl = DIR == 0 ? e % NSL : e / NL;
+                if (line < nlines && sl < nslots) stage_slot(line, sl, nullptr);
+            }
         }
-    }
-    __syncthreads();  // every face has read the split fluxes
-    if (has_face)
-#pragma unroll
-        for (int c = 0; c < 5; c++) HF[c][hidx(line, f)] = hv[c];
-    __syncthreads();
+        const bool rd = __syncthreads_and(rd_ok) != 0 && !K.ieee_div;
 
-    double lmax[3] = {0.0, 0.0, 0.0};
-    StageCounts nfix;
-    double r2 = 0.0;  // residual partial (stage 2)
-    const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
-    if (line < nlines && f >= 1 && f <= nc) {
-        int i, j, k;
-        cell(line, a0 + f - 1, i, j, k);
-        double dF[5];
-#pragma unroll
-        for (int c = 0; c < 5; c++) dF[c] = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
-        if constexpr (VISC)
-            finish_cell_visc_z<WIDE>(blocks, B, K, P, u5, i, j, k, dF, stage, bun, bout, dt, lmax, nfix, r2);
-        else
-            finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
-    }
-    if (DIR == 2 && MODE == 1) {
-        nfix.flush(counters);
-        if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * t.x);
-        if (stage == 2 && K.resid) block_sum_store(r2, K.resid + witem);
+        const int line = DIR == 0 ? tid / NF : tid % NL;
+        const int f = DIR == 0 ? tid % NF : tid / NL;
+        const bool has_face = line < nlines && f <= nc;
+        double hv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (has_face) {
+            const int m = a0 + f;
+    #pragma unroll
+            for (int c = 0; c < 5; c++) {
+                // slots f..f+NST-1 = cells m-H..m+H-1; the six-cell WENO stencil starts at cell m-3
+                double sp[NST], sn[NST];
+    #pragma unroll
+                for (int q = 0; q < NST; q++) {
+                    sp[q] = S[c][sidx(line, f + q)];
+                    sn[q] = S[5 + c][sidx(line, f + q)];
+                }
+                hv[c] = rd ? face_value<SCHEME, true>(B, K, DIR, m, sp + (H - 3), sn + (H - 3))
+                           : face_value<SCHEME, false>(B, K, DIR, m, sp + (H - 3), sn + (H - 3));
+            }
+        }
+        __syncthreads();  // every face has read the split fluxes
+        if (has_face)
+    #pragma unroll
+            for (int c = 0; c < 5; c++) HF[c][hidx(line, f)] = hv[c];
+        __syncthreads();
+
+        double lmax[3] = {0.0, 0.0, 0.0};
+        StageCounts nfix;
+        double r2 = 0.0;  // residual partial (stage 2)
+        const double dt = (DIR == 2 && MODE == 1) ? stage_dt(blocks, K, stage, B.group, dtbuf, lam_cur, lam_next) : 0.0;
+        if (line < nlines && f >= 1 && f <= nc) {
+            int i, j, k;
+            cell(line, a0 + f - 1, i, j, k);
+            double dF[5];
+    #pragma unroll
+            for (int c = 0; c < 5; c++) dF[c] = HF[c][hidx(line, f)] - HF[c][hidx(line, f - 1)];
+            if constexpr (VISC)
+                finish_cell_visc_z<WIDE>(blocks, B, K, P, u5, i, j, k, dF, stage, bun, bout, dt, lmax, nfix, r2);
+            else
+                finish_cell<DIR, NS, MODE>(blocks, B, K, P, i, j, k, dF, stage, bin, bun, bout, dt, lmax, nfix, r2);
+        }
+        if (DIR == 2 && MODE == 1) {
+            nfix.flush(counters);
+            if (stage == 2) block_max3_atomic(lmax, lam_next + 3 * t.x);
+            if (stage == 2 && K.resid) block_sum_store(r2, K.resid + witem);
+        }    };
+    if constexpr (PERS) {
+        // a persistent launch over the device work queue, one tile per
+        // iteration; the thread index is opaque to the compiler per tile, so
+        // values derived from it (LDS addresses of the staging and the faces)
+        // are not kept live across tiles
+        __shared__ int SW[1];
+        WorkQueue wq;
+        for (int it = 0;; it++) {
+            const int witem = wq.next(K, it, SW);
+            if (witem < 0) break;
+            int tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            tile(witem, tid);
+            __syncthreads();  // the next tile reuses the LDS
+        }
+        work_done(K);
+    } else {
+        tile(xcd_tile(blockIdx.x, gridDim.x), threadIdx.x);
     }
 }
 
@@ -1731,49 +1802,73 @@ __global__ __launch_bounds__(256) void KNAME(k_ghost_fill)(const nuc3d_block_dev
 
 // ---------------------------------------------------------------------------
 // host-side launchers (the only symbols the C-ABI layer uses)
+// Workgroups of a direction-sweep launch: one per tile, or for a persistent
+// launch (K.pq set) as many as the GPU holds at once: this kernel's
+// workgroups per CU (its registers and LDS) times the CUs
+#ifndef NUC3D_PERSIST_X
+#define NUC3D_PERSIST_X 1
+#endif
+static int sweep_grid(const void *kernel, int ntiles, const nuc3d_kparams &K)
+{
+    if (!K.pq) return ntiles;
+    int dev = 0, ncu = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess || ncu < 1 || nb < 1)
+        return ntiles;
+    return std::min(ntiles, NUC3D_PERSIST_X * nb * ncu);
+}
+
 template <int DIR, int SCHEME, bool NS, int MODE>
 static void launch_sweep(const nuc3d_block_dev *blocks, const int4 *tiles, int ntiles, nuc3d_kparams K, int stage,
                          int bin, int bun, int bout, double *dtbuf, const unsigned long long *lam_cur,
                          unsigned long long *lam_next, unsigned long long *counters, hipStream_t st, bool decide,
                          int nf)
 {
+    using Kernel = void (*)(const nuc3d_block_dev *, const int4 *, nuc3d_kparams, int, int, int, int, double *,
+                            const unsigned long long *, unsigned long long *, unsigned long long *);
+    // persistent launches exist for the WENO5-JS / WENO5-Z sweeps that keep
+    // their occupancy in that form (x and y, and the Euler z sweep: 77-79
+    // VGPRs, six workgroups per CU; the Navier-Stokes z sweep, which also
+    // takes the viscous divergence and the RK stage, would need 148 VGPRs,
+    // three per CU, against 95 and five); every other kernel takes one
+    // workgroup per tile
+    auto go = [&](Kernel kern, bool pers = false) {
+        nuc3d_kparams Kl = K;
+        if (!pers) Kl.pq = nullptr;
+        hipLaunchKernelGGL(kern, dim3(sweep_grid(reinterpret_cast<const void *>(kern), ntiles, Kl)), dim3(256), 0, st,
+                           blocks, tiles, Kl, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
+    };
     // the fused viscous z sweep of a case with cd8 directions takes the cd8
     // divergence (the kernels of cd6-only cases are built without it)
     if constexpr (NS && DIR == 2 && MODE == 1) {
         if (K.diff[0] || K.diff[1] || K.diff[2]) {
             if constexpr (SCHEME == SCH_HYBRID) {
-                if (decide)
-                    hipLaunchKernelGGL((KNAME(k_sweep_hybrid)<DIR, NS, MODE, true, true>), dim3(ntiles), dim3(256),
-                                       0, st, blocks, tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next,
-                                       counters);
-                else
-                    hipLaunchKernelGGL((KNAME(k_sweep_hybrid)<DIR, NS, MODE, false, true>), dim3(ntiles), dim3(256),
-                                       0, st, blocks, tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next,
-                                       counters);
+                if (decide) go(KNAME(k_sweep_hybrid)<DIR, NS, MODE, true, true>);
+                else go(KNAME(k_sweep_hybrid)<DIR, NS, MODE, false, true>);
             } else {
-                hipLaunchKernelGGL((KNAME(k_sweep)<DIR, SCHEME, NS, MODE, 16, true>), dim3(ntiles), dim3(256), 0, st,
-                                   blocks, tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
+                go(KNAME(k_sweep)<DIR, SCHEME, NS, MODE, 16, true>);
             }
             return;
         }
     }
     if constexpr (SCHEME == SCH_WENO5JS || SCHEME == SCH_WENO5Z) {
+        if (K.pq && !(NS && DIR == 2 && MODE == 1)) {
+            if (nf == 32) go(KNAME(k_sweep)<DIR, SCHEME, NS, MODE, 32, false, true>, true);
+            else go(KNAME(k_sweep)<DIR, SCHEME, NS, MODE, 16, false, true>, true);
+            return;
+        }
         if (nf == 32) {
-            hipLaunchKernelGGL((KNAME(k_sweep)<DIR, SCHEME, NS, MODE, 32>), dim3(ntiles), dim3(256), 0, st, blocks,
-                               tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
+            go(KNAME(k_sweep)<DIR, SCHEME, NS, MODE, 32>);
             return;
         }
     }
     if constexpr (SCHEME == SCH_HYBRID) {
-        if (decide)
-            hipLaunchKernelGGL((KNAME(k_sweep_hybrid)<DIR, NS, MODE, true>), dim3(ntiles), dim3(256), 0, st, blocks,
-                               tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
-        else
-            hipLaunchKernelGGL((KNAME(k_sweep_hybrid)<DIR, NS, MODE, false>), dim3(ntiles), dim3(256), 0, st, blocks,
-                               tiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
-    } else
-        hipLaunchKernelGGL((KNAME(k_sweep)<DIR, SCHEME, NS, MODE>), dim3(ntiles), dim3(256), 0, st, blocks, tiles,
-                           K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters);
+        if (decide) go(KNAME(k_sweep_hybrid)<DIR, NS, MODE, true>);
+        else go(KNAME(k_sweep_hybrid)<DIR, NS, MODE, false>);
+    } else {
+        go(KNAME(k_sweep)<DIR, SCHEME, NS, MODE>);
+    }
 }
 
 template <int SCHEME, bool NS>

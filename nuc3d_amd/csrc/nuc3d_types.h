@@ -110,5 +110,13 @@ struct nuc3d_kparams {
     // fallback), spread over NUC3D_HCOUNT_SLOTS slots of 16 words so that
     // the atomics of different workgroups do not meet on one address
     unsigned long long *hcount;
+    // persistent direction sweeps: eight per-XCD queue counters and an exit
+    // counter (zero when a launch starts; its last workgroup zeroes them), and
+    // the number of work items (tiles) of the launch; null: one workgroup per
+    // tile
+    unsigned *pq;
+    int pq_n;
+    int pad_q;
 };
 #define NUC3D_HCOUNT_SLOTS 256
+#define NUC3D_QUEUE_WORDS 16

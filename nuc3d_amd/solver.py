@@ -146,6 +146,7 @@ def lib():
         "nuc3d_set_sensor_interval": [vp, ctypes.c_int],
         "nuc3d_set_uniform_mesh": [vp, ctypes.c_int],
         "nuc3d_set_overlap": [vp, ctypes.c_int],
+        "nuc3d_set_persistent": [vp, ctypes.c_int],
         "nuc3d_get_uniform_mesh": [vp, ctypes.c_int, ip],
         "nuc3d_op_interpolate": [ctypes.c_int] * 8 + [dp, dp, dp, vp],
         "nuc3d_op_differentiate": [ctypes.c_int] * 6 + [dp, dp, dp, vp],
@@ -411,6 +412,12 @@ class Case:
         """Run the viscous-flux pass on a second stream beside the x and y
         sweeps (default on; results are identical)."""
         _check(lib().nuc3d_set_overlap(self.h, 1 if enable else 0), "set_overlap")
+
+    def set_persistent(self, enable):
+        """Launch each direction sweep as one persistent grid that takes
+        (block, tile) work items from a device queue (default off; results
+        are identical)."""
+        _check(lib().nuc3d_set_persistent(self.h, 1 if enable else 0), "set_persistent")
 
     def set_uniform_mesh(self, enable):
         """Let blocks of one J and one value per metric term take them from
