@@ -724,7 +724,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     struct LdsDecide {
         union {
             float pr[5][NSL * NL];
-            unsigned xf[256][16];
+            unsigned xf[256][20];  // 16 words per face, rows of 80 B: 8 rows cover the 32 banks
             double hf[5][NF * NL];
         } u;
         float zl[256];
