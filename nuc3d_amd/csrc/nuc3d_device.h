@@ -128,6 +128,19 @@ __device__ __forceinline__ bool rd_range(double v)
     const double a = fabs(v);
     return v == 0.0 || (a >= 0x1p-100 && a <= 0x1p60);
 }
+// The exact WENO5-JS weights need only the upper bound: their divisors are
+// (eps + beta)^2 >= 1e-12 and at most 2^254, their numerators the constants
+// C_k and the unnormalised weights (at least C_k / 2^254), so for any finite
+// split flux of magnitude at most 2^60 (a NaN fails the test) every quotient
+// is normal with a normal numerator, however small the fluxes.  WENO5-Z's
+// ratio tau / (beta + eps) and the fast variant's one-division form have
+// numerators that small fluxes make small, and keep rd_range.
+template <int SCHEME>
+__device__ __forceinline__ bool rd_staged(double v)
+{
+    if constexpr (NUC3D_EXACT && (SCHEME == SCH_WENO5JS || SCHEME == SCH_HYBRID)) return fabs(v) <= 0x1p60;
+    else return rd_range(v);
+}
 
 // ---- smoothness indicators on an oriented stencil f0..f4 ----
 // zq: the +inner beta2 of weno5z.cpp:80 ((f2-2f3+f3)^2 in place of (f2-2f3+f4)^2)

@@ -594,7 +594,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
             for (int c = 0; c < 5; c++) {
                 S[c][o] = fp[c];
                 S[5 + c][o] = fn[c];
-                rd_ok = rd_ok && rd_range(fp[c]) && rd_range(fn[c]);
+                rd_ok = rd_ok && rd_staged<SCHEME>(fp[c]) && rd_staged<SCHEME>(fn[c]);
             }
         };
         double u5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // viscous z sweep: the state of the thread's own cell
@@ -798,7 +798,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
             S[c][o] = fp[c];
             S[5 + c][o] = fn[c];
             if constexpr (DECIDE) PR[c][o] = (float)pr[c];
-            rd_ok = rd_ok && rd_range(fp[c]) && rd_range(fn[c]);
+            rd_ok = rd_ok && rd_staged<SCH_HYBRID>(fp[c]) && rd_staged<SCH_HYBRID>(fn[c]);
         }
         if constexpr (!DECIDE) {
             reinterpret_cast<LdsReuse &>(LR).u.prp[0][o] = (float)pr[0];

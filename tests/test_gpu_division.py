@@ -47,9 +47,14 @@ CASES = {
     "hybrid_ns_steps3": (dict(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4)),
                          lambda: cases.taylor_green(dims=(20, 18, 16), noise=5e-2, seed=9), 3, 0.5),
     "hybrid_walls_ns_rhs": (dict(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), t_wall=1.2), lambda: _walls(), 0, 0.5),
-    # split fluxes of ~1e-45 are below the reduced division's range: every
-    # workgroup takes the IEEE division, and so must give the same results
-    "tiny_state_ns_rhs": (dict(), lambda: _tiny(), 0, 0.5),
+    # split fluxes of ~1e-45 (and ~1e-160, whose smoothness indicators are
+    # subnormal): below the range the WENO5-Z and fast-variant weights need,
+    # so their workgroups take the IEEE division; the exact WENO5-JS weights
+    # need only the upper bound and take the reduced one.  Either way the
+    # results must be the same
+    "tiny_state_ns_rhs": (dict(), lambda: _tiny(1e-45), 0, 0.5),
+    "tinier_state_ns_rhs": (dict(), lambda: _tiny(1e-160), 0, 0.5),
+    "tiny_state_ns_weno5z_rhs": (dict(scheme=(solver.WENO5Z,) * 3), lambda: _tiny(1e-45), 0, 0.5),
 }
 
 
@@ -59,9 +64,9 @@ def _walls():
     return cases.split_block_x(spec, 2)
 
 
-def _tiny():
+def _tiny(scale):
     b = cases.taylor_green(dims=(16, 12, 10), noise=1e-3, seed=8)[0]
-    b.q = b.q * 1e-45
+    b.q = b.q * scale
     return [b]
 
 
