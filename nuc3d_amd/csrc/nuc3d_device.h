@@ -43,6 +43,19 @@ enum { BC_INLET = 0, BC_OUTLET = 1, BC_WALL = 2, BC_SYMM = 3 };
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
+// Global-memory access through the block descriptors' field pointers.  Those
+// pointers are generic to the compiler, which makes FLAT instructions of their
+// loads and stores; a FLAT access also counts on the LDS counter, so a wait
+// for an LDS read waits for every global load in flight.  The fields are
+// global memory, and these access them as such.
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) unsigned char gbyte;
+__device__ __forceinline__ const gdouble *gp(const double *p) { return (const gdouble *)p; }
+__device__ __forceinline__ double gld(const double *p) { return *(const gdouble *)p; }
+__device__ __forceinline__ void gst(double *p, double v) { *(gdouble *)p = v; }
+__device__ __forceinline__ unsigned char gldb(const unsigned char *p) { return *(const gbyte *)p; }
+__device__ __forceinline__ void gstb(unsigned char *p, unsigned char v) { *(gbyte *)p = v; }
+
 // ---- FP64 division in two halves ----
 // The compiler's IEEE division a/b on gfx950 is v_div_scale (of b and of a),
 // v_rcp_f64 and two Newton steps for the reciprocal, the quotient a*y with
@@ -543,19 +556,23 @@ __device__ __forceinline__ void split_point(const double *prim, double jac, doub
 }
 
 // Central differences (centraldifference6th.cpp:43-45, centraldifference8th.cpp:49-52);
-// s points at the centre cell, st = element stride along the line.
-__device__ __forceinline__ double cd6(const double *s, long st)
+// s points at the centre cell, st = element stride along the line (T: double
+// for LDS and registers, gdouble for global memory).
+template <typename T>
+__device__ __forceinline__ double cd6(const T *s, long st)
 {
     return CD6_2 * (s[3 * st] - s[-3 * st]) + CD6_1 * (s[2 * st] - s[-2 * st]) + CD6_0 * (s[st] - s[-st]);
 }
-__device__ __forceinline__ double cd8(const double *s, long st)
+template <typename T>
+__device__ __forceinline__ double cd8(const T *s, long st)
 {
     return CD8_3 * (s[4 * st] - s[-4 * st]) + CD8_2 * (s[3 * st] - s[-3 * st]) + CD8_1 * (s[2 * st] - s[-2 * st]) +
            CD8_0 * (s[st] - s[-st]);
 }
 // Exterior derivative closure (boundaryScheme.cpp:100-126); k = distance from
 // the physical face, left = the face is on the low side.
-__device__ __forceinline__ double diff_exterior(const double *s, long st, int k, bool left)
+template <typename T>
+__device__ __forceinline__ double diff_exterior(const T *s, long st, int k, bool left)
 {
     switch (k) {
     case 0: return left ? s[st] - s[0] : s[0] - s[-st];

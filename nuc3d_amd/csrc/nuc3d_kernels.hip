@@ -122,12 +122,12 @@ __device__ __forceinline__ long long dstride(const nuc3d_block_dev &B, int d)
 // same values as the fields hold)
 __device__ __forceinline__ double jac_at(const nuc3d_block_dev &B, long long x)
 {
-    return B.uniform ? B.jac0 : B.jac[x];
+    return B.uniform ? B.jac0 : gld(B.jac + x);
 }
 
 __device__ __forceinline__ double met_at(const nuc3d_block_dev &B, int e, long long x)
 {
-    return B.uniform ? B.met0[e] : B.met[e][x];
+    return B.uniform ? B.met0[e] : gld(B.met[e] + x);
 }
 
 // split fluxes of cell x of buffer qb in direction d; returns the local
@@ -140,7 +140,7 @@ __device__ __forceinline__ double cell_split(const nuc3d_block_dev &B, double *c
 {
     double Q[5];
 #pragma unroll
-    for (int c = 0; c < 5; c++) Q[c] = qb[c][x];
+    for (int c = 0; c < 5; c++) Q[c] = gld(qb[c] + x);
     if (qout)
 #pragma unroll
         for (int c = 0; c < 5; c++) qout[c] = Q[c];
@@ -175,7 +175,7 @@ __device__ __forceinline__ void bc_ghost(const nuc3d_block_dev &B, double *const
     } else {
         double Q[5];
 #pragma unroll
-        for (int c = 0; c < 5; c++) Q[c] = qb[c][xb];
+        for (int c = 0; c < 5; c++) Q[c] = gld(qb[c] + xb);
         double cs;
         con2prim(Q, jac, P, prim[0], prim[1], prim[2], prim[3], prim[4], cs);
     }
@@ -190,7 +190,7 @@ __device__ __forceinline__ void bc_ghost(const nuc3d_block_dev &B, double *const
         const double W = met_at(B, 6, xb) * prim[1] + met_at(B, 7, xb) * prim[2] + met_at(B, 8, xb) * prim[3];
 #pragma unroll
         for (int m = 0; m < 3; m++) {
-            const double xx = B.dxyz[3 * m][xb], xe = B.dxyz[3 * m + 1][xb], xz = B.dxyz[3 * m + 2][xb];
+            const double xx = gld(B.dxyz[3 * m] + xb), xe = gld(B.dxyz[3 * m + 1] + xb), xz = gld(B.dxyz[3 * m + 2] + xb);
             if (d == 0)
                 prim[1 + m] = -xx * U + xe * V + xz * W;
             else if (d == 1)
@@ -266,11 +266,12 @@ __device__ __forceinline__ double line_diff(const nuc3d_block_dev &B, const nuc3
 {
     const long long st = dstride(B, d);
     const int n = B.n[d], w = K.width[d];
-    if (B.ftype[2 * d] == -1 && t < w) return diff_exterior(f, st, t, true);
-    if (B.ftype[2 * d + 1] == -1 && t >= n - w) return diff_exterior(f, st, n - 1 - t, false);
+    const gdouble *g = gp(f);
+    if (B.ftype[2 * d] == -1 && t < w) return diff_exterior(g, st, t, true);
+    if (B.ftype[2 * d + 1] == -1 && t >= n - w) return diff_exterior(g, st, n - 1 - t, false);
     if constexpr (WIDE)
-        if (K.diff[d]) return cd8(f, st);
-    return cd6(f, st);
+        if (K.diff[d]) return cd8(g, st);
+    return cd6(g, st);
 }
 
 // Same derivative on a line held in LDS (element stride st along the line).
@@ -324,7 +325,7 @@ __device__ __forceinline__ void push_q(const nuc3d_block_dev *blocks, const nuc3
         long long x;
         if (ghost_target(blocks, B, f, i, j, k, nb, x)) {
 #pragma unroll
-            for (int c = 0; c < 5; c++) blocks[nb].q[buf][c][x] = vals[c];
+            for (int c = 0; c < 5; c++) gst(blocks[nb].q[buf][c] + x, vals[c]);
         }
     }
 }
@@ -434,7 +435,7 @@ __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, co
         for (int c = 0; c < 5; c++) r2 += (qn[c] - un5[c]) * (qn[c] - un5[c]);
     fixup_energy(qn, jac, P, nfix);
 #pragma unroll
-    for (int c = 0; c < 5; c++) B.q[bout][c][x] = qn[c];
+    for (int c = 0; c < 5; c++) gst(B.q[bout][c] + x, qn[c]);
     push_q(blocks, B, bout, i, j, k, qn);
     if (stage == 2) {
         double rho, u, v, w, pr, cs;
@@ -459,8 +460,8 @@ __device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const n
     double u5[5], un5[5];
 #pragma unroll
     for (int c = 0; c < 5; c++) {
-        u5[c] = B.q[bin][c][x];
-        un5[c] = stage == 0 ? 0.0 : B.q[bun][c][x];
+        u5[c] = gld(B.q[bin][c] + x);
+        un5[c] = stage == 0 ? 0.0 : gld(B.q[bun][c] + x);
     }
     rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, jac_at(B, x), stage, bout, dt, lmax, nfix, r2);
 }
@@ -480,10 +481,10 @@ __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const
     const long long x = cidx(B, i, j, k);
     double r[5];
 #pragma unroll
-    for (int c = 0; c < 5; c++) r[c] = DIR == 0 ? dF[c] : B.rhs[c][x] + dF[c];
+    for (int c = 0; c < 5; c++) r[c] = DIR == 0 ? dF[c] : gld(B.rhs[c] + x) + dF[c];
     if (DIR < 2 || NS || MODE == 0) {  // Navier-Stokes: k_visc_div completes the cell
 #pragma unroll
-        for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
+        for (int c = 0; c < 5; c++) gst(B.rhs[c] + x, r[c]);
         return;
     }
     rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
@@ -506,7 +507,7 @@ __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks
     const long long x = cidx(B, i, j, k);
     double r[5];
 #pragma unroll
-    for (int c = 0; c < 5; c++) r[c] = B.rhs[c][x] + dF[c];
+    for (int c = 0; c < 5; c++) r[c] = gld(B.rhs[c] + x) + dF[c];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         const double gx = line_diff<WIDE>(B, K, B.fv[v] + x, 0, i);
@@ -516,7 +517,7 @@ __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks
     }
     double un5[5];
 #pragma unroll
-    for (int c = 0; c < 5; c++) un5[c] = stage == 0 ? 0.0 : B.q[bun][c][x];
+    for (int c = 0; c < 5; c++) un5[c] = stage == 0 ? 0.0 : gld(B.q[bun][c] + x);
     rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, jac_at(B, x), stage, bout, dt, lmax, nfix, r2);
 }
 
@@ -900,7 +901,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
         // the positivity fallback is evaluated again on this stage's state: a
         // central face whose stencil means or central face values of rho or p
         // are no longer positive takes WENO5-JS and is recorded as a fallback
-        unsigned char fl = decided ? B.hflag[DIR][fidx] : (unsigned char)0;
+        unsigned char fl = decided ? gldb(B.hflag[DIR] + fidx) : (unsigned char)0;
         if (decided && fl == 0) {  // only central faces can fall back
             const float(*PP)[NSL * NL] = reinterpret_cast<LdsReuse &>(LR).u.prp;
             auto getf = [&](int v, int j) { return PP[v == 0 ? 0 : 1][sidx(line, f + 1 + j)]; };
@@ -913,7 +914,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
                              c8(1) > 0.0f;
             if (fl == 0 && !pos) {
                 fl = 2;
-                B.hflag[DIR][fidx] = 2;
+                gstb(B.hflag[DIR] + fidx, 2);
             }
         }
         weno = decided && (fl == 1 || fl == 2);
@@ -933,7 +934,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
         WSTAT[wv][1] = __popcll(bnet);
         WSTAT[wv][2] = __popcll(bfb);
     }
-    if (DECIDE && active) B.hflag[DIR][fidx] = closure ? 3 : (weno ? (posok ? 1 : 2) : 0);
+    if (DECIDE && active) gstb(B.hflag[DIR] + fidx, closure ? 3 : (weno ? (posok ? 1 : 2) : 0));
     __syncthreads();
     int base = 0, total = 0;
 #pragma unroll
@@ -1314,7 +1315,7 @@ __device__ __forceinline__ void prim4_of(const double *q6, const PhysParams &P, 
 __device__ __forceinline__ void load_q6(const nuc3d_block_dev &B, double *const *qb, long long x, double *q6)
 {
 #pragma unroll
-    for (int c = 0; c < 5; c++) q6[c] = qb[c][x];
+    for (int c = 0; c < 5; c++) q6[c] = gld(qb[c] + x);
     q6[5] = jac_at(B, x);
 }
 
@@ -1459,7 +1460,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UNI ? NUC3D
     auto load_state = [&](long long x, double *q6) {
         if constexpr (UNI) {
 #pragma unroll
-            for (int c = 0; c < 5; c++) q6[c] = qb[c][x];
+            for (int c = 0; c < 5; c++) q6[c] = gld(qb[c] + x);
             q6[5] = B.jac0;
         } else {
             load_q6(B, qb, x, q6);
@@ -1521,14 +1522,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UNI ? NUC3D
         visc_flux_cell(K, P, dxi, deta, dzeta, w4, m + 1, m[0], out);
         const long long x = xc + k * sz;
 #pragma unroll
-        for (int e = 0; e < 12; e++) B.fv[e][x] = out[e];
+        for (int e = 0; e < 12; e++) gst(B.fv[e] + x, out[e]);
 #pragma unroll
         for (int f = 0; f < 6; f++) {
             int nb;
             long long y;
             if (ghost_target(blocks, B, f, i, j, k, nb, y)) {
 #pragma unroll
-                for (int e = 0; e < 12; e++) blocks[nb].fv[e][y] = out[e];
+                for (int e = 0; e < 12; e++) gst(blocks[nb].fv[e] + y, out[e]);
             }
         }
     }
@@ -1573,7 +1574,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
     for (int c = 0; c < 5; c++) pr[c] = 0.0;
     if (incell)
 #pragma unroll
-        for (int v = 0; v < 4; v++) pz[v] = B.fv[8 + v][xc + (kb - H) * sz];
+        for (int v = 0; v < 4; v++) pz[v] = gld(B.fv[8 + v] + xc + (kb - H) * sz);
 
     for (int p = kb - H; p < ke + H; p++) {
         const int k = p - H, buf = p & 1;
@@ -1604,17 +1605,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
         // loads for the next plane, in flight during this plane's work
         if (incell && p + 1 < ke + H)
 #pragma unroll
-            for (int v = 0; v < 4; v++) pz[v] = B.fv[8 + v][xc + (p + 1) * sz];
+            for (int v = 0; v < 4; v++) pz[v] = gld(B.fv[8 + v] + xc + (p + 1) * sz);
         if (k + 1 >= kb && k + 1 < ke) {
             const long long x1 = xc + (k + 1) * sz;
             if (hv)
 #pragma unroll
-                for (int v = 0; v < 4; v++) ph[v] = B.fv[(hx ? 0 : 4) + v][xh + (k + 1) * sz];
+                for (int v = 0; v < 4; v++) ph[v] = gld(B.fv[(hx ? 0 : 4) + v] + xh + (k + 1) * sz);
             if (incell) {
 #pragma unroll
-                for (int e = 0; e < 8; e++) pxy[e] = B.fv[e][x1];
+                for (int e = 0; e < 8; e++) pxy[e] = gld(B.fv[e] + x1);
 #pragma unroll
-                for (int c = 0; c < 5; c++) pr[c] = B.rhs[c][x1];
+                for (int c = 0; c < 5; c++) pr[c] = gld(B.rhs[c] + x1);
             }
         }
         __syncthreads();
@@ -1632,7 +1633,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
         }
         if (MODE == 0) {
 #pragma unroll
-            for (int c = 0; c < 5; c++) B.rhs[c][x] = r[c];
+            for (int c = 0; c < 5; c++) gst(B.rhs[c] + x, r[c]);
         } else {
             rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
         }
