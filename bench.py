@@ -516,8 +516,9 @@ def main():
     sweep_tiles = plain.sweep_tiles(0)
     # the dominant kernel's FP64 VALU figures, from the committed counters
     nf_dom = {"x_sweep": sweep_tiles[0], "y_sweep": sweep_tiles[1], "z_sweep": sweep_tiles[2]}.get(dom)
-    dom_kernel = ("k_sweep_%s<%d, 0, true, %d, %d, false>" % (variant, {"x_sweep": 0, "y_sweep": 1, "z_sweep": 2}[dom],
-                                                              1 if dom == "z_sweep" else 0, nf_dom)
+    dom_kernel = ("k_sweep_%s<%d, 0, true, %d, %d, false, false>" % (variant, {"x_sweep": 0, "y_sweep": 1,
+                                                                            "z_sweep": 2}[dom],
+                                                                     1 if dom == "z_sweep" else 0, nf_dom)
                   if nf_dom else "k_visc_flux_%s<3, true>" % variant)
     valu = read_valu(dom_kernel, n, variant)
 

@@ -68,8 +68,8 @@ def test_committed_valu_summary(variant):
     counter summary: every plain stage kernel is listed, its FP64 rate is its
     FP64 instructions (FMA as two flops) over its duration, and the step's
     FP64 instructions per cell-update are three stages of the four kernels."""
-    ks = ["k_sweep_%s<0, 0, true, 0, 32, false>", "k_sweep_%s<1, 0, true, 0, 32, false>",
-          "k_sweep_%s<2, 0, true, 1, 16, false>", "k_visc_flux_%s<3, true>"]
+    ks = ["k_sweep_%s<0, 0, true, 0, 32, false, false>", "k_sweep_%s<1, 0, true, 0, 32, false, false>",
+          "k_sweep_%s<2, 0, true, 1, 16, false, false>", "k_visc_flux_%s<3, true>"]
     vs = [bench.read_valu(k % variant, 256, variant) for k in ks]
     assert all(v is not None for v in vs)
     for v in vs:
