@@ -167,6 +167,9 @@ struct nuc3d_case {
     // persistent direction sweeps over the device work queue (nuc3d_set_persistent)
     bool persistent = false;
     unsigned *dqueue = nullptr;        // NUC3D_QUEUE_WORDS counters, zero between launches
+    // range flags of the three state buffers (K.qok when the case's gamma and
+    // Mach numbers allow the shared reciprocal of rho), zero after a refresh
+    unsigned *dqok = nullptr;
     hipStream_t vstream = nullptr;
     hipEvent_t vfork = nullptr, vjoin = nullptr;
     // captured two-step graphs, one per parity of the eigenvalue buffers
@@ -341,8 +344,16 @@ int refresh(nuc3d_case *c, hipStream_t s)
     }
     int alluni = 1;
     for (const auto &B : c->hb) alluni = alluni && B.uniform;
-    if (alluni != c->K.alluni) {  // the captured steps hold the launch parameters by value
+    // the shared reciprocal of rho needs gamma in [1, 8] and every Mach number
+    // in [2^-40, 2^40] (the temperature's numerator gamma Ma^2 p stays in the
+    // reduced division's range); the flags restart at zero with every refresh
+    bool qok_allowed = c->K.gamma >= 1.0 && c->K.gamma <= 8.0;
+    for (const auto &B : c->hb) qok_allowed = qok_allowed && B.mach >= 0x1p-40 && B.mach <= 0x1p40;
+    unsigned *qok = qok_allowed ? c->dqok : nullptr;
+    HIPCHK(hipMemsetAsync(c->dqok, 0, sizeof(unsigned) * 4, s));
+    if (alluni != c->K.alluni || qok != c->K.qok) {  // the captured steps hold the launch parameters by value
         c->K.alluni = alluni;
+        c->K.qok = qok;
         for (auto &g : c->gexec)
             if (g) {
                 (void)hipGraphExecDestroy(g);
@@ -571,6 +582,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     K.hcount = nullptr;
     K.pq = nullptr;
     K.pq_n = 0;
+    K.qok = nullptr;
     K.ieee_div = 0;
     K.hrhs = 0;
     K.ngroups = 1;
@@ -709,6 +721,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
     if (hipMalloc(&c->duniform, sizeof(int) * nblocks) != hipSuccess ||
         hipMalloc(&c->dconsts, sizeof(double) * 8) != hipSuccess ||
         hipMalloc(&c->dqueue, sizeof(unsigned) * NUC3D_QUEUE_WORDS) != hipSuccess ||
+        hipMalloc(&c->dqok, sizeof(unsigned) * 4) != hipSuccess ||
         hipMalloc(&c->db, sizeof(nuc3d_block_dev) * nblocks) != hipSuccess ||
         hipMalloc(&c->dtiles, sizeof(nuc3d_tile) * tiles.size()) != hipSuccess ||
         hipMalloc(&c->ddt, sizeof(double) * NUC3D_MAX_BLOCKS) != hipSuccess ||
@@ -730,6 +743,7 @@ int nuc3d_case_create(const nuc3d_params *p, int nblocks, const int *dims, nuc3d
         hipMemset(c->dlam, 0, sizeof(unsigned long long) * 6 * nblocks) != hipSuccess ||
         hipMemset(c->dcount, 0, sizeof(unsigned long long) * 4) != hipSuccess ||
         hipMemset(c->dqueue, 0, sizeof(unsigned) * NUC3D_QUEUE_WORDS) != hipSuccess ||
+        hipMemset(c->dqok, 0, sizeof(unsigned) * 4) != hipSuccess ||
         hipMemset(c->dhcount, 0, sizeof(unsigned long long) * 16 * NUC3D_HCOUNT_SLOTS) != hipSuccess) {
         nuc3d_case_destroy(c);
         return fail(NUC3D_ERR_HIP, "initialising control arrays failed");
@@ -756,6 +770,7 @@ int nuc3d_case_destroy(nuc3d_case *c)
     (void)hipFree(c->duniform);
     (void)hipFree(c->dconsts);
     (void)hipFree(c->dqueue);
+    (void)hipFree(c->dqok);
     (void)hipFree(c->dtiles);
     for (int d = 0; d < 3; d++)
         for (int g = 0; g < 2; g++) (void)hipFree(c->dsweep[d][g]);

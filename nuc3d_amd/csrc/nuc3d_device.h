@@ -50,11 +50,14 @@ __device__ __forceinline__ double sq(double x) { return x * x; }
 // global memory, and these access them as such.
 typedef __attribute__((address_space(1))) double gdouble;
 typedef __attribute__((address_space(1))) unsigned char gbyte;
+typedef __attribute__((address_space(1))) unsigned gword;
 __device__ __forceinline__ const gdouble *gp(const double *p) { return (const gdouble *)p; }
 __device__ __forceinline__ double gld(const double *p) { return *(const gdouble *)p; }
 __device__ __forceinline__ void gst(double *p, double v) { *(gdouble *)p = v; }
 __device__ __forceinline__ unsigned char gldb(const unsigned char *p) { return *(const gbyte *)p; }
 __device__ __forceinline__ void gstb(unsigned char *p, unsigned char v) { *(gbyte *)p = v; }
+__device__ __forceinline__ unsigned gldw(const unsigned *p) { return *(const gword *)p; }
+__device__ __forceinline__ void gstw(unsigned *p, unsigned v) { *(gword *)p = v; }
 
 // ---- FP64 division in two halves ----
 // The compiler's IEEE division a/b on gfx950 is v_div_scale (of b and of a),
@@ -374,19 +377,46 @@ __device__ __forceinline__ double face_exterior(const double *s, bool plus, int 
 struct PhysParams {
     double gamma, mach, reynolds, prandtl, sref, twall;
     int riemann;
+    // the state read is in the range of state_in_range: the divisions by rho
+    // share one reciprocal (same quotients)
+    int rdq = 0;
 };
 
+// A state whose divisions by rho = Q0 J may share one reciprocal (recip_nr):
+// rho in [2^-200, 2^200], the momenta Q1..Q3 (conserved / J) and gamma p
+// each zero or of magnitude in [2^-400, 2^400].  With gamma in [1, 8] and
+// Mach numbers in [2^-40, 2^40] the temperature's numerator gamma Ma^2 p is
+// then zero or within [2^-500, 2^500]; every quotient is normal and every
+// numerator far above 2^-969.
+__device__ __forceinline__ bool momenta_in_range(const double *Q, double rho)
+{
+    return rd_denominator(rho) && rd_operand(Q[1]) && rd_operand(Q[2]) && rd_operand(Q[3]);
+}
+
 // con2prim (physicsModel.cpp:313-353).  Returns p; sets rho,u,v,w,c.
+// P.rdq: the four divisions by rho (the velocities and c^2) through one
+// reciprocal; yr (may be null) receives it for further divisions by rho.
 __device__ __forceinline__ void con2prim(const double *Q, double jac, const PhysParams &P, double &rho, double &u,
-                                         double &v, double &w, double &p, double &c)
+                                         double &v, double &w, double &p, double &c, double *yr = nullptr)
 {
     rho = Q[0] * jac;
-    u = Q[1] / rho * jac;
-    v = Q[2] / rho * jac;
-    w = Q[3] / rho * jac;
-    double E = Q[4] * jac;
-    p = (E - 0.5 * rho * (u * u + v * v + w * w)) * (P.gamma - 1.0);
-    c = sqrt(fabs(P.gamma * p / rho));
+    if (P.rdq) {
+        const double y = recip_nr(rho);
+        u = div_nrs(Q[1], rho, y) * jac;
+        v = div_nrs(Q[2], rho, y) * jac;
+        w = div_nrs(Q[3], rho, y) * jac;
+        double E = Q[4] * jac;
+        p = (E - 0.5 * rho * (u * u + v * v + w * w)) * (P.gamma - 1.0);
+        c = sqrt(fabs(div_nrs(P.gamma * p, rho, y)));
+        if (yr) *yr = y;
+    } else {
+        u = Q[1] / rho * jac;
+        v = Q[2] / rho * jac;
+        w = Q[3] / rho * jac;
+        double E = Q[4] * jac;
+        p = (E - 0.5 * rho * (u * u + v * v + w * w)) * (P.gamma - 1.0);
+        c = sqrt(fabs(P.gamma * p / rho));
+    }
 }
 
 // Per-thread counts of a stage: e<0 fix-ups, and states that stay unusable
@@ -408,21 +438,35 @@ struct StageCounts {
 // (E - rho|u|^2/2)(gamma - 1) must not be negative (physicsModel.cpp:1009-1016,
 // where it exits).  Counts the fix-up and a non-positive density or negative
 // pressure (either one NaN) in cnt.  Returns true if Q[4] was replaced.
-__device__ __forceinline__ bool fixup_energy(double *Q, double jac, const PhysParams &P, StageCounts &cnt)
+// rd: the state passed momenta_in_range (the divisions by rho share one
+// reciprocal).  pr (may be null) receives the pressure con2prim computes from
+// the final state, the one every later reader of it sees.
+__device__ __forceinline__ bool fixup_energy(double *Q, double jac, const PhysParams &P, StageCounts &cnt,
+                                             bool rd = false, double *pr = nullptr)
 {
     double rho = Q[0] * jac;
-    double u = Q[1] / rho * jac;
-    double v = Q[2] / rho * jac;
-    double w = Q[3] / rho * jac;
+    double u, v, w;
+    if (rd) {
+        const double y = recip_nr(rho);
+        u = div_nrs(Q[1], rho, y) * jac;
+        v = div_nrs(Q[2], rho, y) * jac;
+        w = div_nrs(Q[3], rho, y) * jac;
+    } else {
+        u = Q[1] / rho * jac;
+        v = Q[2] / rho * jac;
+        w = Q[3] / rho * jac;
+    }
     double E = Q[4] * jac;
     const double ke = 0.5 * rho * (u * u + v * v + w * w);
     double e0 = E - ke;
     bool fixed = false;
+    if (pr) *pr = e0 * (P.gamma - 1.0);
     if (e0 < 0.0) {
         double g = P.gamma, m = P.mach;
         E = (pow(rho, g) / ((g * m * m) * (g - 1.0)) + 0.5 * rho * (u * u + v * v + w * w));
         Q[4] = E / jac;
         e0 = E - ke;
+        if (pr) *pr = (Q[4] * jac - ke) * (P.gamma - 1.0);
         cnt.fix++;
         fixed = true;
     }

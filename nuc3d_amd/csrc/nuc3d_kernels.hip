@@ -47,6 +47,16 @@ __device__ __forceinline__ PhysParams phys_of(const nuc3d_kparams &K, const nuc3
     return P;
 }
 
+// The same for a kernel reading the state of buffer bin: the divisions by rho
+// of its con2prim share one reciprocal where every cell of that buffer passed
+// momenta_in_range when it was written (K.qok[bin], kept by the RK stages)
+__device__ __forceinline__ PhysParams phys_of(const nuc3d_kparams &K, const nuc3d_block_dev &B, int bin)
+{
+    PhysParams P = phys_of(K, B);
+    P.rdq = K.qok && !K.ieee_div && gldw(K.qok + bin) != 0u;
+    return P;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
 // with its own L2.  Tile lists are ordered so that neighbouring tiles (whose
 // halos overlap) are adjacent; this gives XCD x the contiguous range of tiles
@@ -397,6 +407,10 @@ __device__ __forceinline__ double stage_dt(const nuc3d_block_dev *blocks, const 
                                           double *dtbuf, const unsigned long long *lam_cur,
                                           unsigned long long *lam_next)
 {
+    // the buffer the next stage writes (stage 0 writes 1, 1 writes 2, 2 writes
+    // 0) is not read before that stage has rewritten all of it: its range flag
+    // is set here and cleared there by any cell out of range
+    if (K.qok && blockIdx.x == 0 && threadIdx.x == 0) gstw(K.qok + (stage == 0 ? 2 : stage - 1), 1u);
     if (stage != 0) return dtbuf[g];
     const double dt = group_dt(blocks, lam_cur, K.nblocks, g, K.cfl);
     if (blockIdx.x == 0)
@@ -410,11 +424,16 @@ __device__ __forceinline__ double stage_dt(const nuc3d_block_dev *blocks, const 
 // TVD-RK3 stage of cell x from RHS r (TVD-RK3.cpp:37-137) with the cell's
 // current state u5, u_n (un5, stages 1 and 2) and J already loaded; e<0
 // fix-up (physicsModel.cpp:322-332), ghost push, and the eigenvalue maxima of
-// the new state at stage 2 (physicsModel.cpp:557-597).
+// the new state at stage 2 (physicsModel.cpp:557-597).  qok (may be null):
+// the range flag of buffer bout, cleared if the new state is out of the range
+// of momenta_in_range or its gamma p out of rd_operand's; rd: this kernel
+// may use the reduced division (then so do the fix-up and the eigenvalues of
+// a state in range).
 __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
                                                const PhysParams &P, int i, int j, int k, long long x, const double *r,
                                                const double *u5, const double *un5, double jac, int stage, int bout,
-                                               double dt, double *lmax, StageCounts &nfix, double &r2)
+                                               double dt, double *lmax, StageCounts &nfix, double &r2,
+                                               unsigned *qok, bool rd)
 {
     double qn[5];
 #pragma unroll
@@ -433,13 +452,19 @@ __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, co
     if (stage == 2)
 #pragma unroll
         for (int c = 0; c < 5; c++) r2 += (qn[c] - un5[c]) * (qn[c] - un5[c]);
-    fixup_energy(qn, jac, P, nfix);
+    const bool inr = qok && momenta_in_range(qn, qn[0] * jac);
+    double pw = 0.0;
+    fixup_energy(qn, jac, P, nfix, rd && inr, &pw);
+    PhysParams Pn = P;
+    Pn.rdq = inr && rd_operand(P.gamma * pw);
+    if (qok && !Pn.rdq) gstw(qok, 0u);
+    Pn.rdq = Pn.rdq && rd;
 #pragma unroll
     for (int c = 0; c < 5; c++) gst(B.q[bout][c] + x, qn[c]);
     push_q(blocks, B, bout, i, j, k, qn);
     if (stage == 2) {
         double rho, u, v, w, pr, cs;
-        con2prim(qn, jac, P, rho, u, v, w, pr, cs);
+        con2prim(qn, jac, Pn, rho, u, v, w, pr, cs);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const double mx = met_at(B, 3 * d, x), my = met_at(B, 3 * d + 1, x), mz = met_at(B, 3 * d + 2, x);
@@ -453,9 +478,9 @@ __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, co
 
 // The same, loading the cell's state from buffers bin / bun.
 __device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
-                                          const PhysParams &P, int i, int j, int k, long long x, const double *r,
-                                          int stage, int bin, int bun, int bout, double dt, double *lmax,
-                                          StageCounts &nfix, double &r2)
+                                          const nuc3d_kparams &K, const PhysParams &P, int i, int j, int k,
+                                          long long x, const double *r, int stage, int bin, int bun, int bout,
+                                          double dt, double *lmax, StageCounts &nfix, double &r2)
 {
     double u5[5], un5[5];
 #pragma unroll
@@ -463,7 +488,8 @@ __device__ __forceinline__ void rk_update(const nuc3d_block_dev *blocks, const n
         u5[c] = gld(B.q[bin][c] + x);
         un5[c] = stage == 0 ? 0.0 : gld(B.q[bun][c] + x);
     }
-    rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, jac_at(B, x), stage, bout, dt, lmax, nfix, r2);
+    rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, jac_at(B, x), stage, bout, dt, lmax, nfix, r2,
+                   K.qok ? K.qok + bout : nullptr, !K.ieee_div);
 }
 
 // Cell (i,j,k) receives the flux difference dF of direction DIR:
@@ -487,7 +513,7 @@ __device__ __forceinline__ void finish_cell(const nuc3d_block_dev *blocks, const
         for (int c = 0; c < 5; c++) gst(B.rhs[c] + x, r[c]);
         return;
     }
-    rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
+    rk_update(blocks, B, K, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
 }
 
 // Navier-Stokes stages: the z sweep also finishes its cells with the viscous
@@ -518,7 +544,8 @@ __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks
     double un5[5];
 #pragma unroll
     for (int c = 0; c < 5; c++) un5[c] = stage == 0 ? 0.0 : gld(B.q[bun][c] + x);
-    rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, jac_at(B, x), stage, bout, dt, lmax, nfix, r2);
+    rk_update_vals(blocks, B, P, i, j, k, x, r, u5, un5, jac_at(B, x), stage, bout, dt, lmax, nfix, r2,
+                   K.qok ? K.qok + bout : nullptr, !K.ieee_div);
 }
 
 }  // namespace
@@ -544,7 +571,8 @@ __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks
 // or 32 faces x 8 lines (fewer halo cells staged per face; the autotuner picks
 // per block shape and direction).
 template <int DIR, int SCHEME, bool NS, int MODE, int NF = 16, bool WIDE = false, bool PERS = false>
-__global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__restrict__ blocks,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS && DIR == 2 && MODE == 1 ? 5 : 1))) void KNAME(
+    k_sweep)(const nuc3d_block_dev *__restrict__ blocks,
                                                       const int4 *__restrict__ tiles, nuc3d_kparams K, int stage,
                                                       int bin, int bun, int bout, double *dtbuf,
                                                       const unsigned long long *lam_cur,
@@ -566,7 +594,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
     auto tile = [&](const int witem, const int tid) __attribute__((always_inline)) {
         const int4 t = tiles[witem];
         const nuc3d_block_dev &B = blocks[t.x];
-        const PhysParams P = phys_of(K, B);
+        const PhysParams P = phys_of(K, B, bin);
         const int a0 = t.y, l0 = t.z, cc = t.w;
         const int nc = min(NC, B.n[DIR] - a0);
         const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
@@ -707,7 +735,8 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep)(const nuc3d_block_dev *__r
 // spread over slots) and kept per face in B.hflag[DIR]: 0 central, 1 network,
 // 2 fallback, 3 physical-face closure.
 template <int DIR, bool NS, int MODE, bool DECIDE, bool WIDE = false>
-__global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_dev *__restrict__ blocks,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS && DIR == 2 && MODE == 1 ? 5 : 1))) void KNAME(
+    k_sweep_hybrid)(const nuc3d_block_dev *__restrict__ blocks,
                                                              const int4 *__restrict__ tiles, nuc3d_kparams K,
                                                              int stage, int bin, int bun, int bout, double *dtbuf,
                                                              const unsigned long long *lam_cur,
@@ -770,7 +799,7 @@ __global__ __launch_bounds__(256) void KNAME(k_sweep_hybrid)(const nuc3d_block_d
     const int witem = xcd_tile(blockIdx.x, gridDim.x);
     const int4 t = tiles[witem];
     const nuc3d_block_dev &B = blocks[t.x];
-    const PhysParams P = phys_of(K, B);
+    const PhysParams P = phys_of(K, B, bin);
     const int a0 = t.y, l0 = t.z, cc = t.w;
     const int nc = min(NC, B.n[DIR] - a0);
     const int nlines = min(NL, (DIR == 0 ? B.n[1] : B.n[0]) - l0);
@@ -1091,7 +1120,7 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
     const int4 *lg = lines + (long long)witem * LW;
     if (threadIdx.x < LW) lsh[threadIdx.x] = lg[threadIdx.x];
     const nuc3d_block_dev &B = blocks[lg[0].x];
-    const PhysParams P = phys_of(K, B);
+    const PhysParams P = phys_of(K, B, bin);
     const int n = B.n[DIR], w = K.width[DIR], M = n - 2 * w + 1;
     double *const *qin = B.q[bin];
     __syncthreads();
@@ -1304,12 +1333,13 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 // u, v, w, T of a cell from its conserved state Q/J (q6[0..4]) and J (q6[5])
 __device__ __forceinline__ void prim4_of(const double *q6, const PhysParams &P, double *w4)
 {
-    double rho, u, v, w, p, cs;
-    con2prim(q6, q6[5], P, rho, u, v, w, p, cs);
+    double rho, u, v, w, p, cs, y = 0.0;
+    con2prim(q6, q6[5], P, rho, u, v, w, p, cs, &y);
     w4[0] = u;
     w4[1] = v;
     w4[2] = w;
-    w4[3] = P.gamma * P.mach * P.mach * p / rho;  // physicsModel.cpp:1029
+    // physicsModel.cpp:1029
+    w4[3] = P.rdq ? div_nrs(P.gamma * P.mach * P.mach * p, rho, y) : P.gamma * P.mach * P.mach * p / rho;
 }
 
 __device__ __forceinline__ void load_q6(const nuc3d_block_dev &B, double *const *qb, long long x, double *q6)
@@ -1435,7 +1465,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UNI ? NUC3D
     __shared__ double Wp[2][4][LY][LX];  // u,v,w,T of plane k with the xy halo, double-buffered
     const nuc3d_tile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const nuc3d_block_dev &B = blocks[tl.b];
-    const PhysParams P = phys_of(K, B);
+    const PhysParams P = phys_of(K, B, bin);
     double *const *qb = B.q[bin];
     const int ti = threadIdx.x & (TX - 1), tj = threadIdx.x / TX;
     const int tw = min(TX, B.n[0] - tl.i0), th = min(TY, B.n[1] - tl.j0);
@@ -1635,7 +1665,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NUC3D_VISC_
 #pragma unroll
             for (int c = 0; c < 5; c++) gst(B.rhs[c] + x, r[c]);
         } else {
-            rk_update(blocks, B, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
+            rk_update(blocks, B, K, P, i, j, k, x, r, stage, bin, bun, bout, dt, lmax, nfix, r2);
         }
     }
     if (MODE == 1) {

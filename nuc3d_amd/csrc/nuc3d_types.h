@@ -117,6 +117,11 @@ struct nuc3d_kparams {
     unsigned *pq;
     int pq_n;
     int pad_q;
+    // range flags of the three state buffers: 1 where every cell of the buffer
+    // passed momenta_in_range (and gamma p rd_operand's range) when the stage
+    // that wrote it last wrote it, so that the divisions by rho of its readers
+    // share one reciprocal; null: not kept (every reader divides by rho)
+    unsigned *qok;
 };
 #define NUC3D_HCOUNT_SLOTS 256
 #define NUC3D_QUEUE_WORDS 16

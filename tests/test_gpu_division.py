@@ -55,6 +55,11 @@ CASES = {
     "tiny_state_ns_rhs": (dict(), lambda: _tiny(1e-45), 0, 0.5),
     "tinier_state_ns_rhs": (dict(), lambda: _tiny(1e-160), 0, 0.5),
     "tiny_state_ns_weno5z_rhs": (dict(scheme=(solver.WENO5Z,) * 3), lambda: _tiny(1e-45), 0, 0.5),
+    # momenta below the range of the shared reciprocal of rho (subnormal, and
+    # normal below 2^-400) at a quarter of the cells of a resting field, which
+    # keeps them over the steps: every RK stage writes such cells, so every
+    # reader of the state divides by rho with the IEEE division
+    "resting_tiny_momenta_ns_steps3": (dict(), lambda: _resting_tiny_momenta(), 3, 0.5),
 }
 
 
@@ -68,6 +73,31 @@ def _tiny(scale):
     b = cases.taylor_green(dims=(16, 12, 10), noise=1e-3, seed=8)[0]
     b.q = b.q * scale
     return [b]
+
+
+def _resting_tiny_momenta():
+    """Uniform pressure, density with 1e-3 noise, no flow; the x and z momenta
+    of a quarter of the cells set to +-[1e-320, 2e-308] (subnormal) or
+    +-[1e-300, 1e-125] (normal, below 2^-400).  The inviscid momentum fluxes
+    stay uniform to the last bit, so the momenta change only through the
+    viscous terms of their own size and stay out of range."""
+    dims = (16, 12, 10)
+    h = tuple(2.0 * np.pi / d for d in dims)
+    phys = cases.Physics()
+    rng = np.random.default_rng(23)
+    shape = dims[::-1]
+    rho = 1.0 + 1e-3 * rng.uniform(-1.0, 1.0, shape)
+    p = np.full(shape, 1.0 / (phys.gamma * phys.mach * phys.mach))
+    zero = np.zeros(shape)
+    jac, met, dxyz = cases.cartesian_metrics(dims, h)
+    q = cases.conserved_over_j(rho, zero, zero, zero, p, phys.gamma, jac)
+    for c in (1, 3):
+        pick = rng.uniform(size=shape) < 0.25
+        sub = rng.uniform(size=shape) < 0.5
+        mag = np.where(sub, 10.0 ** rng.uniform(-320, np.log10(2e-308), shape), 10.0 ** rng.uniform(-300, -125, shape))
+        q[c] = np.where(pick, np.where(rng.uniform(size=shape) < 0.5, -mag, mag), 0.0)
+    assert (np.abs(q[1]) < 2.0 ** -400).all() and (np.abs(q[1][q[1] != 0.0]) < 2.3e-308).any()
+    return [cases.BlockSpec(dims, cases.periodic_faces(), jac, met, dxyz, q)]
 
 
 @pytest.mark.parametrize("exact", [1, 0])
