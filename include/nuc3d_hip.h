@@ -410,8 +410,9 @@ int nuc3d_has_crweno5(void);
  * compares the two on n hashed operand pairs of that range (x7 quotients:
  * one reciprocal serves three numerators and a signed zero, whose quotient
  * keeps the IEEE sign, and the sign-keeping correction serves two numerators
- * and a signed zero over |b|) on the current device and returns the number of
- * quotients whose bits differ (0 expected). */
+ * and a signed zero over |b|), and the square root of |a / b| by the same
+ * shortening of the compiler's FP64 square root, on the current device, and
+ * returns the number of results whose bits differ (0 expected). */
 int nuc3d_selftest_division(long long n, unsigned long long seed, long long *mismatches);
 /* ieee = 1: the case's kernels use the compiler's IEEE division everywhere
  * instead (results are the same bit for bit; for tests). */

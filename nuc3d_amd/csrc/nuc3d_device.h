@@ -106,6 +106,25 @@ __device__ __forceinline__ double div_nrs(double a, double b, double y)
     const double r = __builtin_fma(-b, q, a);
     return -__builtin_fma(-r, y, -q);
 }
+// sqrt(x) for x normal, at least 2^-767 and finite: the compiler's FP64
+// square root on gfx950 (v_rsq_f64 and the Goldschmidt-style refinement of
+// the LLVM AMDGPU lowering) without its steps for tiny arguments (scaled by
+// 2^256 before and 2^-128 after) and for 0 and infinity (passed through),
+// which leave such an x unchanged: the same instructions on the same values,
+// so the same result.  nuc3d_selftest_division compares it with sqrt.
+__device__ __forceinline__ double sqrt_nr(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
 // Operands of divisions that share a reciprocal.  A positive denominator in
 // [2^-200, 2^200] (rd_denominator); numerators zero or of magnitude in
 // [2^-600, 2^600], such as p (rd_operand: 0 or in [2^-400, 2^400]) and its
@@ -383,19 +402,22 @@ struct PhysParams {
 };
 
 // A state whose divisions by rho = Q0 J may share one reciprocal (recip_nr):
-// rho in [2^-200, 2^200], the momenta Q1..Q3 (conserved / J) and gamma p
-// each zero or of magnitude in [2^-400, 2^400].  With gamma in [1, 8] and
-// Mach numbers in [2^-40, 2^40] the temperature's numerator gamma Ma^2 p is
-// then zero or within [2^-500, 2^500]; every quotient is normal and every
-// numerator far above 2^-969.
+// rho in [2^-200, 2^200], the momenta Q1..Q3 (conserved / J) each zero or of
+// magnitude in [2^-400, 2^400], and gamma p of magnitude in [2^-400, 2^400]
+// (gamma_p_in_range).  With gamma in [1, 8] and Mach numbers in [2^-40, 2^40]
+// the temperature's numerator gamma Ma^2 p is then within [2^-500, 2^500];
+// every quotient is normal and every numerator far above 2^-969, and c^2 =
+// |gamma p / rho| lies in [2^-600, 2^600], where sqrt_nr is the square root.
 __device__ __forceinline__ bool momenta_in_range(const double *Q, double rho)
 {
     return rd_denominator(rho) && rd_operand(Q[1]) && rd_operand(Q[2]) && rd_operand(Q[3]);
 }
+__device__ __forceinline__ bool gamma_p_in_range(double gp) { return gp != 0.0 && rd_operand(gp); }
 
 // con2prim (physicsModel.cpp:313-353).  Returns p; sets rho,u,v,w,c.
 // P.rdq: the four divisions by rho (the velocities and c^2) through one
-// reciprocal; yr (may be null) receives it for further divisions by rho.
+// reciprocal, and c by sqrt_nr; yr (may be null) receives the reciprocal for
+// further divisions by rho.
 __device__ __forceinline__ void con2prim(const double *Q, double jac, const PhysParams &P, double &rho, double &u,
                                          double &v, double &w, double &p, double &c, double *yr = nullptr)
 {
@@ -407,7 +429,7 @@ __device__ __forceinline__ void con2prim(const double *Q, double jac, const Phys
         w = div_nrs(Q[3], rho, y) * jac;
         double E = Q[4] * jac;
         p = (E - 0.5 * rho * (u * u + v * v + w * w)) * (P.gamma - 1.0);
-        c = sqrt(fabs(div_nrs(P.gamma * p, rho, y)));
+        c = sqrt_nr(fabs(div_nrs(P.gamma * p, rho, y)));
         if (yr) *yr = y;
     } else {
         u = Q[1] / rho * jac;

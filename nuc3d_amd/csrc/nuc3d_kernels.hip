@@ -426,7 +426,7 @@ __device__ __forceinline__ double stage_dt(const nuc3d_block_dev *blocks, const 
 // fix-up (physicsModel.cpp:322-332), ghost push, and the eigenvalue maxima of
 // the new state at stage 2 (physicsModel.cpp:557-597).  qok (may be null):
 // the range flag of buffer bout, cleared if the new state is out of the range
-// of momenta_in_range or its gamma p out of rd_operand's; rd: this kernel
+// of momenta_in_range or its gamma p out of gamma_p_in_range's; rd: this kernel
 // may use the reduced division (then so do the fix-up and the eigenvalues of
 // a state in range).
 __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, const nuc3d_block_dev &B,
@@ -456,7 +456,7 @@ __device__ __forceinline__ void rk_update_vals(const nuc3d_block_dev *blocks, co
     double pw = 0.0;
     fixup_energy(qn, jac, P, nfix, rd && inr, &pw);
     PhysParams Pn = P;
-    Pn.rdq = inr && rd_operand(P.gamma * pw);
+    Pn.rdq = inr && gamma_p_in_range(P.gamma * pw);
     if (qok && !Pn.rdq) gstw(qok, 0u);
     Pn.rdq = Pn.rdq && rd;
 #pragma unroll

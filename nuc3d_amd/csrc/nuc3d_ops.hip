@@ -401,13 +401,15 @@ __global__ __launch_bounds__(256) void k_selftest_division(long long n, unsigned
         // and the same over |b| through div_nrs, whose correction keeps the
         // IEEE sign of a zero over a positive denominator
         const double bp = fabs(b), yp = nuc3d_dev::recip_nr(bp);
-        const double q[7] = {nuc3d_dev::div_nr(a, b, y),    nuc3d_dev::div_nr(a2, b, y),
+        // and the square root of |a / b| (in [2^-700, 2^700] for a != 0) by sqrt_nr
+        const double sq = fabs(a / b);
+        const double q[8] = {nuc3d_dev::div_nr(a, b, y),    nuc3d_dev::div_nr(a2, b, y),
                              nuc3d_dev::div_nr(a3, b, y),   nuc3d_dev::div_nrz(z, b, y),
                              nuc3d_dev::div_nrs(a, bp, yp), nuc3d_dev::div_nrs(a3, bp, yp),
-                             nuc3d_dev::div_nrs(z, bp, yp)};
-        const double r[7] = {a / b, a2 / b, a3 / b, z / b, a / bp, a3 / bp, z / bp};
+                             nuc3d_dev::div_nrs(z, bp, yp), a == 0.0 ? 0.0 : nuc3d_dev::sqrt_nr(sq)};
+        const double r[8] = {a / b, a2 / b, a3 / b, z / b, a / bp, a3 / bp, z / bp, sqrt(sq)};
 #pragma unroll
-        for (int k = 0; k < 7; k++)
+        for (int k = 0; k < 8; k++)
             if (__double_as_longlong(q[k]) != __double_as_longlong(r[k])) bad++;
     }
     if (bad) atomicAdd(mismatch, bad);

@@ -118,7 +118,7 @@ struct nuc3d_kparams {
     int pq_n;
     int pad_q;
     // range flags of the three state buffers: 1 where every cell of the buffer
-    // passed momenta_in_range (and gamma p rd_operand's range) when the stage
+    // passed momenta_in_range and gamma_p_in_range when the stage
     // that wrote it last wrote it, so that the divisions by rho of its readers
     // share one reciprocal; null: not kept (every reader divides by rho)
     unsigned *qok;

@@ -167,7 +167,8 @@ def lib():
 
 def selftest_division(n, seed=0):
     """Quotients of the reduced FP64 division that differ from a / b over n
-    hashed operand pairs (x7) on the device."""
+    hashed operand pairs (x7), and square roots of |a / b| by the shortened
+    square root that differ from sqrt, on the device."""
     v = ctypes.c_longlong()
     _check(lib().nuc3d_selftest_division(int(n), int(seed), ctypes.byref(v)), "selftest_division")
     return v.value

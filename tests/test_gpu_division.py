@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 def test_selftest_division():
     n = 1 << 28
-    print("case: %d hashed operand pairs x 7 quotients" % n, flush=True)
+    print("case: %d hashed operand pairs x 7 quotients and a square root" % n, flush=True)
     for seed in (1, 2):
         assert solver.selftest_division(n, seed) == 0
 
