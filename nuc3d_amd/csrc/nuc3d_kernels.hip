@@ -814,6 +814,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS && DIR =
     };
 
     const int nslots = nc + 2 * H;
+    const int line = DIR == 0 ? threadIdx.x / NF : threadIdx.x % NL;
+    const int f = DIR == 0 ? threadIdx.x % NF : threadIdx.x / NL;
+    const int n = B.n[DIR], w = K.width[DIR];
+    const int m = a0 + f;
+    const bool active = line < nlines && f <= nc;
+    const bool closure = (B.ftype[2 * DIR] == -1 && m < w) || (B.ftype[2 * DIR + 1] == -1 && m > n - w);
+    const bool decided = active && !closure;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the face's index in the decision array of direction DIR (a single RHS
+    // evaluation keeps its decisions in the second half)
+    long long fidx = 0;
+    {
+        int i, j, k;
+        cell(line, m, i, j, k);
+        if (DIR == 0) fidx = m + (long long)(B.n[0] + 1) * (j + (long long)B.n[1] * k);
+        else if (DIR == 1) fidx = m + (long long)(B.n[1] + 1) * (k + (long long)B.n[2] * i);
+        else fidx = m + (long long)(B.n[2] + 1) * (i + (long long)B.n[0] * j);
+        if (K.hrhs) fidx += (long long)(B.n[DIR] + 1) * B.n[(DIR + 1) % 3] * B.n[(DIR + 2) % 3];
+    }
+    // the stored decision of the face, loaded before the staging so that the
+    // load's latency overlaps it
+    unsigned char fl0 = 0;
+    if constexpr (!DECIDE) fl0 = decided ? gldb(B.hflag[DIR] + fidx) : (unsigned char)0;
     // whether every staged split flux is in the range of the reduced division
     // (the WENO5-JS faces of the workgroup then use it; same quotients)
     bool rd_ok = true;
@@ -862,25 +885,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS && DIR =
     __syncthreads();
     const bool rd = WCNT[0] && WCNT[1] && WCNT[2] && WCNT[3] && !K.ieee_div;
 
-    const int line = DIR == 0 ? threadIdx.x / NF : threadIdx.x % NL;
-    const int f = DIR == 0 ? threadIdx.x % NF : threadIdx.x / NL;
-    const int n = B.n[DIR], w = K.width[DIR];
-    const int m = a0 + f;
-    const bool active = line < nlines && f <= nc;
-    const bool closure = (B.ftype[2 * DIR] == -1 && m < w) || (B.ftype[2 * DIR + 1] == -1 && m > n - w);
-    const bool decided = active && !closure;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // the face's index in the decision array of direction DIR (a single RHS
-    // evaluation keeps its decisions in the second half)
-    long long fidx = 0;
-    {
-        int i, j, k;
-        cell(line, m, i, j, k);
-        if (DIR == 0) fidx = m + (long long)(B.n[0] + 1) * (j + (long long)B.n[1] * k);
-        else if (DIR == 1) fidx = m + (long long)(B.n[1] + 1) * (k + (long long)B.n[2] * i);
-        else fidx = m + (long long)(B.n[2] + 1) * (i + (long long)B.n[0] * j);
-        if (K.hrhs) fidx += (long long)(B.n[DIR] + 1) * B.n[(DIR + 1) % 3] * B.n[(DIR + 2) % 3];
-    }
     bool posok = false, weno = false;
     if constexpr (DECIDE) {
         // every face: its 30 features and the positivity fallback (the stencil's
@@ -930,7 +934,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS && DIR =
         // the positivity fallback is evaluated again on this stage's state: a
         // central face whose stencil means or central face values of rho or p
         // are no longer positive takes WENO5-JS and is recorded as a fallback
-        unsigned char fl = decided ? gldb(B.hflag[DIR] + fidx) : (unsigned char)0;
+        unsigned char fl = fl0;
         if (decided && fl == 0) {  // only central faces can fall back
             const float(*PP)[NSL * NL] = reinterpret_cast<LdsReuse &>(LR).u.prp;
             auto getf = [&](int v, int j) { return PP[v == 0 ? 0 : 1][sidx(line, f + 1 + j)]; };
