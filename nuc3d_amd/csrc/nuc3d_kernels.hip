@@ -1321,9 +1321,10 @@ __global__ __launch_bounds__(64) void KNAME(k_crweno)(const nuc3d_block_dev *__r
 // arithmetic; one barrier per plane separates the LDS fill from its use.
 
 // Waves per SIMD the viscous kernels are register-limited to.  Measured at
-// 256^3: both run fastest unconstrained (about 224 / 231 VGPRs, 2 waves per
-// SIMD, every load of the next plane in flight); capping them at 3 or 4 waves
-// spills and is slower.
+// 256^3: both run fastest unconstrained (187 VGPRs for the viscous fluxes on
+// the uniform-mesh path, 227 with the metric fields, 235 for the divergence
+// pass: 2 waves per SIMD, every load of the next plane in flight); capping
+// them at 3 or 4 waves spills and is slower.
 #ifndef NUC3D_VISC_FLUX_WAVES
 #define NUC3D_VISC_FLUX_WAVES 2
 #endif
@@ -1865,9 +1866,9 @@ static void launch_sweep(const nuc3d_block_dev *blocks, const int4 *tiles, int n
     // persistent launches exist for the WENO5-JS / WENO5-Z sweeps that keep
     // their occupancy in that form (x and y, and the Euler z sweep: 77-79
     // VGPRs, six workgroups per CU; the Navier-Stokes z sweep, which also
-    // takes the viscous divergence and the RK stage, would need 148 VGPRs,
-    // three per CU, against 95 and five); every other kernel takes one
-    // workgroup per tile
+    // takes the viscous divergence and the RK stage, needed 148 VGPRs in that
+    // form, three workgroups per CU against five); every other kernel takes
+    // one workgroup per tile
     auto go = [&](Kernel kern, bool pers = false) {
         nuc3d_kparams Kl = K;
         if (!pers) Kl.pq = nullptr;

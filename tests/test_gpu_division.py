@@ -112,3 +112,34 @@ def test_reduced_division_bit_identical(name, exact):
     for x, y in zip(a, b):
         assert np.isfinite(x).all()
         np.testing.assert_array_equal(x.view(np.int64), y.view(np.int64))
+
+
+def test_range_flags_across_graphs_and_uploads():
+    """The range flags of the state buffers (set by one RK stage for the
+    buffer the next one writes, cleared by a cell out of range, zeroed by a
+    refresh) through captured and eager steps and through new states uploaded
+    between steps: in range, then momenta out of range, then in range again.
+    The same bits as the IEEE division throughout."""
+    print("case: TGV 16x12x10 NS, graph and eager steps across uploads of in- and out-of-range states", flush=True)
+    blocks = cases.taylor_green(dims=(16, 12, 10), noise=1e-3, seed=31)
+    tiny = _resting_tiny_momenta()[0]
+    outs = []
+    for ieee in (False, True):
+        g = solver.Case(solver.Params(), blocks)
+        g.set_division(ieee)
+        g.steps_graph(2, 0.5)
+        g.steps(1, 0.5)
+        a = g.get_q(0)
+        g.set_q(0, tiny.q)
+        g.steps_graph(2, 0.5)
+        b = g.get_q(0)
+        g.set_q(0, a)
+        g.steps(1, 0.5)
+        g.steps_graph(4, 0.5)
+        c = g.get_q(0)
+        g.sync()
+        outs.append((a, b, c))
+        g.close()
+    for x, y in zip(*outs):
+        assert np.isfinite(x).all()
+        np.testing.assert_array_equal(x.view(np.int64), y.view(np.int64))
