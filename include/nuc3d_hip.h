@@ -237,7 +237,8 @@ int nuc3d_get_dt_group(nuc3d_case *c, int group, double *dt);
  * Runs between full steps. */
 int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out);
 /* The autotuner also picks, per block shape and direction, the tile shape of
- * the WENO5-JS / WENO5-Z direction sweeps: 16 faces x 16 lines per workgroup
+ * the WENO5-JS / WENO5-Z direction sweeps and of the learned hybrid's x and y
+ * sweeps: 16 faces x 16 lines per workgroup
  * (the default) or 32 faces x 8 lines (fewer halo cells staged per face, half
  * as many lines side by side), timing each shape's candidates on the
  * direction's own launches in the context of the whole case (the x and y
@@ -245,8 +246,8 @@ int nuc3d_autotune(nuc3d_case *c, void *stream, int *kseg_out);
  * nuc3d_rhs result is overwritten).  Results do not change.  The choice of
  * block b (faces per tile line for x, y, z) is read with
  * nuc3d_get_sweep_tiles and can be set with nuc3d_set_sweep_tiles (32 only for
- * WENO5-JS and WENO5-Z directions; between full steps; the residual history
- * restarts). */
+ * WENO5-JS and WENO5-Z directions and the hybrid's x and y; between full
+ * steps; the residual history restarts). */
 int nuc3d_get_sweep_tiles(nuc3d_case *c, int b, int *nf3);
 int nuc3d_set_sweep_tiles(nuc3d_case *c, int b, const int *nf3);
 

@@ -1528,24 +1528,29 @@ int set_column_tiles(nuc3d_case *c, const std::vector<int> &kseg)
 // whether direction d may take 32-face tiles
 bool wide_tiles_ok(const nuc3d_case *c, int d)
 {
+    if (c->p.scheme[d] == NUC3D_HYBRID) return d < 2;  // the hybrid's x and y sweeps
     if (c->p.scheme[d] != NUC3D_WENO5JS && c->p.scheme[d] != NUC3D_WENO5Z) return false;
     const bool cd8 = c->K.diff[0] || c->K.diff[1] || c->K.diff[2];
     return !(d == 2 && cd8 && c->p.model == NUC3D_NAVIERSTOKES3D);
 }
 
-// Tile shape of the WENO5-JS / WENO5-Z direction sweeps per block shape and
-// direction: 16 faces x 16 lines or 32 faces x 8 lines (fewer halo cells
-// staged per face, half-width rows in the transverse directions).  Each
-// shape's candidates are timed in the context of the whole case (the other
-// blocks keep their choice) on the direction's own launches: the x and y
-// sweeps as in an RHS evaluation, the z sweep as in stage 0 (into the stage's
-// output buffer, which the next stage 0 writes again; the fix-up and
-// positivity counts are restored).  The last nuc3d_rhs result is overwritten.
+// Tile shape of the WENO5-JS / WENO5-Z direction sweeps (and of the hybrid's
+// x and y sweeps) per block shape and direction: 16 faces x 16 lines or 32
+// faces x 8 lines (fewer halo cells staged per face, half-width rows in the
+// transverse directions).  Each shape's candidates are timed in the context
+// of the whole case (the other blocks keep their choice) on the direction's
+// own launches: the x and y sweeps as in an RHS evaluation (the hybrid's with
+// its stored decisions, whose positivity fallback on this state is the one
+// the next stage makes), the z sweep as in stage 0 (into the stage's output
+// buffer, which the next stage 0 writes again; the fix-up and positivity
+// counts are restored).  The last nuc3d_rhs result is overwritten.
 int tune_sweeps(nuc3d_case *c, hipStream_t s)
 {
     const bool ns = c->p.model == NUC3D_NAVIERSTOKES3D;
+    // the hybrid's sweeps are timed once its sensor is set
+    auto tunable = [c](int d) { return wide_tiles_ok(c, d) && (c->p.scheme[d] != NUC3D_HYBRID || c->K.sensor); };
     bool any = false;
-    for (int d = 0; d < 3; d++) any = any || c->p.scheme[d] == NUC3D_WENO5JS || c->p.scheme[d] == NUC3D_WENO5Z;
+    for (int d = 0; d < 3; d++) any = any || tunable(d);
     if (!any) return NUC3D_OK;
     unsigned long long saved[4];
     HIPCHK(hipStreamSynchronize(s));
@@ -1566,7 +1571,7 @@ int tune_sweeps(nuc3d_case *c, hipStream_t s)
         return e;
     };
     for (int d = 0; d < 3 && rc == 0; d++) {
-        if (!wide_tiles_ok(c, d)) continue;
+        if (!tunable(d)) continue;
         std::vector<int> done(c->nb, 0);
         for (int b = 0; b < c->nb && rc == 0; b++) {
             if (done[b]) continue;
@@ -1707,8 +1712,9 @@ int nuc3d_set_sweep_tiles(nuc3d_case *c, int b, const int *nf3)
     for (int d = 0; d < 3; d++) {
         if (nf3[d] != 16 && nf3[d] != 32) return fail(NUC3D_ERR_BAD_ARG, "faces per tile line must be 16 or 32");
         if (nf3[d] == 32 && !wide_tiles_ok(c, d))
-            return fail(NUC3D_ERR_UNSUPPORTED, "32-face tiles are built for the WENO5-JS and WENO5-Z sweeps only "
-                                               "(and not for the viscous z sweep of a case with cd8 directions)");
+            return fail(NUC3D_ERR_UNSUPPORTED, "32-face tiles are built for the WENO5-JS and WENO5-Z sweeps and the "
+                                               "hybrid's x and y sweeps only (and not for the viscous z sweep of a "
+                                               "case with cd8 directions)");
     }
     for (int d = 0; d < 3; d++) c->swnf[b][d] = nf3[d];
     return build_sweeps(c);

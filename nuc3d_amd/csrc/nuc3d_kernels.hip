@@ -734,7 +734,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS && DIR =
 // network, WENO5-JS by the fallback; one atomic per workgroup and count,
 // spread over slots) and kept per face in B.hflag[DIR]: 0 central, 1 network,
 // 2 fallback, 3 physical-face closure.
-template <int DIR, bool NS, int MODE, bool DECIDE, bool WIDE = false>
+template <int DIR, bool NS, int MODE, bool DECIDE, bool WIDE = false, int NFH = 16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS && DIR == 2 && MODE == 1 ? 5 : 1))) void KNAME(
     k_sweep_hybrid)(const nuc3d_block_dev *__restrict__ blocks,
                                                              const int4 *__restrict__ tiles, nuc3d_kparams K,
@@ -745,7 +745,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS && DIR =
 {
     using namespace nuc3d_sensor;
     constexpr int H = 4;
-    constexpr int NL = 16, NF = 16, NC = NF - 1, NSL = NC + 2 * H, NST = 2 * H;
+    // NFH faces per line and 256 / NFH lines per workgroup: 16 x 16, or 32 x 8
+    // for the x and y sweeps (fewer halo cells staged per face)
+    constexpr int NF = NFH, NL = 256 / NF, NC = NF - 1, NSL = NC + 2 * H, NST = 2 * H;
+    static_assert(NL * NF == 256, "one face per thread");
     __shared__ double S[10][NSL * NL];
     // Where decisions are made, one area has three lives: the FP32 roundings
     // of rho, u, v, w, p of the staged cells; then the bf16 features of the
@@ -1900,6 +1903,13 @@ static void launch_sweep(const nuc3d_block_dev *blocks, const int4 *tiles, int n
         }
     }
     if constexpr (SCHEME == SCH_HYBRID) {
+        if constexpr (DIR < 2) {
+            if (nf == 32) {
+                if (decide) go(KNAME(k_sweep_hybrid)<DIR, NS, MODE, true, false, 32>);
+                else go(KNAME(k_sweep_hybrid)<DIR, NS, MODE, false, false, 32>);
+                return;
+            }
+        }
         if (decide) go(KNAME(k_sweep_hybrid)<DIR, NS, MODE, true>);
         else go(KNAME(k_sweep_hybrid)<DIR, NS, MODE, false>);
     } else {
@@ -1937,7 +1947,8 @@ extern "C" hipError_t KNAME(nuc3d_launch_sweep)(int dir, int scheme, int ns, int
                                                 const unsigned long long *lam_cur, unsigned long long *lam_next,
                                                 unsigned long long *counters, hipStream_t st)
 {
-    if (nf != 16 && !(nf == 32 && (scheme == SCH_WENO5JS || scheme == SCH_WENO5Z))) return hipErrorInvalidValue;
+    if (nf != 16 && !(nf == 32 && (scheme == SCH_WENO5JS || scheme == SCH_WENO5Z || (scheme == SCH_HYBRID && dir < 2))))
+        return hipErrorInvalidValue;
 #define NUC3D_A dir, mode, nf, blocks, tiles, ntiles, K, stage, bin, bun, bout, dtbuf, lam_cur, lam_next, counters, st
     if (scheme == SCH_WENO5JS) {
         if (ns) sweep_dispatch<SCH_WENO5JS, true>(NUC3D_A);

@@ -347,3 +347,65 @@ def test_hybrid_rhs_between_stages_keeps_step_decisions(interval):
     g.stage(0, 0.5)
     assert sum(g.hybrid_counts()) > sum(counts)
     g.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+def test_hybrid_wide_tiles_bit_identical(exact):
+    """The hybrid's x and y sweeps on 32-face x 8-line tiles give the same
+    states, decisions and counts as on 16 x 16 tiles: two blocks with walls
+    (closure faces at tile edges), deciding and reusing stages, a single RHS
+    between stages, eager and graph-launched steps."""
+    print("case: hybrid NS 2 blocks walls, sweep tiles (16,16,16) vs (32,32,16), exact=%d" % exact, flush=True)
+    spec = cases.taylor_green(dims=(40, 22, 12), noise=3e-2, seed=29)[0]
+    spec.faces = [(-1, 2, None), (-1, 2, None), (-1, 3, None), (-1, 3, None), (0, 5, None), (0, 4, None)]
+    blocks = cases.split_block_x(spec, 2)
+    sp, _ = hybrid_params(solver.NAVIERSTOKES3D, exact, t_wall=1.2)
+    params = mixed_params(6)
+    out = []
+    for nf in ((16, 16, 16), (32, 32, 16)):
+        g = solver.Case(sp, blocks)
+        for b in range(2):
+            g.set_sweep_tiles(b, nf)
+        g.set_sensor(params, 0.0)
+        g.set_sensor_interval(2)
+        g.steps(1, 0.5)
+        g.stage(0, 0.5)
+        g.rhs()
+        r = [g.get_rhs(b) for b in range(2)]
+        g.stage(1, 0.5)
+        g.stage(2, 0.5)
+        g.steps_graph(2, 0.5)
+        g.sync()
+        out.append(([g.get_q(b) for b in range(2)], r, [g.hybrid_flags(b, d) for b in range(2) for d in range(3)],
+                    g.hybrid_counts(), g.fixups()))
+        g.close()
+    (qa, ra, fa, ca, xa), (qb, rb, fb, cb, xb) = out
+    for a, b in zip(qa + ra, qb + rb):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(fa, fb):
+        np.testing.assert_array_equal(a, b)
+    assert ca == cb and xa == xb
+    print("  counts %s" % (ca,), flush=True)
+
+
+def test_hybrid_autotune_tiles():
+    """The autotuner times both tile shapes of the hybrid's x and y sweeps
+    (the z sweep keeps 16 x 16) and the steps after it equal an untuned
+    case's."""
+    print("case: hybrid NS TGV 32^3, autotuned sweep tiles", flush=True)
+    blocks = cases.taylor_green(32, noise=1e-2, seed=31)
+    sp, _ = hybrid_params(solver.NAVIERSTOKES3D, 1)
+    params = mixed_params(7)
+    qs = []
+    for tune in (True, False):
+        g = solver.Case(sp, blocks)
+        g.set_sensor(params, 0.0)
+        if tune:
+            g.autotune()
+            nf = g.sweep_tiles(0)
+            print("  sweep tiles %s" % (nf,), flush=True)
+            assert nf[0] in (16, 32) and nf[1] in (16, 32) and nf[2] == 16
+        g.steps_graph(2, 0.5)
+        qs.append(g.get_q(0))
+        g.close()
+    np.testing.assert_array_equal(qs[0], qs[1])
