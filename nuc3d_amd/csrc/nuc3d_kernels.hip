@@ -570,13 +570,15 @@ __device__ __forceinline__ void finish_cell_visc_z(const nuc3d_block_dev *blocks
 // NF faces per line and 256 / NF lines per workgroup: 16 x 16 (the default)
 // or 32 faces x 8 lines (fewer halo cells staged per face; the autotuner picks
 // per block shape and direction).
+// The Navier-Stokes z sweep (viscous divergence and RK stage) is built for
+// five waves per SIMD: 91 VGPRs, where the compiler's own choice (94-97)
+// would drop it to four once the RK stage's range check is added.
+#define NUC3D_SWEEP_WAVES(NS, DIR, MODE) amdgpu_waves_per_eu((NS) && (DIR) == 2 && (MODE) == 1 ? 5 : 1)
 template <int DIR, int SCHEME, bool NS, int MODE, int NF = 16, bool WIDE = false, bool PERS = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS && DIR == 2 && MODE == 1 ? 5 : 1))) void KNAME(
-    k_sweep)(const nuc3d_block_dev *__restrict__ blocks,
-                                                      const int4 *__restrict__ tiles, nuc3d_kparams K, int stage,
-                                                      int bin, int bun, int bout, double *dtbuf,
-                                                      const unsigned long long *lam_cur,
-                                                      unsigned long long *lam_next, unsigned long long *counters)
+__global__ __launch_bounds__(256) __attribute__((NUC3D_SWEEP_WAVES(NS, DIR, MODE))) void KNAME(k_sweep)(
+    const nuc3d_block_dev *__restrict__ blocks, const int4 *__restrict__ tiles, nuc3d_kparams K, int stage, int bin,
+    int bun, int bout, double *dtbuf, const unsigned long long *lam_cur, unsigned long long *lam_next,
+    unsigned long long *counters)
 {
     // 3-cell halo (21 slots per line) for WENO5; the central scheme reaches 4 cells (23 slots)
     constexpr int H = SCHEME == SCH_CENTRAL8 ? 4 : NUC3D_HALO;
@@ -735,13 +737,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS && DIR =
 // spread over slots) and kept per face in B.hflag[DIR]: 0 central, 1 network,
 // 2 fallback, 3 physical-face closure.
 template <int DIR, bool NS, int MODE, bool DECIDE, bool WIDE = false, int NFH = 16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS && DIR == 2 && MODE == 1 ? 5 : 1))) void KNAME(
-    k_sweep_hybrid)(const nuc3d_block_dev *__restrict__ blocks,
-                                                             const int4 *__restrict__ tiles, nuc3d_kparams K,
-                                                             int stage, int bin, int bun, int bout, double *dtbuf,
-                                                             const unsigned long long *lam_cur,
-                                                             unsigned long long *lam_next,
-                                                             unsigned long long *counters)
+__global__ __launch_bounds__(256) __attribute__((NUC3D_SWEEP_WAVES(NS, DIR, MODE))) void KNAME(k_sweep_hybrid)(
+    const nuc3d_block_dev *__restrict__ blocks, const int4 *__restrict__ tiles, nuc3d_kparams K, int stage, int bin,
+    int bun, int bout, double *dtbuf, const unsigned long long *lam_cur, unsigned long long *lam_next,
+    unsigned long long *counters)
 {
     using namespace nuc3d_sensor;
     constexpr int H = 4;
