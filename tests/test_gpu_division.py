@@ -44,6 +44,12 @@ CASES = {
                                                 riemann=solver.AUSMP),
                                            lambda: cases.shu_osher(48, 10, 8, seed=2), 3, 0.4),
     "walls_multiblock_ns_steps2": (dict(t_wall=1.2), lambda: _walls(), 2, 0.5),
+    # CRWENO5 lines read the state; their Euler z lines, or with Navier-Stokes
+    # the separate divergence pass, write it (and keep its range flag)
+    "tgv_ns_crweno5_steps3": (dict(scheme=(solver.CRWENO5,) * 3),
+                              lambda: cases.taylor_green(16, noise=1e-3, seed=6), 3, 0.5),
+    "shuosher_euler_crweno5_steps3": (dict(model=solver.EULER3D, mach=1.0, scheme=(solver.CRWENO5,) * 3),
+                                      lambda: cases.shu_osher(48, 10, 8, seed=3), 3, 0.4),
     "hybrid_ns_steps3": (dict(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4)),
                          lambda: cases.taylor_green(dims=(20, 18, 16), noise=5e-2, seed=9), 3, 0.5),
     "hybrid_walls_ns_rhs": (dict(scheme=(solver.HYBRID,) * 3, width=(4, 4, 4), t_wall=1.2), lambda: _walls(), 0, 0.5),
