@@ -409,3 +409,17 @@ def test_hybrid_autotune_tiles():
         qs.append(g.get_q(0))
         g.close()
     np.testing.assert_array_equal(qs[0], qs[1])
+
+
+def test_hybrid_z_tiles_stay_16():
+    """32-face tiles exist for the hybrid's x and y sweeps only: asking for
+    them in z is refused with NUC3D_ERR_UNSUPPORTED and changes nothing."""
+    blocks = cases.taylor_green(16, noise=1e-3, seed=2)
+    sp, _ = hybrid_params(solver.NAVIERSTOKES3D, 1)
+    g = solver.Case(sp, blocks)
+    with pytest.raises(solver.Nuc3dError, match="32-face tiles"):
+        g.set_sweep_tiles(0, (32, 32, 32))
+    assert tuple(g.sweep_tiles(0)) == (16, 16, 16)
+    g.set_sweep_tiles(0, (32, 32, 16))
+    assert tuple(g.sweep_tiles(0)) == (32, 32, 16)
+    g.close()
