@@ -751,15 +751,21 @@ __global__ __launch_bounds__(256) __attribute__((NUC3D_SWEEP_WAVES(NS, DIR, MODE
     __shared__ double S[10][NSL * NL];
     // Where decisions are made, one area has three lives: the FP32 roundings
     // of rho, u, v, w, p of the staged cells; then the bf16 features of the
-    // 256 faces; then the face fluxes.  Where they are reused, the face fluxes
-    // go to the split-flux area (below).
+    // 256 faces, 15 words (30 features) per face, rows of 60 B (an odd number
+    // of words: consecutive rows start on different banks), whose first word
+    // then takes the face's logit (a wavefront reads and writes only the rows
+    // of its own faces); then the face fluxes and the list of WENO5-JS faces.
+    // 32-face tiles: 40.4 KB of LDS per workgroup, four per CU.  Where
+    // decisions are reused, the face fluxes go to the split-flux area (below).
     struct LdsDecide {
         union {
             float pr[5][NSL * NL];
-            unsigned xf[256][20];  // 16 words per face, rows of 80 B: 8 rows cover the 32 banks
-            double hf[5][NF * NL];
+            unsigned xf[256 * 15];
+            struct {
+                double hf[5][NF * NL];
+                unsigned short lst[256];
+            } fl;
         } u;
-        float zl[256];
     };
     struct LdsNone {
         union {
@@ -773,13 +779,15 @@ __global__ __launch_bounds__(256) __attribute__((NUC3D_SWEEP_WAVES(NS, DIR, MODE
     constexpr bool HF_IN_S = !DECIDE;
     static_assert(10 * NSL >= 5 * NF, "the face fluxes fit the split-flux area");
     __shared__ __align__(16) std::conditional_t<DECIDE, LdsDecide, LdsNone> U;
-    double(*HF)[NF * NL] = HF_IN_S ? reinterpret_cast<double(*)[NF * NL]>(&S[0][0])
-                                   : reinterpret_cast<double(*)[NF * NL]>(&U.u.hf[0][0]);
+    double(*HF)[NF * NL] = nullptr;
     float(*PR)[NSL * NL] = nullptr;
-    float *ZL = nullptr;
+    unsigned *XF = nullptr;
     if constexpr (DECIDE) {
+        HF = U.u.fl.hf;
         PR = U.u.pr;
-        ZL = U.zl;
+        XF = U.u.xf;
+    } else {
+        HF = reinterpret_cast<double(*)[NF * NL]>(&S[0][0]);
     }
     // Where decisions are reused, the FP32 roundings of rho and p of the
     // staged cells, for the positivity check every stage repeats; the list of
@@ -790,11 +798,13 @@ __global__ __launch_bounds__(256) __attribute__((NUC3D_SWEEP_WAVES(NS, DIR, MODE
             unsigned short lst[256];
         } u;
     };
-    struct LdsList {
-        unsigned short lst[256];
+    struct LdsNoReuse {
+        unsigned pad[4];
     };
-    __shared__ std::conditional_t<DECIDE, LdsList, LdsReuse> LR;
-    unsigned short *LST = reinterpret_cast<unsigned short *>(&LR);
+    __shared__ std::conditional_t<DECIDE, LdsNoReuse, LdsReuse> LR;
+    unsigned short *LST = nullptr;
+    if constexpr (DECIDE) LST = U.u.fl.lst;
+    else LST = reinterpret_cast<LdsReuse &>(LR).u.lst;
     __shared__ int WCNT[4];
     __shared__ int WSTAT[4][3];
 
@@ -908,13 +918,15 @@ __global__ __launch_bounds__(256) __attribute__((NUC3D_SWEEP_WAVES(NS, DIR, MODE
         }
         __syncthreads();  // the primitives are dead: the area takes the features
         {
-            uint4 *row = reinterpret_cast<uint4 *>(U.u.xf[threadIdx.x]);
+            // features 30 and 31 (word 15) are zero and not stored
+            unsigned *row = XF + 15 * threadIdx.x;
 #pragma unroll
-            for (int i = 0; i < 4; i++) row[i] = make_uint4(xw[4 * i], xw[4 * i + 1], xw[4 * i + 2], xw[4 * i + 3]);
+            for (int i = 0; i < 15; i++) row[i] = xw[i];
         }
         __syncthreads();
         // the network: wavefront wv evaluates its 64 faces in four MFMA tiles of
-        // 16; lane group g takes features 8g..8g+7 (words 4g..4g+3 of a row)
+        // 16; lane group g takes features 8g..8g+7 (words 4g..4g+3 of a row);
+        // the logit of face tf then takes the first word of its row
         {
             Regs R;
             load_regs_lanes(K.sensor, R);
@@ -922,14 +934,16 @@ __global__ __launch_bounds__(256) __attribute__((NUC3D_SWEEP_WAVES(NS, DIR, MODE
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int tf = 64 * wv + 16 * q + (lane & 15);
-                const uint4 v4 = *reinterpret_cast<const uint4 *>(&U.u.xf[tf][4 * g]);
+                const unsigned *row = XF + 15 * tf + 4 * g;
+                const unsigned w3 = row[g < 3 ? 3 : 2];  // lane group 3: word 15 is zero (not stored)
+                const uint4 v4 = make_uint4(row[0], row[1], row[2], g < 3 ? w3 : 0u);
                 Acts A;
                 const float z = forward16(R, __builtin_bit_cast(bf16x8, v4), A);
-                if (g == 0) ZL[tf] = z;
+                if (g == 0) XF[15 * tf] = __float_as_uint(z);
             }
         }
         __syncthreads();  // the features are dead: the area takes the face fluxes
-        weno = decided && (!posok || ZL[threadIdx.x] >= K.sensor->thr);
+        weno = decided && (!posok || __uint_as_float(XF[15 * threadIdx.x]) >= K.sensor->thr);
     } else {
         // stages 1 and 2 of a step (and every stage of a step that reuses the
         // previous step's decisions) take the stored network decisions, but
