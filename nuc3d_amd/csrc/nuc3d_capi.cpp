@@ -19,6 +19,14 @@
 #include "../../include/nuc3d_ops.h"
 #include "nuc3d_types.h"
 
+// Navier-Stokes z sweeps: i-tiles per group of the work-item order (the tiles
+// of a group that share a run of j rows and a k segment are neighbours in the
+// list, so the i halo of the viscous divergence is read from one XCD's L2;
+// 4 measures 1.5% faster per z sweep on 256^3 than 1, and 1% than 16)
+#ifndef NUC3D_ZRUN_IGROUP
+#define NUC3D_ZRUN_IGROUP 4
+#endif
+
 #define NUC3D_LAUNCHERS(SFX)                                                                                         \
     extern "C" hipError_t nuc3d_launch_sweep_##SFX(int, int, int, int, int, const nuc3d_block_dev *, const int4 *,   \
                                                    int, nuc3d_kparams, int, int, int, int, double *,                 \
@@ -402,10 +410,13 @@ int build_sweeps(nuc3d_case *c)
                 // stencil spans j-3..j+3: runs of 8 rows in j at one k segment are
                 // neighbours in the list (they run together and share one XCD's
                 // L2), and so are the k segments of one run (their k halos overlap)
-                for (int l0 = 0; l0 < B.n[0]; l0 += nl)
+                // and the runs of a group of i-tiles (the fv_xi i halo)
+                for (int g0 = 0; g0 < B.n[0]; g0 += NUC3D_ZRUN_IGROUP * nl)
                     for (int j0 = 0; j0 < B.n[1]; j0 += 8)
                         for (int a0 = 0; a0 < B.n[2]; a0 += ncell)
-                            for (int j = j0; j < std::min(j0 + 8, B.n[1]); j++) L.push_back(make_int4(b, a0, l0, j));
+                            for (int l0 = g0; l0 < std::min(g0 + NUC3D_ZRUN_IGROUP * nl, B.n[0]); l0 += nl)
+                                for (int j = j0; j < std::min(j0 + 8, B.n[1]); j++)
+                                    L.push_back(make_int4(b, a0, l0, j));
             } else {
                 for (int j = 0; j < B.n[1]; j++)
                     for (int l0 = 0; l0 < B.n[0]; l0 += nl)
