@@ -26,6 +26,9 @@
 #ifndef NUC3D_ZRUN_IGROUP
 #define NUC3D_ZRUN_IGROUP 4
 #endif
+#ifndef NUC3D_ZRUN_J
+#define NUC3D_ZRUN_J 8  // j rows per run
+#endif
 
 #define NUC3D_LAUNCHERS(SFX)                                                                                         \
     extern "C" hipError_t nuc3d_launch_sweep_##SFX(int, int, int, int, int, const nuc3d_block_dev *, const int4 *,   \
@@ -412,10 +415,10 @@ int build_sweeps(nuc3d_case *c)
                 // L2), and so are the k segments of one run (their k halos overlap)
                 // and the runs of a group of i-tiles (the fv_xi i halo)
                 for (int g0 = 0; g0 < B.n[0]; g0 += NUC3D_ZRUN_IGROUP * nl)
-                    for (int j0 = 0; j0 < B.n[1]; j0 += 8)
+                    for (int j0 = 0; j0 < B.n[1]; j0 += NUC3D_ZRUN_J)
                         for (int a0 = 0; a0 < B.n[2]; a0 += ncell)
                             for (int l0 = g0; l0 < std::min(g0 + NUC3D_ZRUN_IGROUP * nl, B.n[0]); l0 += nl)
-                                for (int j = j0; j < std::min(j0 + 8, B.n[1]); j++)
+                                for (int j = j0; j < std::min(j0 + NUC3D_ZRUN_J, B.n[1]); j++)
                                     L.push_back(make_int4(b, a0, l0, j));
             } else {
                 for (int j = 0; j < B.n[1]; j++)
